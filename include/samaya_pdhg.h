@@ -1,0 +1,126 @@
+/* C ABI of the MI355X (gfx950) PDHG device layer: the drop-in for samaya's gpu::PdhgDevice seam
+ * and its CSR SpMV. Plain C: opaque handles, pointers and sizes; every call returns an int
+ * status (SMY_OK = 0) and never unwinds. smy_last_error() returns the message of the calling
+ * thread's last failure.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the samaya tree):
+ *   class gpu::PdhgDevice ........ src/gpu/pdhg_core.hpp:24-79 (implementation pdhg_core.cu)
+ *   gpu::csr_spmv ................ src/gpu/spmv.hpp:16-21
+ *   gpu::spmv_threads_per_row .... src/gpu/spmv.hpp:23
+ *   gpu::gpu_available/device_name src/gpu/device.hpp:34-63
+ */
+#ifndef SAMAYA_PDHG_H
+#define SAMAYA_PDHG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SMY_OK = 0,
+  SMY_NO_DEVICE = 1,    /* no usable gfx950 GPU: PdhgDevice::create returned null */
+  SMY_TOO_LARGE = 2,    /* nnz >= INT_MAX (32-bit CSR offsets): create returned null */
+  SMY_ERROR = 3,        /* a HIP failure (driver, out of memory): the reference throws */
+  SMY_INVALID = 4,      /* bad argument */
+  SMY_STATE = 5         /* call out of order (e.g. set_original missing before evaluate) */
+};
+
+enum { SMY_FP64 = 0, SMY_FP32 = 1 }; /* iterate and matrix-value precision of the step kernels */
+enum { SMY_CURRENT = 0, SMY_AVERAGE = 1 }; /* PdhgDevice::Iterate (pdhg_core.hpp:53) */
+
+/* A compressed sparse column matrix as samaya's SparseMatrix holds it
+ * (include/samaya/sparse_matrix.hpp:36-69): start[cols + 1] (64-bit), index[nnz], value[nnz],
+ * row indices sorted within each column. */
+typedef struct smy_csc {
+  int32_t rows;
+  int32_t cols;
+  const int64_t* start;
+  const int32_t* index;
+  const double* value;
+} smy_csc;
+
+typedef struct smy_pdhg smy_pdhg;
+
+/* Replaces gpu::gpu_available (device.hpp:34-48): 1 when a gfx950 device is present. */
+int smy_gpu_available(void);
+/* Replaces gpu::device_name (device.hpp:51-63): writes the device name (empty if none). */
+int smy_gpu_device_name(char* buf, size_t len);
+/* Message of this thread's last failed call ("" if none). */
+const char* smy_last_error(void);
+
+/* Replaces PdhgDevice::create (pdhg_core.hpp:27-30, pdhg_core.cu:242-273): uploads A (m x n,
+ * CSC), its transpose At (CSC, n x m) and cost/lower/upper (n + m each, equilibrated space).
+ * precision SMY_FP32 also keeps FP32 copies of the values and iterates for the step kernels;
+ * evaluation sums stay FP64 either way. *out is NULL unless SMY_OK is returned. */
+int smy_pdhg_create(const smy_csc* A, const smy_csc* At, const double* cost, const double* lower,
+                    const double* upper, int precision, smy_pdhg** out);
+/* Replaces ~PdhgDevice. */
+void smy_pdhg_destroy(smy_pdhg* h);
+/* Replaces PdhgDevice::set_original (pdhg_core.hpp:49-51): equilibration scales and the cost
+ * and bounds before equilibration (n + m), used by evaluate. */
+int smy_pdhg_set_original(smy_pdhg* h, const double* col_scale, const double* row_scale,
+                          const double* cost, const double* lower, const double* upper);
+/* Replaces PdhgDevice::step (pdhg_core.hpp:36): v_new, y_new and the averages with weight
+ * 1 / (iter + 1). Asynchronous. */
+int smy_pdhg_step(smy_pdhg* h, double tau, double sigma, double theta, long long iter);
+/* Replaces PdhgDevice::advance (pdhg_core.hpp:38): v_prev <- v <- v_new, y <- y_new. */
+int smy_pdhg_advance(smy_pdhg* h);
+/* Replaces PdhgDevice::restart (pdhg_core.hpp:40): v_prev <- v. */
+int smy_pdhg_restart(smy_pdhg* h);
+/* Replaces PdhgDevice::download (pdhg_core.hpp:42-44): copies iterates to host arrays (n + m or
+ * m doubles each); any pointer may be NULL. Synchronizes. */
+int smy_pdhg_download(smy_pdhg* h, double* v, double* v_new, double* y, double* y_new,
+                      double* v_erg, double* y_erg);
+/* Replaces PdhgDevice::evaluate (pdhg_core.hpp:67): out[7] = ||Ax-r||^2, dual residual^2,
+ * primal objective, dual objective, ||x||^2, ||r||^2, ||y||^2 of the current or average iterate,
+ * summed in FP64 in a fixed order. Synchronizes. */
+int smy_pdhg_evaluate(smy_pdhg* h, int which, double out[7]);
+/* Replaces PdhgDevice::restart_distance (pdhg_core.hpp:70). Synchronizes. */
+int smy_pdhg_restart_distance(smy_pdhg* h, int which, double* dv_sq, double* dy_sq);
+/* Replaces PdhgDevice::restart_to (pdhg_core.hpp:73). */
+int smy_pdhg_restart_to(smy_pdhg* h, int which);
+
+/* Additions without a reference counterpart (the host solver's fast path):
+ * evaluate both iterates in one pass over A and A' (out[0..6] current, out[7..13] average);
+ * run `count` steps, advancing after each but the last (iterations iter0 ... iter0+count-1
+ * since the last restart), from a captured HIP graph when count == the graph length. */
+int smy_pdhg_evaluate_both(smy_pdhg* h, double out[14]);
+int smy_pdhg_run(smy_pdhg* h, double tau, double sigma, double theta, long long iter0,
+                 int count);
+/* Switches the step precision (SMY_FP64 / SMY_FP32), converting the device state. */
+int smy_pdhg_set_precision(smy_pdhg* h, int precision);
+/* Uploads a primal-dual starting point (v: n + m, y: m; scaled space) as current iterate, restart
+ * point and average (warm start). */
+int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y);
+/* Kernel time of the last smy_pdhg_run, in milliseconds (HIP events on the handle's stream). */
+int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
+
+/* Replaces gpu::csr_spmv (spmv.hpp:16-21): y = A x for a CSR matrix in device memory (rows x ?,
+ * ptr[rows + 1], idx[nnz], val[nnz]), x and y device arrays. threads_per_row is accepted for
+ * the reference's signature; the gfx950 kernel picks its lane groups per row block. Runs on
+ * `stream` (NULL: the default stream) and synchronizes it. */
+int smy_csr_spmv(int rows, const int* ptr, const int* idx, const double* val, const double* x,
+                 double* y, void* stream, int threads_per_row);
+/* The same with FP32 values, x and y (FP32 accumulation). */
+int smy_csr_spmv_f32(int rows, const int* ptr, const int* idx, const float* val, const float* x,
+                     float* y, void* stream);
+/* Replaces gpu::spmv_threads_per_row (spmv.hpp:23): smallest power of two in [2, 32] not below
+ * the mean row length (the reference's CUSP rule, kept for its known answers). */
+int smy_spmv_threads_per_row(long long nnz, int rows);
+
+/* A prepared SpMV plan over a CSR matrix already in device memory (the row-block schedule is
+ * built once): used to time the product alone. */
+typedef struct smy_spmv_plan smy_spmv_plan;
+int smy_spmv_plan_create(int rows, const int* ptr_host, const int* ptr_dev, const int* idx_dev,
+                         const void* val_dev, int precision, smy_spmv_plan** out);
+int smy_spmv_plan_run(smy_spmv_plan* p, const void* x_dev, void* y_dev, void* stream);
+void smy_spmv_plan_destroy(smy_spmv_plan* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAMAYA_PDHG_H */

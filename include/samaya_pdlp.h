@@ -1,0 +1,157 @@
+/* C ABI of the PDLP host solver (C++) that drives the gfx950 device layer of samaya_pdhg.h.
+ * Plain C, never unwinds; status codes are samaya's SimplexStatus / Status values as noted.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the samaya tree):
+ *   samaya::solve_pdlp(LpProblem, PdlpOptions, Logger) ... src/lp/pdlp.hpp:35 (options :11-16,
+ *                                                          result :19-30)
+ *   solve_pdlp_lp (LP by PDLP through Solver::solve) ..... src/core/solver.cpp:392-477, reached
+ *                                                          from include/samaya/solver.hpp:75-86
+ *                                                          with Params::lp_method = kPdlp and
+ *                                                          Params::use_gpu (params.hpp:34-36)
+ *   samaya_solve ......................................... include/samaya_c.h:41
+ *   samaya_model_num_rows / _cols / _nonzeros, _free ..... include/samaya_c.h:33-37
+ *   verify_lp_optimality / _infeasibility / _unbounded_ray include/samaya/verify.hpp:42-67
+ *   samaya::gpu_info (GpuInfo) ........................... include/samaya/solver.hpp:93-97
+ */
+#ifndef SAMAYA_PDLP_H
+#define SAMAYA_PDLP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* SimplexStatus (src/lp/simplex.hpp:28-35). */
+enum {
+  SMY_STATUS_OPTIMAL = 0,
+  SMY_STATUS_INFEASIBLE = 1,
+  SMY_STATUS_UNBOUNDED = 2,
+  SMY_STATUS_ITERATION_LIMIT = 3,
+  SMY_STATUS_TIME_LIMIT = 4,
+  SMY_STATUS_NUMERICAL_ERROR = 5
+};
+
+enum { SMY_PRECISION_FP64 = 0, SMY_PRECISION_MIXED = 1 };
+
+/* Synthetic instance families (deterministic, seeded mt19937_64 with the library's own uniform
+ * and Box-Muller transforms). */
+enum {
+  SMY_GEN_SPARSE_UNIFORM = 1,   /* p0 rows, p1 cols, p2 nnz per column at uniform distinct rows,
+                                   values N(0,1); half equality rows, half <=; feasible x0 */
+  SMY_GEN_SPARSE_POWERLAW = 2,  /* p0 rows, p1 cols, column counts power law on [2, p2] with mean
+                                   p3 (max realized), values N(0,1) x 10^U[-2,2] */
+  SMY_GEN_BLENDING = 3,         /* multi-period blending LP, p0 periods, p1 units, p2 streams */
+  SMY_GEN_KNAPSACK = 4          /* multi-dimensional knapsack LP, p0 rows, p1 cols, p2 nnz/col */
+};
+
+typedef struct smy_model smy_model;
+
+/* A model min/max obj'x + offset s.t. row_lower <= A x <= row_upper, col_lower <= x <= col_upper
+ * (include/samaya/model.hpp:41-80, LP fields); A is CSC (start[n+1], index[nnz], value[nnz]),
+ * infinite bounds are +-INFINITY. The arrays are copied. */
+int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,
+                     const int32_t* index, const double* value, const double* obj,
+                     const double* col_lower, const double* col_upper, const double* row_lower,
+                     const double* row_upper, smy_model** out);
+int smy_model_generate(int kind, uint64_t seed, long long p0, long long p1, long long p2,
+                       double p3, smy_model** out);
+void smy_model_free(smy_model* model);
+int smy_model_num_rows(const smy_model* model);
+int smy_model_num_cols(const smy_model* model);
+long long smy_model_num_nonzeros(const smy_model* model);
+/* Copies the model out; any pointer may be NULL. */
+int smy_model_export(const smy_model* model, int64_t* start, int32_t* index, double* value,
+                     double* obj, double* col_lower, double* col_upper, double* row_lower,
+                     double* row_upper, int* maximize, double* offset);
+
+/* PdlpOptions (src/lp/pdlp.hpp:11-16) plus this library's additions. */
+typedef struct smy_pdlp_params {
+  double tol;                /* 1e-8 (the solver's kPdlpTolerance, src/core/solver.cpp:31) */
+  long long max_iterations;  /* -1: automatic */
+  double time_limit;         /* seconds, INFINITY: none */
+  int use_gpu;               /* must be 1: iterations run on the GPU only */
+  int precision;             /* SMY_PRECISION_FP64 or SMY_PRECISION_MIXED */
+  double fp32_switch_tol;    /* mixed: FP32 steps until this distance, then FP64 */
+  int use_graph;             /* replay the 64 steps between evaluations from a HIP graph */
+  int detect;                /* infeasibility / unboundedness detection */
+  int log_level;             /* 0 silent ... 3 debug, to stdout */
+  int verify;                /* model-level solve: run the verifier */
+  int polish_rounds;         /* model-level solve: FP64 continuations with tol/10 if rejected */
+} smy_pdlp_params;
+
+void smy_pdlp_params_default(smy_pdlp_params* params);
+
+/* PdlpResult (src/lp/pdlp.hpp:19-30) scalars plus timings. */
+typedef struct smy_pdlp_info {
+  int status;
+  long long iterations;
+  long long fp32_iterations;
+  double primal_residual;
+  double dual_residual;
+  double gap;
+  double operator_norm;
+  double setup_seconds;
+  double solve_seconds;
+  double kernel_ms;
+  int restarts;
+  int evaluations;
+  char message[256];
+} smy_pdlp_info;
+
+/* solve_pdlp on an LpProblem in computational form (min cost'v, [A -I] v = 0,
+ * lower <= v <= upper; A is m x n CSC; cost/lower/upper have n + m entries). Outputs (any may be
+ * NULL): x (n + m), y (m), dual_ray (m), primal_ray (n) in that scaled space; the restart log
+ * (iteration, to-average flag, KKT, omega) up to restart_cap entries; the evaluation log
+ * (iteration, current rel primal/dual/gap, average rel primal/dual/gap) up to eval_cap. */
+int smy_solve_pdlp(int m, int n, const int64_t* a_start, const int32_t* a_index,
+                   const double* a_value, const double* cost, const double* lower,
+                   const double* upper, const smy_pdlp_params* params, const double* start_x,
+                   const double* start_y, smy_pdlp_info* info, double* x, double* y,
+                   double* dual_ray, double* primal_ray, long long* restart_iter,
+                   int* restart_to_average, double* restart_kkt, double* restart_omega,
+                   int restart_cap, long long* eval_iter, double* eval_measures, int eval_cap);
+
+/* The model-level answer (samaya::Result fields for an LP, include/samaya/solver.hpp:38-73). */
+typedef struct smy_lp_result {
+  int status;   /* SimplexStatus of the PDLP run */
+  int verified;
+  double objective;
+  double max_primal_violation;
+  double max_dual_violation;
+  long long iterations; /* reported by samaya's CLI as barrier_iterations */
+  smy_pdlp_info pdlp;
+  char message[256];
+} smy_lp_result;
+
+/* LP by PDLP on the GPU (solve_pdlp_lp): power-of-two scaling, computational form, PDLP, mapping
+ * back and the independent verifier. Vectors (any may be NULL): col_value (n), row_dual (m,
+ * d = c - A'y convention), col_dual (n), row_activity (m), certificate (m if infeasible, n if
+ * unbounded). */
+int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_result* result,
+                 double* col_value, double* row_dual, double* col_dual, double* row_activity,
+                 double* certificate);
+
+/* samaya_solve (include/samaya_c.h:41) for this library: LP by PDLP on the GPU with default
+ * parameters. Returns samaya's Status code (1 optimal, 2 infeasible, 3 unbounded, 5 time limit,
+ * 6 iteration limit, 8 numerical error) and the objective (NaN without an answer). */
+int smy_solve(const smy_model* model, double* objective);
+
+/* Independent verifier (double-double) on model-space x (n) and y (m) / a certificate. out3 =
+ * {max primal violation, max dual violation, objective}. Returns 1 when the check passes. */
+int smy_verify_lp(const smy_model* model, const double* x, const double* y, double primal_tol,
+                  double dual_tol, double* out3, char* message, size_t message_len);
+int smy_verify_infeasibility(const smy_model* model, const double* y, char* message,
+                             size_t message_len);
+int smy_verify_unbounded_ray(const smy_model* model, const double* ray, char* message,
+                             size_t message_len);
+
+/* GpuInfo (include/samaya/solver.hpp:93-97): built_with_gpu_kernels is 1 in this library. */
+int smy_gpu_info(int* built_with_gpu_kernels, char* device, size_t device_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAMAYA_PDLP_H */

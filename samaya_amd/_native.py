@@ -1,0 +1,148 @@
+"""ctypes bindings of the native library (include/samaya_pdhg.h, include/samaya_pdlp.h).
+
+The library is samaya_amd/_lib/libsamaya_amd.so, built in-tree by samaya_amd/build.py. There is
+no fallback: load() raises when the library is missing, and every GPU entry point returns
+SMY_NO_DEVICE without a gfx950 GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libsamaya_amd.so"
+
+SMY_OK, SMY_NO_DEVICE, SMY_TOO_LARGE, SMY_ERROR, SMY_INVALID, SMY_STATE = range(6)
+SMY_FP64, SMY_FP32 = 0, 1
+SMY_CURRENT, SMY_AVERAGE = 0, 1
+SMY_PRECISION_FP64, SMY_PRECISION_MIXED = 0, 1
+SMY_GEN_SPARSE_UNIFORM, SMY_GEN_SPARSE_POWERLAW, SMY_GEN_BLENDING, SMY_GEN_KNAPSACK = 1, 2, 3, 4
+
+# Every symbol the headers declare (checked by the CPU test suite).
+EXPORTS = (
+    "smy_gpu_available", "smy_gpu_device_name", "smy_last_error", "smy_pdhg_create",
+    "smy_pdhg_destroy", "smy_pdhg_set_original", "smy_pdhg_step", "smy_pdhg_advance",
+    "smy_pdhg_restart", "smy_pdhg_download", "smy_pdhg_evaluate", "smy_pdhg_restart_distance",
+    "smy_pdhg_restart_to", "smy_pdhg_evaluate_both", "smy_pdhg_run", "smy_pdhg_set_precision",
+    "smy_pdhg_set_start", "smy_pdhg_last_run_ms", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
+    "smy_spmv_plan_destroy",
+    "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
+    "smy_model_num_cols", "smy_model_num_nonzeros", "smy_model_export", "smy_pdlp_params_default",
+    "smy_solve_pdlp", "smy_solve_lp", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
+    "smy_verify_unbounded_ray", "smy_gpu_info",
+)
+
+
+class SmyCsc(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("cols", C.c_int32), ("start", C.c_void_p),
+                ("index", C.c_void_p), ("value", C.c_void_p)]
+
+
+class SmyPdlpParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iterations", C.c_longlong), ("time_limit", C.c_double),
+                ("use_gpu", C.c_int), ("precision", C.c_int), ("fp32_switch_tol", C.c_double),
+                ("use_graph", C.c_int), ("detect", C.c_int), ("log_level", C.c_int),
+                ("verify", C.c_int), ("polish_rounds", C.c_int)]
+
+
+class SmyPdlpInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong),
+                ("fp32_iterations", C.c_longlong), ("primal_residual", C.c_double),
+                ("dual_residual", C.c_double), ("gap", C.c_double),
+                ("operator_norm", C.c_double), ("setup_seconds", C.c_double),
+                ("solve_seconds", C.c_double), ("kernel_ms", C.c_double), ("restarts", C.c_int),
+                ("evaluations", C.c_int), ("message", C.c_char * 256)]
+
+
+class SmyLpResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("verified", C.c_int), ("objective", C.c_double),
+                ("max_primal_violation", C.c_double), ("max_dual_violation", C.c_double),
+                ("iterations", C.c_longlong), ("pdlp", SmyPdlpInfo), ("message", C.c_char * 256)]
+
+
+_lib = None
+
+vp = C.c_void_p
+i32 = C.c_int
+i64 = C.c_longlong
+f64 = C.c_double
+
+
+def _proto(lib, name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+
+
+def load() -> C.CDLL:
+    """The native library; raises FileNotFoundError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python samaya_amd/build.py` "
+            "(there is no CPU fallback for the PDLP iterations)")
+    lib = C.CDLL(str(LIB_PATH))
+    _proto(lib, "smy_gpu_available", i32)
+    _proto(lib, "smy_gpu_device_name", i32, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_last_error", C.c_char_p)
+    _proto(lib, "smy_pdhg_create", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), vp, vp, vp, i32,
+           C.POINTER(vp))
+    _proto(lib, "smy_pdhg_destroy", None, vp)
+    _proto(lib, "smy_pdhg_set_original", i32, vp, vp, vp, vp, vp, vp)
+    _proto(lib, "smy_pdhg_step", i32, vp, f64, f64, f64, i64)
+    _proto(lib, "smy_pdhg_advance", i32, vp)
+    _proto(lib, "smy_pdhg_restart", i32, vp)
+    _proto(lib, "smy_pdhg_download", i32, vp, vp, vp, vp, vp, vp, vp)
+    _proto(lib, "smy_pdhg_evaluate", i32, vp, i32, vp)
+    _proto(lib, "smy_pdhg_evaluate_both", i32, vp, vp)
+    _proto(lib, "smy_pdhg_restart_distance", i32, vp, i32, C.POINTER(f64), C.POINTER(f64))
+    _proto(lib, "smy_pdhg_restart_to", i32, vp, i32)
+    _proto(lib, "smy_pdhg_run", i32, vp, f64, f64, f64, i64, i32)
+    _proto(lib, "smy_pdhg_set_precision", i32, vp, i32)
+    _proto(lib, "smy_pdhg_set_start", i32, vp, vp, vp)
+    _proto(lib, "smy_pdhg_last_run_ms", i32, vp, C.POINTER(f64))
+    _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)
+    _proto(lib, "smy_csr_spmv_f32", i32, i32, vp, vp, vp, vp, vp, vp)
+    _proto(lib, "smy_spmv_threads_per_row", i32, i64, i32)
+    _proto(lib, "smy_spmv_plan_create", i32, i32, vp, vp, vp, vp, i32, C.POINTER(vp))
+    _proto(lib, "smy_spmv_plan_run", i32, vp, vp, vp, vp)
+    _proto(lib, "smy_spmv_plan_destroy", None, vp)
+    _proto(lib, "smy_model_create", i32, i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp,
+           C.POINTER(vp))
+    _proto(lib, "smy_model_generate", i32, i32, C.c_uint64, i64, i64, i64, f64, C.POINTER(vp))
+    _proto(lib, "smy_model_free", None, vp)
+    _proto(lib, "smy_model_num_rows", i32, vp)
+    _proto(lib, "smy_model_num_cols", i32, vp)
+    _proto(lib, "smy_model_num_nonzeros", i64, vp)
+    _proto(lib, "smy_model_export", i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(i32),
+           C.POINTER(f64))
+    _proto(lib, "smy_pdlp_params_default", None, C.POINTER(SmyPdlpParams))
+    _proto(lib, "smy_solve_pdlp", i32, i32, i32, vp, vp, vp, vp, vp, vp,
+           C.POINTER(SmyPdlpParams), vp, vp, C.POINTER(SmyPdlpInfo), vp, vp, vp, vp, vp, vp, vp,
+           vp, i32, vp, vp, i32)
+    _proto(lib, "smy_solve_lp", i32, vp, C.POINTER(SmyPdlpParams), C.POINTER(SmyLpResult), vp, vp,
+           vp, vp, vp)
+    _proto(lib, "smy_solve", i32, vp, C.POINTER(f64))
+    _proto(lib, "smy_verify_lp", i32, vp, vp, vp, f64, f64, vp, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_verify_infeasibility", i32, vp, vp, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_verify_unbounded_ray", i32, vp, vp, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_gpu_info", i32, C.POINTER(i32), C.c_char_p, C.c_size_t)
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().smy_last_error().decode(errors="replace")
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what} (code {code}): {last_error()}")
+        self.code = code
+
+
+def check(code: int, what: str) -> None:
+    if code != SMY_OK:
+        raise NativeError(code, what)

@@ -1,0 +1,337 @@
+// C ABI of include/samaya_pdlp.h over the C++ host solver.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <memory>
+
+#include "../../../include/samaya_pdhg.h"
+#include "../../../include/samaya_pdlp.h"
+#include "generators.hpp"
+#include "pdlp.hpp"
+
+using namespace smy::host;
+
+struct smy_model {
+  Model model;
+};
+
+namespace {
+
+void copy_text(char* dst, size_t len, const std::string& s) {
+  if (dst == nullptr || len == 0) return;
+  const size_t k = std::min(s.size(), len - 1);
+  std::memcpy(dst, s.data(), k);
+  dst[k] = '\0';
+}
+
+template <typename T>
+void put(const std::vector<T>& v, T* dst) {
+  if (dst != nullptr && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+}
+
+PdlpOptions to_options(const smy_pdlp_params* p) {
+  smy_pdlp_params d;
+  smy_pdlp_params_default(&d);
+  if (p == nullptr) p = &d;
+  PdlpOptions o;
+  o.tol = p->tol;
+  o.max_iterations = p->max_iterations;
+  o.time_limit = p->time_limit;
+  o.use_gpu = p->use_gpu != 0;
+  o.precision = p->precision == SMY_PRECISION_MIXED ? Precision::kMixed : Precision::kFp64;
+  o.fp32_switch_tol = p->fp32_switch_tol;
+  o.use_graph = p->use_graph != 0;
+  o.detect = p->detect != 0;
+  o.log_level = p->log_level;
+  return o;
+}
+
+void fill_info(const PdlpResult& r, smy_pdlp_info* info) {
+  if (info == nullptr) return;
+  info->status = static_cast<int>(r.status);
+  info->iterations = r.iterations;
+  info->fp32_iterations = r.fp32_iterations;
+  info->primal_residual = r.primal_residual;
+  info->dual_residual = r.dual_residual;
+  info->gap = r.gap;
+  info->operator_norm = r.operator_norm;
+  info->setup_seconds = r.setup_seconds;
+  info->solve_seconds = r.solve_seconds;
+  info->kernel_ms = r.kernel_ms;
+  info->restarts = static_cast<int>(r.restarts.size());
+  info->evaluations = static_cast<int>(r.evals.size());
+  copy_text(info->message, sizeof info->message, r.message);
+}
+
+bool valid_csc(int rows, int cols, const int64_t* start, const int32_t* index) {
+  if (rows < 0 || cols < 0 || start == nullptr) return false;
+  if (start[0] != 0) return false;
+  for (int j = 0; j < cols; ++j) {
+    if (start[j + 1] < start[j]) return false;
+    for (int64_t p = start[j]; p < start[j + 1]; ++p) {
+      if (index[p] < 0 || index[p] >= rows) return false;
+      if (p > start[j] && index[p] <= index[p - 1]) return false;  // sorted, unique
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void smy_pdlp_params_default(smy_pdlp_params* p) {
+  if (p == nullptr) return;
+  p->tol = 1e-8;
+  p->max_iterations = -1;
+  p->time_limit = std::numeric_limits<double>::infinity();
+  p->use_gpu = 1;
+  p->precision = SMY_PRECISION_FP64;
+  p->fp32_switch_tol = 1e-5;
+  p->use_graph = 1;
+  p->detect = 1;
+  p->log_level = 0;
+  p->verify = 1;
+  p->polish_rounds = 0;
+}
+
+int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,
+                     const int32_t* index, const double* value, const double* obj,
+                     const double* col_lower, const double* col_upper, const double* row_lower,
+                     const double* row_upper, smy_model** out) {
+  if (out == nullptr) return SMY_INVALID;
+  *out = nullptr;
+  if (!valid_csc(m, n, start, index) || obj == nullptr || col_lower == nullptr ||
+      col_upper == nullptr || (m > 0 && (row_lower == nullptr || row_upper == nullptr)))
+    return SMY_INVALID;
+  try {
+    auto md = std::make_unique<smy_model>();
+    Model& M = md->model;
+    M.m = m;
+    M.n = n;
+    M.maximize = maximize != 0;
+    M.offset = offset;
+    M.A.rows = m;
+    M.A.cols = n;
+    M.A.start.assign(start, start + n + 1);
+    M.A.index.assign(index, index + start[n]);
+    M.A.value.assign(value, value + start[n]);
+    M.obj.assign(obj, obj + n);
+    M.col_lower.assign(col_lower, col_lower + n);
+    M.col_upper.assign(col_upper, col_upper + n);
+    M.row_lower.assign(row_lower, row_lower + m);
+    M.row_upper.assign(row_upper, row_upper + m);
+    *out = md.release();
+    return SMY_OK;
+  } catch (...) {
+    return SMY_ERROR;
+  }
+}
+
+int smy_model_generate(int kind, uint64_t seed, long long p0, long long p1, long long p2,
+                       double p3, smy_model** out) {
+  if (out == nullptr) return SMY_INVALID;
+  *out = nullptr;
+  try {
+    auto md = std::make_unique<smy_model>();
+    switch (kind) {
+      case SMY_GEN_SPARSE_UNIFORM:
+        md->model = generate_sparse_uniform(seed, static_cast<int>(p0), static_cast<int>(p1),
+                                            static_cast<int>(p2));
+        break;
+      case SMY_GEN_SPARSE_POWERLAW:
+        md->model = generate_sparse_powerlaw(seed, static_cast<int>(p0), static_cast<int>(p1),
+                                             static_cast<int>(p2), p3);
+        break;
+      default:
+        return SMY_INVALID;
+    }
+    *out = md.release();
+    return SMY_OK;
+  } catch (const std::invalid_argument&) {
+    return SMY_INVALID;
+  } catch (...) {
+    return SMY_ERROR;
+  }
+}
+
+void smy_model_free(smy_model* model) { delete model; }
+int smy_model_num_rows(const smy_model* model) { return model ? model->model.m : 0; }
+int smy_model_num_cols(const smy_model* model) { return model ? model->model.n : 0; }
+long long smy_model_num_nonzeros(const smy_model* model) {
+  return model ? static_cast<long long>(model->model.A.nnz()) : 0;
+}
+
+int smy_model_export(const smy_model* model, int64_t* start, int32_t* index, double* value,
+                     double* obj, double* col_lower, double* col_upper, double* row_lower,
+                     double* row_upper, int* maximize, double* offset) {
+  if (model == nullptr) return SMY_INVALID;
+  const Model& M = model->model;
+  put(M.A.start, start);
+  put(M.A.index, index);
+  put(M.A.value, value);
+  put(M.obj, obj);
+  put(M.col_lower, col_lower);
+  put(M.col_upper, col_upper);
+  put(M.row_lower, row_lower);
+  put(M.row_upper, row_upper);
+  if (maximize) *maximize = M.maximize ? 1 : 0;
+  if (offset) *offset = M.offset;
+  return SMY_OK;
+}
+
+int smy_solve_pdlp(int m, int n, const int64_t* a_start, const int32_t* a_index,
+                   const double* a_value, const double* cost, const double* lower,
+                   const double* upper, const smy_pdlp_params* params, const double* start_x,
+                   const double* start_y, smy_pdlp_info* info, double* x, double* y,
+                   double* dual_ray, double* primal_ray, long long* restart_iter,
+                   int* restart_to_average, double* restart_kkt, double* restart_omega,
+                   int restart_cap, long long* eval_iter, double* eval_measures, int eval_cap) {
+  if (!valid_csc(m, n, a_start, a_index) || cost == nullptr || lower == nullptr || upper == nullptr)
+    return SMY_INVALID;
+  try {
+    LpProblem lp;
+    lp.m = m;
+    lp.n = n;
+    lp.A.rows = m;
+    lp.A.cols = n;
+    lp.A.start.assign(a_start, a_start + n + 1);
+    lp.A.index.assign(a_index, a_index + a_start[n]);
+    lp.A.value.assign(a_value, a_value + a_start[n]);
+    lp.At = csc_transpose(lp.A);
+    const size_t nt = static_cast<size_t>(n + m);
+    lp.cost.assign(cost, cost + nt);
+    lp.lower.assign(lower, lower + nt);
+    lp.upper.assign(upper, upper + nt);
+    PdlpOptions o = to_options(params);
+    if (start_x != nullptr) o.start_x.assign(start_x, start_x + nt);
+    if (start_y != nullptr) o.start_y.assign(start_y, start_y + m);
+    const PdlpResult r = solve_pdlp(lp, o);
+    fill_info(r, info);
+    put(r.x, x);
+    put(r.y, y);
+    put(r.dual_ray, dual_ray);
+    put(r.primal_ray, primal_ray);
+    for (int k = 0; k < static_cast<int>(r.restarts.size()) && k < restart_cap; ++k) {
+      if (restart_iter) restart_iter[k] = r.restarts[k].iter;
+      if (restart_to_average) restart_to_average[k] = r.restarts[k].to_average;
+      if (restart_kkt) restart_kkt[k] = r.restarts[k].kkt;
+      if (restart_omega) restart_omega[k] = r.restarts[k].omega;
+    }
+    for (int k = 0; k < static_cast<int>(r.evals.size()) && k < eval_cap; ++k) {
+      if (eval_iter) eval_iter[k] = r.evals[k].iter;
+      if (eval_measures)
+        for (int q = 0; q < 3; ++q) {
+          eval_measures[6 * k + q] = r.evals[k].current[q];
+          eval_measures[6 * k + 3 + q] = r.evals[k].average[q];
+        }
+    }
+    return r.status == Status::kNumericalError && !r.message.empty() ? SMY_ERROR : SMY_OK;
+  } catch (const std::exception& e) {
+    if (info) {
+      info->status = static_cast<int>(Status::kNumericalError);
+      copy_text(info->message, sizeof info->message, e.what());
+    }
+    return SMY_ERROR;
+  }
+}
+
+int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_result* result,
+                 double* col_value, double* row_dual, double* col_dual, double* row_activity,
+                 double* certificate) {
+  if (model == nullptr || result == nullptr) return SMY_INVALID;
+  std::memset(result, 0, sizeof *result);
+  try {
+    smy_pdlp_params d;
+    smy_pdlp_params_default(&d);
+    if (params == nullptr) params = &d;
+    LpSolveOptions o;
+    o.pdlp = to_options(params);
+    o.verify = params->verify != 0;
+    o.polish_rounds = params->polish_rounds;
+    const LpSolveResult r = solve_lp(model->model, o);
+    result->status = static_cast<int>(r.status);
+    result->verified = r.verified ? 1 : 0;
+    result->objective = r.objective;
+    result->max_primal_violation = r.max_primal_violation;
+    result->max_dual_violation = r.max_dual_violation;
+    result->iterations = r.iterations;
+    fill_info(r.pdlp, &result->pdlp);
+    copy_text(result->message, sizeof result->message, r.message);
+    put(r.col_value, col_value);
+    put(r.row_dual, row_dual);
+    put(r.col_dual, col_dual);
+    put(r.row_activity, row_activity);
+    if (!r.infeasibility_certificate.empty()) put(r.infeasibility_certificate, certificate);
+    if (!r.unbounded_ray.empty()) put(r.unbounded_ray, certificate);
+    return SMY_OK;
+  } catch (const std::exception& e) {
+    result->status = static_cast<int>(Status::kNumericalError);
+    copy_text(result->message, sizeof result->message, e.what());
+    return SMY_ERROR;
+  }
+}
+
+int smy_solve(const smy_model* model, double* objective) {
+  // samaya::Status values (include/samaya_c.h:14-28).
+  if (objective != nullptr) *objective = std::numeric_limits<double>::quiet_NaN();
+  if (model == nullptr) return 9;  // SAMAYA_INVALID_MODEL
+  smy_lp_result r;
+  if (smy_solve_lp(model, nullptr, &r, nullptr, nullptr, nullptr, nullptr, nullptr) != SMY_OK)
+    return 8;
+  if (r.status == SMY_STATUS_OPTIMAL && !r.verified) return 8;
+  switch (r.status) {
+    case SMY_STATUS_OPTIMAL:
+      if (objective != nullptr) *objective = r.objective;
+      return 1;
+    case SMY_STATUS_INFEASIBLE: return 2;
+    case SMY_STATUS_UNBOUNDED: return 3;
+    case SMY_STATUS_TIME_LIMIT: return 5;
+    case SMY_STATUS_ITERATION_LIMIT: return 6;
+    default: return 8;
+  }
+}
+
+int smy_verify_lp(const smy_model* model, const double* x, const double* y, double primal_tol,
+                  double dual_tol, double* out3, char* message, size_t message_len) {
+  if (model == nullptr || x == nullptr || (y == nullptr && model->model.m > 0)) return -1;
+  const VerifyReport r = verify_lp_optimality(model->model, x, y, primal_tol, dual_tol);
+  if (out3 != nullptr) {
+    out3[0] = std::max(r.max_bound_violation, r.max_row_violation);
+    out3[1] = r.max_dual_violation;
+    out3[2] = r.objective;
+  }
+  copy_text(message, message_len, r.message);
+  return r.ok ? 1 : 0;
+}
+
+int smy_verify_infeasibility(const smy_model* model, const double* y, char* message,
+                             size_t message_len) {
+  if (model == nullptr || y == nullptr) return -1;
+  const VerifyReport r = verify_infeasibility(model->model, y);
+  copy_text(message, message_len, r.message);
+  return r.ok ? 1 : 0;
+}
+
+int smy_verify_unbounded_ray(const smy_model* model, const double* ray, char* message,
+                             size_t message_len) {
+  if (model == nullptr || ray == nullptr) return -1;
+  const VerifyReport r = verify_unbounded_ray(model->model, ray);
+  copy_text(message, message_len, r.message);
+  return r.ok ? 1 : 0;
+}
+
+int smy_gpu_info(int* built_with_gpu_kernels, char* device, size_t device_len) {
+  if (built_with_gpu_kernels) *built_with_gpu_kernels = 1;
+  if (device != nullptr && device_len > 0) {
+    device[0] = '\0';
+    if (smy_gpu_available()) smy_gpu_device_name(device, device_len);
+  }
+  return SMY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// The PDLP host solver: every algorithmic decision (equilibration, step sizes, termination,
+// restarts, primal weight, infeasibility and unboundedness detection) on the host, every
+// iteration on the gfx950 device through the C ABI of include/samaya_pdhg.h.
+//
+// Mirrors samaya's solve_pdlp (src/lp/pdlp.hpp:11-35, src/lp/pdlp.cpp:228-1193): PdlpOptions and
+// PdlpResult keep the reference's fields and meaning, and with the default options ("parity"
+// algorithm, FP64) the solver takes the reference's decisions with the reference's constants.
+// Additions, each off by default or selected explicitly:
+//   * precision kMixed: FP32 values and iterates with FP32 accumulation in the steps, FP64
+//     evaluation sums, restarts and termination, then FP64 steps (the polish) to the tolerance;
+//   * HIP-graph replay of the 64 steps between evaluations;
+//   * polish_to_verify: after an optimal answer that the verifier rejects, continue the FP64
+//     iteration with a tighter tolerance (the reference hands that case to crossover);
+//   * warm start from a given primal-dual point.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lp_data.hpp"
+
+namespace smy::host {
+
+// SimplexStatus (src/lp/simplex.hpp:28-35), same order and values.
+enum class Status : int {
+  kOptimal = 0,
+  kInfeasible,
+  kUnbounded,
+  kIterationLimit,
+  kTimeLimit,
+  kNumericalError,
+};
+const char* to_string(Status s);
+
+enum class Precision : int { kFp64 = 0, kMixed = 1 };
+
+struct PdlpOptions {
+  double tol = 1e-8;              // max(rel primal, rel dual, |rel gap|) < tol
+  long long max_iterations = -1;  // -1: max(1e4, 1000 (m + n)), capped at 1e6 without time limit
+  double time_limit = kInf;       // seconds, setup included
+  bool use_gpu = true;            // iterations run on the GPU only: false is an error here
+  Precision precision = Precision::kFp64;
+  // Mixed precision: FP32 steps until the better of current / average reaches this (FP64)
+  // termination distance, or FP32 stops improving; then FP64 steps to `tol`.
+  double fp32_switch_tol = 1e-5;
+  bool use_graph = true;          // replay each 64-step run from a HIP graph
+  bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
+  int log_level = 0;
+  // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m).
+  std::vector<double> start_x, start_y;
+};
+
+struct RestartEvent {
+  long long iter;
+  int to_average;
+  double kkt;
+  double omega;
+};
+
+struct EvalEvent {
+  long long iter;
+  double current[3];  // rel primal, rel dual, rel gap
+  double average[3];
+};
+
+struct PdlpResult {
+  Status status = Status::kNumericalError;
+  std::vector<double> x;           // n + m, scaled computational-form space
+  std::vector<double> y;           // m
+  std::vector<double> dual_ray;    // m, when infeasible
+  std::vector<double> primal_ray;  // n, when unbounded
+  long long iterations = 0;
+  long long fp32_iterations = 0;   // of which in FP32 (mixed precision)
+  double primal_residual = 0.0;
+  double dual_residual = 0.0;
+  double gap = 0.0;
+  double operator_norm = 0.0;      // L, the bound on ||[A_eq -I]||
+  double setup_seconds = 0.0;      // equilibration, norm estimate, upload
+  double solve_seconds = 0.0;      // everything
+  double kernel_ms = 0.0;          // device time of the step runs (HIP events)
+  std::string message;
+  std::vector<RestartEvent> restarts;
+  std::vector<EvalEvent> evals;
+};
+
+PdlpResult solve_pdlp(const LpProblem& lp, const PdlpOptions& options);
+
+// ---- Model-level path (solve_pdlp_lp, src/core/solver.cpp:392-477) ----
+
+struct VerifyReport {
+  bool ok = true;
+  double max_bound_violation = 0.0;
+  double max_row_violation = 0.0;
+  double max_dual_violation = 0.0;
+  double objective = 0.0;
+  std::string message;
+};
+
+// Independent checks in double-double arithmetic (src/verify/verify.cpp:132-401), tolerances
+// 1e-6 primal / dual relative (include/samaya/verify.hpp:17-21).
+VerifyReport verify_lp_optimality(const Model& model, const double* x, const double* y,
+                                  double primal_tol = 1e-6, double dual_tol = 1e-6);
+VerifyReport verify_infeasibility(const Model& model, const double* y, double primal_tol = 1e-6);
+VerifyReport verify_unbounded_ray(const Model& model, const double* ray, double primal_tol = 1e-6,
+                                  double dual_tol = 1e-6);
+
+struct LpSolveOptions {
+  PdlpOptions pdlp;
+  bool verify = true;
+  // Optimal but rejected by the verifier: continue FP64 iterations with tol / 10, up to this many
+  // times, before reporting the answer as unverified.
+  int polish_rounds = 0;
+};
+
+struct LpSolveResult {
+  Status status = Status::kNumericalError;
+  bool verified = false;
+  double objective = 0.0;
+  std::vector<double> col_value;  // n
+  std::vector<double> row_activity;  // m
+  std::vector<double> row_dual;   // m, d = c - A'y convention
+  std::vector<double> col_dual;   // n
+  std::vector<double> infeasibility_certificate;  // m
+  std::vector<double> unbounded_ray;              // n
+  double max_primal_violation = 0.0;
+  double max_dual_violation = 0.0;
+  long long iterations = 0;       // reported by the reference as barrier_iterations
+  std::string message;
+  PdlpResult pdlp;
+};
+
+LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options);
+
+}  // namespace smy::host

@@ -1,0 +1,382 @@
+// Independent LP answer checks in double-double arithmetic and the model-level LP-by-PDLP path.
+//
+// The checks recompute row activities and reduced costs from the model alone and share no code
+// with the solver; they follow samaya's verifier (src/verify/verify.cpp:28-401, LP parts) and its
+// double-double type (src/core/xreal.hpp): Knuth's TwoSum and Dekker's TwoProduct with a
+// Veltkamp split, so the sums do not depend on FMA or on the compiler (this file is built with
+// -ffp-contract=off).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "pdlp.hpp"
+
+namespace smy::host {
+namespace {
+
+class DoubleDouble {
+ public:
+  DoubleDouble() = default;
+  DoubleDouble(double v) : hi_(v) {}  // NOLINT: widening
+  double value() const { return hi_ + lo_; }
+  bool is_zero() const { return hi_ == 0.0 && lo_ == 0.0; }
+  bool less(const DoubleDouble& b) const { return hi_ < b.hi_ || (hi_ == b.hi_ && lo_ < b.lo_); }
+  DoubleDouble negated() const {
+    DoubleDouble r;
+    r.hi_ = -hi_;
+    r.lo_ = -lo_;
+    return r;
+  }
+  DoubleDouble magnitude() const { return (hi_ < 0.0 || (hi_ == 0.0 && lo_ < 0.0)) ? negated() : *this; }
+
+  DoubleDouble& operator+=(const DoubleDouble& o) {
+    if (!std::isfinite(hi_) || !std::isfinite(o.hi_)) return nonfinite(hi_ + o.hi_);
+    double s, e, t, f;
+    sum_exact(hi_, o.hi_, s, e);
+    sum_exact(lo_, o.lo_, t, f);
+    e += t;
+    sum_ordered(s, e, s, e);
+    e += f;
+    sum_ordered(s, e, hi_, lo_);
+    if (!std::isfinite(hi_)) lo_ = 0.0;
+    return *this;
+  }
+  DoubleDouble& operator*=(double b) {
+    if (!std::isfinite(hi_) || !std::isfinite(b)) return nonfinite(hi_ * b);
+    double p, e;
+    product_exact(hi_, b, p, e);
+    if (!std::isfinite(p)) return nonfinite(p);
+    e += lo_ * b;
+    sum_ordered(p, e, hi_, lo_);
+    return *this;
+  }
+  static DoubleDouble product(double a, double b) {
+    DoubleDouble r(a);
+    r *= b;
+    return r;
+  }
+
+ private:
+  static void sum_exact(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bv = s - a;
+    e = (a - (s - bv)) + (b - bv);
+  }
+  static void sum_ordered(double a, double b, double& s, double& e) {
+    s = a + b;
+    e = b - (s - a);
+  }
+  static void halves(double a, double& h, double& l) {
+    constexpr double kSplit = 134217729.0;  // 2^27 + 1
+    constexpr double kBig = 6.69692879491417e+299;  // 2^996
+    if (a > kBig || a < -kBig) {
+      a *= 0x1p-28;
+      const double t = kSplit * a;
+      h = (t - (t - a)) * 0x1p28;
+      l = (a - (t - (t - a))) * 0x1p28;
+      return;
+    }
+    const double t = kSplit * a;
+    h = t - (t - a);
+    l = a - h;
+  }
+  static void product_exact(double a, double b, double& p, double& e) {
+    p = a * b;
+    if (!std::isfinite(p)) {
+      e = 0.0;
+      return;
+    }
+    double ah, al, bh, bl;
+    halves(a, ah, al);
+    halves(b, bh, bl);
+    e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+  }
+  DoubleDouble& nonfinite(double v) {
+    hi_ = v;
+    lo_ = 0.0;
+    return *this;
+  }
+  double hi_ = 0.0;
+  double lo_ = 0.0;
+};
+
+using DD = DoubleDouble;
+
+void record(VerifyReport& r, double violation, double tol, const char* what, int index) {
+  if (violation <= tol) return;
+  if (r.ok || r.message.empty()) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s %d violated by %.3e (tolerance %.1e)", what, index, violation, tol);
+    r.message = buf;
+  }
+  r.ok = false;
+}
+
+double bound_violation(double v, double lo, double up, double scale) {
+  if (v < lo) return (lo - v) / (1.0 + std::max(std::fabs(lo), scale));
+  if (v > up) return (v - up) / (1.0 + std::max(std::fabs(up), scale));
+  return 0.0;
+}
+
+VerifyReport check_primal(const Model& md, const double* x, double tol) {
+  VerifyReport r;
+  for (int j = 0; j < md.n; ++j) {
+    if (!std::isfinite(x[j])) {
+      r.ok = false;
+      r.message = "solution contains a non-finite value";
+      return r;
+    }
+    const double v = bound_violation(x[j], md.col_lower[j], md.col_upper[j], 0.0);
+    r.max_bound_violation = std::max(r.max_bound_violation, v);
+    record(r, v, tol, "column bound", j);
+  }
+  std::vector<DD> act(static_cast<size_t>(md.m));
+  std::vector<double> mag(static_cast<size_t>(md.m), 0.0);
+  for (int j = 0; j < md.n; ++j)
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+      const DD t = DD::product(md.A.value[p], x[j]);
+      act[md.A.index[p]] += t;
+      mag[md.A.index[p]] = std::max(mag[md.A.index[p]], t.magnitude().value());
+    }
+  for (int i = 0; i < md.m; ++i) {
+    const double a = act[i].value();
+    if (!std::isfinite(a) || !std::isfinite(mag[i])) {
+      r.ok = false;
+      r.max_row_violation = kInf;
+      if (r.message.empty()) r.message = "row " + std::to_string(i) + " activity is not finite";
+      continue;
+    }
+    const double v = bound_violation(a, md.row_lower[i], md.row_upper[i], mag[i]);
+    r.max_row_violation = std::max(r.max_row_violation, v);
+    record(r, v, tol, "row", i);
+  }
+  DD obj(md.offset);
+  for (int j = 0; j < md.n; ++j) obj += DD::product(md.obj[j], x[j]);
+  r.objective = obj.value();
+  return r;
+}
+
+}  // namespace
+
+VerifyReport verify_lp_optimality(const Model& md, const double* x, const double* y,
+                                  double primal_tol, double dual_tol) {
+  VerifyReport r = check_primal(md, x, primal_tol);
+  const double sense = md.maximize ? -1.0 : 1.0;
+  const auto at_bound = [&](double v, double b, double scale) {
+    return std::isfinite(b) && std::fabs(v - b) <= primal_tol * (1.0 + std::max(std::fabs(b), scale));
+  };
+  std::vector<DD> act(static_cast<size_t>(md.m));
+  std::vector<double> mag(static_cast<size_t>(md.m), 0.0);
+  for (int j = 0; j < md.n; ++j) {
+    DD d(md.obj[j]);
+    double scale = std::fabs(md.obj[j]);
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+      const DD t = DD::product(md.A.value[p], y[md.A.index[p]]);
+      d += t.negated();
+      scale += t.magnitude().value();
+      act[md.A.index[p]] += DD::product(md.A.value[p], x[j]);
+      mag[md.A.index[p]] = std::max(mag[md.A.index[p]], std::fabs(md.A.value[p] * x[j]));
+    }
+    const double dj = sense * d.value();
+    if (!std::isfinite(dj) || !std::isfinite(scale)) {
+      r.ok = false;
+      r.max_dual_violation = kInf;
+      if (r.message.empty()) r.message = "reduced cost of column " + std::to_string(j) + " is not finite";
+      continue;
+    }
+    double v = 0.0;
+    if (!at_bound(x[j], md.col_lower[j], 0.0)) v = std::max(v, dj);
+    if (!at_bound(x[j], md.col_upper[j], 0.0)) v = std::max(v, -dj);
+    v /= 1.0 + scale;
+    r.max_dual_violation = std::max(r.max_dual_violation, v);
+    record(r, v, dual_tol, "reduced cost of column", j);
+  }
+  for (int i = 0; i < md.m; ++i) {
+    if (!std::isfinite(y[i])) {
+      r.ok = false;
+      r.message = "dual vector contains a non-finite value";
+      return r;
+    }
+    const double yi = sense * y[i];
+    const double a = act[i].value();
+    double v = 0.0;
+    if (!at_bound(a, md.row_lower[i], mag[i])) v = std::max(v, yi);
+    if (!at_bound(a, md.row_upper[i], mag[i])) v = std::max(v, -yi);
+    v /= 1.0 + std::fabs(yi);
+    r.max_dual_violation = std::max(r.max_dual_violation, v);
+    record(r, v, dual_tol, "dual of row", i);
+  }
+  return r;
+}
+
+VerifyReport verify_infeasibility(const Model& md, const double* y, double primal_tol) {
+  VerifyReport r;
+  double y_max = 0.0;
+  for (int i = 0; i < md.m; ++i) {
+    if (!std::isfinite(y[i])) {
+      r.ok = false;
+      r.message = "certificate contains a non-finite value";
+      return r;
+    }
+    y_max = std::max(y_max, std::fabs(y[i]));
+  }
+  if (y_max == 0.0) {
+    r.ok = false;
+    r.message = "certificate is zero";
+    return r;
+  }
+  DD lo(0.0), hi(0.0);
+  double scale = 0.0;
+  const auto range = [&](const DD& c, double l, double u) {
+    if (DD(0.0).less(c)) {
+      lo += l > -kInf ? DD(c) *= l : DD(-INFINITY);
+      hi += u < kInf ? DD(c) *= u : DD(INFINITY);
+    } else if (c.less(DD(0.0))) {
+      lo += u < kInf ? DD(c) *= u : DD(-INFINITY);
+      hi += l > -kInf ? DD(c) *= l : DD(INFINITY);
+    }
+  };
+  for (int j = 0; j < md.n; ++j) {
+    DD w(0.0);
+    double w_scale = 0.0;
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+      const DD t = DD::product(md.A.value[p], y[md.A.index[p]] / y_max);
+      w += t;
+      w_scale += t.magnitude().value();
+    }
+    if (std::fabs(w.value()) <= 1e-9 * w_scale) w = DD(0.0);
+    range(w, md.col_lower[j], md.col_upper[j]);
+    if (!w.is_zero())
+      scale = std::max({scale, std::fabs(w.value() * md.col_lower[j]), std::fabs(w.value() * md.col_upper[j])});
+  }
+  for (int i = 0; i < md.m; ++i) {
+    const DD c = DD(y[i] / y_max).negated();
+    range(c, md.row_lower[i], md.row_upper[i]);
+    if (!c.is_zero())
+      scale = std::max({scale, std::fabs(c.value() * md.row_lower[i]), std::fabs(c.value() * md.row_upper[i])});
+  }
+  if (!std::isfinite(scale)) scale = 0.0;
+  const double margin = primal_tol * (1.0 + scale);
+  if (DD(margin).less(lo) || hi.less(DD(-margin))) return r;
+  r.ok = false;
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "certificate range [%.3e, %.3e] does not exclude zero", lo.value(), hi.value());
+  r.message = buf;
+  return r;
+}
+
+VerifyReport verify_unbounded_ray(const Model& md, const double* ray, double primal_tol,
+                                  double dual_tol) {
+  VerifyReport r;
+  double v_max = 0.0;
+  for (int j = 0; j < md.n; ++j) v_max = std::max(v_max, std::fabs(ray[j]));
+  if (!(v_max > 0.0) || !std::isfinite(v_max)) {
+    r.ok = false;
+    r.message = "ray is zero or not finite";
+    return r;
+  }
+  const double sense = md.maximize ? -1.0 : 1.0;
+  DD slope(0.0);
+  double c_max = 0.0;
+  for (int j = 0; j < md.n; ++j) {
+    const double v = ray[j] / v_max;
+    slope += DD::product(sense * md.obj[j], v);
+    c_max = std::max(c_max, std::fabs(md.obj[j]));
+    double viol = 0.0;
+    if (v < 0.0 && md.col_lower[j] > -kInf) viol = -v;
+    if (v > 0.0 && md.col_upper[j] < kInf) viol = v;
+    r.max_bound_violation = std::max(r.max_bound_violation, viol);
+    record(r, viol, primal_tol, "ray leaves the bounds of column", j);
+  }
+  std::vector<DD> av(static_cast<size_t>(md.m));
+  std::vector<double> mag(static_cast<size_t>(md.m), 0.0);
+  for (int j = 0; j < md.n; ++j)
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+      const double t = md.A.value[p] * ray[j] / v_max;
+      av[md.A.index[p]] += DD(t);
+      mag[md.A.index[p]] = std::max(mag[md.A.index[p]], std::fabs(t));
+    }
+  for (int i = 0; i < md.m; ++i) {
+    const double a = av[i].value();
+    double viol = 0.0;
+    if (a < 0.0 && md.row_lower[i] > -kInf) viol = -a / (1.0 + mag[i]);
+    if (a > 0.0 && md.row_upper[i] < kInf) viol = a / (1.0 + mag[i]);
+    r.max_row_violation = std::max(r.max_row_violation, viol);
+    record(r, viol, primal_tol, "ray leaves the bounds of row", i);
+  }
+  if (!slope.less(DD(-dual_tol * (1.0 + c_max)))) {
+    r.ok = false;
+    r.message = "ray does not improve the objective";
+  }
+  return r;
+}
+
+LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options) {
+  // solve_pdlp_lp (src/core/solver.cpp:392-477): scaling, computational form, PDLP, mapping back
+  // (unscale; y <- -sense y since PDLP's reduced costs are c + A'y; d = c - A'y; objective in
+  // double-double) and the verifier.
+  LpSolveResult out;
+  const Scaling scaled = compute_scaling(model.A);
+  const LpProblem lp = make_problem(model, scaled);
+  PdlpOptions popt = options.pdlp;
+  const int n = model.n, m = model.m;
+  const double sense = model.maximize ? -1.0 : 1.0;
+  for (int round = 0;; ++round) {
+    out.pdlp = solve_pdlp(lp, popt);
+    const PdlpResult& r = out.pdlp;
+    out.status = r.status;
+    out.iterations += r.iterations;
+    out.message = r.message;
+    if (r.x.size() == static_cast<size_t>(n + m)) {
+      out.col_value.assign(r.x.begin(), r.x.begin() + n);
+      out.row_activity.assign(r.x.begin() + n, r.x.end());
+      for (int j = 0; j < n; ++j) out.col_value[j] *= scaled.col[j];
+      for (int i = 0; i < m; ++i) out.row_activity[i] /= scaled.row[i];
+      out.row_dual = r.y;
+      for (int i = 0; i < m; ++i) out.row_dual[i] *= scaled.row[i];
+      for (double& yv : out.row_dual) yv *= -sense;
+      out.col_dual.assign(static_cast<size_t>(n), 0.0);
+      DD obj(model.offset);
+      for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int64_t p = model.A.start[j]; p < model.A.start[j + 1]; ++p)
+          s += model.A.value[p] * out.row_dual[model.A.index[p]];
+        out.col_dual[j] = model.obj[j] - s;
+        obj += DD::product(model.obj[j], out.col_value[j]);
+      }
+      out.objective = obj.value();
+    }
+    VerifyReport rep;
+    bool answered = true;
+    if (r.status == Status::kOptimal) {
+      if (options.verify) rep = verify_lp_optimality(model, out.col_value.data(), out.row_dual.data());
+    } else if (r.status == Status::kInfeasible && !r.dual_ray.empty()) {
+      out.infeasibility_certificate = r.dual_ray;
+      for (int i = 0; i < m; ++i) out.infeasibility_certificate[i] *= scaled.row[i];
+      for (double& t : out.infeasibility_certificate) t *= sense;
+      if (options.verify) rep = verify_infeasibility(model, out.infeasibility_certificate.data());
+    } else if (r.status == Status::kUnbounded && !r.primal_ray.empty()) {
+      out.unbounded_ray = r.primal_ray;
+      for (int j = 0; j < n; ++j) out.unbounded_ray[j] *= scaled.col[j];
+      if (options.verify) rep = verify_unbounded_ray(model, out.unbounded_ray.data());
+    } else {
+      answered = false;
+    }
+    out.verified = answered && options.verify && rep.ok;
+    out.max_primal_violation = std::max(rep.max_bound_violation, rep.max_row_violation);
+    out.max_dual_violation = rep.max_dual_violation;
+    if (answered && !rep.ok) out.message = "PDLP: " + std::string(to_string(r.status)) + " failed verification (" + rep.message + ")";
+    if (out.verified || !answered || r.status != Status::kOptimal || round >= options.polish_rounds) break;
+    // Polish: the same solve from the rejected point, in FP64, with a tighter tolerance.
+    popt.tol /= 10.0;
+    popt.precision = Precision::kFp64;
+    popt.start_x = r.x;
+    popt.start_y = r.y;
+  }
+  return out;
+}
+
+}  // namespace smy::host

@@ -1,0 +1,158 @@
+"""Python mirror of samaya's GPU seam: gpu::PdhgDevice (src/gpu/pdhg_core.hpp:24-79) and
+gpu::csr_spmv / spmv_threads_per_row (src/gpu/spmv.hpp:16-23), over the C ABI of
+include/samaya_pdhg.h. Device arrays for csr_spmv are torch tensors on cuda:0 (memory plumbing
+only; the product is the HIP kernel).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .lp import _f64, _p
+
+CURRENT, AVERAGE = N.SMY_CURRENT, N.SMY_AVERAGE
+
+
+def _csc(rows: int, cols: int, start: np.ndarray, index: np.ndarray, value: np.ndarray):
+    keep = (np.ascontiguousarray(start, np.int64), np.ascontiguousarray(index, np.int32),
+            _f64(value))
+    s = N.SmyCsc(rows, cols, keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data)
+    return s, keep
+
+
+class PdhgDevice:
+    """One LP's PDHG state on the GPU. create() returns None without a usable GPU or when the
+    matrix is too large for 32-bit offsets (PdhgDevice::create's null); driver failures raise."""
+
+    def __init__(self, h, n: int, m: int):
+        self.lib = N.load()
+        self.h = h
+        self.n, self.m = n, m
+
+    @classmethod
+    def create(cls, A: tuple, At: tuple, cost, lower, upper, precision: str = "fp64"):
+        """A = (rows, cols, start, index, value) CSC of the m x n matrix; At the same for A'."""
+        lib = N.load()
+        a, keep_a = _csc(*A)
+        at, keep_at = _csc(*At)
+        h = C.c_void_p()
+        prec = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
+        rc = lib.smy_pdhg_create(C.byref(a), C.byref(at), _p(_f64(cost)), _p(_f64(lower)),
+                                 _p(_f64(upper)), prec, C.byref(h))
+        if rc in (N.SMY_NO_DEVICE, N.SMY_TOO_LARGE):
+            return None
+        N.check(rc, "pdhg_create")
+        return cls(h, A[1], A[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.smy_pdhg_destroy(self.h)
+            self.h = None
+
+    def set_original(self, col_scale, row_scale, cost, lower, upper):
+        N.check(self.lib.smy_pdhg_set_original(self.h, _p(_f64(col_scale)), _p(_f64(row_scale)),
+                                               _p(_f64(cost)), _p(_f64(lower)), _p(_f64(upper))),
+                "set_original")
+
+    def step(self, tau: float, sigma: float, theta: float, iteration: int):
+        N.check(self.lib.smy_pdhg_step(self.h, tau, sigma, theta, iteration), "step")
+
+    def run(self, tau: float, sigma: float, theta: float, iter0: int, count: int):
+        N.check(self.lib.smy_pdhg_run(self.h, tau, sigma, theta, iter0, count), "run")
+
+    def last_run_ms(self) -> float:
+        ms = C.c_double()
+        N.check(self.lib.smy_pdhg_last_run_ms(self.h, C.byref(ms)), "last_run_ms")
+        return ms.value
+
+    def advance(self):
+        N.check(self.lib.smy_pdhg_advance(self.h), "advance")
+
+    def restart(self):
+        N.check(self.lib.smy_pdhg_restart(self.h), "restart")
+
+    def download(self):
+        """dict of v, v_new, y, y_new, v_erg, y_erg (host copies)."""
+        nt, m = self.n + self.m, self.m
+        out = {k: np.zeros(nt) for k in ("v", "v_new", "v_erg")}
+        out.update({k: np.zeros(m) for k in ("y", "y_new", "y_erg")})
+        N.check(self.lib.smy_pdhg_download(self.h, _p(out["v"]), _p(out["v_new"]), _p(out["y"]),
+                                           _p(out["y_new"]), _p(out["v_erg"]), _p(out["y_erg"])),
+                "download")
+        return out
+
+    def evaluate(self, which: int) -> np.ndarray:
+        out = np.zeros(7)
+        N.check(self.lib.smy_pdhg_evaluate(self.h, which, _p(out)), "evaluate")
+        return out
+
+    def evaluate_both(self) -> np.ndarray:
+        out = np.zeros(14)
+        N.check(self.lib.smy_pdhg_evaluate_both(self.h, _p(out)), "evaluate_both")
+        return out
+
+    def restart_distance(self, which: int) -> tuple[float, float]:
+        dv, dy = C.c_double(), C.c_double()
+        N.check(self.lib.smy_pdhg_restart_distance(self.h, which, C.byref(dv), C.byref(dy)),
+                "restart_distance")
+        return dv.value, dy.value
+
+    def restart_to(self, which: int):
+        N.check(self.lib.smy_pdhg_restart_to(self.h, which), "restart_to")
+
+    def set_precision(self, precision: str):
+        N.check(self.lib.smy_pdhg_set_precision(
+            self.h, {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]), "set_precision")
+
+    def set_start(self, v, y):
+        N.check(self.lib.smy_pdhg_set_start(self.h, _p(_f64(v)), _p(_f64(y))), "set_start")
+
+
+def spmv_threads_per_row(nnz: int, rows: int) -> int:
+    return int(N.load().smy_spmv_threads_per_row(nnz, rows))
+
+
+def csr_spmv(ptr, idx, val, x, y, threads_per_row: int = 32) -> None:
+    """y = A x on the GPU for torch tensors on cuda:0 (int32 ptr/idx, float64 val/x/y)."""
+    import torch
+    for t, dt in ((ptr, torch.int32), (idx, torch.int32), (val, torch.float64),
+                  (x, torch.float64), (y, torch.float64)):
+        if t.dtype != dt or not t.is_cuda or not t.is_contiguous():
+            raise ValueError("csr_spmv expects contiguous cuda tensors (int32 ptr/idx, float64)")
+    rows = ptr.numel() - 1
+    stream = torch.cuda.current_stream().cuda_stream
+    N.check(N.load().smy_csr_spmv(rows, ptr.data_ptr(), idx.data_ptr(), val.data_ptr(),
+                                  x.data_ptr(), y.data_ptr(), stream, threads_per_row), "csr_spmv")
+
+
+def csr_spmv_f32(ptr, idx, val, x, y) -> None:
+    import torch
+    rows = ptr.numel() - 1
+    stream = torch.cuda.current_stream().cuda_stream
+    N.check(N.load().smy_csr_spmv_f32(rows, ptr.data_ptr(), idx.data_ptr(), val.data_ptr(),
+                                      x.data_ptr(), y.data_ptr(), stream), "csr_spmv_f32")
+
+
+class SpmvPlan:
+    """A row-block schedule over a CSR matrix already on the GPU (torch tensors)."""
+
+    def __init__(self, ptr_host: np.ndarray, ptr, idx, val, precision: str = "fp64"):
+        self.lib = N.load()
+        self.keep = (np.ascontiguousarray(ptr_host, np.int32), ptr, idx, val)
+        self.h = C.c_void_p()
+        prec = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
+        N.check(self.lib.smy_spmv_plan_create(len(ptr_host) - 1, _p(self.keep[0]), ptr.data_ptr(),
+                                              idx.data_ptr(), val.data_ptr(), prec,
+                                              C.byref(self.h)), "spmv_plan_create")
+
+    def run(self, x, y, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        N.check(self.lib.smy_spmv_plan_run(self.h, x.data_ptr(), y.data_ptr(), s), "spmv_plan_run")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.smy_spmv_plan_destroy(self.h)
+            self.h = None

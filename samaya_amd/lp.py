@@ -1,0 +1,339 @@
+"""Host-side LP objects and solver entry points (Python mirror of samaya's interfaces).
+
+  Model ............. include/samaya/model.hpp:41-80 (LP fields), CSC matrix A
+  LpProblem ......... src/lp/simplex.hpp:18-26 (computational form, A m x n CSC)
+  PdlpOptions ....... src/lp/pdlp.hpp:11-16 (+ precision, graph, detection switches)
+  PdlpResult ........ src/lp/pdlp.hpp:19-30
+  solve_pdlp ........ src/lp/pdlp.hpp:35
+  solve_lp .......... solve_pdlp_lp, src/core/solver.cpp:392-477 (LP by PDLP through Solver::solve)
+  verify_* .......... include/samaya/verify.hpp:42-67
+  gpu_info .......... include/samaya/solver.hpp:93-97
+
+All arithmetic happens in the native library (C++ host solver + HIP kernels); this module only
+marshals numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+STATUS_NAMES = ("optimal", "infeasible", "unbounded", "iteration limit", "time limit",
+                "numerical error")
+OPTIMAL, INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR = range(6)
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+@dataclass
+class Model:
+    """min/max obj'x + offset  s.t.  row_lower <= A x <= row_upper, col_lower <= x <= col_upper.
+
+    A is given as CSC arrays (start int64[n+1], index int32[nnz] sorted within columns, value
+    float64[nnz]); infinite bounds are +-inf.
+    """
+
+    m: int
+    n: int
+    start: np.ndarray
+    index: np.ndarray
+    value: np.ndarray
+    obj: np.ndarray
+    col_lower: np.ndarray
+    col_upper: np.ndarray
+    row_lower: np.ndarray
+    row_upper: np.ndarray
+    maximize: bool = False
+    offset: float = 0.0
+
+    def __post_init__(self):
+        self.start = np.ascontiguousarray(self.start, dtype=np.int64)
+        self.index = np.ascontiguousarray(self.index, dtype=np.int32)
+        self.value = _f64(self.value)
+        for name in ("obj", "col_lower", "col_upper", "row_lower", "row_upper"):
+            setattr(self, name, _f64(getattr(self, name)))
+
+    @property
+    def nnz(self) -> int:
+        return int(self.start[-1])
+
+    @classmethod
+    def from_dense(cls, A, obj, col_lower, col_upper, row_lower, row_upper, maximize=False,
+                   offset=0.0) -> "Model":
+        A = np.asarray(A, dtype=np.float64)
+        m, n = A.shape
+        start, index, value = [0], [], []
+        for j in range(n):
+            rows = np.nonzero(A[:, j])[0]
+            index.extend(rows.tolist())
+            value.extend(A[rows, j].tolist())
+            start.append(len(index))
+        return cls(m, n, np.array(start), np.array(index, dtype=np.int32), np.array(value),
+                   obj, col_lower, col_upper, row_lower, row_upper, maximize, offset)
+
+    @classmethod
+    def generate(cls, kind: int, seed: int, p0: int, p1: int, p2: int, p3: float = 0.0) -> "Model":
+        """A synthetic instance from the library's deterministic generators."""
+        lib = N.load()
+        h = C.c_void_p()
+        N.check(lib.smy_model_generate(kind, seed, p0, p1, p2, p3, C.byref(h)), "generate")
+        try:
+            return cls._export(h)
+        finally:
+            lib.smy_model_free(h)
+
+    @classmethod
+    def _export(cls, h) -> "Model":
+        lib = N.load()
+        m, n, nnz = lib.smy_model_num_rows(h), lib.smy_model_num_cols(h), lib.smy_model_num_nonzeros(h)
+        start = np.empty(n + 1, np.int64)
+        index = np.empty(nnz, np.int32)
+        value = np.empty(nnz)
+        obj, cl, cu = np.empty(n), np.empty(n), np.empty(n)
+        rl, ru = np.empty(m), np.empty(m)
+        mx = C.c_int()
+        off = C.c_double()
+        N.check(lib.smy_model_export(h, _p(start), _p(index), _p(value), _p(obj), _p(cl), _p(cu),
+                                     _p(rl), _p(ru), C.byref(mx), C.byref(off)), "export")
+        return cls(m, n, start, index, value, obj, cl, cu, rl, ru, bool(mx.value), off.value)
+
+    def native(self) -> "NativeModel":
+        return NativeModel(self)
+
+
+class NativeModel:
+    """A model copied into the native library (smy_model*)."""
+
+    def __init__(self, md: Model):
+        self.lib = N.load()
+        self.h = C.c_void_p()
+        N.check(self.lib.smy_model_create(md.m, md.n, int(md.maximize), md.offset, _p(md.start),
+                                          _p(md.index), _p(md.value), _p(md.obj), _p(md.col_lower),
+                                          _p(md.col_upper), _p(md.row_lower), _p(md.row_upper),
+                                          C.byref(self.h)), "model_create")
+        self.m, self.n = md.m, md.n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.smy_model_free(self.h)
+            self.h = None
+
+
+@dataclass
+class LpProblem:
+    """Computational form: min cost'v s.t. [A -I] v = 0, lower <= v <= upper (v = (x, r))."""
+
+    m: int
+    n: int
+    start: np.ndarray
+    index: np.ndarray
+    value: np.ndarray
+    cost: np.ndarray
+    lower: np.ndarray
+    upper: np.ndarray
+
+    def __post_init__(self):
+        self.start = np.ascontiguousarray(self.start, dtype=np.int64)
+        self.index = np.ascontiguousarray(self.index, dtype=np.int32)
+        self.value = _f64(self.value)
+        self.cost, self.lower, self.upper = _f64(self.cost), _f64(self.lower), _f64(self.upper)
+
+
+@dataclass
+class PdlpOptions:
+    tol: float = 1e-8
+    max_iterations: int = -1
+    time_limit: float = math.inf
+    use_gpu: bool = True
+    precision: str = "fp64"          # "fp64" or "mixed"
+    fp32_switch_tol: float = 1e-5
+    use_graph: bool = True
+    detect: bool = True
+    log_level: int = 0
+    verify: bool = True
+    polish_rounds: int = 0
+
+    def native(self) -> N.SmyPdlpParams:
+        p = N.SmyPdlpParams()
+        N.load().smy_pdlp_params_default(C.byref(p))
+        p.tol = self.tol
+        p.max_iterations = self.max_iterations
+        p.time_limit = self.time_limit
+        p.use_gpu = int(self.use_gpu)
+        p.precision = {"fp64": N.SMY_PRECISION_FP64, "mixed": N.SMY_PRECISION_MIXED}[self.precision]
+        p.fp32_switch_tol = self.fp32_switch_tol
+        p.use_graph = int(self.use_graph)
+        p.detect = int(self.detect)
+        p.log_level = self.log_level
+        p.verify = int(self.verify)
+        p.polish_rounds = self.polish_rounds
+        return p
+
+
+@dataclass
+class PdlpResult:
+    status: int
+    x: np.ndarray
+    y: np.ndarray
+    dual_ray: np.ndarray | None
+    primal_ray: np.ndarray | None
+    iterations: int
+    fp32_iterations: int
+    primal_residual: float
+    dual_residual: float
+    gap: float
+    operator_norm: float
+    setup_seconds: float
+    solve_seconds: float
+    kernel_ms: float
+    message: str
+    restarts: list = field(default_factory=list)   # (iter, to_average, kkt, omega)
+    evals: list = field(default_factory=list)      # (iter, (cur p, d, g), (avg p, d, g))
+
+    @property
+    def status_name(self) -> str:
+        return STATUS_NAMES[self.status]
+
+
+def solve_pdlp(lp: LpProblem, options: PdlpOptions | None = None, start_x=None,
+               start_y=None) -> PdlpResult:
+    """samaya::solve_pdlp (src/lp/pdlp.hpp:35) on the GPU."""
+    options = options or PdlpOptions()
+    lib = N.load()
+    m, n = lp.m, lp.n
+    x, y = np.zeros(n + m), np.zeros(m)
+    dray, pray = np.zeros(m), np.zeros(n)
+    cap = 1 << 16
+    r_it = np.zeros(cap, np.int64)
+    r_av = np.zeros(cap, np.int32)
+    r_kkt, r_om = np.zeros(cap), np.zeros(cap)
+    ecap = 1 << 16
+    e_it = np.zeros(ecap, np.int64)
+    e_ms = np.zeros(6 * ecap)
+    info = N.SmyPdlpInfo()
+    params = options.native()
+    sx = None if start_x is None else _f64(start_x)
+    sy = None if start_y is None else _f64(start_y)
+    lib.smy_solve_pdlp(m, n, _p(lp.start), _p(lp.index), _p(lp.value), _p(lp.cost), _p(lp.lower),
+                       _p(lp.upper), C.byref(params), _p(sx), _p(sy), C.byref(info), _p(x), _p(y),
+                       _p(dray), _p(pray), _p(r_it), _p(r_av), _p(r_kkt), _p(r_om), cap, _p(e_it),
+                       _p(e_ms), ecap)
+    nr = min(info.restarts, cap)
+    ne = min(info.evaluations, ecap)
+    return PdlpResult(
+        status=info.status, x=x, y=y,
+        dual_ray=dray if info.status == INFEASIBLE else None,
+        primal_ray=pray if info.status == UNBOUNDED else None,
+        iterations=info.iterations, fp32_iterations=info.fp32_iterations,
+        primal_residual=info.primal_residual, dual_residual=info.dual_residual, gap=info.gap,
+        operator_norm=info.operator_norm, setup_seconds=info.setup_seconds,
+        solve_seconds=info.solve_seconds, kernel_ms=info.kernel_ms,
+        message=info.message.decode(errors="replace"),
+        restarts=[(int(r_it[k]), int(r_av[k]), float(r_kkt[k]), float(r_om[k])) for k in range(nr)],
+        evals=[(int(e_it[k]), tuple(e_ms[6 * k:6 * k + 3]), tuple(e_ms[6 * k + 3:6 * k + 6]))
+               for k in range(ne)],
+    )
+
+
+@dataclass
+class LpResult:
+    status: int
+    verified: bool
+    objective: float
+    col_value: np.ndarray
+    row_dual: np.ndarray
+    col_dual: np.ndarray
+    row_activity: np.ndarray
+    certificate: np.ndarray | None
+    max_primal_violation: float
+    max_dual_violation: float
+    iterations: int
+    fp32_iterations: int
+    primal_residual: float
+    dual_residual: float
+    gap: float
+    setup_seconds: float
+    solve_seconds: float
+    kernel_ms: float
+    message: str
+
+    @property
+    def status_name(self) -> str:
+        return STATUS_NAMES[self.status]
+
+
+def solve_lp(model: Model | NativeModel, options: PdlpOptions | None = None) -> LpResult:
+    """LP by PDLP on the GPU with verification (solve_pdlp_lp, src/core/solver.cpp:392-477)."""
+    options = options or PdlpOptions()
+    nm = model if isinstance(model, NativeModel) else model.native()
+    lib = N.load()
+    m, n = nm.m, nm.n
+    x, yd, d, act = np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(m)
+    cert = np.zeros(max(m, n))
+    res = N.SmyLpResult()
+    params = options.native()
+    lib.smy_solve_lp(nm.h, C.byref(params), C.byref(res), _p(x), _p(yd), _p(d), _p(act), _p(cert))
+    st = res.status
+    return LpResult(
+        status=st, verified=bool(res.verified), objective=res.objective, col_value=x,
+        row_dual=yd, col_dual=d, row_activity=act,
+        certificate=cert[:m] if st == INFEASIBLE else cert[:n] if st == UNBOUNDED else None,
+        max_primal_violation=res.max_primal_violation, max_dual_violation=res.max_dual_violation,
+        iterations=res.iterations, fp32_iterations=res.pdlp.fp32_iterations,
+        primal_residual=res.pdlp.primal_residual, dual_residual=res.pdlp.dual_residual,
+        gap=res.pdlp.gap, setup_seconds=res.pdlp.setup_seconds,
+        solve_seconds=res.pdlp.solve_seconds, kernel_ms=res.pdlp.kernel_ms,
+        message=res.message.decode(errors="replace"))
+
+
+def verify_lp_optimality(model: Model | NativeModel, x, y, primal_tol=1e-6, dual_tol=1e-6):
+    """(ok, max primal violation, max dual violation, objective, message)."""
+    nm = model if isinstance(model, NativeModel) else model.native()
+    out = np.zeros(3)
+    msg = C.create_string_buffer(256)
+    ok = N.load().smy_verify_lp(nm.h, _p(_f64(x)), _p(_f64(y)), primal_tol, dual_tol, _p(out), msg, 256)
+    return bool(ok == 1), float(out[0]), float(out[1]), float(out[2]), msg.value.decode()
+
+
+def verify_infeasibility(model: Model | NativeModel, y):
+    nm = model if isinstance(model, NativeModel) else model.native()
+    msg = C.create_string_buffer(256)
+    ok = N.load().smy_verify_infeasibility(nm.h, _p(_f64(y)), msg, 256)
+    return bool(ok == 1), msg.value.decode()
+
+
+def verify_unbounded_ray(model: Model | NativeModel, ray):
+    nm = model if isinstance(model, NativeModel) else model.native()
+    msg = C.create_string_buffer(256)
+    ok = N.load().smy_verify_unbounded_ray(nm.h, _p(_f64(ray)), msg, 256)
+    return bool(ok == 1), msg.value.decode()
+
+
+def gpu_available() -> bool:
+    return bool(N.load().smy_gpu_available())
+
+
+def gpu_info() -> tuple[bool, str]:
+    """GpuInfo: (built with GPU kernels, device name or '')."""
+    built = C.c_int()
+    buf = C.create_string_buffer(256)
+    N.load().smy_gpu_info(C.byref(built), buf, 256)
+    return bool(built.value), buf.value.decode()
+
+
+def solve(model: Model | NativeModel) -> tuple[int, float]:
+    """samaya_solve (include/samaya_c.h:41): (samaya Status code, objective)."""
+    nm = model if isinstance(model, NativeModel) else model.native()
+    obj = C.c_double()
+    st = N.load().smy_solve(nm.h, C.byref(obj))
+    return st, obj.value

@@ -1,0 +1,403 @@
+"""Test-side bindings of the CPU oracle (oracle/_lib/liboracle.so) and, where it was built, of
+the reference's own PDLP sources (oracle/_ref/libsamaya_ref.so). Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import tempfile
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "_lib" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libsamaya_ref.so"
+
+vp = C.c_void_p
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+class OModel(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("maximize", C.c_int), ("offset", C.c_double),
+                ("start", vp), ("index", vp), ("value", vp), ("obj", vp), ("col_lower", vp),
+                ("col_upper", vp), ("row_lower", vp), ("row_upper", vp)]
+
+
+class OLp(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("start", vp), ("index", vp), ("value", vp),
+                ("cost", vp), ("lower", vp), ("upper", vp)]
+
+
+class OPdlpOut(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("primal_residual", C.c_double),
+                ("dual_residual", C.c_double), ("gap", C.c_double), ("L", C.c_double),
+                ("x", vp), ("y", vp), ("dual_ray", vp), ("primal_ray", vp), ("row_scale", vp),
+                ("col_scale", vp), ("restart_iter", vp), ("restart_to_average", vp),
+                ("restart_kkt", vp), ("restart_omega", vp), ("restart_cap", C.c_int),
+                ("restart_count", C.c_int), ("eval_iter", vp), ("eval_measures", vp),
+                ("eval_cap", C.c_int), ("eval_count", C.c_int), ("seconds", C.c_double)]
+
+
+class OLpOut(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("objective", C.c_double),
+                ("verified", C.c_int), ("max_primal_violation", C.c_double),
+                ("max_dual_violation", C.c_double), ("col_value", vp), ("row_dual", vp),
+                ("col_dual", vp), ("ray", vp), ("message", C.c_char * 256),
+                ("seconds", C.c_double)]
+
+
+class RModelBuf(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("maximize", C.c_int), ("offset", C.c_double),
+                ("nnz", C.c_longlong), ("start", vp), ("index", vp), ("value", vp), ("obj", vp),
+                ("col_lower", vp), ("col_upper", vp), ("row_lower", vp), ("row_upper", vp),
+                ("rows_cap", C.c_int), ("cols_cap", C.c_int), ("nnz_cap", C.c_longlong)]
+
+
+class RLpOut(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_longlong), ("objective", C.c_double),
+                ("verified", C.c_int), ("max_primal_violation", C.c_double),
+                ("max_dual_violation", C.c_double), ("primal_residual", C.c_double),
+                ("dual_residual", C.c_double), ("gap", C.c_double), ("col_value", vp),
+                ("row_dual", vp), ("col_dual", vp), ("ray", vp), ("pdlp_x", vp), ("pdlp_y", vp),
+                ("message", C.c_char * 256)]
+
+
+@dataclass
+class M:
+    """A model as plain arrays (the fields of oracle_model / ref_model)."""
+    m: int
+    n: int
+    maximize: int
+    offset: float
+    start: np.ndarray
+    index: np.ndarray
+    value: np.ndarray
+    obj: np.ndarray
+    col_lower: np.ndarray
+    col_upper: np.ndarray
+    row_lower: np.ndarray
+    row_upper: np.ndarray
+
+    def c(self, cls=OModel):
+        self.start = np.ascontiguousarray(self.start, np.int64)
+        self.index = np.ascontiguousarray(self.index, np.int32)
+        for k in ("value", "obj", "col_lower", "col_upper", "row_lower", "row_upper"):
+            setattr(self, k, np.ascontiguousarray(getattr(self, k), np.float64))
+        return cls(self.m, self.n, int(self.maximize), float(self.offset), _p(self.start),
+                   _p(self.index), _p(self.value), _p(self.obj), _p(self.col_lower),
+                   _p(self.col_upper), _p(self.row_lower), _p(self.row_upper))
+
+    def to_json(self):
+        f = lambda a: [float(t) if np.isfinite(t) else ("inf" if t > 0 else "-inf") for t in a]
+        return {"m": self.m, "n": self.n, "maximize": int(self.maximize), "offset": self.offset,
+                "start": self.start.tolist(), "index": self.index.tolist(),
+                "value": f(self.value), "obj": f(self.obj), "col_lower": f(self.col_lower),
+                "col_upper": f(self.col_upper), "row_lower": f(self.row_lower),
+                "row_upper": f(self.row_upper)}
+
+    @classmethod
+    def from_json(cls, d):
+        g = lambda a: np.array([float(t) for t in a], np.float64)
+        return cls(d["m"], d["n"], d["maximize"], d["offset"], np.array(d["start"], np.int64),
+                   np.array(d["index"], np.int32), g(d["value"]), g(d["obj"]),
+                   g(d["col_lower"]), g(d["col_upper"]), g(d["row_lower"]), g(d["row_upper"]))
+
+    def product_model(self):
+        import samaya_amd as smy
+        return smy.Model(self.m, self.n, self.start, self.index, self.value, self.obj,
+                         self.col_lower, self.col_upper, self.row_lower, self.row_upper,
+                         bool(self.maximize), self.offset)
+
+    @classmethod
+    def from_product(cls, md):
+        return cls(md.m, md.n, int(md.maximize), md.offset, md.start, md.index, md.value, md.obj,
+                   md.col_lower, md.col_upper, md.row_lower, md.row_upper)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            raise FileNotFoundError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        lib = C.CDLL(str(ORACLE_SO))
+        lib.oracle_solve_pdlp.argtypes = [C.POINTER(OLp), C.c_double, C.c_longlong, C.POINTER(OPdlpOut)]
+        lib.oracle_make_problem.argtypes = [C.POINTER(OModel), vp, vp, vp, vp, vp, vp]
+        lib.oracle_ruiz.argtypes = [C.POINTER(OLp), C.c_int, vp, vp]
+        lib.oracle_operator_norm.argtypes = [C.c_int, C.c_int, vp, vp, vp]
+        lib.oracle_operator_norm.restype = C.c_double
+        lib.oracle_solve_lp.argtypes = [C.POINTER(OModel), C.c_double, C.c_longlong,
+                                        C.POINTER(OLpOut), C.POINTER(OPdlpOut)]
+        lib.oracle_verify_lp.argtypes = [C.POINTER(OModel), vp, vp, C.c_double, C.c_double, vp,
+                                         C.c_char_p, C.c_int]
+        lib.oracle_sums.argtypes = [C.POINTER(OLp), vp, vp, vp, vp, vp]
+        lib.oracle_pdhg_steps.argtypes = [C.POINTER(OLp), C.c_double, C.c_double, C.c_double,
+                                          C.c_int, vp, vp, vp, vp]
+        lib.oracle_csr_spmv.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(str(REF_SO))
+        lib.ref_rng_new.argtypes = [C.c_uint]
+        lib.ref_rng_new.restype = vp
+        lib.ref_rng_free.argtypes = [vp]
+        lib.ref_random_lp.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(RModelBuf)]
+        lib.ref_reference_lp.argtypes = [C.POINTER(OModel), C.POINTER(C.c_double)]
+        lib.ref_solve_lp.argtypes = [C.POINTER(OModel), C.c_double, C.c_longlong, C.c_int,
+                                     C.POINTER(RLpOut)]
+        lib.ref_verify_lp.argtypes = [C.POINTER(OModel), vp, vp, vp]
+        lib.ref_make_problem.argtypes = [C.POINTER(OModel), vp, vp, vp, vp, vp, vp]
+        _ref = lib
+    return _ref
+
+
+# ------------------------------------------------------------------------------------------
+# Computational form (via the oracle's restatement of compute_scaling + make_problem).
+# ------------------------------------------------------------------------------------------
+@dataclass
+class LP:
+    m: int
+    n: int
+    start: np.ndarray
+    index: np.ndarray
+    value: np.ndarray
+    cost: np.ndarray
+    lower: np.ndarray
+    upper: np.ndarray
+
+    def c(self):
+        return OLp(self.m, self.n, _p(self.start), _p(self.index), _p(self.value), _p(self.cost),
+                   _p(self.lower), _p(self.upper))
+
+    def product(self):
+        import samaya_amd as smy
+        return smy.LpProblem(self.m, self.n, self.start, self.index, self.value, self.cost,
+                             self.lower, self.upper)
+
+
+def make_problem(md: M):
+    """(LP, col_scale, row_scale) from the oracle's compute_scaling + make_problem."""
+    o = oracle()
+    nt = md.n + md.m
+    cs, rs = np.zeros(md.n), np.zeros(md.m)
+    cost, lo, up = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+    av = np.zeros(md.nnz if hasattr(md, "nnz") else int(md.start[-1]))
+    cm = md.c()
+    o.oracle_make_problem(C.byref(cm), _p(cs), _p(rs), _p(cost), _p(lo), _p(up), _p(av))
+    lp = LP(md.m, md.n, md.start.copy(), md.index.copy(), av, cost, lo, up)
+    return lp, cs, rs
+
+
+@dataclass
+class OracleRun:
+    status: int
+    iterations: int
+    primal_residual: float
+    dual_residual: float
+    gap: float
+    L: float
+    x: np.ndarray
+    y: np.ndarray
+    dual_ray: np.ndarray
+    primal_ray: np.ndarray
+    row_scale: np.ndarray
+    col_scale: np.ndarray
+    restarts: list
+    evals: list
+    seconds: float
+
+
+def _pdlp_out(m, n, cap=1 << 15):
+    nt = n + m
+    bufs = dict(x=np.zeros(nt), y=np.zeros(m), dual_ray=np.zeros(m), primal_ray=np.zeros(n),
+                row_scale=np.zeros(m), col_scale=np.zeros(n), r_it=np.zeros(cap, np.int64),
+                r_av=np.zeros(cap, np.int32), r_kkt=np.zeros(cap), r_om=np.zeros(cap),
+                e_it=np.zeros(cap, np.int64), e_ms=np.zeros(6 * cap))
+    out = OPdlpOut()
+    out.x, out.y = _p(bufs["x"]), _p(bufs["y"])
+    out.dual_ray, out.primal_ray = _p(bufs["dual_ray"]), _p(bufs["primal_ray"])
+    out.row_scale, out.col_scale = _p(bufs["row_scale"]), _p(bufs["col_scale"])
+    out.restart_iter, out.restart_to_average = _p(bufs["r_it"]), _p(bufs["r_av"])
+    out.restart_kkt, out.restart_omega = _p(bufs["r_kkt"]), _p(bufs["r_om"])
+    out.restart_cap = cap
+    out.eval_iter, out.eval_measures, out.eval_cap = _p(bufs["e_it"]), _p(bufs["e_ms"]), cap
+    return out, bufs
+
+
+def _collect(out, b):
+    nr, ne = min(out.restart_count, len(b["r_it"])), min(out.eval_count, len(b["e_it"]))
+    return OracleRun(out.status, out.iterations, out.primal_residual, out.dual_residual, out.gap,
+                     out.L, b["x"], b["y"], b["dual_ray"], b["primal_ray"], b["row_scale"],
+                     b["col_scale"],
+                     [(int(b["r_it"][k]), int(b["r_av"][k]), float(b["r_kkt"][k]),
+                       float(b["r_om"][k])) for k in range(nr)],
+                     [(int(b["e_it"][k]), tuple(b["e_ms"][6 * k:6 * k + 3]),
+                       tuple(b["e_ms"][6 * k + 3:6 * k + 6])) for k in range(ne)],
+                     out.seconds)
+
+
+def oracle_solve_pdlp(lp: LP, tol: float, max_iterations: int = -1) -> OracleRun:
+    out, b = _pdlp_out(lp.m, lp.n)
+    olp = lp.c()
+    oracle().oracle_solve_pdlp(C.byref(olp), tol, max_iterations, C.byref(out))
+    return _collect(out, b)
+
+
+def oracle_solve_lp(md: M, tol: float, max_iterations: int = -1):
+    """(model-space result dict, OracleRun)."""
+    out, b = _pdlp_out(md.m, md.n)
+    res = OLpOut()
+    x, y, d, ray = np.zeros(md.n), np.zeros(md.m), np.zeros(md.n), np.zeros(max(md.m, md.n))
+    res.col_value, res.row_dual, res.col_dual, res.ray = _p(x), _p(y), _p(d), _p(ray)
+    cm = md.c()
+    oracle().oracle_solve_lp(C.byref(cm), tol, max_iterations, C.byref(res), C.byref(out))
+    return dict(status=res.status, iterations=res.iterations, objective=res.objective,
+                verified=bool(res.verified), max_primal_violation=res.max_primal_violation,
+                max_dual_violation=res.max_dual_violation, col_value=x, row_dual=y, col_dual=d,
+                ray=ray, message=res.message.decode(), seconds=res.seconds), _collect(out, b)
+
+
+def oracle_sums(lp: LP, rs, cs, v, y) -> np.ndarray:
+    out = np.zeros(7)
+    olp = lp.c()
+    oracle().oracle_sums(C.byref(olp), _p(np.ascontiguousarray(rs, float)),
+                         _p(np.ascontiguousarray(cs, float)), _p(np.ascontiguousarray(v, float)),
+                         _p(np.ascontiguousarray(y, float)), _p(out))
+    return out
+
+
+def oracle_pdhg_steps(lp: LP, tau, sigma, theta, steps):
+    nt = lp.n + lp.m
+    vn, yn, ve, ye = np.zeros(nt), np.zeros(lp.m), np.zeros(nt), np.zeros(lp.m)
+    olp = lp.c()
+    oracle().oracle_pdhg_steps(C.byref(olp), tau, sigma, theta, steps, _p(vn), _p(yn), _p(ve), _p(ye))
+    return vn, yn, ve, ye
+
+
+def oracle_verify_lp(md: M, x, y, ptol=1e-6, dtol=1e-6):
+    out = np.zeros(3)
+    msg = C.create_string_buffer(256)
+    cm = md.c()
+    ok = oracle().oracle_verify_lp(C.byref(cm), _p(np.ascontiguousarray(x, float)),
+                                   _p(np.ascontiguousarray(y, float)), ptol, dtol, _p(out), msg, 256)
+    return bool(ok), out, msg.value.decode()
+
+
+def oracle_operator_norm(m, n, start, index, value) -> float:
+    start = np.ascontiguousarray(start, np.int64)
+    index = np.ascontiguousarray(index, np.int32)
+    value = np.ascontiguousarray(value, np.float64)
+    return oracle().oracle_operator_norm(m, n, _p(start), _p(index), _p(value))
+
+
+def oracle_ruiz(lp: LP, passes=10):
+    rs, cs = np.zeros(lp.m), np.zeros(lp.n)
+    olp = lp.c()
+    oracle().oracle_ruiz(C.byref(olp), passes, _p(rs), _p(cs))
+    return rs, cs
+
+
+def oracle_csr_spmv(ptr, idx, val, x) -> np.ndarray:
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(len(ptr) - 1)
+    oracle().oracle_csr_spmv(len(ptr) - 1, _p(ptr), _p(idx), _p(val), _p(x), _p(y))
+    return y
+
+
+# ------------------------------------------------------------------------------------------
+# Reference build (only in the container that has the reference tree).
+# ------------------------------------------------------------------------------------------
+class RefRng:
+    def __init__(self, seed: int):
+        self.lib = ref()
+        self.h = self.lib.ref_rng_new(seed)
+
+    def random_lp(self, family: int, max_rows: int, max_cols: int) -> M:
+        cap_r, cap_c = max_rows, max_cols
+        nnz_cap = cap_r * cap_c
+        b = dict(start=np.zeros(cap_c + 1, np.int64), index=np.zeros(nnz_cap, np.int32),
+                 value=np.zeros(nnz_cap), obj=np.zeros(cap_c), col_lower=np.zeros(cap_c),
+                 col_upper=np.zeros(cap_c), row_lower=np.zeros(cap_r), row_upper=np.zeros(cap_r))
+        buf = RModelBuf()
+        for k, a in b.items():
+            setattr(buf, k, _p(a))
+        buf.rows_cap, buf.cols_cap, buf.nnz_cap = cap_r, cap_c, nnz_cap
+        rc = self.lib.ref_random_lp(self.h, family, max_rows, max_cols, C.byref(buf))
+        assert rc == 0
+        m, n, nnz = buf.m, buf.n, buf.nnz
+        return M(m, n, buf.maximize, buf.offset, b["start"][: n + 1].copy(), b["index"][:nnz].copy(),
+                 b["value"][:nnz].copy(), b["obj"][:n].copy(), b["col_lower"][:n].copy(),
+                 b["col_upper"][:n].copy(), b["row_lower"][:m].copy(), b["row_upper"][:m].copy())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_rng_free(self.h)
+            self.h = None
+
+
+def ref_reference_lp(md: M):
+    obj = C.c_double()
+    cm = md.c()
+    st = ref().ref_reference_lp(C.byref(cm), C.byref(obj))
+    return st, obj.value
+
+
+_RESTART = re.compile(r"pdlp: restart at iter (\d+) to the (average|current iterate) "
+                      r"\(KKT ([0-9.e+-]+)\), omega ([0-9.e+-]+)")
+
+
+def ref_solve_lp(md: M, tol: float, max_iterations: int = -1, capture_log: bool = True):
+    """Runs the reference's PDLP path; returns a dict (log level 2 captured for the restart
+    schedule)."""
+    lib = ref()
+    nt = md.n + md.m
+    x, y, d = np.zeros(md.n), np.zeros(md.m), np.zeros(md.n)
+    ray = np.zeros(max(md.m, md.n))
+    px, py = np.zeros(nt), np.zeros(md.m)
+    out = RLpOut()
+    out.col_value, out.row_dual, out.col_dual, out.ray = _p(x), _p(y), _p(d), _p(ray)
+    out.pdlp_x, out.pdlp_y = _p(px), _p(py)
+    cm = md.c()
+    log = ""
+    if capture_log:
+        libc = C.CDLL(None)
+        libc.fflush(None)
+        with tempfile.TemporaryFile(mode="w+b") as tmp:
+            saved = os.dup(1)
+            os.dup2(tmp.fileno(), 1)
+            try:
+                lib.ref_solve_lp(C.byref(cm), tol, max_iterations, 2, C.byref(out))
+                libc.fflush(None)
+            finally:
+                os.dup2(saved, 1)
+                os.close(saved)
+            tmp.seek(0)
+            log = tmp.read().decode(errors="replace")
+    else:
+        lib.ref_solve_lp(C.byref(cm), tol, max_iterations, 0, C.byref(out))
+    restarts = [(int(a), 1 if b == "average" else 0, float(c), float(e))
+                for a, b, c, e in _RESTART.findall(log)]
+    return dict(status=out.status, iterations=out.iterations, objective=out.objective,
+                verified=bool(out.verified), max_primal_violation=out.max_primal_violation,
+                max_dual_violation=out.max_dual_violation, primal_residual=out.primal_residual,
+                dual_residual=out.dual_residual, gap=out.gap, col_value=x, row_dual=y, col_dual=d,
+                ray=ray, pdlp_x=px, pdlp_y=py, restarts=restarts, message=out.message.decode())
