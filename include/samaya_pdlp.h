@@ -66,6 +66,13 @@ int smy_model_export(const smy_model* model, int64_t* start, int32_t* index, dou
                      double* obj, double* col_lower, double* col_upper, double* row_lower,
                      double* row_upper, int* maximize, double* offset);
 
+/* compute_scaling (src/linalg/scaling.cpp:130-200): power-of-two column (n) and row (m) factors
+ * of the model's A. make_problem (src/lp/lp_solver.cpp:28-51): the computational form's cost,
+ * lower and upper (n + m each) and the scaled values of A in the model's CSC order. */
+int smy_compute_scaling(const smy_model* model, double* col_scale, double* row_scale);
+int smy_make_problem(const smy_model* model, double* cost, double* lower, double* upper,
+                     double* a_value);
+
 /* PdlpOptions (src/lp/pdlp.hpp:11-16) plus this library's additions. */
 typedef struct smy_pdlp_params {
   double tol;                /* 1e-8 (the solver's kPdlpTolerance, src/core/solver.cpp:31) */

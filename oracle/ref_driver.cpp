@@ -107,6 +107,12 @@ extern "C" {
 void* ref_rng_new(unsigned seed) { return new std::mt19937(seed); }
 void ref_rng_free(void* rng) { delete static_cast<std::mt19937*>(rng); }
 
+// std::uniform_real_distribution<double>(a, b) drawn from the caller's generator, as the
+// reference's tests draw their random equilibration scales (tests/test_gpu.cpp:522-526).
+double ref_uniform(void* rng, double a, double b) {
+  return std::uniform_real_distribution<double>(a, b)(*static_cast<std::mt19937*>(rng));
+}
+
 // One draw of tests/lp_generators.hpp:33 random_lp from the caller's generator state. Returns 0,
 // or -1 when a buffer is too small.
 int ref_random_lp(void* rng, int family, int max_rows, int max_cols, ref_model_buf* out) {

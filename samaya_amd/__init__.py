@@ -5,7 +5,7 @@ here only marshals arrays into the native library's C ABI (include/samaya_pdhg.h
 include/samaya_pdlp.h); it has no compute path of its own.
 """
 from ._native import LIB_PATH, load, last_error, NativeError
-from .lp import (Model, NativeModel, LpProblem, PdlpOptions, PdlpResult, LpResult, solve_pdlp,
+from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, PdlpOptions, PdlpResult, LpResult, solve_pdlp,
                  solve_lp, solve, verify_lp_optimality, verify_infeasibility,
                  verify_unbounded_ray, gpu_available, gpu_info, STATUS_NAMES, OPTIMAL,
                  INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR)
@@ -13,6 +13,7 @@ from . import generators
 
 __all__ = [
     "LIB_PATH", "load", "last_error", "NativeError", "Model", "NativeModel", "LpProblem",
+    "compute_scaling", "make_problem",
     "PdlpOptions", "PdlpResult", "LpResult", "solve_pdlp", "solve_lp", "solve",
     "verify_lp_optimality", "verify_infeasibility", "verify_unbounded_ray", "gpu_available",
     "gpu_info", "STATUS_NAMES", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "ITERATION_LIMIT",

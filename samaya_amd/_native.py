@@ -27,7 +27,8 @@ EXPORTS = (
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
-    "smy_model_num_cols", "smy_model_num_nonzeros", "smy_model_export", "smy_pdlp_params_default",
+    "smy_model_num_cols", "smy_model_num_nonzeros", "smy_model_export", "smy_compute_scaling",
+    "smy_make_problem", "smy_pdlp_params_default",
     "smy_solve_pdlp", "smy_solve_lp", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
     "smy_verify_unbounded_ray", "smy_gpu_info",
 )
@@ -118,6 +119,8 @@ def load() -> C.CDLL:
     _proto(lib, "smy_model_num_nonzeros", i64, vp)
     _proto(lib, "smy_model_export", i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(i32),
            C.POINTER(f64))
+    _proto(lib, "smy_compute_scaling", i32, vp, vp, vp)
+    _proto(lib, "smy_make_problem", i32, vp, vp, vp, vp, vp)
     _proto(lib, "smy_pdlp_params_default", None, C.POINTER(SmyPdlpParams))
     _proto(lib, "smy_solve_pdlp", i32, i32, i32, vp, vp, vp, vp, vp, vp,
            C.POINTER(SmyPdlpParams), vp, vp, C.POINTER(SmyPdlpInfo), vp, vp, vp, vp, vp, vp, vp,

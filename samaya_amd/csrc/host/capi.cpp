@@ -184,6 +184,34 @@ int smy_model_export(const smy_model* model, int64_t* start, int32_t* index, dou
   return SMY_OK;
 }
 
+int smy_compute_scaling(const smy_model* model, double* col_scale, double* row_scale) {
+  if (model == nullptr) return SMY_INVALID;
+  try {
+    const Scaling s = compute_scaling(model->model.A);
+    put(s.col, col_scale);
+    put(s.row, row_scale);
+    return SMY_OK;
+  } catch (...) {
+    return SMY_ERROR;
+  }
+}
+
+int smy_make_problem(const smy_model* model, double* cost, double* lower, double* upper,
+                     double* a_value) {
+  if (model == nullptr) return SMY_INVALID;
+  try {
+    const LpProblem lp = make_problem(model->model, compute_scaling(model->model.A));
+    if (lp.A.nnz() != model->model.A.nnz()) return SMY_ERROR;  // a scaled entry underflowed
+    put(lp.cost, cost);
+    put(lp.lower, lower);
+    put(lp.upper, upper);
+    put(lp.A.value, a_value);
+    return SMY_OK;
+  } catch (...) {
+    return SMY_ERROR;
+  }
+}
+
 int smy_solve_pdlp(int m, int n, const int64_t* a_start, const int32_t* a_index,
                    const double* a_value, const double* cost, const double* lower,
                    const double* upper, const smy_pdlp_params* params, const double* start_x,

@@ -129,6 +129,26 @@ class NativeModel:
             self.h = None
 
 
+def compute_scaling(model: "Model | NativeModel"):
+    """compute_scaling (src/linalg/scaling.cpp:130-200): (col_scale, row_scale), powers of two."""
+    nm = model if isinstance(model, NativeModel) else model.native()
+    cs, rs = np.zeros(nm.n), np.zeros(nm.m)
+    N.check(N.load().smy_compute_scaling(nm.h, _p(cs), _p(rs)), "compute_scaling")
+    return cs, rs
+
+
+def make_problem(model: "Model") -> "LpProblem":
+    """make_problem(model, compute_scaling(model.A)) (src/lp/lp_solver.cpp:28-51)."""
+    nm = model.native()
+    nt = model.n + model.m
+    cost, lower, upper = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+    value = np.zeros(model.nnz)
+    N.check(N.load().smy_make_problem(nm.h, _p(cost), _p(lower), _p(upper), _p(value)),
+            "make_problem")
+    return LpProblem(model.m, model.n, model.start.copy(), model.index.copy(), value, cost, lower,
+                     upper)
+
+
 @dataclass
 class LpProblem:
     """Computational form: min cost'v s.t. [A -I] v = 0, lower <= v <= upper (v = (x, r))."""

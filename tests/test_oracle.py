@@ -1,0 +1,297 @@
+"""Pins the CPU oracle (oracle/pdlp_oracle.cpp) before anything is compared with it:
+
+* against the reference's own answers (golden fixtures written by the reference's PDLP built from
+  its sources): status, iteration count, restart schedule, the scaled PDLP point and the
+  model-space answer, bit for bit;
+* against the reference build itself (oracle/_ref) on further seeded families, when present;
+* against what the mathematics fixes: closed forms of the 2-norm Ruiz variant, the power
+  iteration on rank-one and diagonal matrices, hand-computed evaluation sums, known verifier
+  verdicts, scaling invariants, and HiGHS (scipy) objectives.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.optimize import linprog
+
+import oracle_lib as O
+from conftest import load_golden
+
+
+def fx_float(a):
+    return np.array([float(t) for t in a])
+
+
+def assert_matches_reference(case, md):
+    got, run = O.oracle_solve_lp(md, case["tol"], case["max_iterations"])
+    assert got["status"] == case["status"]
+    assert got["iterations"] == case["iterations"]
+    assert [[a, b] for a, b, *_ in run.restarts] == case["restarts"]
+    assert np.array_equal(run.x, fx_float(case["pdlp_x"]))
+    assert np.array_equal(run.y, fx_float(case["pdlp_y"]))
+    assert got["objective"] == case["objective"]
+    assert got["verified"] == case["verified"]
+    assert np.array_equal(got["col_value"], fx_float(case["col_value"]))
+    assert np.array_equal(got["row_dual"], fx_float(case["row_dual"]))
+
+
+@pytest.mark.parametrize("name", ["pdlp_random_feasible", "pdlp_vs_simplex",
+                                  "pdlp_through_solver", "gpu_vs_cpu", "gpu_repeatable"])
+def test_oracle_reproduces_reference_runs_bit_for_bit(name):
+    for case in load_golden(name)["cases"]:
+        assert_matches_reference(case, O.M.from_json(case["model"]))
+
+
+def test_oracle_reproduces_reference_certificates():
+    fx = load_golden("pdlp_infeasible_unbounded")
+    assert fx["infeasible"] and fx["unbounded"]
+    for key in ("infeasible", "unbounded"):
+        for case in fx[key]:
+            md = O.M.from_json(case["model"])
+            assert_matches_reference(case, md)
+            if "ray" in case:
+                got, _ = O.oracle_solve_lp(md, case["tol"], case["max_iterations"])
+                k = md.m if case["status"] == 1 else md.n
+                assert np.array_equal(got["ray"][:k], fx_float(case["ray"]))
+
+
+def test_reference_test_gates_hold_on_oracle():
+    # tests/test_pdlp.cpp:222-231: mean iterations to 1e-8 over the verified optima <= 1500
+    # (the reference reports 874), more than 10 optima, every answer equal to the simplex's.
+    cases = load_golden("pdlp_random_feasible")["cases"]
+    its = []
+    for c in cases:
+        md = O.M.from_json(c["model"])
+        got, _ = O.oracle_solve_lp(md, 1e-8, 100000)
+        if got["status"] == 0 and got["verified"]:
+            its.append(got["iterations"])
+            assert abs(got["objective"] - c["simplex_objective"]) <= 1e-6 * (1 + abs(c["simplex_objective"]))
+    assert len(its) > 10
+    assert np.mean(its) <= 1500
+    assert int(np.mean(its)) == 874
+
+
+def test_c1_matches_reference_fixture():
+    import samaya_amd as smy
+    ref = load_golden("c1_reference")
+    md = O.M.from_product(smy.generators.c1())
+    import hashlib
+    assert hashlib.sha256(md.value.tobytes()).hexdigest() == ref["model_sha"]
+    got, run = O.oracle_solve_lp(md, 1e-6)
+    assert got["status"] == ref["status"] and got["iterations"] == ref["iterations"]
+    assert got["objective"] == ref["objective"]
+    assert [[a, b] for a, b, *_ in run.restarts] == ref["restarts"]
+    assert got["verified"] == ref["verified"]
+
+
+def highs_objective(md: O.M) -> float:
+    A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+    sense = -1.0 if md.maximize else 1.0
+    eq = md.row_lower == md.row_upper
+    ub_rows, ub_rhs = [], []
+    fin_u = np.isfinite(md.row_upper) & ~eq
+    fin_l = np.isfinite(md.row_lower) & ~eq
+    A_ub = sp.vstack([A[fin_u], -A[fin_l]]).tocsr()
+    b_ub = np.concatenate([md.row_upper[fin_u], -md.row_lower[fin_l]])
+    bounds = list(zip([None if not np.isfinite(t) else t for t in md.col_lower],
+                      [None if not np.isfinite(t) else t for t in md.col_upper]))
+    res = linprog(sense * md.obj, A_ub=A_ub if A_ub.shape[0] else None,
+                  b_ub=b_ub if A_ub.shape[0] else None,
+                  A_eq=A[eq] if eq.any() else None, b_eq=md.row_lower[eq] if eq.any() else None,
+                  bounds=bounds, method="highs")
+    assert res.status == 0
+    return sense * res.fun + md.offset
+
+
+def test_c1_objective_matches_highs():
+    import samaya_amd as smy
+    md = O.M.from_product(smy.generators.c1())
+    got, _ = O.oracle_solve_lp(md, 1e-6)
+    ref = highs_objective(md)
+    assert abs(got["objective"] - ref) <= 1e-5 * (1 + abs(ref))
+
+
+def test_random_feasible_objectives_match_highs():
+    for c in load_golden("pdlp_random_feasible")["cases"][:20]:
+        if c["simplex_status"] != 0:
+            continue
+        md = O.M.from_json(c["model"])
+        got, _ = O.oracle_solve_lp(md, 1e-8, 100000)
+        ref = highs_objective(md)
+        assert abs(got["objective"] - ref) <= 1e-6 * (1 + abs(ref))
+        assert abs(c["simplex_objective"] - ref) <= 1e-9 * (1 + abs(ref))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("family,seed,size,tol", [(2, 77, 12, 1e-6), (4, 78, 12, 1e-6),
+                                                  (1, 79, 10, 1e-4), (3, 80, 10, 1e-4),
+                                                  (0, 81, 30, 1e-6)])
+def test_oracle_equals_reference_build(family, seed, size, tol):
+    # kDegenerate, kBoxed, kRandom, kLoose and larger kFeasible models, drawn by the reference's
+    # own generator: the oracle and the reference give the same bits.
+    rng = O.RefRng(seed)
+    for _ in range(8):
+        md = rng.random_lp(family, size, size)
+        r = O.ref_solve_lp(md, tol, 50000)
+        got, run = O.oracle_solve_lp(md, tol, 50000)
+        assert got["status"] == r["status"] and got["iterations"] == r["iterations"]
+        assert np.array_equal(run.x, r["pdlp_x"]) and np.array_equal(run.y, r["pdlp_y"])
+        assert got["objective"] == r["objective"] or (math.isnan(got["objective"]) and math.isnan(r["objective"]))
+        assert [(a, b) for a, b, *_ in run.restarts] == [(a, b) for a, b, *_ in r["restarts"]]
+
+
+# ---------------------------------------------------------------------------------------------
+# Closed forms and invariants (no reference involved).
+# ---------------------------------------------------------------------------------------------
+def lp_from_dense(A, cost=None, lower=None, upper=None) -> O.LP:
+    A = np.asarray(A, float)
+    m, n = A.shape
+    S = sp.csc_matrix(A)
+    S.sort_indices()
+    nt = n + m
+    return O.LP(m, n, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(float),
+                np.zeros(nt) if cost is None else np.asarray(cost, float),
+                np.zeros(nt) if lower is None else np.asarray(lower, float),
+                np.ones(nt) if upper is None else np.asarray(upper, float))
+
+
+def test_ruiz_oscillates_with_period_two_on_a_diagonal():
+    # The simultaneous full-step update of pdlp.cpp:131-178: pass 1 sets every scale to
+    # 1/|a_ii|, pass 2 measures norms of 1/|a_ii| and resets them to 1; ten passes end at 1.
+    rs, cs = O.oracle_ruiz(lp_from_dense(np.diag([4.0, 0.5])), 10)
+    assert np.array_equal(rs, [1.0, 1.0]) and np.array_equal(cs, [1.0, 1.0])
+    rs, cs = O.oracle_ruiz(lp_from_dense(np.diag([4.0, 0.5])), 9)
+    assert np.allclose(rs, [0.25, 2.0]) and np.allclose(cs, [0.25, 2.0])
+
+
+def test_ruiz_stops_after_one_pass_when_changes_are_small():
+    rs, cs = O.oracle_ruiz(lp_from_dense(np.diag([1.05, 0.97])), 10)
+    assert np.allclose(rs, [1 / 1.05, 1 / 0.97], rtol=0, atol=1e-15)
+    assert np.allclose(cs, [1 / 1.05, 1 / 0.97], rtol=0, atol=1e-15)
+
+
+def test_ruiz_clamps_extreme_scales():
+    rs, cs = O.oracle_ruiz(lp_from_dense(np.array([[1e-20]])), 1)
+    assert rs[0] == 1e8 and cs[0] == 1e8
+
+
+def test_operator_norm_rank_one_and_diagonal():
+    u = np.array([1.0, 2.0, -2.0])
+    v = np.array([3.0, 0.0, 4.0, 1.0])
+    A = np.outer(u, v)
+    S = sp.csc_matrix(A)
+    S.sort_indices()
+    sigma = np.linalg.norm(u) * np.linalg.norm(v)
+    L = O.oracle_operator_norm(3, 4, S.indptr, S.indices, S.data)
+    assert math.isclose(L, math.sqrt((1.01 * sigma) ** 2 + 1), rel_tol=1e-12)
+    D = sp.csc_matrix(np.diag([3.0, 1.0, 0.5]))
+    L = O.oracle_operator_norm(3, 3, D.indptr, D.indices, D.data)
+    assert math.isclose(L, math.sqrt((1.01 * 3.0) ** 2 + 1), rel_tol=1e-9)
+    assert O.oracle_operator_norm(0, 2, np.zeros(3, np.int64), np.zeros(0, np.int32), np.zeros(0)) == 1.0
+
+
+def test_operator_norm_power_iteration_from_below():
+    # The 20-step power iteration's Rayleigh quotient never exceeds sigma_max and approaches it;
+    # with 20 steps on random matrices it need not reach it, so L = sqrt((1.01 s)^2 + 1) is an
+    # estimate of ||[A -I]||_2 = sqrt(sigma_max^2 + 1), not a guaranteed upper bound.
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        A = rng.normal(size=(40, 30)) * (rng.random((40, 30)) < 0.2)
+        S = sp.csc_matrix(A)
+        S.sort_indices()
+        L = O.oracle_operator_norm(40, 30, S.indptr, S.indices, S.data)
+        est = math.sqrt(L * L - 1.0) / 1.01
+        smax = np.linalg.norm(A, 2)
+        assert 0.8 * smax <= est <= smax * (1 + 1e-12)
+
+
+def test_sums_hand_computed():
+    # min x1 + 2 x2, row: x1 + x2 (logical r in [1, 1]); 0 <= x <= 10; scales 1.
+    lp = lp_from_dense([[1.0, 1.0]], cost=[1.0, 2.0, 0.0], lower=[0, 0, 1], upper=[10, 10, 1])
+    v = np.array([0.25, 0.5, 1.0])   # A x - r = -0.25
+    y = np.array([-1.0])             # g = c + A'y = (0, 1) for x, (0 - (-1)) = 1 for r
+    s = O.oracle_sums(lp, np.ones(1), np.ones(2), v, y)
+    # primal_sq .0625; x boxed: lambda = g, residual 0; r fixed: lambda = g; dual_sq 0.
+    # primal obj 0.25 + 1.0; dual obj: x1 lam 0 -> 0, x2 lam 1 > 0 -> 1*0, r lam 1 -> 1*1 = 1.
+    assert np.allclose(s, [0.0625, 0.0, 1.25, 1.0, 0.3125, 1.0, 1.0], atol=0, rtol=0)
+    # A free variable's whole reduced cost is residual; one-sided bounds absorb one sign.
+    lp2 = lp_from_dense([[1.0, 1.0]], cost=[1.0, 2.0, 0.0], lower=[-np.inf, 0, 1],
+                        upper=[np.inf, np.inf, 1])
+    s2 = O.oracle_sums(lp2, np.ones(1), np.ones(2), v, np.array([-3.0]))
+    # g = (1 - 3, 2 - 3, 3) = (-2, -1, 3): x1 free -> 4; x2 lower-only, g < 0 -> 1; r fixed -> 0.
+    assert s2[1] == 5.0
+    assert s2[3] == 3.0 * 1.0  # only r's lambda = 3 > 0 meets a finite bound (lower 1)
+
+
+def tiny_lp_model() -> O.M:
+    # tests/instances/tiny_lp.mps: max 3x + 2y, x + y <= 4, x + 3y <= 6, 0 <= x <= 3, y >= 0;
+    # optimum x = 3, y = 1, objective 11 (tests/CMakeLists.txt:48-51).
+    A = sp.csc_matrix(np.array([[1.0, 1.0], [1.0, 3.0]]))
+    return O.M(2, 2, 1, 0.0, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data,
+               np.array([3.0, 2.0]), np.array([0.0, 0.0]), np.array([3.0, np.inf]),
+               np.array([-np.inf, -np.inf]), np.array([4.0, 6.0]))
+
+
+def test_tiny_lp_known_answer():
+    md = tiny_lp_model()
+    got, _ = O.oracle_solve_lp(md, 1e-8)
+    assert got["status"] == 0 and got["verified"]
+    assert abs(got["objective"] - 11.0) <= 1e-6
+    assert np.allclose(got["col_value"], [3.0, 1.0], atol=1e-6)
+
+
+def test_verifier_verdicts():
+    md = tiny_lp_model()
+    # Optimal x = (3, 1); duals: row 1 binding with y1 = 2 (max: d = c - A'y must be >= 0 ...).
+    ok, viol, _ = O.oracle_verify_lp(md, [3.0, 1.0], [2.0, 0.0])
+    assert ok, viol
+    ok, viol, msg = O.oracle_verify_lp(md, [3.0, 1.0], [2.5, 0.0])  # wrong dual
+    assert not ok and "column" in msg
+    ok, viol, msg = O.oracle_verify_lp(md, [3.5, 1.0], [2.0, 0.0])  # bound violated
+    assert not ok and viol[0] > 0.1
+
+
+def test_scaling_invariants():
+    rng = np.random.default_rng(1)
+    for seed in range(1, 11):
+        rng = np.random.default_rng(seed)
+        m, n = 15, 20
+        rs = 10.0 ** rng.integers(-6, 7, m)
+        cs = 10.0 ** rng.integers(-6, 7, n)
+        A = np.zeros((m, n))
+        for j in range(n):
+            A[j % m, j] = rs[j % m] * cs[j] * rng.uniform(0.5, 2.0)
+            keep = rng.random(m) < 0.3
+            A[keep, j] = rs[keep] * cs[j] * rng.uniform(0.5, 2.0, keep.sum())
+        S = sp.csc_matrix(A)
+        S.sort_indices()
+        md = O.M(m, n, 0, 0.0, S.indptr, S.indices, S.data, np.zeros(n), np.zeros(n), np.ones(n),
+                 np.zeros(m), np.ones(m))
+        lp, col, row = O.make_problem(md)
+        for f in np.concatenate([col, row]):
+            mant, _ = math.frexp(f)
+            assert mant == 0.5
+        before = np.max(np.abs(S.data)) / np.min(np.abs(S.data))
+        after = np.max(np.abs(lp.value)) / np.min(np.abs(lp.value))
+        assert before > 1e6 and after < 1e3
+        # Exact round trip: every factor is a power of two.
+        assert np.array_equal(lp.value / row[S.indices] / col[np.repeat(np.arange(n), np.diff(S.indptr))], S.data)
+
+
+def test_scaling_empty_rows_keep_factor_one():
+    S = sp.csc_matrix(np.array([[1e4, 1e-4], [0, 0], [0, 0]]))
+    S.sort_indices()
+    md = O.M(3, 2, 0, 0.0, S.indptr, S.indices, S.data, np.zeros(2), np.zeros(2), np.ones(2),
+             np.zeros(3), np.ones(3))
+    lp, col, row = O.make_problem(md)
+    assert row[1] == 1.0 and row[2] == 1.0
+    assert np.max(np.abs(lp.value)) / np.min(np.abs(lp.value)) <= 2.0
+
+
+def test_csr_spmv_brute_force_known_values():
+    ptr = np.array([0, 2, 2, 3], np.int32)
+    idx = np.array([0, 2, 1], np.int32)
+    val = np.array([2.0, -1.0, 4.0])
+    y = O.oracle_csr_spmv(ptr, idx, val, np.array([1.0, 0.5, 3.0]))
+    assert np.array_equal(y, [-1.0, 0.0, 2.0])
