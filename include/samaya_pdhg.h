@@ -98,6 +98,10 @@ int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y);
 /* Kernel time of the last smy_pdhg_run, in milliseconds (HIP events on the handle's stream). */
 int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
 
+/* Measurement helper: average milliseconds of the dual (rows of A) and primal (rows of A') step
+ * kernels over `reps` back-to-back launches on the handle's stream. Changes the iterates. */
+int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
+
 /* Replaces gpu::csr_spmv (spmv.hpp:16-21): y = A x for a CSR matrix in device memory (rows x ?,
  * ptr[rows + 1], idx[nnz], val[nnz]), x and y device arrays. threads_per_row is accepted for
  * the reference's signature; the gfx950 kernel picks its lane groups per row block. Runs on

@@ -128,7 +128,9 @@ typedef struct smy_lp_result {
   double max_primal_violation;
   double max_dual_violation;
   long long iterations; /* reported by samaya's CLI as barrier_iterations */
-  smy_pdlp_info pdlp;
+  long long fp32_iterations; /* of which FP32 steps (mixed precision) */
+  int rounds;           /* PDLP solves run: 1 + polish rounds used */
+  smy_pdlp_info pdlp;   /* the last solve */
   char message[256];
 } smy_lp_result;
 

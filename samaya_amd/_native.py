@@ -23,7 +23,7 @@ EXPORTS = (
     "smy_pdhg_destroy", "smy_pdhg_set_original", "smy_pdhg_step", "smy_pdhg_advance",
     "smy_pdhg_restart", "smy_pdhg_download", "smy_pdhg_evaluate", "smy_pdhg_restart_distance",
     "smy_pdhg_restart_to", "smy_pdhg_evaluate_both", "smy_pdhg_run", "smy_pdhg_set_precision",
-    "smy_pdhg_set_start", "smy_pdhg_last_run_ms", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_set_start", "smy_pdhg_last_run_ms", "smy_pdhg_time_kernels", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
@@ -58,7 +58,8 @@ class SmyPdlpInfo(C.Structure):
 class SmyLpResult(C.Structure):
     _fields_ = [("status", C.c_int), ("verified", C.c_int), ("objective", C.c_double),
                 ("max_primal_violation", C.c_double), ("max_dual_violation", C.c_double),
-                ("iterations", C.c_longlong), ("pdlp", SmyPdlpInfo), ("message", C.c_char * 256)]
+                ("iterations", C.c_longlong), ("fp32_iterations", C.c_longlong),
+                ("rounds", C.c_int), ("pdlp", SmyPdlpInfo), ("message", C.c_char * 256)]
 
 
 _lib = None
@@ -84,6 +85,13 @@ def load() -> C.CDLL:
         raise FileNotFoundError(
             f"{LIB_PATH} is missing: build it with `python samaya_amd/build.py` "
             "(there is no CPU fallback for the PDLP iterations)")
+    # PyTorch ships its own HIP runtime. When both are used in one process, torch must be loaded
+    # first so that this library binds to the same runtime (libamdhip64 is shared by soname);
+    # loading this library first leaves torch unable to see the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     _proto(lib, "smy_gpu_available", i32)
     _proto(lib, "smy_gpu_device_name", i32, C.c_char_p, C.c_size_t)
@@ -104,6 +112,7 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_set_precision", i32, vp, i32)
     _proto(lib, "smy_pdhg_set_start", i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_last_run_ms", i32, vp, C.POINTER(f64))
+    _proto(lib, "smy_pdhg_time_kernels", i32, vp, i32, C.POINTER(f64), C.POINTER(f64))
     _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)
     _proto(lib, "smy_csr_spmv_f32", i32, i32, vp, vp, vp, vp, vp, vp)
     _proto(lib, "smy_spmv_threads_per_row", i32, i64, i32)

@@ -300,6 +300,61 @@ int step_impl(smy_pdhg* h, double tau, double sigma, double theta, long long ite
   return SMY_OK;
 }
 
+// Average duration of each of the two step kernels over `reps` back-to-back launches (HIP events
+// on the handle's stream). The launches repeat one step from the current state, so the iterates
+// and averages change: a measurement helper, not part of a solve.
+template <typename T>
+int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
+  Iter<T>& S = state<T>(h);
+  const int p = h->p;
+  hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, h->stream, h->scalars, 0.5, 0.5,
+                     1.0, 0LL);
+  DualRowsOp<T> dual;
+  dual.n = h->n;
+  dual.j = 0;
+  dual.sc = h->scalars;
+  dual.vbar = S.vbar[p];
+  dual.y = S.y[p];
+  dual.y_new = S.y[1 - p];
+  dual.y_erg = S.y_erg;
+  dual.v = S.v[p];
+  dual.v_new = S.v[1 - p];
+  dual.v_erg = S.v_erg;
+  dual.vbar_next = S.vbar[1 - p];
+  dual.cost = S.cost;
+  dual.lower = S.lower;
+  dual.upper = S.upper;
+  PrimalColsOp<T> primal;
+  primal.j = 0;
+  primal.sc = h->scalars;
+  primal.y_new = S.y[1 - p];
+  primal.v = S.v[p];
+  primal.v_new = S.v[1 - p];
+  primal.v_erg = S.v_erg;
+  primal.vbar_next = S.vbar[1 - p];
+  primal.cost = S.cost;
+  primal.lower = S.lower;
+  primal.upper = S.upper;
+  float t = 0.0f;
+  launch_rows(h->a, avals<T>(h->a), dual, h->stream);  // warm
+  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+  for (int r = 0; r < reps; ++r) launch_rows(h->a, avals<T>(h->a), dual, h->stream);
+  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+  SMY_HIP(hipEventSynchronize(h->ev_stop));
+  SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
+  *dual_ms = static_cast<double>(t) / reps;
+  launch_rows(h->at, avals<T>(h->at), primal, h->stream);
+  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+  for (int r = 0; r < reps; ++r) launch_rows(h->at, avals<T>(h->at), primal, h->stream);
+  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+  SMY_HIP(hipEventSynchronize(h->ev_stop));
+  SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
+  *primal_ms = static_cast<double>(t) / reps;
+  SMY_HIP(hipGetLastError());
+  h->timed = false;
+  return SMY_OK;
+}
+
 template <typename T>
 int run_impl(smy_pdhg* h, double tau, double sigma, double theta, long long iter0, int count) {
   if (count <= 0) return SMY_OK;
@@ -556,7 +611,9 @@ int smy_pdhg_create(const smy_csc* A, const smy_csc* At, const double* cost, con
                     const double* upper, int precision, smy_pdhg** out) {
   if (out == nullptr) return fail(SMY_INVALID, "null output handle");
   *out = nullptr;
-  if (A == nullptr || At == nullptr || cost == nullptr || lower == nullptr || upper == nullptr)
+  if (A == nullptr || At == nullptr || A->start == nullptr || At->start == nullptr)
+    return fail(SMY_INVALID, "null argument");
+  if (A->rows + A->cols > 0 && (cost == nullptr || lower == nullptr || upper == nullptr))
     return fail(SMY_INVALID, "null argument");
   if (precision != SMY_FP64 && precision != SMY_FP32) return fail(SMY_INVALID, "bad precision");
   if (A->rows != At->cols || A->cols != At->rows || A->rows < 0 || A->cols < 0)
@@ -595,8 +652,9 @@ void smy_pdhg_destroy(smy_pdhg* h) { delete h; }
 
 int smy_pdhg_set_original(smy_pdhg* h, const double* col_scale, const double* row_scale,
                           const double* cost, const double* lower, const double* upper) {
-  if (h == nullptr || col_scale == nullptr || row_scale == nullptr || cost == nullptr ||
-      lower == nullptr || upper == nullptr)
+  // Empty vectors may come as null pointers (an LP without rows has no row scales).
+  if (h == nullptr || (h->n > 0 && col_scale == nullptr) || (h->m > 0 && row_scale == nullptr) ||
+      (h->nt > 0 && (cost == nullptr || lower == nullptr || upper == nullptr)))
     return fail(SMY_INVALID, "null argument");
   return guarded([&]() -> int {
     h->cs = h->upload(col_scale, h->n);
@@ -622,6 +680,15 @@ int smy_pdhg_run(smy_pdhg* h, double tau, double sigma, double theta, long long 
   return guarded([&]() -> int {
     return h->precision == SMY_FP64 ? run_impl<double>(h, tau, sigma, theta, iter0, count)
                                     : run_impl<float>(h, tau, sigma, theta, iter0, count);
+  });
+}
+
+int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
+  if (h == nullptr || dual_ms == nullptr || primal_ms == nullptr || reps <= 0)
+    return fail(SMY_INVALID, "bad argument");
+  return guarded([&]() -> int {
+    return h->precision == SMY_FP64 ? time_kernels_impl<double>(h, reps, dual_ms, primal_ms)
+                                    : time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
   });
 }
 
@@ -735,7 +802,8 @@ int smy_pdhg_set_precision(smy_pdhg* h, int precision) {
 }
 
 int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y) {
-  if (h == nullptr || v == nullptr || (y == nullptr && h->m > 0)) return fail(SMY_INVALID, "null argument");
+  if (h == nullptr || (v == nullptr && h->nt > 0) || (y == nullptr && h->m > 0))
+    return fail(SMY_INVALID, "null argument");
   return guarded([&]() -> int {
     if (h->precision == SMY_FP64) set_start_impl<double>(h, v, y); else set_start_impl<float>(h, v, y);
     return SMY_OK;

@@ -288,6 +288,8 @@ int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_r
     result->max_primal_violation = r.max_primal_violation;
     result->max_dual_violation = r.max_dual_violation;
     result->iterations = r.iterations;
+    result->fp32_iterations = r.fp32_iterations;
+    result->rounds = r.rounds;
     fill_info(r.pdlp, &result->pdlp);
     copy_text(result->message, sizeof result->message, r.message);
     put(r.col_value, col_value);

@@ -126,6 +126,8 @@ struct LpSolveResult {
   double max_primal_violation = 0.0;
   double max_dual_violation = 0.0;
   long long iterations = 0;       // reported by the reference as barrier_iterations
+  long long fp32_iterations = 0;  // of which FP32 steps (mixed precision)
+  int rounds = 0;                 // solves run (1 + polish rounds used)
   std::string message;
   PdlpResult pdlp;
 };

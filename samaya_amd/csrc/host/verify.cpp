@@ -329,6 +329,8 @@ LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options) {
     const PdlpResult& r = out.pdlp;
     out.status = r.status;
     out.iterations += r.iterations;
+    out.fp32_iterations += r.fp32_iterations;
+    out.rounds = round + 1;
     out.message = r.message;
     if (r.x.size() == static_cast<size_t>(n + m)) {
       out.col_value.assign(r.x.begin(), r.x.begin() + n);

@@ -67,6 +67,12 @@ class PdhgDevice:
         N.check(self.lib.smy_pdhg_last_run_ms(self.h, C.byref(ms)), "last_run_ms")
         return ms.value
 
+    def time_kernels(self, reps: int = 20) -> tuple[float, float]:
+        """Average ms of the dual and primal step kernels (changes the iterates)."""
+        d, p = C.c_double(), C.c_double()
+        N.check(self.lib.smy_pdhg_time_kernels(self.h, reps, C.byref(d), C.byref(p)), "time_kernels")
+        return d.value, p.value
+
     def advance(self):
         N.check(self.lib.smy_pdhg_advance(self.h), "advance")
 

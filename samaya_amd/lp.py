@@ -279,6 +279,7 @@ class LpResult:
     max_dual_violation: float
     iterations: int
     fp32_iterations: int
+    rounds: int
     primal_residual: float
     dual_residual: float
     gap: float
@@ -309,7 +310,7 @@ def solve_lp(model: Model | NativeModel, options: PdlpOptions | None = None) -> 
         row_dual=yd, col_dual=d, row_activity=act,
         certificate=cert[:m] if st == INFEASIBLE else cert[:n] if st == UNBOUNDED else None,
         max_primal_violation=res.max_primal_violation, max_dual_violation=res.max_dual_violation,
-        iterations=res.iterations, fp32_iterations=res.pdlp.fp32_iterations,
+        iterations=res.iterations, fp32_iterations=res.fp32_iterations, rounds=res.rounds,
         primal_residual=res.pdlp.primal_residual, dual_residual=res.pdlp.dual_residual,
         gap=res.pdlp.gap, setup_seconds=res.pdlp.setup_seconds,
         solve_seconds=res.pdlp.solve_seconds, kernel_ms=res.pdlp.kernel_ms,

@@ -71,8 +71,11 @@ def test_device_steps_match_oracle_iterates(gpu):
         dev.step(0.3, 0.3, 1.0, 7)
         got = dev.download()
         vn, yn, ve, ye = oracle_pdhg_steps(eq, 0.3, 0.3, 1.0, 8)
-        assert close(got["v_new"], vn, 1e-12) and close(got["y_new"], yn, 1e-12)
-        assert close(got["v_erg"], ve, 1e-12) and close(got["y_erg"], ye, 1e-12)
+        # The device sums each row in another order than the host's column scatter; with these
+        # fixed step sizes the iterates grow to ~1e8 in 8 steps, so the bar is the reference's
+        # own for device-vs-host iterates (1e-10 relative, docs/GETTING_STARTED.md:246-248).
+        assert close(got["v_new"], vn, 1e-10) and close(got["y_new"], yn, 1e-10)
+        assert close(got["v_erg"], ve, 1e-10) and close(got["y_erg"], ye, 1e-10)
 
 
 def test_evaluate_matches_host_sums_and_repeats(gpu):
