@@ -4,7 +4,7 @@
 //   A by rows (CSR: ptr int32[m+1], idx int32[nnz], val) = the CSC of A' read as rows, and
 //   A' by rows (ptr int32[n+1], ...) = the CSC of A read as rows; FP64 values always (the
 //   evaluation sums and the FP64 steps), FP32 values too when FP32 steps are asked for; each
-//   with its row-block schedule (int32[nblocks+1]).
+//   with its row-block schedule ((first row, first nonzero) int32 pairs, nblocks + 1).
 //   Iterates in the step precision T: v[2] and y[2] (current / next, swapped by a parity bit
 //   instead of pointers), vbar[2] (the extrapolated point of this and of the next step), the
 //   running averages v_erg, y_erg, the restart point v_rst, y_rst, and cost/lower/upper.
@@ -68,7 +68,7 @@ struct DevMatrix {
   int* idx = nullptr;
   double* val64 = nullptr;
   float* val32 = nullptr;
-  int* blocks = nullptr;
+  int2* meta = nullptr;  // row-block schedule: nblocks + 1 (first row, first nonzero)
   int nblocks = 0;
 };
 
@@ -87,16 +87,42 @@ struct Iter {
   bool ready = false;
 };
 
+// Workgroups of one csr_rows_kernel instantiation that fit on the device at once: the kernel is
+// persistent, so its grid is min(row blocks, resident workgroups). Fixed per instantiation and
+// device model, so the evaluation sums keep their order from run to run.
 template <typename V, typename Op>
-void launch_rows(const DevMatrix& M, const V* val, const Op& op, hipStream_t s) {
-  if (M.nblocks == 0) return;
+int resident_workgroups() {
+  static const int count = [] {
+    int per_cu = 0, device = 0;
+    hipDeviceProp_t prop;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, csr_rows_kernel<V, Op>, kBlock, 0) !=
+            hipSuccess ||
+        hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess)
+      return 1024;
+    return std::max(1, per_cu) * std::max(1, prop.multiProcessorCount);
+  }();
+  return count;
+}
+
+template <typename V, typename Op>
+int rows_grid(const DevMatrix& M) {
+  return std::min(M.nblocks, resident_workgroups<V, Op>());
+}
+
+// Returns the grid used (0: no rows, nothing launched).
+template <typename V, typename Op>
+int launch_rows(const DevMatrix& M, const V* val, const Op& op, hipStream_t s) {
+  if (M.nblocks == 0) return 0;
   CsrRows<V> A;
   A.rows = M.rows;
+  A.nblocks = M.nblocks;
   A.ptr = M.ptr;
   A.idx = M.idx;
   A.val = val;
-  A.blocks = M.blocks;
-  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(M.nblocks), dim3(kBlock), 0, s, A, op);
+  A.meta = M.meta;
+  const int grid = rows_grid<V, Op>(M);
+  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(grid), dim3(kBlock), 0, s, A, op);
+  return grid;
 }
 
 }  // namespace
@@ -179,12 +205,13 @@ void upload_rows(smy_pdhg* h, const smy_csc* csc, DevMatrix& M, bool fp32) {
   M.nnz = csc->start[csc->cols];
   std::vector<int> ptr(static_cast<size_t>(csc->cols) + 1);
   for (int r = 0; r <= csc->cols; ++r) ptr[r] = static_cast<int>(csc->start[r]);
-  const std::vector<int> starts = build_row_blocks(ptr.data(), M.rows);
-  M.nblocks = static_cast<int>(starts.size()) - 1;
+  const std::vector<RowBlockMeta> meta = build_row_blocks(ptr.data(), M.rows);
+  M.nblocks = static_cast<int>(meta.size()) - 1;
   M.ptr = h->upload(ptr.data(), static_cast<long long>(ptr.size()));
   M.idx = h->upload(csc->index, M.nnz);
   M.val64 = h->upload(csc->value, M.nnz);
-  M.blocks = h->upload(starts.data(), static_cast<long long>(starts.size()));
+  static_assert(sizeof(RowBlockMeta) == sizeof(int2), "schedule entries are read as int2");
+  M.meta = h->upload(reinterpret_cast<const int2*>(meta.data()), static_cast<long long>(meta.size()));
   if (fp32) {
     M.val32 = h->alloc<float>(M.nnz);
     if (M.nnz > 0)
@@ -398,9 +425,9 @@ int run_impl(smy_pdhg* h, double tau, double sigma, double theta, long long iter
 // Evaluation sums of NV iterates into h->sums_host (7 per iterate). Synchronizes.
 template <typename T, int NV>
 void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
-  const int nb1 = h->a.nblocks, nb2 = h->at.nblocks;
+  // Each persistent workgroup leaves one partial per sum: at most nblocks of them per matrix.
   double* part1 = h->partial;
-  double* part2 = h->partial + static_cast<size_t>(7 * NV) * nb1;
+  double* part2 = h->partial + static_cast<size_t>(7 * NV) * h->a.nblocks;
   MeasureRowsOp<T, NV> rows;
   rows.n = h->n;
   for (int v = 0; v < NV; ++v) {
@@ -412,7 +439,7 @@ void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
   rows.lower = h->lower_o;
   rows.upper = h->upper_o;
   rows.partial = part1;
-  launch_rows(h->a, h->a.val64, rows, h->stream);
+  const int nb1 = launch_rows(h->a, h->a.val64, rows, h->stream);
   MeasureColsOp<T, NV> cols;
   cols.m = h->m;
   for (int v = 0; v < NV; ++v) {
@@ -424,7 +451,7 @@ void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
   cols.lower = h->lower_o;
   cols.upper = h->upper_o;
   cols.partial = part2;
-  launch_rows(h->at, h->at.val64, cols, h->stream);
+  const int nb2 = launch_rows(h->at, h->at.val64, cols, h->stream);
   // Both partial arrays in one fixed-order pass: the rows of A first, then the rows of A'.
   constexpr int K = 7 * NV;
   hipLaunchKernelGGL((finish_sums_kernel<K>), dim3(1), dim3(kBlock), 0, h->stream, nb1, part1,
@@ -491,9 +518,8 @@ int restart_to_impl(smy_pdhg* h, int which) {
   const T* cy = which == SMY_CURRENT ? S.y[1 - p] : S.y_erg;
   const int grid = blocks_for(std::max(h->nt, h->m));
   hipLaunchKernelGGL((restart_to_kernel<T>), dim3(grid), dim3(kBlock), 0, h->stream, h->nt, h->m,
-                     cv, cy, S.v_rst, S.v[p], S.v[1 - p], S.vbar[p], S.y_rst, S.y[p], S.y[1 - p]);
-  hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks_for(h->nt)), dim3(kBlock), 0, h->stream,
-                     static_cast<long long>(h->nt), S.vbar[p], S.vbar[1 - p]);
+                     cv, cy, S.v_rst, S.v[p], S.v[1 - p], S.vbar[0], S.vbar[1], S.y_rst, S.y[p],
+                     S.y[1 - p]);
   SMY_HIP(hipGetLastError());
   h->vbar_any = true;
   h->stepped = false;
@@ -828,10 +854,10 @@ struct smy_spmv_plan {
   const int* ptr = nullptr;
   const int* idx = nullptr;
   const void* val = nullptr;
-  int* blocks = nullptr;
+  int2* meta = nullptr;
   int nblocks = 0;
   ~smy_spmv_plan() {
-    if (blocks) (void)hipFree(blocks);
+    if (meta) (void)hipFree(meta);
   }
 };
 
@@ -843,7 +869,7 @@ int plan_run(smy_spmv_plan* p, const void* x, void* y, hipStream_t s) {
   M.rows = p->rows;
   M.ptr = const_cast<int*>(p->ptr);
   M.idx = const_cast<int*>(p->idx);
-  M.blocks = p->blocks;
+  M.meta = p->meta;
   M.nblocks = p->nblocks;
   if (p->precision == SMY_FP64) {
     SpmvOp<double, double> op{static_cast<const double*>(x), static_cast<double*>(y)};
@@ -864,10 +890,10 @@ int plan_make(int rows, const int* ptr_host, const int* ptr, const int* idx, con
   p->ptr = ptr;
   p->idx = idx;
   p->val = val;
-  const std::vector<int> starts = build_row_blocks(ptr_host, rows);
-  p->nblocks = static_cast<int>(starts.size()) - 1;
-  SMY_HIP(hipMalloc(&p->blocks, starts.size() * sizeof(int)));
-  SMY_HIP(hipMemcpy(p->blocks, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice));
+  const std::vector<RowBlockMeta> meta = build_row_blocks(ptr_host, rows);
+  p->nblocks = static_cast<int>(meta.size()) - 1;
+  SMY_HIP(hipMalloc(&p->meta, meta.size() * sizeof(int2)));
+  SMY_HIP(hipMemcpy(p->meta, meta.data(), meta.size() * sizeof(int2), hipMemcpyHostToDevice));
   *out = p.release();
   return SMY_OK;
 }

@@ -1,12 +1,13 @@
 // Row-block schedule for the LDS-staged CSR kernels (host side).
 //
-// A matrix stored by rows is cut into blocks of consecutive rows; one 256-thread workgroup
-// processes one block. A block holds at most kBlockNnz nonzeros and at most kBlockRows rows, so
-// its products fit in LDS and every thread of the workgroup streams its share of the block's
-// values and indices with fully coalesced loads. A row longer than kBlockNnz gets a block of its
-// own, which the kernel processes with a strided loop over the row instead of LDS staging.
-// The schedule depends only on the sparsity pattern, so every sum the kernels form is taken in
-// the same order on every run.
+// A matrix stored by rows is cut into blocks of consecutive rows. A block holds at most
+// kBlockNnz nonzeros and at most kBlockRows rows, so its products fit in LDS and every thread of
+// a 256-thread workgroup streams kBlockNnz / 256 of the block's values and indices with fully
+// coalesced loads. A row longer than kLongRow gets a block of its own, which the kernel sums with
+// a strided loop over the whole workgroup instead of LDS staging (so no lane group ever reduces
+// more than kLongRow / g products serially). Each block is described by the pair (first row,
+// first nonzero); the next pair closes it. The schedule depends only on the sparsity pattern, so
+// every sum the kernels form is taken in the same order on every run.
 #pragma once
 
 #include <cstdint>
@@ -14,30 +15,35 @@
 
 namespace smy {
 
-constexpr int kBlockNnz = 1024;  // Nonzeros staged in LDS per workgroup.
-constexpr int kBlockRows = 256;  // Rows per workgroup (one epilogue thread per row).
+constexpr int kBlockNnz = 1024;  // Nonzeros staged in LDS per block.
+constexpr int kBlockRows = 256;  // Rows per block (one epilogue thread per row).
+constexpr int kLongRow = 256;    // Longer rows get a block of their own.
 
-inline std::vector<int> build_row_blocks(const int* ptr, int rows, int cap = kBlockNnz,
-                                         int max_rows = kBlockRows) {
-  std::vector<int> starts{0};
+struct RowBlockMeta {
+  int row;  // first row of the block
+  int nz;   // offset of its first nonzero
+};
+
+inline std::vector<RowBlockMeta> build_row_blocks(const int* ptr, int rows) {
+  std::vector<RowBlockMeta> meta;
   int r = 0;
   while (r < rows) {
-    if (ptr[r + 1] - ptr[r] > cap) {
+    meta.push_back({r, ptr[r]});
+    if (ptr[r + 1] - ptr[r] > kLongRow) {
       ++r;
-      starts.push_back(r);
       continue;
     }
     const int first = r;
     long long nnz = 0;
-    while (r < rows && r - first < max_rows) {
+    while (r < rows && r - first < kBlockRows) {
       const int len = ptr[r + 1] - ptr[r];
-      if (len > cap || nnz + len > cap) break;
+      if (len > kLongRow || nnz + len > kBlockNnz) break;
       nnz += len;
       ++r;
     }
-    starts.push_back(r);
   }
-  return starts;
+  meta.push_back({rows, ptr[rows]});
+  return meta;
 }
 
 }  // namespace smy
