@@ -34,6 +34,9 @@ enum {
 };
 
 enum { SMY_PRECISION_FP64 = 0, SMY_PRECISION_MIXED = 1 };
+/* Enhanced: infinity-norm Ruiz + Pock-Chambolle preconditioning in place of the reference's
+ * 2-norm equilibration variant; every other decision unchanged. */
+enum { SMY_ALGORITHM_REFERENCE = 0, SMY_ALGORITHM_ENHANCED = 1 };
 
 /* Synthetic instance families (deterministic, seeded mt19937_64 with the library's own uniform
  * and Box-Muller transforms). */
@@ -86,6 +89,7 @@ typedef struct smy_pdlp_params {
   int log_level;             /* 0 silent ... 3 debug, to stdout */
   int verify;                /* model-level solve: run the verifier */
   int polish_rounds;         /* model-level solve: FP64 continuations with tol/10 if rejected */
+  int algorithm;             /* SMY_ALGORITHM_REFERENCE (the reference's decisions) or _ENHANCED */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);

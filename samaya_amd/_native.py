@@ -43,7 +43,7 @@ class SmyPdlpParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iterations", C.c_longlong), ("time_limit", C.c_double),
                 ("use_gpu", C.c_int), ("precision", C.c_int), ("fp32_switch_tol", C.c_double),
                 ("use_graph", C.c_int), ("detect", C.c_int), ("log_level", C.c_int),
-                ("verify", C.c_int), ("polish_rounds", C.c_int)]
+                ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int)]
 
 
 class SmyPdlpInfo(C.Structure):

@@ -189,16 +189,32 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
         for (int v = 0; v < NV; ++v) rowsum[v][grp] = s[v];
       }
     } else {
-      // One long row: the whole workgroup strides over it.
+      // One long row: the whole workgroup strides over it, kPer entries per thread per round
+      // with every load of a round issued before the first product.
       Acc s[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) s[v] = Acc(0);
       const int end = cur.p0 + cur.nnz;
-      for (int p = cur.p0 + tid; p < end; p += kBlock) {
-        const Acc a = static_cast<Acc>(__builtin_nontemporal_load(A.val + p));
-        const int c = __builtin_nontemporal_load(A.idx + p);
+      for (int p = cur.p0 + tid; p < end; p += kPer * kBlock) {
+        V a[kPer];
+        int c[kPer];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) s[v] += a * op.gather(v, c);
+        for (int u = 0; u < kPer; ++u) {
+          const int q = p + u * kBlock;
+          a[u] = q < end ? __builtin_nontemporal_load(A.val + q) : V(0);
+          c[u] = q < end ? __builtin_nontemporal_load(A.idx + q) : 0;
+        }
+        Acc g[NV][kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) g[v][u] = p + u * kBlock < end ? op.gather(v, c[u]) : Acc(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) s[v] += static_cast<Acc>(a[u]) * g[v][u];
+        }
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {

@@ -17,7 +17,7 @@ namespace smy {
 
 constexpr int kBlockNnz = 1024;  // Nonzeros staged in LDS per block.
 constexpr int kBlockRows = 256;  // Rows per block (one epilogue thread per row).
-constexpr int kLongRow = 256;    // Longer rows get a block of their own.
+constexpr int kLongRow = kBlockNnz;  // Longer rows get a block of their own.
 
 struct RowBlockMeta {
   int row;  // first row of the block

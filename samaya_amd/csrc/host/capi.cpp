@@ -47,6 +47,7 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.use_graph = p->use_graph != 0;
   o.detect = p->detect != 0;
   o.log_level = p->log_level;
+  o.algorithm = p->algorithm == 1 ? Algorithm::kEnhanced : Algorithm::kReference;
   return o;
 }
 
@@ -97,6 +98,7 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->log_level = 0;
   p->verify = 1;
   p->polish_rounds = 0;
+  p->algorithm = SMY_ALGORITHM_REFERENCE;
 }
 
 int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,

@@ -166,6 +166,44 @@ void equilibrate(const Csc& A, const Csc& At, std::vector<double>& rs, std::vect
   }
 }
 
+// Diagonal preconditioner of the enhanced algorithm: `passes` rounds of infinity-norm Ruiz
+// equilibration (every row and column of D_r A D_c divided by the square root of its largest
+// magnitude, rows and columns from the same current matrix), then one Pock-Chambolle step with
+// alpha = 1 (rows by the square root of their 1-norm, columns likewise). Factors of empty rows and
+// columns stay 1.
+void ruiz_pock_chambolle(const Csc& A, const Csc& At, std::vector<double>& rs,
+                         std::vector<double>& cs, int passes) {
+  const int m = A.rows, n = A.cols;
+  rs.assign(static_cast<size_t>(m), 1.0);
+  cs.assign(static_cast<size_t>(n), 1.0);
+  if (m == 0 || n == 0) return;
+  std::vector<double> rn(static_cast<size_t>(m)), cn(static_cast<size_t>(n));
+  const auto norms = [&](bool one_norm) {
+    for (int j = 0; j < n; ++j) {
+      double acc = 0.0;
+      for (int64_t p = A.start[j]; p < A.start[j + 1]; ++p) {
+        const double t = std::fabs(A.value[p] * rs[A.index[p]] * cs[j]);
+        acc = one_norm ? acc + t : std::max(acc, t);
+      }
+      cn[j] = acc;
+    }
+    for (int i = 0; i < m; ++i) {
+      double acc = 0.0;
+      for (int64_t p = At.start[i]; p < At.start[i + 1]; ++p) {
+        const double t = std::fabs(At.value[p] * rs[i] * cs[At.index[p]]);
+        acc = one_norm ? acc + t : std::max(acc, t);
+      }
+      rn[i] = acc;
+    }
+    for (int j = 0; j < n; ++j)
+      if (cn[j] > 0.0) cs[j] /= std::sqrt(cn[j]);
+    for (int i = 0; i < m; ++i)
+      if (rn[i] > 0.0) rs[i] /= std::sqrt(rn[i]);
+  };
+  for (int pass = 0; pass < passes; ++pass) norms(false);
+  norms(true);
+}
+
 // Upper bound on ||[A -I]||_2 by 20 power iterations on A'A from a seeded random start
 // (src/lp/pdlp.cpp:36-103): sqrt((1.01 sigma)^2 + 1).
 double operator_norm_bound(const Csc& A, const Csc& At) {
@@ -391,7 +429,10 @@ PdlpResult Solve::run() {
   }
 
   // --- Setup (src/lp/pdlp.cpp:260-318) ---
-  equilibrate(lp_.A, lp_.At, rs_, cs_, 10);
+  if (opt_.algorithm == Algorithm::kEnhanced)
+    ruiz_pock_chambolle(lp_.A, lp_.At, rs_, cs_, 10);
+  else
+    equilibrate(lp_.A, lp_.At, rs_, cs_, 10);
   const Csc A_eq = csc_scaled(lp_.A, rs_, cs_);
   const Csc At_eq = csc_scaled(lp_.At, cs_, rs_);
   std::vector<double> cost = lp_.cost, lower = lp_.lower, upper = lp_.upper;

@@ -35,12 +35,18 @@ const char* to_string(Status s);
 
 enum class Precision : int { kFp64 = 0, kMixed = 1 };
 
+// kReference: the reference's decisions exactly (2-norm equilibration variant, pdlp.cpp:109-187).
+// kEnhanced: infinity-norm Ruiz (10 passes) followed by Pock-Chambolle (alpha = 1) diagonal
+// preconditioning, the usual PDLP preconditioner; every other decision unchanged.
+enum class Algorithm : int { kReference = 0, kEnhanced = 1 };
+
 struct PdlpOptions {
   double tol = 1e-8;              // max(rel primal, rel dual, |rel gap|) < tol
   long long max_iterations = -1;  // -1: max(1e4, 1000 (m + n)), capped at 1e6 without time limit
   double time_limit = kInf;       // seconds, setup included
   bool use_gpu = true;            // iterations run on the GPU only: false is an error here
   Precision precision = Precision::kFp64;
+  Algorithm algorithm = Algorithm::kReference;
   // Mixed precision: FP32 steps until the better of current / average reaches this (FP64)
   // termination distance, or FP32 stops improving; then FP64 steps to `tol`.
   double fp32_switch_tol = 1e-5;

@@ -182,6 +182,7 @@ class PdlpOptions:
     log_level: int = 0
     verify: bool = True
     polish_rounds: int = 0
+    algorithm: str = "reference"    # "reference" or "enhanced"
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -197,6 +198,7 @@ class PdlpOptions:
         p.log_level = self.log_level
         p.verify = int(self.verify)
         p.polish_rounds = self.polish_rounds
+        p.algorithm = {"reference": 0, "enhanced": 1}[self.algorithm]
         return p
 
 
