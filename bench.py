@@ -112,7 +112,7 @@ def kernel_roofline(md, precision: str) -> dict:
     return {"dual_ms": dual_ms, "primal_ms": primal_ms, "bytes": b}
 
 
-def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 192)) -> dict | None:
+def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 128)) -> dict | None:
     """The reference's CPU PDLP (oracle/_ref) on the same LP for two short runs; the difference
     gives the time per iteration, extrapolated to the GPU FP64 run's iteration count."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -147,6 +147,9 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmv", action="store_true")
+    ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
+    ap.add_argument("--max-iterations", type=int, default=-1,
+                    help="cap each solve (profiling runs); the line then reports no speedup")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the batched and "
@@ -164,8 +167,8 @@ def main() -> int:
     md = smy.generators.c2(args.scale)
     log(f"LP m={md.m} n={md.n} nnz={md.nnz} generated in {time.perf_counter() - t0:.1f} s")
     nm = md.native()
-    opts = smy.PdlpOptions(tol=args.tol, precision="mixed", detect=False, verify=True,
-                           polish_rounds=0)
+    opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, detect=False,
+                           verify=True, polish_rounds=0, max_iterations=args.max_iterations)
 
     for w in range(args.warmup):
         r = smy.solve_lp(nm, opts)
@@ -185,11 +188,14 @@ def main() -> int:
     iterate_s = sum(r.solve_seconds - r.setup_seconds for r in runs) / len(runs)
     setup_s = sum(r.setup_seconds for r in runs) / len(runs)
     last = runs[-1]
-    assert all(r.status == smy.OPTIMAL for r in runs), [r.message for r in runs]
+    capped = args.max_iterations > 0
+    assert capped or all(r.status == smy.OPTIMAL for r in runs), [r.message for r in runs]
 
     fp64 = None
-    if not args.no_fp64:
-        f = smy.solve_lp(nm, smy.PdlpOptions(tol=args.tol, precision="fp64", detect=False))
+    if not args.no_fp64 and not capped:
+        # Plain FP64 PDLP: the reference's algorithm and precision on the same GPU.
+        f = smy.solve_lp(nm, smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference",
+                                             detect=False))
         fp64 = f
         log(f"fp64: {f.status_name} {f.iterations} it, iterate {f.solve_seconds - f.setup_seconds:.3f} s")
 
@@ -217,6 +223,8 @@ def main() -> int:
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "fp32 steps + fp64 evaluation/polish",
+        "algorithm": args.algorithm,
+        "status": last.status_name,
         "data": "synthetic (seeded generator)",
         "config": {"workload": "sparse LP m=500000 n=1000000 nnz~12M power-law columns, "
                                "mixed-precision PDLP to rel KKT 1e-6" if args.scale == 1.0 else
