@@ -53,6 +53,19 @@ def step_kernel_bytes(nnz: int, m: int, n: int, s: int) -> dict:
     return {"dual": dual, "primal": primal}
 
 
+def profiled_traffic(op: str) -> int | None:
+    """Calibrated HBM bytes per launch of a step kernel, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/step_kernels_c2_mixed.md), if present."""
+    path = ROOT / "profiles" / "step_kernels_c2_mixed.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == op:
+            return int(parts[2])
+    return None
+
+
 def spmv_sweep(md) -> dict:
     import numpy as np
     import scipy.sparse as sp
@@ -236,7 +249,9 @@ def main() -> int:
         "verified": last.verified,
         "objective": last.objective,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
+                     "traffic": (profiled_traffic("PrimalColsOp<float>" if dom == "primal" else "DualRowsOp<float>")
+                                 if args.scale == 1.0 else None),
                      "kernel": f"csr_rows_kernel<float, {'PrimalColsOp' if dom == 'primal' else 'DualRowsOp'}<float>>",
                      "avg_us": round(dom_ms * 1e3, 2), "bytes_per_launch": rl["bytes"][dom]},
         "cpu_baseline": cpu,

@@ -507,21 +507,20 @@ __global__ __launch_bounds__(kBlock) void distance_kernel(int nt, int m, const T
   block_partials<2>(local, partial);
 }
 
-// out[q] = sum over blocks of partial[q * nblocks + b], in a fixed order (one workgroup).
-template <int K>
+// out[q] = sum over blocks of partial[q * nblocks + b], in a fixed order: workgroup q sums row q
+// (launch with one workgroup per sum).
 __global__ __launch_bounds__(kBlock) void finish_sums_kernel(int nblocks,
                                                              const double* __restrict__ partial,
                                                              double* __restrict__ out) {
-  __shared__ double wsum[K][kBlock / kWave];
+  __shared__ double wsum[kBlock / kWave];
   const int tid = threadIdx.x;
-  for (int q = 0; q < K; ++q) {
-    double acc = 0.0;
-    for (int b = tid; b < nblocks; b += kBlock) acc += partial[q * nblocks + b];
-    acc = group_sum<kWave>(acc);
-    if ((tid & (kWave - 1)) == 0) wsum[q][tid / kWave] = acc;
-  }
+  const int q = blockIdx.x;
+  double acc = 0.0;
+  for (int b = tid; b < nblocks; b += kBlock) acc += partial[static_cast<size_t>(q) * nblocks + b];
+  acc = group_sum<kWave>(acc);
+  if ((tid & (kWave - 1)) == 0) wsum[tid / kWave] = acc;
   __syncthreads();
-  if (tid < K) out[tid] = ((wsum[tid][0] + wsum[tid][1]) + wsum[tid][2]) + wsum[tid][3];
+  if (tid == 0) out[q] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
 // A restart at an iterate (cv, cy): it becomes the restart point, the current and next iterate,

@@ -440,9 +440,9 @@ void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
   const int nb2 = launch_rows(h->at, h->at.val64, cols, h->stream);
   // Both partial arrays in one fixed-order pass: the rows of A first, then the rows of A'.
   constexpr int K = 7 * NV;
-  hipLaunchKernelGGL((finish_sums_kernel<K>), dim3(1), dim3(kBlock), 0, h->stream, nb1, part1,
+  hipLaunchKernelGGL(finish_sums_kernel, dim3(K), dim3(kBlock), 0, h->stream, nb1, part1,
                      h->sums_dev);
-  hipLaunchKernelGGL((finish_sums_kernel<K>), dim3(1), dim3(kBlock), 0, h->stream, nb2, part2,
+  hipLaunchKernelGGL(finish_sums_kernel, dim3(K), dim3(kBlock), 0, h->stream, nb2, part2,
                      h->sums_dev + K);
   SMY_HIP(hipGetLastError());
   SMY_HIP(hipMemcpyAsync(h->sums_host, h->sums_dev, 2 * K * sizeof(double), hipMemcpyDeviceToHost,
@@ -483,8 +483,8 @@ int distance_impl(smy_pdhg* h, int which, double* dv, double* dy) {
   const int grid = std::max(1, std::min(1024, blocks_for(h->nt)));
   hipLaunchKernelGGL((distance_kernel<T>), dim3(grid), dim3(kBlock), 0, h->stream, h->nt, h->m, v,
                      S.v_rst, y, S.y_rst, h->partial);
-  hipLaunchKernelGGL((finish_sums_kernel<2>), dim3(1), dim3(kBlock), 0, h->stream, grid,
-                     h->partial, h->sums_dev);
+  hipLaunchKernelGGL(finish_sums_kernel, dim3(2), dim3(kBlock), 0, h->stream, grid, h->partial,
+                     h->sums_dev);
   SMY_HIP(hipGetLastError());
   SMY_HIP(hipMemcpyAsync(h->sums_host, h->sums_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
                          h->stream));
