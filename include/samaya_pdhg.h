@@ -102,6 +102,43 @@ int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
  * kernels over `reps` back-to-back launches on the handle's stream. Changes the iterates. */
 int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
 
+/* ---- Batches: many LPs in one handle (no reference counterpart; the reference solves one LP
+ * per PdhgDevice). The `count` instances (at most 16384) are stacked block-diagonally and step
+ * together, two kernel launches per step and precision for the whole batch; every instance keeps
+ * its own step sizes, step count since restart, restart point, precision, and can be frozen once
+ * its solve has ended. An instance's arithmetic is the same as in a handle of its own: a batch
+ * of K gives each LP the bits smy_pdhg_create would give it. The single-LP functions above take
+ * only handles of one instance; advance, last_run_ms, time_kernels and destroy take both. ---- */
+
+/* A[i] (m_i x n_i CSC), At[i] its transpose, cost[i] / lower[i] / upper[i] (n_i + m_i). */
+int smy_pdhg_create_batch(int count, const smy_csc* A, const smy_csc* At,
+                          const double* const* cost, const double* const* lower,
+                          const double* const* upper, int precision, smy_pdhg** out);
+/* Number of instances (0 for NULL). */
+int smy_pdhg_batch_size(smy_pdhg* h);
+/* set_original / set_start / download for instance i (vectors of that instance's sizes). */
+int smy_pdhg_set_original_at(smy_pdhg* h, int instance, const double* col_scale,
+                             const double* row_scale, const double* cost, const double* lower,
+                             const double* upper);
+int smy_pdhg_set_start_at(smy_pdhg* h, int instance, const double* v, const double* y);
+int smy_pdhg_download_at(smy_pdhg* h, int instance, double* v, double* v_new, double* y,
+                         double* y_new, double* v_erg, double* y_erg);
+/* `count` steps of every instance that is not frozen, instance i with tau[i], sigma[i] and its
+ * iterations since restart starting at iter0[i] (arrays of batch size). */
+int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, double theta,
+                       const long long* iter0, int count);
+/* out[14 i + q]: smy_pdhg_evaluate_both of instance i (meaningless for frozen instances). */
+int smy_pdhg_evaluate_batch(smy_pdhg* h, double* out);
+/* which[i]: -1 (skip), SMY_CURRENT or SMY_AVERAGE; out[2 i], out[2 i + 1]: squared distances of
+ * that candidate from instance i's restart point (v, then y). */
+int smy_pdhg_restart_distance_batch(smy_pdhg* h, const int* which, double* out);
+/* Restarts instance i at its candidate which[i] (-1: no restart). */
+int smy_pdhg_restart_to_batch(smy_pdhg* h, const int* which);
+/* precision[i]: SMY_FP64 / SMY_FP32 (state converted when it changes) or -1 (unchanged). */
+int smy_pdhg_set_precision_batch(smy_pdhg* h, const int* precision);
+/* Instance i takes no further steps and is not evaluated; its state stays downloadable. */
+int smy_pdhg_freeze(smy_pdhg* h, int instance);
+
 /* Replaces gpu::csr_spmv (spmv.hpp:16-21): y = A x for a CSR matrix in device memory (rows x ?,
  * ptr[rows + 1], idx[nnz], val[nnz]), x and y device arrays. threads_per_row is accepted for
  * the reference's signature; the gfx950 kernel picks its lane groups per row block. Runs on

@@ -45,7 +45,8 @@ enum {
                                    values N(0,1); half equality rows, half <=; feasible x0 */
   SMY_GEN_SPARSE_POWERLAW = 2,  /* p0 rows, p1 cols, column counts power law on [2, p2] with mean
                                    p3 (max realized), values N(0,1) x 10^U[-2,2] */
-  SMY_GEN_BLENDING = 3,         /* multi-period blending LP, p0 periods, p1 units, p2 streams */
+  SMY_GEN_BLENDING = 3,         /* multi-period blending LP: p0 periods (0: U{4..48}), p1 units,
+                                   p2 streams, p3 qualities (0: U{1..30}); staircase structure */
   SMY_GEN_KNAPSACK = 4          /* multi-dimensional knapsack LP, p0 rows, p1 cols, p2 nnz/col */
 };
 
@@ -145,6 +146,16 @@ typedef struct smy_lp_result {
 int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_result* result,
                  double* col_value, double* row_dual, double* col_dual, double* row_activity,
                  double* certificate);
+
+/* smy_solve_lp of `count` models in one batch (no reference counterpart): the PDLP iterations of
+ * all models share the kernel launches on one device handle, every other decision is each
+ * model's own, so results[i] equals smy_solve_lp of models[i] alone. col_values / row_duals:
+ * NULL, or `count` pointers (each NULL or of the model's size). times[3] (may be NULL): host
+ * setup seconds, solve seconds (setup included) and device milliseconds of the step runs, summed
+ * over the rounds. */
+int smy_solve_lp_batch(const smy_model* const* models, int count, const smy_pdlp_params* params,
+                       smy_lp_result* results, double* const* col_values,
+                       double* const* row_duals, double* times);
 
 /* samaya_solve (include/samaya_c.h:41) for this library: LP by PDLP on the GPU with default
  * parameters. Returns samaya's Status code (1 optimal, 2 infeasible, 3 unbounded, 5 time limit,

@@ -5,8 +5,8 @@ here only marshals arrays into the native library's C ABI (include/samaya_pdhg.h
 include/samaya_pdlp.h); it has no compute path of its own.
 """
 from ._native import LIB_PATH, load, last_error, NativeError
-from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, PdlpOptions, PdlpResult, LpResult, solve_pdlp,
-                 solve_lp, solve, verify_lp_optimality, verify_infeasibility,
+from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, PdlpOptions,
+                 PdlpResult, LpResult, BatchTimes, solve_pdlp, solve_lp, solve_lp_batch, solve, verify_lp_optimality, verify_infeasibility,
                  verify_unbounded_ray, gpu_available, gpu_info, STATUS_NAMES, OPTIMAL,
                  INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR)
 from . import generators
@@ -14,7 +14,8 @@ from . import generators
 __all__ = [
     "LIB_PATH", "load", "last_error", "NativeError", "Model", "NativeModel", "LpProblem",
     "compute_scaling", "make_problem",
-    "PdlpOptions", "PdlpResult", "LpResult", "solve_pdlp", "solve_lp", "solve",
+    "PdlpOptions", "PdlpResult", "LpResult", "BatchTimes", "solve_pdlp", "solve_lp",
+    "solve_lp_batch", "solve",
     "verify_lp_optimality", "verify_infeasibility", "verify_unbounded_ray", "gpu_available",
     "gpu_info", "STATUS_NAMES", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "ITERATION_LIMIT",
     "TIME_LIMIT", "NUMERICAL_ERROR", "generators",

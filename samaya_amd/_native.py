@@ -23,13 +23,17 @@ EXPORTS = (
     "smy_pdhg_destroy", "smy_pdhg_set_original", "smy_pdhg_step", "smy_pdhg_advance",
     "smy_pdhg_restart", "smy_pdhg_download", "smy_pdhg_evaluate", "smy_pdhg_restart_distance",
     "smy_pdhg_restart_to", "smy_pdhg_evaluate_both", "smy_pdhg_run", "smy_pdhg_set_precision",
-    "smy_pdhg_set_start", "smy_pdhg_last_run_ms", "smy_pdhg_time_kernels", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_set_start", "smy_pdhg_last_run_ms", "smy_pdhg_time_kernels",
+    "smy_pdhg_create_batch", "smy_pdhg_batch_size", "smy_pdhg_set_original_at",
+    "smy_pdhg_set_start_at", "smy_pdhg_download_at", "smy_pdhg_run_batch",
+    "smy_pdhg_evaluate_batch", "smy_pdhg_restart_distance_batch", "smy_pdhg_restart_to_batch",
+    "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
     "smy_model_num_cols", "smy_model_num_nonzeros", "smy_model_export", "smy_compute_scaling",
     "smy_make_problem", "smy_pdlp_params_default",
-    "smy_solve_pdlp", "smy_solve_lp", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
+    "smy_solve_pdlp", "smy_solve_lp", "smy_solve_lp_batch", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
     "smy_verify_unbounded_ray", "smy_gpu_info",
 )
 
@@ -113,6 +117,18 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_set_start", i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_last_run_ms", i32, vp, C.POINTER(f64))
     _proto(lib, "smy_pdhg_time_kernels", i32, vp, i32, C.POINTER(f64), C.POINTER(f64))
+    _proto(lib, "smy_pdhg_create_batch", i32, i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), vp, vp,
+           vp, i32, C.POINTER(vp))
+    _proto(lib, "smy_pdhg_batch_size", i32, vp)
+    _proto(lib, "smy_pdhg_set_original_at", i32, vp, i32, vp, vp, vp, vp, vp)
+    _proto(lib, "smy_pdhg_set_start_at", i32, vp, i32, vp, vp)
+    _proto(lib, "smy_pdhg_download_at", i32, vp, i32, vp, vp, vp, vp, vp, vp)
+    _proto(lib, "smy_pdhg_run_batch", i32, vp, vp, vp, f64, vp, i32)
+    _proto(lib, "smy_pdhg_evaluate_batch", i32, vp, vp)
+    _proto(lib, "smy_pdhg_restart_distance_batch", i32, vp, vp, vp)
+    _proto(lib, "smy_pdhg_restart_to_batch", i32, vp, vp)
+    _proto(lib, "smy_pdhg_set_precision_batch", i32, vp, vp)
+    _proto(lib, "smy_pdhg_freeze", i32, vp, i32)
     _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)
     _proto(lib, "smy_csr_spmv_f32", i32, i32, vp, vp, vp, vp, vp, vp)
     _proto(lib, "smy_spmv_threads_per_row", i32, i64, i32)
@@ -136,6 +152,8 @@ def load() -> C.CDLL:
            vp, i32, vp, vp, i32)
     _proto(lib, "smy_solve_lp", i32, vp, C.POINTER(SmyPdlpParams), C.POINTER(SmyLpResult), vp, vp,
            vp, vp, vp)
+    _proto(lib, "smy_solve_lp_batch", i32, vp, i32, C.POINTER(SmyPdlpParams),
+           C.POINTER(SmyLpResult), vp, vp, vp)
     _proto(lib, "smy_solve", i32, vp, C.POINTER(f64))
     _proto(lib, "smy_verify_lp", i32, vp, vp, vp, f64, f64, vp, C.c_char_p, C.c_size_t)
     _proto(lib, "smy_verify_infeasibility", i32, vp, vp, C.c_char_p, C.c_size_t)

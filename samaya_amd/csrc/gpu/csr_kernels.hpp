@@ -1,8 +1,7 @@
 // Device kernels for the PDHG hot path on gfx950 (CDNA4, wavefront 64).
 //
-// Every product with A or A' runs through one kernel shape, csr_rows_kernel: a persistent grid
-// of 256-thread workgroups (as many as fit on the chip) walks the row-block schedule
-// (row_blocks.hpp), block b, b + grid, b + 2 grid, ... For each block the workgroup
+// Every product with A or A' runs through one kernel shape, csr_rows_kernel: one 256-thread
+// workgroup per row block of the schedule (row_blocks.hpp). The workgroup
 //   1. streams the block's values and column indices once, 4 per thread, with coalesced
 //      non-temporal loads (read once per product, they must not evict the gathered vector from
 //      L2 / Infinity Cache), together with the block's row offsets and, for each row, the inputs
@@ -13,10 +12,12 @@
 //      with a strided loop over the whole workgroup;
 //   4. applies the epilogue functor (Op::row) to each row total: the PDHG update of that row's
 //      variable, or the evaluation sums, so no product makes a round trip through HBM.
-// The loads of step 1 for the next block are issued before steps 2-4 of the current one, and the
-// schedule entry of the block after that is loaded one block further ahead, so each workgroup
-// keeps two blocks' worth of memory traffic in flight instead of one dependent chain. All sums
-// are formed in an order fixed by the schedule and the grid: repeated runs are bit-identical.
+//
+// Batches. Several LPs stacked block-diagonally share the kernels: every block belongs to one
+// LP (its instance, binst[b]), whose step scalars the epilogue reads, and a launch covers a list
+// of blocks (the blocks of the instances that step in that precision). A single LP is the batch
+// of one. All sums are formed in an order fixed by the schedule: repeated runs are bit-identical,
+// and an LP's sums do not depend on the other LPs of its batch.
 #pragma once
 
 #include "hip_common.hpp"
@@ -33,12 +34,14 @@ struct CsrRows {
   const int* __restrict__ ptr = nullptr;
   const int* __restrict__ idx = nullptr;
   const V* __restrict__ val = nullptr;
-  const int2* __restrict__ meta = nullptr;  // nblocks + 1 entries (first row, first nonzero)
+  const int2* __restrict__ meta = nullptr;   // nblocks + 1 entries (first row, first nonzero)
+  const int* __restrict__ binst = nullptr;   // instance of each block (null: all instance 0)
+  const int* __restrict__ list = nullptr;    // blocks of this launch (null: block = workgroup)
 };
 
-// Scalars of one PDHG step, read by the step kernels from device memory so that the same kernel
-// nodes can be replayed from a HIP graph with new values. alpha = 1 / (k0 + j + 1) for the j-th
-// step of a run whose first step is the k0-th since the last restart.
+// Scalars of one PDHG step for one instance, read by the step kernels from device memory so that
+// the same kernel nodes can be replayed from a HIP graph with new values. alpha = 1 / (k0 + j + 1)
+// for the j-th step of a run whose first step is the k0-th since the instance's last restart.
 struct StepScalars {
   double tau;
   double sigma;
@@ -52,9 +55,10 @@ __device__ inline T clamp_to(T v, T lo, T up) {
 }
 
 // Sums of K per-thread values over the workgroup (wavefront butterfly, then the four wavefront
-// totals in index order); thread q < K writes total q to partial[q * gridDim.x + blockIdx.x].
+// totals in index order); thread q < K writes total q to partial[q * stride].
 template <int K>
-__device__ inline void block_partials(const double (&local)[K], double* __restrict__ partial) {
+__device__ inline void block_partials(const double (&local)[K], double* __restrict__ partial,
+                                      int stride) {
   __shared__ double wsum[K][kBlock / kWave];
   const int tid = threadIdx.x;
 #pragma unroll
@@ -64,47 +68,8 @@ __device__ inline void block_partials(const double (&local)[K], double* __restri
   }
   __syncthreads();
   if (tid < K) {
-    partial[tid * gridDim.x + blockIdx.x] =
+    partial[static_cast<size_t>(tid) * stride] =
         ((wsum[tid][0] + wsum[tid][1]) + wsum[tid][2]) + wsum[tid][3];
-  }
-}
-
-// What one thread holds of a block between issuing its loads and using them.
-template <typename V, typename Pre>
-struct BlockStage {
-  int r0, p0, nrows, nnz;
-  bool long_row;
-  V a[kPer];
-  int c[kPer];
-  int rp;   // ptr[r0 + tid] - p0 for tid < nrows
-  Pre pre;  // the epilogue inputs of row r0 + tid
-};
-
-template <typename V, typename Op>
-__device__ inline void stage_block(const CsrRows<V>& A, const Op& op, int2 m0, int2 m1,
-                                   BlockStage<V, typename Op::Pre>& s) {
-  const int tid = threadIdx.x;
-  s.r0 = m0.x;
-  s.p0 = m0.y;
-  s.nrows = m1.x - m0.x;
-  s.nnz = m1.y - m0.y;
-  s.long_row = s.nrows == 1 && s.nnz > kLongRow;
-  if (!s.long_row) {
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int k = tid + u * kBlock;
-      if (k < s.nnz) {
-        s.a[u] = __builtin_nontemporal_load(A.val + s.p0 + k);
-        s.c[u] = __builtin_nontemporal_load(A.idx + s.p0 + k);
-      } else {
-        s.a[u] = V(0);
-        s.c[u] = 0;
-      }
-    }
-  }
-  if (tid < s.nrows) {
-    s.rp = A.ptr[s.r0 + tid] - s.p0;
-    s.pre = op.pre(s.r0 + tid);
   }
 }
 
@@ -119,131 +84,128 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
   __shared__ Acc wsum[NV][kBlock / kWave];
   __shared__ int rp[kBlockRows + 1];
   const int tid = threadIdx.x;
-  const int grid = gridDim.x;
+  const int b = A.list != nullptr ? A.list[blockIdx.x] : static_cast<int>(blockIdx.x);
+  const int2 m0 = A.meta[b];
+  const int2 m1 = A.meta[b + 1];
+  const int inst = A.binst != nullptr ? A.binst[b] : 0;
+  const int r0 = m0.x, p0 = m0.y;
+  const int nrows = m1.x - m0.x;
+  const int nnz = m1.y - m0.y;
+  const bool long_row = nrows == 1 && nnz > kLongRow;
+
+  // Streams of values and indices, row offsets and epilogue inputs: all issued before any use.
+  V a[kPer];
+  int c[kPer];
+  if (!long_row) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int k = tid + u * kBlock;
+      if (k < nnz) {
+        a[u] = __builtin_nontemporal_load(A.val + p0 + k);
+        c[u] = __builtin_nontemporal_load(A.idx + p0 + k);
+      } else {
+        a[u] = V(0);
+        c[u] = 0;
+      }
+    }
+  }
+  Pre pre{};
+  if (tid < nrows) {
+    rp[tid] = A.ptr[r0 + tid] - p0;
+    pre = op.pre(r0 + tid, inst);
+  }
+  if (tid == 0) rp[nrows] = nnz;
+
   double sums[NS];
 #pragma unroll
   for (int q = 0; q < NS; ++q) sums[q] = 0.0;
 
-  int b = blockIdx.x;
-  BlockStage<V, Pre> cur;
-  if (b < A.nblocks) stage_block(A, op, A.meta[b], A.meta[b + 1], cur);
-  int bn = b + grid;
-  int2 n0 = make_int2(0, 0), n1 = make_int2(0, 0);
-  if (bn < A.nblocks) {
-    n0 = A.meta[bn];
-    n1 = A.meta[bn + 1];
-  }
-  while (b < A.nblocks) {
-    // Gathers of this block (their loads are the oldest in flight after the streams).
+  if (!long_row) {
     Acc xs[NV][kPer];
-    if (!cur.long_row) {
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const bool live = tid + u * kBlock < cur.nnz;
+    for (int u = 0; u < kPer; ++u) {
+      const bool live = tid + u * kBlock < nnz;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, cur.c[u]) : Acc(0);
-      }
+      for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, c[u]) : Acc(0);
     }
-    // The next block's streams and epilogue inputs, and the schedule entry after it.
-    BlockStage<V, Pre> nxt;
-    const bool more = bn < A.nblocks;
-    if (more) stage_block(A, op, n0, n1, nxt);
-    const int bnn = bn + grid;
-    int2 q0 = make_int2(0, 0), q1 = make_int2(0, 0);
-    if (bnn < A.nblocks) {
-      q0 = A.meta[bnn];
-      q1 = A.meta[bnn + 1];
-    }
-
-    if (!cur.long_row) {
 #pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int k = tid + u * kBlock;
-        if (k < cur.nnz) {
+    for (int u = 0; u < kPer; ++u) {
+      const int k = tid + u * kBlock;
+      if (k < nnz) {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) prod[v][k] = static_cast<Acc>(cur.a[u]) * xs[v][u];
-        }
-      }
-      if (tid < cur.nrows) rp[tid] = cur.rp;
-      if (tid == 0) rp[cur.nrows] = cur.nnz;
-      __syncthreads();
-      // Lanes per row: the largest power of two <= 64 that still gives every row a group.
-      int g = kWave;
-      while (g > 1 && g * cur.nrows > kBlock) g >>= 1;
-      const int grp = tid / g;
-      const int lane = tid & (g - 1);
-      Acc s[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) s[v] = Acc(0);
-      if (grp < cur.nrows) {
-        const int e = rp[grp + 1];
-        for (int q = rp[grp] + lane; q < e; q += g) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) s[v] += prod[v][q];
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < NV; ++v) s[v] = group_sum_dyn(s[v], g);
-      if (grp < cur.nrows && lane == 0) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) rowsum[v][grp] = s[v];
-      }
-    } else {
-      // One long row: the whole workgroup strides over it, kPer entries per thread per round
-      // with every load of a round issued before the first product.
-      Acc s[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) s[v] = Acc(0);
-      const int end = cur.p0 + cur.nnz;
-      for (int p = cur.p0 + tid; p < end; p += kPer * kBlock) {
-        V a[kPer];
-        int c[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-          const int q = p + u * kBlock;
-          a[u] = q < end ? __builtin_nontemporal_load(A.val + q) : V(0);
-          c[u] = q < end ? __builtin_nontemporal_load(A.idx + q) : 0;
-        }
-        Acc g[NV][kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) g[v][u] = p + u * kBlock < end ? op.gather(v, c[u]) : Acc(0);
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) s[v] += static_cast<Acc>(a[u]) * g[v][u];
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const Acc t = group_sum<kWave>(s[v]);
-        if ((tid & (kWave - 1)) == 0) wsum[v][tid / kWave] = t;
-      }
-      __syncthreads();
-      if (tid == 0) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-          rowsum[v][0] = ((wsum[v][0] + wsum[v][1]) + wsum[v][2]) + wsum[v][3];
+        for (int v = 0; v < NV; ++v) prod[v][k] = static_cast<Acc>(a[u]) * xs[v][u];
       }
     }
     __syncthreads();
-    if (tid < cur.nrows) {
-      Acc acc[NV];
+    // Lanes per row: the largest power of two <= 64 that still gives every row a group.
+    int g = kWave;
+    while (g > 1 && g * nrows > kBlock) g >>= 1;
+    const int grp = tid / g;
+    const int lane = tid & (g - 1);
+    Acc s[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) acc[v] = rowsum[v][tid];
-      op.row(cur.r0 + tid, acc, cur.pre, sums);
+    for (int v = 0; v < NV; ++v) s[v] = Acc(0);
+    if (grp < nrows) {
+      const int e = rp[grp + 1];
+      for (int q = rp[grp] + lane; q < e; q += g) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s[v] += prod[v][q];
+      }
     }
-    __syncthreads();  // every LDS read of this block precedes the next block's writes
-    if (!more) break;
-    cur = nxt;
-    b = bn;
-    bn = bnn;
-    n0 = q0;
-    n1 = q1;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = group_sum_dyn(s[v], g);
+    if (grp < nrows && lane == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) rowsum[v][grp] = s[v];
+    }
+  } else {
+    // One long row: the whole workgroup strides over it, kPer entries per thread per round
+    // with every load of a round issued before the first product.
+    Acc s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = Acc(0);
+    const int end = p0 + nnz;
+    for (int p = p0 + tid; p < end; p += kPer * kBlock) {
+      V al[kPer];
+      int cl[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int q = p + u * kBlock;
+        al[u] = q < end ? __builtin_nontemporal_load(A.val + q) : V(0);
+        cl[u] = q < end ? __builtin_nontemporal_load(A.idx + q) : 0;
+      }
+      Acc gv[NV][kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) gv[v][u] = p + u * kBlock < end ? op.gather(v, cl[u]) : Acc(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s[v] += static_cast<Acc>(al[u]) * gv[v][u];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const Acc t = group_sum<kWave>(s[v]);
+      if ((tid & (kWave - 1)) == 0) wsum[v][tid / kWave] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        rowsum[v][0] = ((wsum[v][0] + wsum[v][1]) + wsum[v][2]) + wsum[v][3];
+    }
   }
-  if constexpr (Op::kSums > 0) block_partials<Op::kSums>(sums, op.partial);
+  __syncthreads();
+  if (tid < nrows) {
+    Acc acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = rowsum[v][tid];
+    op.row(r0 + tid, inst, acc, pre, sums);
+  }
+  if constexpr (Op::kSums > 0) block_partials<Op::kSums>(sums, op.partial + b, op.stride);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -261,9 +223,9 @@ struct SpmvOp {
   static constexpr int kSums = 0;
   const X* __restrict__ x;
   X* __restrict__ y;
-  __device__ Pre pre(int) const { return {}; }
+  __device__ Pre pre(int, int) const { return {}; }
   __device__ Acc gather(int, int c) const { return x[c]; }
-  __device__ void row(int r, const Acc* acc, const Pre&, double*) const { y[r] = acc[0]; }
+  __device__ void row(int r, int, const Acc* acc, const Pre&, double*) const { y[r] = acc[0]; }
 };
 
 template <typename T>
@@ -274,6 +236,7 @@ struct DualPre {
 // Rows of A: the dual update of row i and the primal update of its logical variable n + i.
 //   y_new_i = y_i + sigma (A vbar_x - vbar_{n+i})_i,  y_erg_i += alpha (y_new_i - y_erg_i)
 //   v_new_{n+i} = clamp(v_{n+i} - tau (cost_{n+i} - y_new_i)),  its average and next vbar.
+// (n: the number of structural variables of the whole stack; the logical variables follow them.)
 template <typename T>
 struct DualRowsOp {
   using Acc = T;
@@ -294,16 +257,17 @@ struct DualRowsOp {
   const T* __restrict__ cost;
   const T* __restrict__ lower;
   const T* __restrict__ upper;
-  __device__ Pre pre(int i) const {
+  __device__ Pre pre(int i, int) const {
     const int k = n + i;
     return Pre{y[i], y_erg[i], vbar[k], v[k], v_erg[k], cost[k], lower[k], upper[k]};
   }
   __device__ Acc gather(int, int c) const { return vbar[c]; }
-  __device__ void row(int i, const Acc* acc, const Pre& p, double*) const {
-    const T sigma = static_cast<T>(sc->sigma);
-    const T tau = static_cast<T>(sc->tau);
-    const T theta = static_cast<T>(sc->theta);
-    const T alpha = static_cast<T>(1.0 / static_cast<double>(sc->k0 + j + 1));
+  __device__ void row(int i, int inst, const Acc* acc, const Pre& p, double*) const {
+    const StepScalars& s = sc[inst];
+    const T sigma = static_cast<T>(s.sigma);
+    const T tau = static_cast<T>(s.tau);
+    const T theta = static_cast<T>(s.theta);
+    const T alpha = static_cast<T>(1.0 / static_cast<double>(s.k0 + j + 1));
     const int k = n + i;
     const T yn = p.y + sigma * (acc[0] - p.vbar);
     y_new[i] = yn;
@@ -339,14 +303,15 @@ struct PrimalColsOp {
   const T* __restrict__ cost;
   const T* __restrict__ lower;
   const T* __restrict__ upper;
-  __device__ Pre pre(int col) const {
+  __device__ Pre pre(int col, int) const {
     return Pre{v[col], v_erg[col], cost[col], lower[col], upper[col]};
   }
   __device__ Acc gather(int, int c) const { return y_new[c]; }
-  __device__ void row(int col, const Acc* acc, const Pre& p, double*) const {
-    const T tau = static_cast<T>(sc->tau);
-    const T theta = static_cast<T>(sc->theta);
-    const T alpha = static_cast<T>(1.0 / static_cast<double>(sc->k0 + j + 1));
+  __device__ void row(int col, int inst, const Acc* acc, const Pre& p, double*) const {
+    const StepScalars& s = sc[inst];
+    const T tau = static_cast<T>(s.tau);
+    const T theta = static_cast<T>(s.theta);
+    const T alpha = static_cast<T>(1.0 / static_cast<double>(s.k0 + j + 1));
     const T grad = p.cost + acc[0];
     const T vn = clamp_to(p.v - tau * grad, p.lower, p.upper);
     v_new[col] = vn;
@@ -373,7 +338,8 @@ struct MeasurePre {
 // Evaluation sums (7 per iterate, in the model space before equilibration): 0 ||A x - r||^2,
 // 1 dual residual^2, 2 primal objective, 3 dual objective, 4 ||x||^2, 5 ||r||^2, 6 ||y||^2.
 // x = cs v_x, r = v_r / rs, y_o = rs y; A x = (A_eq v_x) / rs and A' y_o = (A_eq' y) / cs.
-// Rows of A carry the logical variables' terms.
+// Rows of A carry the logical variables' terms. Block b leaves its totals at
+// partial[q * stride + b].
 template <typename T, int NV>
 struct MeasureRowsOp {
   using Acc = double;
@@ -388,7 +354,8 @@ struct MeasureRowsOp {
   const double* __restrict__ lower;
   const double* __restrict__ upper;
   double* __restrict__ partial;
-  __device__ Pre pre(int i) const {
+  int stride;
+  __device__ Pre pre(int i, int) const {
     const int k = n + i;
     Pre p;
     p.scale = rs[i];
@@ -403,7 +370,7 @@ struct MeasureRowsOp {
     return p;
   }
   __device__ Acc gather(int v, int c) const { return static_cast<double>(vv[v][c]); }
-  __device__ void row(int, const Acc* acc, const Pre& p, double* s) const {
+  __device__ void row(int, int, const Acc* acc, const Pre& p, double* s) const {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double value = p.v[v] / p.scale;
@@ -429,7 +396,6 @@ struct MeasureColsOp {
   using Pre = MeasurePre<NV>;
   static constexpr int kVec = NV;
   static constexpr int kSums = 7 * NV;
-  int m;
   const T* __restrict__ vv[NV];
   const T* __restrict__ yy[NV];
   const double* __restrict__ cs;
@@ -437,7 +403,8 @@ struct MeasureColsOp {
   const double* __restrict__ lower;
   const double* __restrict__ upper;
   double* __restrict__ partial;
-  __device__ Pre pre(int col) const {
+  int stride;
+  __device__ Pre pre(int col, int) const {
     Pre p;
     p.scale = cs[col];
     p.cost = cost[col];
@@ -451,11 +418,11 @@ struct MeasureColsOp {
     return p;
   }
   __device__ Acc gather(int v, int c) const { return static_cast<double>(yy[v][c]); }
-  __device__ void row(int, const Acc* acc, const Pre& p, double* s) const {
+  __device__ void row(int, int, const Acc* acc, const Pre& p, double* s) const {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double value = p.v[v] * p.scale;
-      const double g = p.cost + (m > 0 ? acc[v] / p.scale : 0.0);
+      const double g = p.cost + acc[v] / p.scale;
       s[7 * v + 4] += value * value;
       s[7 * v + 2] += p.cost * value;
       const double lam = absorbable(g, p.lower, p.upper);
@@ -467,17 +434,155 @@ struct MeasureColsOp {
 };
 
 // ------------------------------------------------------------------------------------------
-// Elementwise and reduction kernels outside the products.
+// Row-wise kernels over both schedules (no products): workgroup w < nba takes block w of the A
+// schedule (rows i: y_i and the logical variable v_{n+i}), w >= nba takes block w - nba of the A'
+// schedule (columns j: v_j). Every entry of every vector belongs to exactly one block, hence to
+// one instance, whose flag says whether the workgroup works at all.
 // ------------------------------------------------------------------------------------------
 
-__global__ void set_scalars_kernel(StepScalars* sc, double tau, double sigma, double theta,
-                                   long long k0) {
-  if (threadIdx.x == 0) {
-    sc->tau = tau;
-    sc->sigma = sigma;
-    sc->theta = theta;
-    sc->k0 = k0;
+struct BlockRows {
+  const int2* __restrict__ meta_a;
+  const int* __restrict__ binst_a;
+  int nba;
+  const int2* __restrict__ meta_t;
+  const int* __restrict__ binst_t;
+  int n;  // structural variables of the stack: v_{n+i} is row i's logical variable
+};
+
+template <typename Op>
+__global__ __launch_bounds__(kBlock) void block_rows_kernel(BlockRows R, Op op) {
+  constexpr int NS = Op::kSums > 0 ? Op::kSums : 1;
+  const int w = blockIdx.x;
+  const bool is_a = w < R.nba;
+  const int b = is_a ? w : w - R.nba;
+  const int2* meta = is_a ? R.meta_a : R.meta_t;
+  const int inst = (is_a ? R.binst_a : R.binst_t)[b];
+  const int r0 = meta[b].x;
+  const int nrows = meta[b + 1].x - r0;
+  const int tid = threadIdx.x;
+  double local[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) local[q] = 0.0;
+  // Blocks of instances without a flag leave their partials untouched: launches for the two
+  // precisions cover disjoint instances and must not overwrite each other's totals.
+  const int flag = op.flag(inst);  // uniform over the workgroup
+  if (flag < 0) return;
+  if (tid < nrows) {
+    if (is_a)
+      op.row_a(r0 + tid, R.n + r0 + tid, flag, local);
+    else
+      op.col(r0 + tid, flag, local);
   }
+  if constexpr (Op::kSums > 0) block_partials<Op::kSums>(local, op.partial + w, op.stride);
+}
+
+// Squared distances of the candidate (current or average, by the instance's flag) from the
+// restart point: 0: ||v - v_rst||^2, 1: ||y - y_rst||^2.
+template <typename T>
+struct DistanceOp {
+  static constexpr int kSums = 2;
+  const int* __restrict__ which;  // per instance: -1 none, 0 current (v_new, y_new), 1 average
+  const T* __restrict__ v_new;
+  const T* __restrict__ y_new;
+  const T* __restrict__ v_erg;
+  const T* __restrict__ y_erg;
+  const T* __restrict__ v_rst;
+  const T* __restrict__ y_rst;
+  double* __restrict__ partial;
+  int stride;
+  __device__ int flag(int inst) const { return which[inst]; }
+  __device__ void row_a(int i, int k, int f, double* s) const {
+    const double dv = static_cast<double>(f == 0 ? v_new[k] : v_erg[k]) - static_cast<double>(v_rst[k]);
+    const double dy = static_cast<double>(f == 0 ? y_new[i] : y_erg[i]) - static_cast<double>(y_rst[i]);
+    s[0] += dv * dv;
+    s[1] += dy * dy;
+  }
+  __device__ void col(int k, int f, double* s) const {
+    const double dv = static_cast<double>(f == 0 ? v_new[k] : v_erg[k]) - static_cast<double>(v_rst[k]);
+    s[0] += dv * dv;
+  }
+};
+
+// A restart at the candidate (current or average): it becomes the restart point, the current and
+// next iterate, and both extrapolation points (so the next step starts without momentum). The
+// candidate may be v_new / y_new itself: each entry is read before it is written, by one thread.
+template <typename T>
+struct RestartOp {
+  static constexpr int kSums = 0;
+  const int* __restrict__ which;
+  T* v;
+  T* v_new;
+  T* y;
+  T* y_new;
+  const T* v_erg;
+  const T* y_erg;
+  T* __restrict__ v_rst;
+  T* __restrict__ y_rst;
+  T* __restrict__ vbar0;
+  T* __restrict__ vbar1;
+  __device__ int flag(int inst) const { return which[inst]; }
+  __device__ void set_v(int k, int f) const {
+    const T c = f == 0 ? v_new[k] : v_erg[k];
+    v_rst[k] = c;
+    v[k] = c;
+    v_new[k] = c;
+    vbar0[k] = c;
+    vbar1[k] = c;
+  }
+  __device__ void row_a(int i, int k, int f, double*) const {
+    set_v(k, f);
+    const T c = f == 0 ? y_new[i] : y_erg[i];
+    y_rst[i] = c;
+    y[i] = c;
+    y_new[i] = c;
+  }
+  __device__ void col(int k, int f, double*) const { set_v(k, f); }
+};
+
+// The iterate state of the flagged instances from one precision to the other.
+template <typename From, typename To>
+struct ConvertOp {
+  static constexpr int kSums = 0;
+  const int* __restrict__ which;
+  const From* vin[6];  // v[2], vbar[2], v_erg, v_rst
+  To* vout[6];
+  const From* yin[4];  // y[2], y_erg, y_rst
+  To* yout[4];
+  __device__ int flag(int inst) const { return which[inst]; }
+  __device__ void col(int k, int, double*) const {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) vout[q][k] = static_cast<To>(vin[q][k]);
+  }
+  __device__ void row_a(int i, int k, int f, double* s) const {
+    col(k, f, s);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) yout[q][i] = static_cast<To>(yin[q][i]);
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// Reductions and elementwise kernels.
+// ------------------------------------------------------------------------------------------
+
+// Per-instance totals of block partials, in a fixed order. Workgroup (q, i * parts + part) sums
+// partial[q * stride + b] over b in ranges[i * parts + part] = [first, last) and writes
+// out[(i * parts + part) * nsums + q].
+__global__ __launch_bounds__(kBlock) void finish_ranges_kernel(int stride, int nsums,
+                                                               const double* __restrict__ partial,
+                                                               const int2* __restrict__ ranges,
+                                                               double* __restrict__ out) {
+  __shared__ double wsum[kBlock / kWave];
+  const int tid = threadIdx.x;
+  const int q = blockIdx.x;
+  const int seg = blockIdx.y;
+  const int2 r = ranges[seg];
+  const double* row = partial + static_cast<size_t>(q) * stride;
+  double acc = 0.0;
+  for (int b = r.x + tid; b < r.y; b += kBlock) acc += row[b];
+  acc = group_sum<kWave>(acc);
+  if ((tid & (kWave - 1)) == 0) wsum[tid / kWave] = acc;
+  __syncthreads();
+  if (tid == 0) out[static_cast<size_t>(seg) * nsums + q] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
 // vbar = v + theta (v - v_prev): only when theta changed since vbar was formed.
@@ -486,67 +591,6 @@ __global__ void extrapolate_kernel(int nt, const T* __restrict__ v, const T* __r
                                    T theta, T* __restrict__ vbar) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nt) vbar[k] = v[k] + theta * (v[k] - v_prev[k]);
-}
-
-// Squared distances ||v - v0||^2 (nt entries) and ||y - y0||^2 (m entries), block partials.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void distance_kernel(int nt, int m, const T* __restrict__ v,
-                                                          const T* __restrict__ v0,
-                                                          const T* __restrict__ y,
-                                                          const T* __restrict__ y0,
-                                                          double* __restrict__ partial) {
-  double local[2] = {0.0, 0.0};
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nt; k += gridDim.x * blockDim.x) {
-    const double d = static_cast<double>(v[k]) - static_cast<double>(v0[k]);
-    local[0] += d * d;
-    if (k < m) {
-      const double e = static_cast<double>(y[k]) - static_cast<double>(y0[k]);
-      local[1] += e * e;
-    }
-  }
-  block_partials<2>(local, partial);
-}
-
-// out[q] = sum over blocks of partial[q * nblocks + b], in a fixed order: workgroup q sums row q
-// (launch with one workgroup per sum).
-__global__ __launch_bounds__(kBlock) void finish_sums_kernel(int nblocks,
-                                                             const double* __restrict__ partial,
-                                                             double* __restrict__ out) {
-  __shared__ double wsum[kBlock / kWave];
-  const int tid = threadIdx.x;
-  const int q = blockIdx.x;
-  double acc = 0.0;
-  for (int b = tid; b < nblocks; b += kBlock) acc += partial[static_cast<size_t>(q) * nblocks + b];
-  acc = group_sum<kWave>(acc);
-  if ((tid & (kWave - 1)) == 0) wsum[tid / kWave] = acc;
-  __syncthreads();
-  if (tid == 0) out[q] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
-}
-
-// A restart at an iterate (cv, cy): it becomes the restart point, the current and next iterate,
-// and both extrapolation points (so the next step starts without momentum).
-template <typename T>
-__global__ void restart_to_kernel(int nt, int m, const T* __restrict__ cv,
-                                   const T* __restrict__ cy, T* __restrict__ v_rst,
-                                   T* __restrict__ v, T* __restrict__ v_new,
-                                   T* __restrict__ vbar0, T* __restrict__ vbar1,
-                                   T* __restrict__ y_rst, T* __restrict__ y,
-                                   T* __restrict__ y_new) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nt) {
-    const T c = cv[k];
-    v_rst[k] = c;
-    v[k] = c;
-    v_new[k] = c;
-    vbar0[k] = c;
-    vbar1[k] = c;
-  }
-  if (k < m) {
-    const T c = cy[k];
-    y_rst[k] = c;
-    y[k] = c;
-    y_new[k] = c;
-  }
 }
 
 template <typename T, typename U>

@@ -1,19 +1,29 @@
 // The PDHG device layer for gfx950 behind the C ABI of include/samaya_pdhg.h.
 //
-// Device state of one LP (all resident in HBM for the whole solve):
-//   A by rows (CSR: ptr int32[m+1], idx int32[nnz], val) = the CSC of A' read as rows, and
-//   A' by rows (ptr int32[n+1], ...) = the CSC of A read as rows; FP64 values always (the
-//   evaluation sums and the FP64 steps), FP32 values too when FP32 steps are asked for; each
-//   with its row-block schedule ((first row, first nonzero) int32 pairs, nblocks + 1).
+// A handle holds a batch of K LPs (K = 1 for the single-LP interface), stacked block-diagonally
+// into one LP: the structural variables of all instances first (instance i's at x offset ox_i),
+// then their logical variables / rows (instance i's rows at y offset oy_i, their logical variables
+// at n + oy_i, n the total structural count). Device state (all resident in HBM for the solve):
+//   A by rows (CSR: ptr int32[m+1], idx int32[nnz], val) = the CSC of each A' read as rows, and
+//   A' by rows (ptr int32[n+1], ...) = the CSC of each A read as rows, with global indices; FP64
+//   values always (the evaluation sums and the FP64 steps), FP32 values too once any instance
+//   steps in FP32; each with its row-block schedule ((first row, first nonzero) int32 pairs,
+//   nblocks + 1), the instance of every block, and per precision the list of blocks whose
+//   instances step in it. No block holds rows of two instances.
 //   Iterates in the step precision T: v[2] and y[2] (current / next, swapped by a parity bit
 //   instead of pointers), vbar[2] (the extrapolated point of this and of the next step), the
 //   running averages v_erg, y_erg, the restart point v_rst, y_rst, and cost/lower/upper.
-//   The model-space data of evaluate (scales, cost and bounds before equilibration), FP64.
+//   Per instance: step scalars (tau, sigma, theta, steps since restart) and the precision of its
+//   steps (or frozen: no further steps), so instances restart, change precision and stop on their
+//   own schedule. The model-space data of evaluate (scales, cost and bounds before
+//   equilibration), FP64.
 //
-// One step is two launches of csr_rows_kernel: over the rows of A (dual update, fused with the
-// primal update of the row's logical variable) and over the rows of A' (primal update of the
-// structural variables). Each also forms the next step's extrapolated point, so the reference's
-// separate extrapolate / dual / primal / average kernels and the ax / aty vectors disappear.
+// One step is two launches of csr_rows_kernel per precision in use: over the rows of A (dual
+// update, fused with the primal update of the row's logical variable) and over the rows of A'
+// (primal update of the structural variables). Each also forms the next step's extrapolated
+// point, so the reference's separate extrapolate / dual / primal / average kernels and the ax /
+// aty vectors disappear. An instance's arithmetic does not depend on the rest of its batch: a
+// batch of K gives each LP the bits it gets alone.
 
 #include <hip/hip_runtime.h>
 
@@ -61,6 +71,9 @@ bool device_is_gfx950(int device) {
 
 int blocks_for(long long count) { return static_cast<int>((count + kBlock - 1) / kBlock); }
 
+constexpr int kFrozen = -1;
+constexpr int kMaxBatch = 16384;  // instances per handle (grid y of the per-instance reductions)
+
 struct DevMatrix {
   int rows = 0;
   long long nnz = 0;
@@ -68,8 +81,11 @@ struct DevMatrix {
   int* idx = nullptr;
   double* val64 = nullptr;
   float* val32 = nullptr;
-  int2* meta = nullptr;  // row-block schedule: nblocks + 1 (first row, first nonzero)
+  int2* meta = nullptr;   // row-block schedule: nblocks + 1 (first row, first nonzero)
+  int* binst = nullptr;   // instance of each block
   int nblocks = 0;
+  int* list[2] = {nullptr, nullptr};  // blocks of the instances stepping in FP64 / FP32
+  int nlist[2] = {0, 0};
 };
 
 template <typename T>
@@ -87,18 +103,26 @@ struct Iter {
   bool ready = false;
 };
 
-// One workgroup per row block: the hardware dispatcher balances blocks of uneven cost (the long
-// rows of a power-law column-count matrix) better than a static persistent assignment, which
-// measured 10-15% slower on such matrices; the kernel's next-block prefetch then never triggers.
-template <typename V, typename Op>
-int rows_grid(const DevMatrix& M) {
-  return M.nblocks;
+struct Layout {
+  int n = 0, m = 0;
+  int ox = 0, oy = 0;    // first structural variable and first row in the stack
+  int ba0 = 0, ba1 = 0;  // its blocks in the A schedule
+  int bt0 = 0, bt1 = 0;  // its blocks in the A' schedule
+  int prec = SMY_FP64;   // precision of its steps, or kFrozen
+  int last_prec = SMY_FP64;  // precision of its state (also when frozen)
+  bool has_original = false;
+};
+
+template <typename T>
+constexpr int prec_of() {
+  return std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
 }
 
-// Returns the grid used (0: no rows, nothing launched).
+// Returns the grid used (0: nothing launched).
 template <typename V, typename Op>
-int launch_rows(const DevMatrix& M, const V* val, const Op& op, hipStream_t s) {
-  if (M.nblocks == 0) return 0;
+int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, const Op& op,
+                hipStream_t s) {
+  if (nlist == 0) return 0;
   CsrRows<V> A;
   A.rows = M.rows;
   A.nblocks = M.nblocks;
@@ -106,9 +130,10 @@ int launch_rows(const DevMatrix& M, const V* val, const Op& op, hipStream_t s) {
   A.idx = M.idx;
   A.val = val;
   A.meta = M.meta;
-  const int grid = rows_grid<V, Op>(M);
-  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(grid), dim3(kBlock), 0, s, A, op);
-  return grid;
+  A.binst = M.binst;
+  A.list = list;
+  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(nlist), dim3(kBlock), 0, s, A, op);
+  return nlist;
 }
 
 }  // namespace
@@ -117,8 +142,9 @@ int launch_rows(const DevMatrix& M, const V* val, const Op& op, hipStream_t s) {
 using namespace smy;
 
 struct smy_pdhg {
+  int K = 1;
   int n = 0, m = 0, nt = 0;
-  int precision = SMY_FP64;
+  std::vector<Layout> inst;
   DevMatrix a, at;  // A by rows, A' by rows
   Iter<double> d;
   Iter<float> f;
@@ -126,23 +152,35 @@ struct smy_pdhg {
   double theta_vbar = 1.0;   // theta the current vbar was formed with
   bool vbar_any = true;      // vbar == v (valid for every theta): start, restart
   bool stepped = false;      // a step since the last advance / restart_to
-  bool has_original = false;
   double* cs = nullptr;
   double* rs = nullptr;
   double* cost_o = nullptr;
   double* lower_o = nullptr;
   double* upper_o = nullptr;
-  double* partial = nullptr;  // 14 x (a.nblocks + at.nblocks) or 2 x 1024
-  double* sums_dev = nullptr;
+  double* partial = nullptr;  // 14 x (a.nblocks + at.nblocks): block b of A', column a.nblocks + b
+  int2* ranges = nullptr;     // per instance: its A blocks, then its A' blocks (partial columns)
+  double* sums_dev = nullptr;   // K x 2 x 14
   double* sums_host = nullptr;  // pinned
   StepScalars* scalars = nullptr;
+  StepScalars* scalars_host = nullptr;  // pinned staging
+  int* flags = nullptr;                 // 2 x K per-instance flags (by precision)
+  int* flags_host = nullptr;            // pinned staging
+  hipEvent_t ev_staged = nullptr;       // the last copy out of the pinned staging buffers
+  bool staged = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   bool timed = false;
-  // Captured runs of kGraphSteps steps, by (precision, parity).
+  // Captured runs of kGraphSteps steps, by (precisions in use, parity); valid for one generation
+  // of the block lists.
   static constexpr int kGraphSteps = 64;
-  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-  hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  struct Captured {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int gen = -1;
+  };
+  Captured graphs[4][2];
+  int list_gen = 0;
+  bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
   std::vector<void*> allocations;
 
   template <typename T>
@@ -164,17 +202,21 @@ struct smy_pdhg {
     SMY_HIP(hipMemsetAsync(dev, 0, static_cast<size_t>(std::max<long long>(count, 1)) * sizeof(T), stream));
     return dev;
   }
+  void drop_graph(Captured& c) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c = Captured{};
+  }
 
   ~smy_pdhg() {
     if (stream) (void)hipStreamSynchronize(stream);
-    for (auto& row : graph_exec)
-      for (auto& g : row)
-        if (g) (void)hipGraphExecDestroy(g);
-    for (auto& row : graph)
-      for (auto& g : row)
-        if (g) (void)hipGraphDestroy(g);
+    for (auto& row : graphs)
+      for (auto& c : row) drop_graph(c);
     for (void* ptr : allocations) (void)hipFree(ptr);
     if (sums_host) (void)hipHostFree(sums_host);
+    if (scalars_host) (void)hipHostFree(scalars_host);
+    if (flags_host) (void)hipHostFree(flags_host);
+    if (ev_staged) (void)hipEventDestroy(ev_staged);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
     if (stream) (void)hipStreamDestroy(stream);
@@ -184,26 +226,104 @@ struct smy_pdhg {
 namespace smy {
 namespace {
 
-// CSR view of a CSC matrix read as the rows of its transpose (no data movement on the host
-// beyond narrowing the offsets to 32 bits).
-void upload_rows(smy_pdhg* h, const smy_csc* csc, DevMatrix& M, bool fp32) {
-  M.rows = csc->cols;
-  M.nnz = csc->start[csc->cols];
-  std::vector<int> ptr(static_cast<size_t>(csc->cols) + 1);
-  for (int r = 0; r <= csc->cols; ++r) ptr[r] = static_cast<int>(csc->start[r]);
-  const std::vector<RowBlockMeta> meta = build_row_blocks(ptr.data(), M.rows);
+// The pinned staging buffers may be rewritten once the copy that last read them has run.
+void staging_wait(smy_pdhg* h) {
+  if (h->staged) {
+    SMY_HIP(hipEventSynchronize(h->ev_staged));
+    h->staged = false;
+  }
+}
+
+void staging_sent(smy_pdhg* h) {
+  SMY_HIP(hipEventRecord(h->ev_staged, h->stream));
+  h->staged = true;
+}
+
+// The stacked matrix by rows from each instance's CSC of the transpose (`rows_of` = the CSC whose
+// columns are the rows), with the column indices moved by the instances' offsets, and its
+// row-block schedule cut per instance.
+void upload_stack(smy_pdhg* h, const smy_csc* parts, bool rows_are_a, DevMatrix& M) {
+  const int K = h->K;
+  long long nnz = 0;
+  int rows = 0;
+  for (int i = 0; i < K; ++i) {
+    nnz += parts[i].start[parts[i].cols];
+    rows += parts[i].cols;
+  }
+  M.rows = rows;
+  M.nnz = nnz;
+  std::vector<int> ptr(static_cast<size_t>(rows) + 1);
+  std::vector<int> idx(static_cast<size_t>(nnz));
+  std::vector<double> val(static_cast<size_t>(nnz));
+  std::vector<RowBlockMeta> meta;
+  std::vector<int> binst;
+  int r = 0;
+  long long z = 0;
+  for (int i = 0; i < K; ++i) {
+    const smy_csc& c = parts[i];
+    Layout& L = h->inst[i];
+    const int col_shift = rows_are_a ? L.ox : L.oy;
+    const int row_first = r;
+    std::vector<int> local(static_cast<size_t>(c.cols) + 1);
+    for (int q = 0; q <= c.cols; ++q) local[q] = static_cast<int>(c.start[q]);
+    for (int q = 0; q < c.cols; ++q) ptr[r + q] = static_cast<int>(z) + local[q];
+    const long long cnt = c.start[c.cols];
+    for (long long e = 0; e < cnt; ++e) {
+      idx[z + e] = c.index[e] + col_shift;
+      val[z + e] = c.value[e];
+    }
+    std::vector<RowBlockMeta> bl = build_row_blocks(local.data(), c.cols);
+    const int first_block = static_cast<int>(meta.size());
+    for (size_t q = 0; q + 1 < bl.size(); ++q) {
+      meta.push_back({bl[q].row + row_first, bl[q].nz + static_cast<int>(z)});
+      binst.push_back(i);
+    }
+    if (rows_are_a) {
+      L.ba0 = first_block;
+      L.ba1 = static_cast<int>(meta.size());
+    } else {
+      L.bt0 = first_block;
+      L.bt1 = static_cast<int>(meta.size());
+    }
+    r += c.cols;
+    z += cnt;
+  }
+  ptr[rows] = static_cast<int>(nnz);
+  meta.push_back({rows, static_cast<int>(nnz)});
   M.nblocks = static_cast<int>(meta.size()) - 1;
   M.ptr = h->upload(ptr.data(), static_cast<long long>(ptr.size()));
-  M.idx = h->upload(csc->index, M.nnz);
-  M.val64 = h->upload(csc->value, M.nnz);
+  M.idx = h->upload(idx.data(), nnz);
+  M.val64 = h->upload(val.data(), nnz);
   static_assert(sizeof(RowBlockMeta) == sizeof(int2), "schedule entries are read as int2");
   M.meta = h->upload(reinterpret_cast<const int2*>(meta.data()), static_cast<long long>(meta.size()));
-  if (fp32) {
-    M.val32 = h->alloc<float>(M.nnz);
-    if (M.nnz > 0)
-      hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(M.nnz)), dim3(kBlock), 0,
-                         h->stream, M.nnz, M.val64, M.val32);
+  M.binst = h->upload(binst.data(), static_cast<long long>(binst.size()));
+  M.list[0] = h->alloc<int>(M.nblocks);
+  M.list[1] = h->alloc<int>(M.nblocks);
+}
+
+// The block lists of both precisions from the instances' current precisions.
+void rebuild_lists(smy_pdhg* h) {
+  SMY_HIP(hipStreamSynchronize(h->stream));
+  for (DevMatrix* M : {&h->a, &h->at}) {
+    const bool is_a = M == &h->a;
+    for (int q = 0; q < 2; ++q) {
+      std::vector<int> list;
+      for (const Layout& L : h->inst) {
+        if (L.prec != q) continue;
+        const int b0 = is_a ? L.ba0 : L.bt0, b1 = is_a ? L.ba1 : L.bt1;
+        for (int b = b0; b < b1; ++b) list.push_back(b);
+      }
+      M->nlist[q] = static_cast<int>(list.size());
+      if (!list.empty())
+        SMY_HIP(hipMemcpy(M->list[q], list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
   }
+  ++h->list_gen;
+  h->lists_dirty = false;
+}
+
+void refresh_lists(smy_pdhg* h) {
+  if (h->lists_dirty) rebuild_lists(h);
 }
 
 template <typename T>
@@ -221,8 +341,7 @@ template <>
 const float* avals<float>(const DevMatrix& M) { return M.val32; }
 
 template <typename T>
-void alloc_iter(smy_pdhg* h, Iter<T>& S, const double* cost, const double* lower,
-                const double* upper) {
+void alloc_iter(smy_pdhg* h, Iter<T>& S) {
   const long long nt = h->nt, m = h->m;
   for (int q = 0; q < 2; ++q) {
     S.v[q] = h->zeros<T>(nt);
@@ -236,25 +355,44 @@ void alloc_iter(smy_pdhg* h, Iter<T>& S, const double* cost, const double* lower
   S.cost = h->alloc<T>(nt);
   S.lower = h->alloc<T>(nt);
   S.upper = h->alloc<T>(nt);
-  if (cost != nullptr) {
-    if constexpr (std::is_same_v<T, double>) {
-      SMY_HIP(hipMemcpy(S.cost, cost, nt * sizeof(double), hipMemcpyHostToDevice));
-      SMY_HIP(hipMemcpy(S.lower, lower, nt * sizeof(double), hipMemcpyHostToDevice));
-      SMY_HIP(hipMemcpy(S.upper, upper, nt * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-      std::vector<float> tmp(static_cast<size_t>(nt));
-      for (const auto& [src, dst] : {std::pair{cost, S.cost}, std::pair{lower, S.lower},
-                                     std::pair{upper, S.upper}}) {
-        for (long long k = 0; k < nt; ++k) tmp[k] = static_cast<float>(src[k]);
-        if (nt > 0) SMY_HIP(hipMemcpy(dst, tmp.data(), nt * sizeof(float), hipMemcpyHostToDevice));
-      }
-    }
-  }
   S.ready = true;
+}
+
+// FP32 values of both matrices, FP32 iterates, cost and bounds (from the FP64 ones).
+void ensure_fp32(smy_pdhg* h) {
+  if (h->f.ready) return;
+  for (DevMatrix* M : {&h->a, &h->at}) {
+    M->val32 = h->alloc<float>(M->nnz);
+    if (M->nnz > 0)
+      hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(M->nnz)), dim3(kBlock), 0,
+                         h->stream, M->nnz, M->val64, M->val32);
+  }
+  alloc_iter<float>(h, h->f);
+  for (auto [src, dst] : {std::pair{h->d.cost, h->f.cost}, std::pair{h->d.lower, h->f.lower},
+                          std::pair{h->d.upper, h->f.upper}})
+    if (h->nt > 0)
+      hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(h->nt)), dim3(kBlock), 0,
+                         h->stream, static_cast<long long>(h->nt), src, dst);
+  SMY_HIP(hipGetLastError());
+}
+
+BlockRows block_rows(const smy_pdhg* h) {
+  return BlockRows{h->a.meta, h->a.binst, h->a.nblocks, h->at.meta, h->at.binst, h->n};
+}
+
+int block_rows_grid(const smy_pdhg* h) { return h->a.nblocks + h->at.nblocks; }
+
+int stride(const smy_pdhg* h) { return h->a.nblocks + h->at.nblocks; }
+
+bool uses(const smy_pdhg* h, int q) {
+  for (const Layout& L : h->inst)
+    if (L.prec == q) return true;
+  return false;
 }
 
 template <typename T>
 void step_kernels(smy_pdhg* h, int j) {
+  constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
   const int p = h->p;
   DualRowsOp<T> dual;
@@ -272,7 +410,7 @@ void step_kernels(smy_pdhg* h, int j) {
   dual.cost = S.cost;
   dual.lower = S.lower;
   dual.upper = S.upper;
-  launch_rows(h->a, avals<T>(h->a), dual, h->stream);
+  launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], dual, h->stream);
   PrimalColsOp<T> primal;
   primal.j = j;
   primal.sc = h->scalars;
@@ -284,28 +422,47 @@ void step_kernels(smy_pdhg* h, int j) {
   primal.cost = S.cost;
   primal.lower = S.lower;
   primal.upper = S.upper;
-  launch_rows(h->at, avals<T>(h->at), primal, h->stream);
+  launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
+}
+
+void step_all(smy_pdhg* h, int j) {
+  if (uses(h, SMY_FP32)) step_kernels<float>(h, j);
+  if (uses(h, SMY_FP64)) step_kernels<double>(h, j);
 }
 
 // Re-forms vbar for a theta other than the one it was formed with (the reference recomputes it
 // from v and v_prev at every step; here only a change of theta needs it).
-template <typename T>
 int refresh_vbar(smy_pdhg* h, double theta) {
   if (h->vbar_any || theta == h->theta_vbar) return SMY_OK;
   if (h->stepped) return fail(SMY_STATE, "theta changed between two steps without advance()");
-  Iter<T>& S = state<T>(h);
-  hipLaunchKernelGGL((extrapolate_kernel<T>), dim3(blocks_for(h->nt)), dim3(kBlock), 0, h->stream,
-                     h->nt, S.v[h->p], S.v[1 - h->p], static_cast<T>(theta), S.vbar[h->p]);
+  const auto go = [&](auto& S) {
+    using T = std::remove_pointer_t<decltype(S.cost)>;
+    hipLaunchKernelGGL((extrapolate_kernel<T>), dim3(blocks_for(h->nt)), dim3(kBlock), 0, h->stream,
+                       h->nt, S.v[h->p], S.v[1 - h->p], static_cast<T>(theta), S.vbar[h->p]);
+  };
+  if (h->nt > 0) {
+    if (h->d.ready) go(h->d);
+    if (h->f.ready) go(h->f);
+  }
   return SMY_OK;
 }
 
-template <typename T>
-int step_impl(smy_pdhg* h, double tau, double sigma, double theta, long long iter) {
-  const int rc = refresh_vbar<T>(h, theta);
+void set_scalars(smy_pdhg* h, const double* tau, const double* sigma, double theta,
+                 const long long* k0) {
+  staging_wait(h);
+  for (int i = 0; i < h->K; ++i) h->scalars_host[i] = StepScalars{tau[i], sigma[i], theta, k0[i]};
+  SMY_HIP(hipMemcpyAsync(h->scalars, h->scalars_host, h->K * sizeof(StepScalars),
+                         hipMemcpyHostToDevice, h->stream));
+  staging_sent(h);
+}
+
+int step_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
+              const long long* k0) {
+  refresh_lists(h);
+  const int rc = refresh_vbar(h, theta);
   if (rc != SMY_OK) return rc;
-  hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, h->stream, h->scalars, tau, sigma,
-                     theta, iter);
-  step_kernels<T>(h, 0);
+  set_scalars(h, tau, sigma, theta, k0);
+  step_all(h, 0);
   SMY_HIP(hipGetLastError());
   h->theta_vbar = theta;
   h->vbar_any = false;
@@ -313,15 +470,59 @@ int step_impl(smy_pdhg* h, double tau, double sigma, double theta, long long ite
   return SMY_OK;
 }
 
-// Average duration of each of the two step kernels over `reps` back-to-back launches (HIP events
-// on the handle's stream). The launches repeat one step from the current state, so the iterates
-// and averages change: a measurement helper, not part of a solve.
+int run_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
+             const long long* k0, int count) {
+  if (count <= 0) return SMY_OK;
+  refresh_lists(h);
+  const int rc = refresh_vbar(h, theta);
+  if (rc != SMY_OK) return rc;
+  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+  set_scalars(h, tau, sigma, theta, k0);
+  const int mask = (uses(h, SMY_FP64) ? 1 : 0) | (uses(h, SMY_FP32) ? 2 : 0);
+  if (count == smy_pdhg::kGraphSteps && mask != 0) {
+    const int p0 = h->p;
+    smy_pdhg::Captured& c = h->graphs[mask][p0];
+    if (c.exec == nullptr || c.gen != h->list_gen) {
+      h->drop_graph(c);
+      SMY_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      for (int j = 0; j < count; ++j) {
+        step_all(h, j);
+        if (j + 1 < count) h->p ^= 1;
+      }
+      SMY_HIP(hipStreamEndCapture(h->stream, &c.graph));
+      h->p = p0;
+      SMY_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+      c.gen = h->list_gen;
+    }
+    SMY_HIP(hipGraphLaunch(c.exec, h->stream));
+    if ((count - 1) & 1) h->p ^= 1;
+  } else {
+    for (int j = 0; j < count; ++j) {
+      step_all(h, j);
+      if (j + 1 < count) h->p ^= 1;
+    }
+  }
+  SMY_HIP(hipGetLastError());
+  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+  h->timed = true;
+  h->theta_vbar = theta;
+  h->vbar_any = false;
+  h->stepped = true;
+  return SMY_OK;
+}
+
+// Average duration of each of the two step kernels of the first precision in use (FP32 before
+// FP64) over `reps` back-to-back launches (HIP events on the handle's stream). The launches repeat
+// one step from the current state, so the iterates and averages change: a measurement helper,
+// not part of a solve.
 template <typename T>
 int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
+  constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
   const int p = h->p;
-  hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, h->stream, h->scalars, 0.5, 0.5,
-                     1.0, 0LL);
+  std::vector<double> tau(h->K, 0.5), sigma(h->K, 0.5);
+  std::vector<long long> k0(h->K, 0);
+  set_scalars(h, tau.data(), sigma.data(), 1.0, k0.data());
   DualRowsOp<T> dual;
   dual.n = h->n;
   dual.j = 0;
@@ -349,16 +550,20 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   primal.lower = S.lower;
   primal.upper = S.upper;
   float t = 0.0f;
-  launch_rows(h->a, avals<T>(h->a), dual, h->stream);  // warm
+  const auto dual_go = [&] { launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], dual, h->stream); };
+  const auto primal_go = [&] {
+    launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
+  };
+  dual_go();  // warm
   SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-  for (int r = 0; r < reps; ++r) launch_rows(h->a, avals<T>(h->a), dual, h->stream);
+  for (int r = 0; r < reps; ++r) dual_go();
   SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
   SMY_HIP(hipEventSynchronize(h->ev_stop));
   SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
   *dual_ms = static_cast<double>(t) / reps;
-  launch_rows(h->at, avals<T>(h->at), primal, h->stream);
+  primal_go();
   SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-  for (int r = 0; r < reps; ++r) launch_rows(h->at, avals<T>(h->at), primal, h->stream);
+  for (int r = 0; r < reps; ++r) primal_go();
   SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
   SMY_HIP(hipEventSynchronize(h->ev_stop));
   SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
@@ -368,52 +573,10 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   return SMY_OK;
 }
 
-template <typename T>
-int run_impl(smy_pdhg* h, double tau, double sigma, double theta, long long iter0, int count) {
-  if (count <= 0) return SMY_OK;
-  const int rc = refresh_vbar<T>(h, theta);
-  if (rc != SMY_OK) return rc;
-  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-  hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, h->stream, h->scalars, tau, sigma,
-                     theta, iter0);
-  const int prec = std::is_same_v<T, double> ? 0 : 1;
-  if (count == smy_pdhg::kGraphSteps) {
-    const int p0 = h->p;
-    hipGraphExec_t& exec = h->graph_exec[prec][p0];
-    if (exec == nullptr) {
-      hipGraph_t& g = h->graph[prec][p0];
-      SMY_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      for (int j = 0; j < count; ++j) {
-        step_kernels<T>(h, j);
-        if (j + 1 < count) h->p ^= 1;
-      }
-      SMY_HIP(hipStreamEndCapture(h->stream, &g));
-      h->p = p0;
-      SMY_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-    }
-    SMY_HIP(hipGraphLaunch(exec, h->stream));
-    if ((count - 1) & 1) h->p ^= 1;
-  } else {
-    for (int j = 0; j < count; ++j) {
-      step_kernels<T>(h, j);
-      if (j + 1 < count) h->p ^= 1;
-    }
-  }
-  SMY_HIP(hipGetLastError());
-  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
-  h->timed = true;
-  h->theta_vbar = theta;
-  h->vbar_any = false;
-  h->stepped = true;
-  return SMY_OK;
-}
-
-// Evaluation sums of NV iterates into h->sums_host (7 per iterate). Synchronizes.
+// Block partials of the evaluation sums of NV iterates of the instances stepping in precision T.
 template <typename T, int NV>
-void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
-  // Each persistent workgroup leaves one partial per sum: at most nblocks of them per matrix.
-  double* part1 = h->partial;
-  double* part2 = h->partial + static_cast<size_t>(7 * NV) * h->a.nblocks;
+void measure_launch(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
+  constexpr int q = prec_of<T>();
   MeasureRowsOp<T, NV> rows;
   rows.n = h->n;
   for (int v = 0; v < NV; ++v) {
@@ -424,10 +587,10 @@ void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
   rows.cost = h->cost_o;
   rows.lower = h->lower_o;
   rows.upper = h->upper_o;
-  rows.partial = part1;
-  const int nb1 = launch_rows(h->a, h->a.val64, rows, h->stream);
+  rows.partial = h->partial;
+  rows.stride = stride(h);
+  launch_rows(h->a, h->a.val64, h->a.list[q], h->a.nlist[q], rows, h->stream);
   MeasureColsOp<T, NV> cols;
-  cols.m = h->m;
   for (int v = 0; v < NV; ++v) {
     cols.vv[v] = vv[v];
     cols.yy[v] = yy[v];
@@ -436,84 +599,186 @@ void measure(smy_pdhg* h, const T* const (&vv)[NV], const T* const (&yy)[NV]) {
   cols.cost = h->cost_o;
   cols.lower = h->lower_o;
   cols.upper = h->upper_o;
-  cols.partial = part2;
-  const int nb2 = launch_rows(h->at, h->at.val64, cols, h->stream);
-  // Both partial arrays in one fixed-order pass: the rows of A first, then the rows of A'.
-  constexpr int K = 7 * NV;
-  hipLaunchKernelGGL(finish_sums_kernel, dim3(K), dim3(kBlock), 0, h->stream, nb1, part1,
-                     h->sums_dev);
-  hipLaunchKernelGGL(finish_sums_kernel, dim3(K), dim3(kBlock), 0, h->stream, nb2, part2,
-                     h->sums_dev + K);
+  cols.partial = h->partial + h->a.nblocks;
+  cols.stride = stride(h);
+  launch_rows(h->at, h->at.val64, h->at.list[q], h->at.nlist[q], cols, h->stream);
+}
+
+// Per-instance totals of `nsums` block partials into out[i * nsums + q]: the instance's A blocks
+// first, then its A' blocks, added on the host in that order. Synchronizes. Entries of frozen
+// instances are meaningless.
+void finish(smy_pdhg* h, int nsums, double* out) {
+  hipLaunchKernelGGL(finish_ranges_kernel, dim3(nsums, 2 * h->K), dim3(kBlock), 0, h->stream,
+                     stride(h), nsums, h->partial, h->ranges, h->sums_dev);
   SMY_HIP(hipGetLastError());
-  SMY_HIP(hipMemcpyAsync(h->sums_host, h->sums_dev, 2 * K * sizeof(double), hipMemcpyDeviceToHost,
-                         h->stream));
+  SMY_HIP(hipMemcpyAsync(h->sums_host, h->sums_dev, 2 * h->K * nsums * sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
   SMY_HIP(hipStreamSynchronize(h->stream));
-  for (int q = 0; q < K; ++q) h->sums_host[q] = h->sums_host[q] + h->sums_host[K + q];
+  for (int i = 0; i < h->K; ++i)
+    for (int q = 0; q < nsums; ++q)
+      out[i * nsums + q] = h->sums_host[(2 * i) * nsums + q] + h->sums_host[(2 * i + 1) * nsums + q];
 }
 
 template <typename T>
-int evaluate_impl(smy_pdhg* h, int which, double* out) {
-  if (!h->has_original) return fail(SMY_STATE, "set_original() must be called before evaluate()");
-  Iter<T>& S = state<T>(h);
-  const T* v = which == SMY_CURRENT ? S.v[1 - h->p] : S.v_erg;
-  const T* y = which == SMY_CURRENT ? S.y[1 - h->p] : S.y_erg;
-  const T* const vv[1] = {v};
-  const T* const yy[1] = {y};
-  measure<T, 1>(h, vv, yy);
-  std::memcpy(out, h->sums_host, 7 * sizeof(double));
-  return SMY_OK;
-}
-
-template <typename T>
-int evaluate_both_impl(smy_pdhg* h, double* out) {
-  if (!h->has_original) return fail(SMY_STATE, "set_original() must be called before evaluate()");
+void measure_both(smy_pdhg* h) {
   Iter<T>& S = state<T>(h);
   const T* const vv[2] = {S.v[1 - h->p], S.v_erg};
   const T* const yy[2] = {S.y[1 - h->p], S.y_erg};
-  measure<T, 2>(h, vv, yy);
-  std::memcpy(out, h->sums_host, 14 * sizeof(double));
-  return SMY_OK;
+  measure_launch<T, 2>(h, vv, yy);
 }
 
-template <typename T>
-int distance_impl(smy_pdhg* h, int which, double* dv, double* dy) {
-  Iter<T>& S = state<T>(h);
-  const T* v = which == SMY_CURRENT ? S.v[1 - h->p] : S.v_erg;
-  const T* y = which == SMY_CURRENT ? S.y[1 - h->p] : S.y_erg;
-  const int grid = std::max(1, std::min(1024, blocks_for(h->nt)));
-  hipLaunchKernelGGL((distance_kernel<T>), dim3(grid), dim3(kBlock), 0, h->stream, h->nt, h->m, v,
-                     S.v_rst, y, S.y_rst, h->partial);
-  hipLaunchKernelGGL(finish_sums_kernel, dim3(2), dim3(kBlock), 0, h->stream, grid, h->partial,
-                     h->sums_dev);
+int evaluate_batch_impl(smy_pdhg* h, double* out) {
+  for (const Layout& L : h->inst)
+    if (L.prec != kFrozen && !L.has_original)
+      return fail(SMY_STATE, "set_original() must be called before evaluate()");
+  refresh_lists(h);
+  if (uses(h, SMY_FP32)) measure_both<float>(h);
+  if (uses(h, SMY_FP64)) measure_both<double>(h);
   SMY_HIP(hipGetLastError());
-  SMY_HIP(hipMemcpyAsync(h->sums_host, h->sums_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                         h->stream));
-  SMY_HIP(hipStreamSynchronize(h->stream));
-  *dv = h->sums_host[0];
-  *dy = h->sums_host[1];
+  finish(h, 14, out);
   return SMY_OK;
 }
 
 template <typename T>
-int restart_to_impl(smy_pdhg* h, int which) {
+void measure_one(smy_pdhg* h, int which) {
+  Iter<T>& S = state<T>(h);
+  const T* const vv[1] = {which == SMY_CURRENT ? S.v[1 - h->p] : S.v_erg};
+  const T* const yy[1] = {which == SMY_CURRENT ? S.y[1 - h->p] : S.y_erg};
+  measure_launch<T, 1>(h, vv, yy);
+}
+
+// Stages per-precision instance flags: flags[q * K + i] = value[i] if instance i's state is in
+// precision q (and it is not frozen), else -1.
+void stage_flags(smy_pdhg* h, const int* value) {
+  staging_wait(h);
+  for (int q = 0; q < 2; ++q)
+    for (int i = 0; i < h->K; ++i)
+      h->flags_host[q * h->K + i] = h->inst[i].prec == q ? value[i] : -1;
+  SMY_HIP(hipMemcpyAsync(h->flags, h->flags_host, 2 * h->K * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+}
+
+bool any_flag(const smy_pdhg* h, const int* which, int q) {
+  for (int i = 0; i < h->K; ++i)
+    if (which[i] >= 0 && h->inst[i].prec == q) return true;
+  return false;
+}
+
+template <typename T>
+void distance_launch(smy_pdhg* h) {
+  constexpr int q = prec_of<T>();
+  Iter<T>& S = state<T>(h);
+  DistanceOp<T> op;
+  op.which = h->flags + q * h->K;
+  op.v_new = S.v[1 - h->p];
+  op.y_new = S.y[1 - h->p];
+  op.v_erg = S.v_erg;
+  op.y_erg = S.y_erg;
+  op.v_rst = S.v_rst;
+  op.y_rst = S.y_rst;
+  op.partial = h->partial;
+  op.stride = stride(h);
+  hipLaunchKernelGGL((block_rows_kernel<DistanceOp<T>>), dim3(block_rows_grid(h)), dim3(kBlock), 0,
+                     h->stream, block_rows(h), op);
+}
+
+int distance_batch_impl(smy_pdhg* h, const int* which, double* out) {
+  if (block_rows_grid(h) == 0) {
+    std::fill(out, out + 2 * h->K, 0.0);
+    return SMY_OK;
+  }
+  stage_flags(h, which);
+  if (any_flag(h, which, SMY_FP32)) distance_launch<float>(h);
+  if (any_flag(h, which, SMY_FP64)) distance_launch<double>(h);
+  SMY_HIP(hipGetLastError());
+  finish(h, 2, out);
+  return SMY_OK;
+}
+
+template <typename T>
+void restart_launch(smy_pdhg* h) {
+  constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
   const int p = h->p;
-  // The candidate may be v_new / y_new itself: each element is read before it is written, by the
-  // same thread.
-  const T* cv = which == SMY_CURRENT ? S.v[1 - p] : S.v_erg;
-  const T* cy = which == SMY_CURRENT ? S.y[1 - p] : S.y_erg;
-  const int grid = blocks_for(std::max(h->nt, h->m));
-  hipLaunchKernelGGL((restart_to_kernel<T>), dim3(grid), dim3(kBlock), 0, h->stream, h->nt, h->m,
-                     cv, cy, S.v_rst, S.v[p], S.v[1 - p], S.vbar[0], S.vbar[1], S.y_rst, S.y[p],
-                     S.y[1 - p]);
+  RestartOp<T> op;
+  op.which = h->flags + q * h->K;
+  op.v = S.v[p];
+  op.v_new = S.v[1 - p];
+  op.y = S.y[p];
+  op.y_new = S.y[1 - p];
+  op.v_erg = S.v_erg;
+  op.y_erg = S.y_erg;
+  op.v_rst = S.v_rst;
+  op.y_rst = S.y_rst;
+  op.vbar0 = S.vbar[0];
+  op.vbar1 = S.vbar[1];
+  hipLaunchKernelGGL((block_rows_kernel<RestartOp<T>>), dim3(block_rows_grid(h)), dim3(kBlock), 0,
+                     h->stream, block_rows(h), op);
+}
+
+int restart_to_batch_impl(smy_pdhg* h, const int* which) {
+  if (block_rows_grid(h) == 0) return SMY_OK;
+  stage_flags(h, which);
+  if (any_flag(h, which, SMY_FP32)) restart_launch<float>(h);
+  if (any_flag(h, which, SMY_FP64)) restart_launch<double>(h);
   SMY_HIP(hipGetLastError());
   h->vbar_any = true;
   h->stepped = false;
   return SMY_OK;
 }
 
+template <typename From, typename To>
+void convert_launch(smy_pdhg* h) {
+  Iter<From>& A = state<From>(h);
+  Iter<To>& B = state<To>(h);
+  ConvertOp<From, To> op;
+  op.which = h->flags + prec_of<From>() * h->K;
+  const From* vin[6] = {A.v[0], A.v[1], A.vbar[0], A.vbar[1], A.v_erg, A.v_rst};
+  To* vout[6] = {B.v[0], B.v[1], B.vbar[0], B.vbar[1], B.v_erg, B.v_rst};
+  const From* yin[4] = {A.y[0], A.y[1], A.y_erg, A.y_rst};
+  To* yout[4] = {B.y[0], B.y[1], B.y_erg, B.y_rst};
+  for (int q = 0; q < 6; ++q) {
+    op.vin[q] = vin[q];
+    op.vout[q] = vout[q];
+  }
+  for (int q = 0; q < 4; ++q) {
+    op.yin[q] = yin[q];
+    op.yout[q] = yout[q];
+  }
+  hipLaunchKernelGGL((block_rows_kernel<ConvertOp<From, To>>), dim3(block_rows_grid(h)),
+                     dim3(kBlock), 0, h->stream, block_rows(h), op);
+}
+
+// New step precisions (SMY_FP64 / SMY_FP32; -1 keeps the instance as it is); the state of each
+// instance that changes is converted.
+int set_precision_batch_impl(smy_pdhg* h, const int* precision) {
+  std::vector<int> flag(h->K, -1);
+  bool any = false, to32 = false;
+  for (int i = 0; i < h->K; ++i) {
+    const int want = precision[i];
+    if (want < 0 || want == h->inst[i].prec) continue;
+    if (h->inst[i].prec == kFrozen) return fail(SMY_STATE, "instance " + std::to_string(i) + " is frozen");
+    flag[i] = 1;
+    any = true;
+    to32 = to32 || want == SMY_FP32;
+  }
+  if (!any) return SMY_OK;
+  if (to32) ensure_fp32(h);
+  if (block_rows_grid(h) > 0) {
+    stage_flags(h, flag.data());
+    if (any_flag(h, flag.data(), SMY_FP64)) convert_launch<double, float>(h);
+    if (any_flag(h, flag.data(), SMY_FP32)) convert_launch<float, double>(h);
+    SMY_HIP(hipGetLastError());
+  }
+  for (int i = 0; i < h->K; ++i)
+    if (flag[i] > 0) h->inst[i].prec = h->inst[i].last_prec = precision[i];
+  h->lists_dirty = true;
+  return SMY_OK;
+}
+
 template <typename T>
-void download_vec(smy_pdhg* h, const T* src, long long count, double* dst) {
+void download_slice(smy_pdhg* h, const T* src, long long count, double* dst) {
   if (dst == nullptr || count == 0) return;
   if constexpr (std::is_same_v<T, double>) {
     SMY_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -525,45 +790,35 @@ void download_vec(smy_pdhg* h, const T* src, long long count, double* dst) {
   }
 }
 
+// Instance i's v (its n structural entries, then its m logical ones) and y.
 template <typename T>
-int download_impl(smy_pdhg* h, double* v, double* v_new, double* y, double* y_new, double* v_erg,
-                  double* y_erg) {
+int download_impl(smy_pdhg* h, int i, double* v, double* v_new, double* y, double* y_new,
+                  double* v_erg, double* y_erg) {
   Iter<T>& S = state<T>(h);
+  const Layout& L = h->inst[i];
   const int p = h->p;
-  download_vec(h, S.v[p], h->nt, v);
-  download_vec(h, S.v[1 - p], h->nt, v_new);
-  download_vec(h, S.y[p], h->m, y);
-  download_vec(h, S.y[1 - p], h->m, y_new);
-  download_vec(h, S.v_erg, h->nt, v_erg);
-  download_vec(h, S.y_erg, h->m, y_erg);
+  const auto vec = [&](const T* src, double* dst) {
+    if (dst == nullptr) return;
+    download_slice(h, src + L.ox, L.n, dst);
+    download_slice(h, src + h->n + L.oy, L.m, dst + L.n);
+  };
+  const auto row = [&](const T* src, double* dst) {
+    if (dst != nullptr) download_slice(h, src + L.oy, L.m, dst);
+  };
+  vec(S.v[p], v);
+  vec(S.v[1 - p], v_new);
+  row(S.y[p], y);
+  row(S.y[1 - p], y_new);
+  vec(S.v_erg, v_erg);
+  row(S.y_erg, y_erg);
   SMY_HIP(hipStreamSynchronize(h->stream));
   return SMY_OK;
 }
 
-template <typename From, typename To>
-void convert_state(smy_pdhg* h) {
-  Iter<From>& A = state<From>(h);
-  Iter<To>& B = state<To>(h);
-  const auto cv = [&](const From* src, To* dst, long long count) {
-    if (count > 0)
-      hipLaunchKernelGGL((convert_kernel<From, To>), dim3(blocks_for(count)), dim3(kBlock), 0,
-                         h->stream, count, src, dst);
-  };
-  for (int q = 0; q < 2; ++q) {
-    cv(A.v[q], B.v[q], h->nt);
-    cv(A.y[q], B.y[q], h->m);
-    cv(A.vbar[q], B.vbar[q], h->nt);
-  }
-  cv(A.v_erg, B.v_erg, h->nt);
-  cv(A.y_erg, B.y_erg, h->m);
-  cv(A.v_rst, B.v_rst, h->nt);
-  cv(A.y_rst, B.y_rst, h->m);
-  SMY_HIP(hipGetLastError());
-}
-
 template <typename T>
-void set_start_impl(smy_pdhg* h, const double* v, const double* y) {
+void set_start_impl(smy_pdhg* h, int i, const double* v, const double* y) {
   Iter<T>& S = state<T>(h);
+  const Layout& L = h->inst[i];
   const auto up = [&](const double* src, T* dst, long long count) {
     if (count == 0) return;
     if constexpr (std::is_same_v<T, double>) {
@@ -574,19 +829,131 @@ void set_start_impl(smy_pdhg* h, const double* v, const double* y) {
       SMY_HIP(hipMemcpy(dst, tmp.data(), count * sizeof(float), hipMemcpyHostToDevice));
     }
   };
+  const auto vec = [&](T* dst) {
+    up(v, dst + L.ox, L.n);
+    up(v + L.n, dst + h->n + L.oy, L.m);
+  };
   SMY_HIP(hipStreamSynchronize(h->stream));
   for (int q = 0; q < 2; ++q) {
-    up(v, S.v[q], h->nt);
-    up(v, S.vbar[q], h->nt);
-    up(y, S.y[q], h->m);
+    vec(S.v[q]);
+    vec(S.vbar[q]);
+    up(y, S.y[q] + L.oy, L.m);
   }
-  up(v, S.v_erg, h->nt);
-  up(v, S.v_rst, h->nt);
-  up(y, S.y_erg, h->m);
-  up(y, S.y_rst, h->m);
+  vec(S.v_erg);
+  vec(S.v_rst);
+  up(y, S.y_erg + L.oy, L.m);
+  up(y, S.y_rst + L.oy, L.m);
   h->vbar_any = true;
   h->stepped = false;
 }
+
+// Instance i's cost / bounds into the stacked FP64 arrays (x part at ox, logical part at n + oy).
+void put_instance_vec(smy_pdhg* h, double* dst, int i, const double* src) {
+  const Layout& L = h->inst[i];
+  if (L.n > 0) SMY_HIP(hipMemcpy(dst + L.ox, src, L.n * sizeof(double), hipMemcpyHostToDevice));
+  if (L.m > 0) SMY_HIP(hipMemcpy(dst + h->n + L.oy, src + L.n, L.m * sizeof(double), hipMemcpyHostToDevice));
+}
+
+int create_impl(int count, const smy_csc* A, const smy_csc* At, const double* const* cost,
+                const double* const* lower, const double* const* upper, int precision,
+                smy_pdhg** out) {
+  long long nnz_a = 0, nnz_t = 0, n = 0, m = 0;
+  for (int i = 0; i < count; ++i) {
+    if (A[i].start == nullptr || At[i].start == nullptr) return fail(SMY_INVALID, "null argument");
+    if (A[i].rows + A[i].cols > 0 && (cost[i] == nullptr || lower[i] == nullptr || upper[i] == nullptr))
+      return fail(SMY_INVALID, "null argument");
+    if (A[i].rows != At[i].cols || A[i].cols != At[i].rows || A[i].rows < 0 || A[i].cols < 0)
+      return fail(SMY_INVALID, "A and At dimensions do not match");
+    if (A[i].start[A[i].cols] != At[i].start[At[i].cols]) return fail(SMY_INVALID, "A and At nnz differ");
+    nnz_a += At[i].start[At[i].cols];
+    nnz_t += A[i].start[A[i].cols];
+    n += A[i].cols;
+    m += A[i].rows;
+  }
+  if (!smy_gpu_available()) return fail(SMY_NO_DEVICE, "no usable gfx950 GPU");
+  if (nnz_a >= INT_MAX || nnz_t >= INT_MAX || n + m >= INT_MAX)
+    return fail(SMY_TOO_LARGE, "matrix too large for 32-bit CSR offsets");
+  return guarded([&]() -> int {
+    auto h = std::make_unique<smy_pdhg>();
+    h->K = count;
+    h->n = static_cast<int>(n);
+    h->m = static_cast<int>(m);
+    h->nt = h->n + h->m;
+    h->inst.resize(count);
+    int ox = 0, oy = 0;
+    for (int i = 0; i < count; ++i) {
+      Layout& L = h->inst[i];
+      L.n = A[i].cols;
+      L.m = A[i].rows;
+      L.ox = ox;
+      L.oy = oy;
+      L.prec = L.last_prec = precision;
+      ox += L.n;
+      oy += L.m;
+    }
+    SMY_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    SMY_HIP(hipEventCreate(&h->ev_start));
+    SMY_HIP(hipEventCreate(&h->ev_stop));
+    SMY_HIP(hipEventCreateWithFlags(&h->ev_staged, hipEventDisableTiming));
+    // A by rows is the CSC of each A' read as CSR; A' by rows is the CSC of each A.
+    upload_stack(h.get(), At, true, h->a);
+    upload_stack(h.get(), A, false, h->at);
+    alloc_iter(h.get(), h->d);
+    for (int i = 0; i < count; ++i) {
+      put_instance_vec(h.get(), h->d.cost, i, cost[i]);
+      put_instance_vec(h.get(), h->d.lower, i, lower[i]);
+      put_instance_vec(h.get(), h->d.upper, i, upper[i]);
+    }
+    if (precision == SMY_FP32) ensure_fp32(h.get());
+    h->partial = h->alloc<double>(14LL * std::max(1, stride(h.get())));
+    std::vector<int2> ranges(2 * static_cast<size_t>(count));
+    for (int i = 0; i < count; ++i) {
+      const Layout& L = h->inst[i];
+      ranges[2 * i] = make_int2(L.ba0, L.ba1);
+      ranges[2 * i + 1] = make_int2(h->a.nblocks + L.bt0, h->a.nblocks + L.bt1);
+    }
+    h->ranges = h->upload(ranges.data(), static_cast<long long>(ranges.size()));
+    h->sums_dev = h->alloc<double>(28LL * count);
+    h->scalars = h->alloc<StepScalars>(count);
+    h->flags = h->alloc<int>(2LL * count);
+    h->cs = h->alloc<double>(h->n);
+    h->rs = h->alloc<double>(h->m);
+    h->cost_o = h->alloc<double>(h->nt);
+    h->lower_o = h->alloc<double>(h->nt);
+    h->upper_o = h->alloc<double>(h->nt);
+    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->sums_host), 28 * sizeof(double) * count,
+                          hipHostMallocDefault));
+    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->scalars_host), sizeof(StepScalars) * count,
+                          hipHostMallocDefault));
+    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->flags_host), 2 * sizeof(int) * count,
+                          hipHostMallocDefault));
+    rebuild_lists(h.get());
+    SMY_HIP(hipStreamSynchronize(h->stream));
+    *out = h.release();
+    return SMY_OK;
+  });
+}
+
+int set_original_impl(smy_pdhg* h, int i, const double* col_scale, const double* row_scale,
+                      const double* cost, const double* lower, const double* upper) {
+  const Layout& L = h->inst[i];
+  // Empty vectors may come as null pointers (an LP without rows has no row scales).
+  if ((L.n > 0 && col_scale == nullptr) || (L.m > 0 && row_scale == nullptr) ||
+      (L.n + L.m > 0 && (cost == nullptr || lower == nullptr || upper == nullptr)))
+    return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int {
+    SMY_HIP(hipStreamSynchronize(h->stream));
+    if (L.n > 0) SMY_HIP(hipMemcpy(h->cs + L.ox, col_scale, L.n * sizeof(double), hipMemcpyHostToDevice));
+    if (L.m > 0) SMY_HIP(hipMemcpy(h->rs + L.oy, row_scale, L.m * sizeof(double), hipMemcpyHostToDevice));
+    put_instance_vec(h, h->cost_o, i, cost);
+    put_instance_vec(h, h->lower_o, i, lower);
+    put_instance_vec(h, h->upper_o, i, upper);
+    h->inst[i].has_original = true;
+    return SMY_OK;
+  });
+}
+
+bool single(smy_pdhg* h) { return h->K == 1; }
 
 }  // namespace
 }  // namespace smy
@@ -623,84 +990,77 @@ int smy_pdhg_create(const smy_csc* A, const smy_csc* At, const double* cost, con
                     const double* upper, int precision, smy_pdhg** out) {
   if (out == nullptr) return fail(SMY_INVALID, "null output handle");
   *out = nullptr;
-  if (A == nullptr || At == nullptr || A->start == nullptr || At->start == nullptr)
-    return fail(SMY_INVALID, "null argument");
-  if (A->rows + A->cols > 0 && (cost == nullptr || lower == nullptr || upper == nullptr))
-    return fail(SMY_INVALID, "null argument");
+  if (A == nullptr || At == nullptr) return fail(SMY_INVALID, "null argument");
   if (precision != SMY_FP64 && precision != SMY_FP32) return fail(SMY_INVALID, "bad precision");
-  if (A->rows != At->cols || A->cols != At->rows || A->rows < 0 || A->cols < 0)
-    return fail(SMY_INVALID, "A and At dimensions do not match");
-  if (!smy_gpu_available()) return fail(SMY_NO_DEVICE, "no usable gfx950 GPU");
-  if (A->start[A->cols] >= INT_MAX || At->start[At->cols] >= INT_MAX)
-    return fail(SMY_TOO_LARGE, "matrix too large for 32-bit CSR offsets");
-  if (A->start[A->cols] != At->start[At->cols]) return fail(SMY_INVALID, "A and At nnz differ");
-  return guarded([&]() -> int {
-    auto h = std::make_unique<smy_pdhg>();
-    h->m = A->rows;
-    h->n = A->cols;
-    h->nt = h->n + h->m;
-    h->precision = precision;
-    SMY_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    SMY_HIP(hipEventCreate(&h->ev_start));
-    SMY_HIP(hipEventCreate(&h->ev_stop));
-    // A by rows is the CSC of A' read as CSR; A' by rows is the CSC of A.
-    upload_rows(h.get(), At, h->a, precision == SMY_FP32);
-    upload_rows(h.get(), A, h->at, precision == SMY_FP32);
-    alloc_iter(h.get(), h->d, cost, lower, upper);
-    if (precision == SMY_FP32) alloc_iter(h.get(), h->f, cost, lower, upper);
-    const long long nparts = std::max<long long>(14LL * (h->a.nblocks + h->at.nblocks), 2048);
-    h->partial = h->alloc<double>(nparts);
-    h->sums_dev = h->alloc<double>(32);
-    h->scalars = h->alloc<StepScalars>(1);
-    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->sums_host), 32 * sizeof(double),
-                          hipHostMallocDefault));
-    SMY_HIP(hipStreamSynchronize(h->stream));
-    *out = h.release();
-    return SMY_OK;
-  });
+  return create_impl(1, A, At, &cost, &lower, &upper, precision, out);
+}
+
+int smy_pdhg_create_batch(int count, const smy_csc* A, const smy_csc* At,
+                          const double* const* cost, const double* const* lower,
+                          const double* const* upper, int precision, smy_pdhg** out) {
+  if (out == nullptr) return fail(SMY_INVALID, "null output handle");
+  *out = nullptr;
+  if (count <= 0 || count > kMaxBatch || A == nullptr || At == nullptr || cost == nullptr ||
+      lower == nullptr || upper == nullptr)
+    return fail(SMY_INVALID, "bad argument");
+  if (precision != SMY_FP64 && precision != SMY_FP32) return fail(SMY_INVALID, "bad precision");
+  return create_impl(count, A, At, cost, lower, upper, precision, out);
 }
 
 void smy_pdhg_destroy(smy_pdhg* h) { delete h; }
 
+int smy_pdhg_batch_size(smy_pdhg* h) { return h == nullptr ? 0 : h->K; }
+
+#define SMY_SINGLE(h)                                                                   \
+  do {                                                                                  \
+    if ((h) == nullptr) return fail(SMY_INVALID, "null handle");                        \
+    if (!single(h)) return fail(SMY_STATE, "a batch handle needs the batch functions"); \
+  } while (0)
+
+#define SMY_INSTANCE(h, i)                                                   \
+  do {                                                                       \
+    if ((h) == nullptr) return fail(SMY_INVALID, "null handle");             \
+    if ((i) < 0 || (i) >= (h)->K) return fail(SMY_INVALID, "bad instance");  \
+  } while (0)
+
 int smy_pdhg_set_original(smy_pdhg* h, const double* col_scale, const double* row_scale,
                           const double* cost, const double* lower, const double* upper) {
-  // Empty vectors may come as null pointers (an LP without rows has no row scales).
-  if (h == nullptr || (h->n > 0 && col_scale == nullptr) || (h->m > 0 && row_scale == nullptr) ||
-      (h->nt > 0 && (cost == nullptr || lower == nullptr || upper == nullptr)))
-    return fail(SMY_INVALID, "null argument");
-  return guarded([&]() -> int {
-    h->cs = h->upload(col_scale, h->n);
-    h->rs = h->upload(row_scale, h->m);
-    h->cost_o = h->upload(cost, h->nt);
-    h->lower_o = h->upload(lower, h->nt);
-    h->upper_o = h->upload(upper, h->nt);
-    h->has_original = true;
-    return SMY_OK;
-  });
+  SMY_SINGLE(h);
+  return set_original_impl(h, 0, col_scale, row_scale, cost, lower, upper);
+}
+
+int smy_pdhg_set_original_at(smy_pdhg* h, int instance, const double* col_scale,
+                             const double* row_scale, const double* cost, const double* lower,
+                             const double* upper) {
+  SMY_INSTANCE(h, instance);
+  return set_original_impl(h, instance, col_scale, row_scale, cost, lower, upper);
 }
 
 int smy_pdhg_step(smy_pdhg* h, double tau, double sigma, double theta, long long iter) {
-  if (h == nullptr) return fail(SMY_INVALID, "null handle");
-  return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? step_impl<double>(h, tau, sigma, theta, iter)
-                                    : step_impl<float>(h, tau, sigma, theta, iter);
-  });
+  SMY_SINGLE(h);
+  return guarded([&]() -> int { return step_impl(h, &tau, &sigma, theta, &iter); });
 }
 
 int smy_pdhg_run(smy_pdhg* h, double tau, double sigma, double theta, long long iter0, int count) {
-  if (h == nullptr) return fail(SMY_INVALID, "null handle");
-  return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? run_impl<double>(h, tau, sigma, theta, iter0, count)
-                                    : run_impl<float>(h, tau, sigma, theta, iter0, count);
-  });
+  SMY_SINGLE(h);
+  return guarded([&]() -> int { return run_impl(h, &tau, &sigma, theta, &iter0, count); });
+}
+
+int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, double theta,
+                       const long long* iter0, int count) {
+  if (h == nullptr || tau == nullptr || sigma == nullptr || iter0 == nullptr)
+    return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int { return run_impl(h, tau, sigma, theta, iter0, count); });
 }
 
 int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   if (h == nullptr || dual_ms == nullptr || primal_ms == nullptr || reps <= 0)
     return fail(SMY_INVALID, "bad argument");
   return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? time_kernels_impl<double>(h, reps, dual_ms, primal_ms)
-                                    : time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
+    refresh_lists(h);
+    if (uses(h, SMY_FP32)) return time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
+    if (uses(h, SMY_FP64)) return time_kernels_impl<double>(h, reps, dual_ms, primal_ms);
+    return fail(SMY_STATE, "no instance steps");
   });
 }
 
@@ -724,7 +1084,7 @@ int smy_pdhg_advance(smy_pdhg* h) {
 }
 
 int smy_pdhg_restart(smy_pdhg* h) {
-  if (h == nullptr) return fail(SMY_INVALID, "null handle");
+  SMY_SINGLE(h);
   return guarded([&]() -> int {
     const auto go = [&](auto& S) {
       using T = std::remove_pointer_t<decltype(S.cost)>;
@@ -733,7 +1093,9 @@ int smy_pdhg_restart(smy_pdhg* h) {
       hipLaunchKernelGGL((copy_kernel<T>), dim3(blocks_for(h->nt)), dim3(kBlock), 0, h->stream,
                          static_cast<long long>(h->nt), S.v[h->p], S.vbar[h->p]);
     };
-    if (h->precision == SMY_FP64) go(h->d); else go(h->f);
+    if (h->nt > 0) {
+      if (h->inst[0].last_prec == SMY_FP64) go(h->d); else go(h->f);
+    }
     SMY_HIP(hipGetLastError());
     h->vbar_any = true;
     return SMY_OK;
@@ -742,82 +1104,106 @@ int smy_pdhg_restart(smy_pdhg* h) {
 
 int smy_pdhg_download(smy_pdhg* h, double* v, double* v_new, double* y, double* y_new,
                       double* v_erg, double* y_erg) {
-  if (h == nullptr) return fail(SMY_INVALID, "null handle");
+  SMY_SINGLE(h);
+  return smy_pdhg_download_at(h, 0, v, v_new, y, y_new, v_erg, y_erg);
+}
+
+int smy_pdhg_download_at(smy_pdhg* h, int instance, double* v, double* v_new, double* y,
+                         double* y_new, double* v_erg, double* y_erg) {
+  SMY_INSTANCE(h, instance);
   return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? download_impl<double>(h, v, v_new, y, y_new, v_erg, y_erg)
-                                    : download_impl<float>(h, v, v_new, y, y_new, v_erg, y_erg);
+    return h->inst[instance].last_prec == SMY_FP64
+               ? download_impl<double>(h, instance, v, v_new, y, y_new, v_erg, y_erg)
+               : download_impl<float>(h, instance, v, v_new, y, y_new, v_erg, y_erg);
   });
 }
 
 int smy_pdhg_evaluate(smy_pdhg* h, int which, double out[7]) {
-  if (h == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  SMY_SINGLE(h);
+  if (out == nullptr) return fail(SMY_INVALID, "null argument");
+  if (!h->inst[0].has_original) return fail(SMY_STATE, "set_original() must be called before evaluate()");
+  if (h->inst[0].prec == kFrozen) return fail(SMY_STATE, "instance 0 is frozen");
   return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? evaluate_impl<double>(h, which, out)
-                                    : evaluate_impl<float>(h, which, out);
-  });
-}
-
-int smy_pdhg_evaluate_both(smy_pdhg* h, double out[14]) {
-  if (h == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
-  return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? evaluate_both_impl<double>(h, out)
-                                    : evaluate_both_impl<float>(h, out);
-  });
-}
-
-int smy_pdhg_restart_distance(smy_pdhg* h, int which, double* dv_sq, double* dy_sq) {
-  if (h == nullptr || dv_sq == nullptr || dy_sq == nullptr) return fail(SMY_INVALID, "null argument");
-  return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? distance_impl<double>(h, which, dv_sq, dy_sq)
-                                    : distance_impl<float>(h, which, dv_sq, dy_sq);
-  });
-}
-
-int smy_pdhg_restart_to(smy_pdhg* h, int which) {
-  if (h == nullptr) return fail(SMY_INVALID, "null handle");
-  return guarded([&]() -> int {
-    return h->precision == SMY_FP64 ? restart_to_impl<double>(h, which)
-                                    : restart_to_impl<float>(h, which);
-  });
-}
-
-int smy_pdhg_set_precision(smy_pdhg* h, int precision) {
-  if (h == nullptr) return fail(SMY_INVALID, "null handle");
-  if (precision != SMY_FP64 && precision != SMY_FP32) return fail(SMY_INVALID, "bad precision");
-  if (precision == h->precision) return SMY_OK;
-  return guarded([&]() -> int {
-    if (precision == SMY_FP32) {
-      if (!h->f.ready) {
-        // FP32 values of both matrices and FP32 iterates, cost and bounds from the FP64 ones.
-        for (DevMatrix* M : {&h->a, &h->at}) {
-          if (M->val32 == nullptr) {
-            M->val32 = h->alloc<float>(M->nnz);
-            if (M->nnz > 0)
-              hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(M->nnz)),
-                                 dim3(kBlock), 0, h->stream, M->nnz, M->val64, M->val32);
-          }
-        }
-        alloc_iter<float>(h, h->f, nullptr, nullptr, nullptr);
-        for (auto [src, dst] : {std::pair{h->d.cost, h->f.cost}, std::pair{h->d.lower, h->f.lower},
-                                std::pair{h->d.upper, h->f.upper}})
-          hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(h->nt)), dim3(kBlock),
-                             0, h->stream, static_cast<long long>(h->nt), src, dst);
-      }
-      convert_state<double, float>(h);
-    } else {
-      convert_state<float, double>(h);
-    }
-    SMY_HIP(hipStreamSynchronize(h->stream));
-    h->precision = precision;
+    refresh_lists(h);
+    if (h->inst[0].prec == SMY_FP64) measure_one<double>(h, which); else measure_one<float>(h, which);
+    SMY_HIP(hipGetLastError());
+    finish(h, 7, out);
     return SMY_OK;
   });
 }
 
-int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y) {
-  if (h == nullptr || (v == nullptr && h->nt > 0) || (y == nullptr && h->m > 0))
-    return fail(SMY_INVALID, "null argument");
+int smy_pdhg_evaluate_both(smy_pdhg* h, double out[14]) {
+  SMY_SINGLE(h);
+  if (out == nullptr) return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int { return evaluate_batch_impl(h, out); });
+}
+
+int smy_pdhg_evaluate_batch(smy_pdhg* h, double* out) {
+  if (h == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int { return evaluate_batch_impl(h, out); });
+}
+
+int smy_pdhg_restart_distance(smy_pdhg* h, int which, double* dv_sq, double* dy_sq) {
+  SMY_SINGLE(h);
+  if (dv_sq == nullptr || dy_sq == nullptr) return fail(SMY_INVALID, "null argument");
   return guarded([&]() -> int {
-    if (h->precision == SMY_FP64) set_start_impl<double>(h, v, y); else set_start_impl<float>(h, v, y);
+    double out[2] = {0.0, 0.0};
+    const int rc = distance_batch_impl(h, &which, out);
+    *dv_sq = out[0];
+    *dy_sq = out[1];
+    return rc;
+  });
+}
+
+int smy_pdhg_restart_distance_batch(smy_pdhg* h, const int* which, double* out) {
+  if (h == nullptr || which == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int { return distance_batch_impl(h, which, out); });
+}
+
+int smy_pdhg_restart_to(smy_pdhg* h, int which) {
+  SMY_SINGLE(h);
+  return guarded([&]() -> int { return restart_to_batch_impl(h, &which); });
+}
+
+int smy_pdhg_restart_to_batch(smy_pdhg* h, const int* which) {
+  if (h == nullptr || which == nullptr) return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int { return restart_to_batch_impl(h, which); });
+}
+
+int smy_pdhg_set_precision(smy_pdhg* h, int precision) {
+  SMY_SINGLE(h);
+  if (precision != SMY_FP64 && precision != SMY_FP32) return fail(SMY_INVALID, "bad precision");
+  return guarded([&]() -> int { return set_precision_batch_impl(h, &precision); });
+}
+
+int smy_pdhg_set_precision_batch(smy_pdhg* h, const int* precision) {
+  if (h == nullptr || precision == nullptr) return fail(SMY_INVALID, "null argument");
+  for (int i = 0; i < h->K; ++i)
+    if (precision[i] != -1 && precision[i] != SMY_FP64 && precision[i] != SMY_FP32)
+      return fail(SMY_INVALID, "bad precision");
+  return guarded([&]() -> int { return set_precision_batch_impl(h, precision); });
+}
+
+int smy_pdhg_freeze(smy_pdhg* h, int instance) {
+  SMY_INSTANCE(h, instance);
+  if (h->inst[instance].prec == kFrozen) return SMY_OK;
+  h->inst[instance].prec = kFrozen;
+  h->lists_dirty = true;
+  return SMY_OK;
+}
+
+int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y) {
+  SMY_SINGLE(h);
+  return smy_pdhg_set_start_at(h, 0, v, y);
+}
+
+int smy_pdhg_set_start_at(smy_pdhg* h, int instance, const double* v, const double* y) {
+  SMY_INSTANCE(h, instance);
+  const Layout& L = h->inst[instance];
+  if ((v == nullptr && L.n + L.m > 0) || (y == nullptr && L.m > 0)) return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int {
+    if (L.last_prec == SMY_FP64) set_start_impl<double>(h, instance, v, y);
+    else set_start_impl<float>(h, instance, v, y);
     return SMY_OK;
   });
 }
@@ -859,10 +1245,10 @@ int plan_run(smy_spmv_plan* p, const void* x, void* y, hipStream_t s) {
   M.nblocks = p->nblocks;
   if (p->precision == SMY_FP64) {
     SpmvOp<double, double> op{static_cast<const double*>(x), static_cast<double*>(y)};
-    launch_rows(M, static_cast<const double*>(p->val), op, s);
+    launch_rows(M, static_cast<const double*>(p->val), nullptr, M.nblocks, op, s);
   } else {
     SpmvOp<float, float> op{static_cast<const float*>(x), static_cast<float*>(y)};
-    launch_rows(M, static_cast<const float*>(p->val), op, s);
+    launch_rows(M, static_cast<const float*>(p->val), nullptr, M.nblocks, op, s);
   }
   SMY_HIP(hipGetLastError());
   return SMY_OK;

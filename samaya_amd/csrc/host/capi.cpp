@@ -81,6 +81,30 @@ bool valid_csc(int rows, int cols, const int64_t* start, const int32_t* index) {
   return true;
 }
 
+LpSolveOptions to_lp_options(const smy_pdlp_params* params) {
+  smy_pdlp_params d;
+  smy_pdlp_params_default(&d);
+  if (params == nullptr) params = &d;
+  LpSolveOptions o;
+  o.pdlp = to_options(params);
+  o.verify = params->verify != 0;
+  o.polish_rounds = params->polish_rounds;
+  return o;
+}
+
+void fill_result(const LpSolveResult& r, smy_lp_result* result) {
+  result->status = static_cast<int>(r.status);
+  result->verified = r.verified ? 1 : 0;
+  result->objective = r.objective;
+  result->max_primal_violation = r.max_primal_violation;
+  result->max_dual_violation = r.max_dual_violation;
+  result->iterations = r.iterations;
+  result->fp32_iterations = r.fp32_iterations;
+  result->rounds = r.rounds;
+  fill_info(r.pdlp, &result->pdlp);
+  copy_text(result->message, sizeof result->message, r.message);
+}
+
 }  // namespace
 
 extern "C" {
@@ -148,6 +172,10 @@ int smy_model_generate(int kind, uint64_t seed, long long p0, long long p1, long
       case SMY_GEN_SPARSE_POWERLAW:
         md->model = generate_sparse_powerlaw(seed, static_cast<int>(p0), static_cast<int>(p1),
                                              static_cast<int>(p2), p3);
+        break;
+      case SMY_GEN_BLENDING:
+        md->model = generate_blending(seed, static_cast<int>(p0), static_cast<int>(p1),
+                                      static_cast<int>(p2), static_cast<int>(p3));
         break;
       default:
         return SMY_INVALID;
@@ -276,24 +304,8 @@ int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_r
   if (model == nullptr || result == nullptr) return SMY_INVALID;
   std::memset(result, 0, sizeof *result);
   try {
-    smy_pdlp_params d;
-    smy_pdlp_params_default(&d);
-    if (params == nullptr) params = &d;
-    LpSolveOptions o;
-    o.pdlp = to_options(params);
-    o.verify = params->verify != 0;
-    o.polish_rounds = params->polish_rounds;
-    const LpSolveResult r = solve_lp(model->model, o);
-    result->status = static_cast<int>(r.status);
-    result->verified = r.verified ? 1 : 0;
-    result->objective = r.objective;
-    result->max_primal_violation = r.max_primal_violation;
-    result->max_dual_violation = r.max_dual_violation;
-    result->iterations = r.iterations;
-    result->fp32_iterations = r.fp32_iterations;
-    result->rounds = r.rounds;
-    fill_info(r.pdlp, &result->pdlp);
-    copy_text(result->message, sizeof result->message, r.message);
+    const LpSolveResult r = solve_lp(model->model, to_lp_options(params));
+    fill_result(r, result);
     put(r.col_value, col_value);
     put(r.row_dual, row_dual);
     put(r.col_dual, col_dual);
@@ -304,6 +316,38 @@ int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_r
   } catch (const std::exception& e) {
     result->status = static_cast<int>(Status::kNumericalError);
     copy_text(result->message, sizeof result->message, e.what());
+    return SMY_ERROR;
+  }
+}
+
+int smy_solve_lp_batch(const smy_model* const* models, int count, const smy_pdlp_params* params,
+                       smy_lp_result* results, double* const* col_values,
+                       double* const* row_duals, double* times) {
+  if (models == nullptr || results == nullptr || count <= 0) return SMY_INVALID;
+  for (int i = 0; i < count; ++i)
+    if (models[i] == nullptr) return SMY_INVALID;
+  std::memset(results, 0, sizeof(smy_lp_result) * static_cast<size_t>(count));
+  try {
+    std::vector<const Model*> ms;
+    for (int i = 0; i < count; ++i) ms.push_back(&models[i]->model);
+    PdlpBatchInfo info;
+    const std::vector<LpSolveResult> rs = solve_lp_batch(ms, to_lp_options(params), &info);
+    for (int i = 0; i < count; ++i) {
+      fill_result(rs[i], &results[i]);
+      if (col_values != nullptr) put(rs[i].col_value, col_values[i]);
+      if (row_duals != nullptr) put(rs[i].row_dual, row_duals[i]);
+    }
+    if (times != nullptr) {
+      times[0] = info.setup_seconds;
+      times[1] = info.solve_seconds;
+      times[2] = info.kernel_ms;
+    }
+    return SMY_OK;
+  } catch (const std::exception& e) {
+    for (int i = 0; i < count; ++i) {
+      results[i].status = static_cast<int>(Status::kNumericalError);
+      copy_text(results[i].message, sizeof results[i].message, e.what());
+    }
     return SMY_ERROR;
   }
 }

@@ -172,4 +172,84 @@ Model generate_sparse_powerlaw(uint64_t seed, int m, int n, int kmax, double mea
   return md;
 }
 
+Model generate_blending(uint64_t seed, int periods, int units, int streams, int qualities) {
+  if (periods < 0 || units <= 0 || streams <= 0 || qualities < 0) throw std::invalid_argument("bad sizes");
+  Rng rng(seed);
+  const int T = periods > 0 ? periods : 4 + static_cast<int>(rng.below(45));     // U{4..48}
+  const int Q = qualities > 0 ? qualities : 1 + static_cast<int>(rng.below(30));  // U{1..30}
+  const int U = units, S = streams;
+  // Stream qualities and weights, each unit's quality limits.
+  std::vector<double> qual(static_cast<size_t>(S) * Q), weight(static_cast<size_t>(S));
+  for (double& t : qual) t = rng.uniform(0.5, 1.5);
+  for (double& t : weight) t = rng.uniform(0.5, 1.5);
+  std::vector<double> spec(static_cast<size_t>(U) * Q);
+  for (double& t : spec) t = rng.uniform(0.9, 1.3);
+
+  // Columns of period t: use(t,u,s), buy(t,u,s), inv(t,u,s) (unit u's tank of stream s), then
+  // out(t,u). Rows of period t: tank balances (U S, equality: inv(t-1,u,s) + buy(t,u,s) -
+  // use(t,u,s) - inv(t,u,s)), unit balances (U, equality: sum_s use(t,u,s) - out(t,u)),
+  // capacities (U, <=: sum_s w_s use(t,u,s)), quality rows (U Q, <=:
+  // sum_s q_sk use(t,u,s) - spec_uk out(t,u)).
+  const int US = U * S;
+  const int cols_per = 3 * US + U;
+  const int rows_per = US + 2 * U + U * Q;
+  Model md;
+  md.m = T * rows_per;
+  md.n = T * cols_per;
+  const auto use = [&](int t, int u, int s) { return t * cols_per + u * S + s; };
+  const auto buy = [&](int t, int u, int s) { return t * cols_per + US + u * S + s; };
+  const auto inv = [&](int t, int u, int s) { return t * cols_per + 2 * US + u * S + s; };
+  const auto out = [&](int t, int u) { return t * cols_per + 3 * US + u; };
+  std::vector<Triplet> e;
+  std::vector<char> equality(static_cast<size_t>(md.m), 0);
+  for (int t = 0; t < T; ++t) {
+    const int r0 = t * rows_per;
+    for (int u = 0; u < U; ++u) {
+      for (int s = 0; s < S; ++s) {
+        const int r = r0 + u * S + s;
+        equality[r] = 1;
+        if (t > 0) e.push_back({r, inv(t - 1, u, s), 1.0});
+        e.push_back({r, buy(t, u, s), 1.0});
+        e.push_back({r, use(t, u, s), -1.0});
+        e.push_back({r, inv(t, u, s), -1.0});
+      }
+      const int rb = r0 + US + u, rc = r0 + US + U + u;
+      equality[rb] = 1;
+      for (int s = 0; s < S; ++s) {
+        e.push_back({rb, use(t, u, s), 1.0});
+        e.push_back({rc, use(t, u, s), weight[s]});
+      }
+      e.push_back({rb, out(t, u), -1.0});
+      for (int q = 0; q < Q; ++q) {
+        const int rq = r0 + US + 2 * U + u * Q + q;
+        for (int s = 0; s < S; ++s) e.push_back({rq, use(t, u, s), qual[static_cast<size_t>(s) * Q + q]});
+        e.push_back({rq, out(t, u), -spec[static_cast<size_t>(u) * Q + q]});
+      }
+    }
+  }
+  md.A = csc_from_triplets(md.m, md.n, std::move(e));
+  // Feasible by construction around x0 ~ U[0, 5]: equalities at A x0, the other rows <= A x0 +
+  // U[0, 1]; costs U[-1, 1]; 0 <= x <= 10.
+  std::vector<double> x0(static_cast<size_t>(md.n));
+  for (double& t : x0) t = rng.uniform(0.0, 5.0);
+  std::vector<double> b(static_cast<size_t>(md.m), 0.0);
+  for (int j = 0; j < md.n; ++j)
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) b[md.A.index[p]] += md.A.value[p] * x0[j];
+  md.row_lower.resize(static_cast<size_t>(md.m));
+  md.row_upper.resize(static_cast<size_t>(md.m));
+  for (int i = 0; i < md.m; ++i) {
+    if (equality[i]) {
+      md.row_lower[i] = md.row_upper[i] = b[i];
+    } else {
+      md.row_lower[i] = -kInf;
+      md.row_upper[i] = b[i] + rng.uniform(0.0, 1.0);
+    }
+  }
+  md.obj.resize(static_cast<size_t>(md.n));
+  for (double& c : md.obj) c = rng.uniform(-1.0, 1.0);
+  md.col_lower.assign(static_cast<size_t>(md.n), 0.0);
+  md.col_upper.assign(static_cast<size_t>(md.n), 10.0);
+  return md;
+}
+
 }  // namespace smy::host

@@ -16,4 +16,13 @@ Model generate_sparse_uniform(uint64_t seed, int m, int n, int per_col);
 // [2, kmax] whose mean is `mean` (column 0 gets kmax), values N(0,1) x 10^U[-2,2].
 Model generate_sparse_powerlaw(uint64_t seed, int m, int n, int kmax, double mean);
 
+// Multi-period blending LP with a block-staircase structure: `periods` periods (0: drawn from
+// U{4..48}); in each, `units` blending units hold a tank of each of `streams` streams (use, buy,
+// inventory per tank and period, inventories linking each period to the next through the tank
+// balances), unit balances, capacities with weights U[0.5, 1.5], and `qualities` quality rows per
+// unit (0: drawn from U{1..30}) with stream qualities U[0.5, 1.5] against a limit U[0.9, 1.3] on
+// the unit's output. Feasible by construction around x0 ~ U[0, 5]; costs U[-1, 1]; 0 <= x <= 10.
+// With 20 units and 30 streams: 660..1240 rows per period, 2.6k..60k rows.
+Model generate_blending(uint64_t seed, int periods, int units, int streams, int qualities);
+
 }  // namespace smy::host

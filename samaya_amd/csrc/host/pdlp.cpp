@@ -1,14 +1,18 @@
 #include "pdlp.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <exception>
 #include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "../../../include/samaya_pdhg.h"
 
@@ -44,63 +48,80 @@ class Clock {
   std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
 };
 
-void logf(const PdlpOptions& o, int level, const char* fmt, ...) {
-  if (level > o.log_level) return;
-  va_list args;
-  va_start(args, fmt);
-  std::vfprintf(stdout, fmt, args);
-  va_end(args);
-  std::fputc('\n', stdout);
-  std::fflush(stdout);
-}
-
 void check(int rc) {
   if (rc != SMY_OK) throw std::runtime_error(smy_last_error());
 }
 
-// Owner of one smy_pdhg handle; every failure becomes an exception (the reference's
-// SAMAYA_CUDA_CHECK behaviour, src/gpu/device.hpp:83-91).
+// Owner of one smy_pdhg handle over a batch of LPs (one for a single solve); every failure
+// becomes an exception (the reference's SAMAYA_CUDA_CHECK behaviour, src/gpu/device.hpp:83-91).
 class Device {
  public:
-  // Null when there is no usable GPU or the matrix is too large (PdhgDevice::create's null).
-  static std::unique_ptr<Device> create(const Csc& A, const Csc& At, const std::vector<double>& cost,
-                                        const std::vector<double>& lower,
-                                        const std::vector<double>& upper, int precision) {
-    const smy_csc a{A.rows, A.cols, A.start.data(), A.index.data(), A.value.data()};
-    const smy_csc at{At.rows, At.cols, At.start.data(), At.index.data(), At.value.data()};
+  // Null when there is no usable GPU or the matrices are too large (PdhgDevice::create's null).
+  static std::unique_ptr<Device> create(const std::vector<const Csc*>& A, const std::vector<const Csc*>& At,
+                                        const std::vector<const std::vector<double>*>& cost,
+                                        const std::vector<const std::vector<double>*>& lower,
+                                        const std::vector<const std::vector<double>*>& upper,
+                                        int precision) {
+    const size_t K = A.size();
+    std::vector<smy_csc> a(K), at(K);
+    std::vector<const double*> c(K), l(K), u(K);
+    for (size_t i = 0; i < K; ++i) {
+      a[i] = smy_csc{A[i]->rows, A[i]->cols, A[i]->start.data(), A[i]->index.data(), A[i]->value.data()};
+      at[i] = smy_csc{At[i]->rows, At[i]->cols, At[i]->start.data(), At[i]->index.data(), At[i]->value.data()};
+      c[i] = cost[i]->data();
+      l[i] = lower[i]->data();
+      u[i] = upper[i]->data();
+    }
     smy_pdhg* h = nullptr;
-    const int rc = smy_pdhg_create(&a, &at, cost.data(), lower.data(), upper.data(), precision, &h);
+    const int rc = smy_pdhg_create_batch(static_cast<int>(K), a.data(), at.data(), c.data(), l.data(),
+                                         u.data(), precision, &h);
     if (rc == SMY_NO_DEVICE || rc == SMY_TOO_LARGE) return nullptr;
     check(rc);
-    return std::unique_ptr<Device>(new Device(h));
+    return std::unique_ptr<Device>(new Device(h, static_cast<int>(K)));
   }
   ~Device() { smy_pdhg_destroy(h_); }
   Device(const Device&) = delete;
   Device& operator=(const Device&) = delete;
 
-  smy_pdhg* handle() { return h_; }
-  void set_original(const std::vector<double>& cs, const std::vector<double>& rs,
+  int size() const { return k_; }
+  void set_original(int i, const std::vector<double>& cs, const std::vector<double>& rs,
                     const std::vector<double>& cost, const std::vector<double>& lower,
                     const std::vector<double>& upper) {
-    check(smy_pdhg_set_original(h_, cs.data(), rs.data(), cost.data(), lower.data(), upper.data()));
+    check(smy_pdhg_set_original_at(h_, i, cs.data(), rs.data(), cost.data(), lower.data(), upper.data()));
   }
-  void run(double tau, double sigma, double theta, long long k0, int count, bool graph) {
+  void set_start(int i, const std::vector<double>& v, const std::vector<double>& y) {
+    check(smy_pdhg_set_start_at(h_, i, v.data(), y.data()));
+  }
+  // `count` steps of every instance still stepping; single steps with advances when `graph` is
+  // off (the reference's one launch sequence per iteration).
+  void run(const std::vector<double>& tau, const std::vector<double>& sigma, double theta,
+           const std::vector<long long>& k0, int count, bool graph) {
     if (graph) {
-      check(smy_pdhg_run(h_, tau, sigma, theta, k0, count));
+      check(smy_pdhg_run_batch(h_, tau.data(), sigma.data(), theta, k0.data(), count));
     } else {
+      std::vector<long long> k = k0;
       for (int j = 0; j < count; ++j) {
-        check(smy_pdhg_step(h_, tau, sigma, theta, k0 + j));
+        check(smy_pdhg_run_batch(h_, tau.data(), sigma.data(), theta, k.data(), 1));
+        for (long long& t : k) ++t;
         if (j + 1 < count) check(smy_pdhg_advance(h_));
       }
     }
   }
-  void evaluate_both(double out[14]) { check(smy_pdhg_evaluate_both(h_, out)); }
-  void restart_distance(int which, double& dv, double& dy) {
-    check(smy_pdhg_restart_distance(h_, which, &dv, &dy));
+  void evaluate(std::vector<double>& out) {
+    out.resize(14 * static_cast<size_t>(k_));
+    check(smy_pdhg_evaluate_batch(h_, out.data()));
   }
-  void restart_to(int which) { check(smy_pdhg_restart_to(h_, which)); }
+  void restart_distance(const std::vector<int>& which, std::vector<double>& out) {
+    out.resize(2 * static_cast<size_t>(k_));
+    check(smy_pdhg_restart_distance_batch(h_, which.data(), out.data()));
+  }
+  void restart_to(const std::vector<int>& which) { check(smy_pdhg_restart_to_batch(h_, which.data())); }
+  void set_precision(const std::vector<int>& precision) {
+    check(smy_pdhg_set_precision_batch(h_, precision.data()));
+  }
+  void freeze(int i) { check(smy_pdhg_freeze(h_, i)); }
   void advance() { check(smy_pdhg_advance(h_)); }
-  void download(std::vector<double>* v, std::vector<double>* v_new, std::vector<double>* y,
+  void download(int i, std::vector<double>* v, std::vector<double>* v_new, std::vector<double>* y,
                 std::vector<double>* y_new, std::vector<double>* v_erg, std::vector<double>* y_erg,
                 int nt, int m) {
     for (auto* p : {v, v_new, v_erg})
@@ -108,11 +129,7 @@ class Device {
     for (auto* p : {y, y_new, y_erg})
       if (p) p->resize(m);
     const auto d = [](std::vector<double>* p) { return p ? p->data() : nullptr; };
-    check(smy_pdhg_download(h_, d(v), d(v_new), d(y), d(y_new), d(v_erg), d(y_erg)));
-  }
-  void set_precision(int p) { check(smy_pdhg_set_precision(h_, p)); }
-  void set_start(const std::vector<double>& v, const std::vector<double>& y) {
-    check(smy_pdhg_set_start(h_, v.data(), y.data()));
+    check(smy_pdhg_download_at(h_, i, d(v), d(v_new), d(y), d(y_new), d(v_erg), d(y_erg)));
   }
   double last_run_ms() {
     double ms = 0.0;
@@ -120,8 +137,9 @@ class Device {
   }
 
  private:
-  explicit Device(smy_pdhg* h) : h_(h) {}
+  Device(smy_pdhg* h, int k) : h_(h), k_(k) {}
   smy_pdhg* h_;
+  int k_;
 };
 
 // The equilibration of solve_pdlp (src/lp/pdlp.cpp:109-187): up to `passes` simultaneous updates
@@ -253,16 +271,21 @@ double worst(const Measure& e) {
   return std::max({e.rel_primal, e.rel_dual, std::fabs(e.rel_gap)});
 }
 
-class Solve {
+// One LP of a solve: its setup, its decisions, its result. A single solve has one lane; a batch
+// has one per LP, each taking exactly the decisions it would take alone.
+class Lane {
  public:
-  Solve(const LpProblem& lp, const PdlpOptions& opt) : lp_(lp), opt_(opt) {}
-  PdlpResult run();
+  Lane(const LpProblem& lp, const PdlpOptions& opt, std::string tag)
+      : lp_(lp), opt_(opt), tag_(std::move(tag)) {}
 
- private:
+  // Host setup (src/lp/pdlp.cpp:260-318, 365-394): equilibration, the scaled matrix, cost and
+  // bounds, the step size, the iteration limit and the norms of the termination test.
+  void prepare();
+  void log(int level, const char* fmt, ...) const;
   Measure finish(const Sums& s) const;
   double kkt(const Measure& e) const {
-    return std::sqrt(omega_ * omega_ * e.rel_primal * e.rel_primal +
-                     e.rel_dual * e.rel_dual / (omega_ * omega_) + e.rel_gap * e.rel_gap);
+    return std::sqrt(omega * omega * e.rel_primal * e.rel_primal +
+                     e.rel_dual * e.rel_dual / (omega * omega) + e.rel_gap * e.rel_gap);
   }
   bool detect_farkas(long long iter, const std::vector<double>& y_erg);
   bool detect_ray(long long iter, const std::vector<double>& v, const std::vector<double>& v_new);
@@ -270,18 +293,104 @@ class Solve {
 
   const LpProblem& lp_;
   const PdlpOptions& opt_;
-  PdlpResult res_;
-  Clock clock_;
-  std::vector<double> rs_, cs_;
-  double a_colmax_ = 0.0, at_rowmax_ = 0.0, c_norm_ = 0.0;
-  double omega_ = 1.0;
+  std::string tag_;  // "pdlp" alone, "pdlp[i]" in a batch
+  // Warm start of this LP (null: the options' start_x / start_y).
+  const std::vector<double>* start_x = nullptr;
+  const std::vector<double>* start_y = nullptr;
+  PdlpResult res;
+  std::vector<double> rs, cs;
+  Csc A_eq, At_eq;
+  std::vector<double> cost, lower, upper;  // equilibrated
+  std::vector<double> v0, y0;              // warm start, equilibrated (empty: cold start)
+  double eta = 0.0;
+  double a_colmax = 0.0, at_rowmax = 0.0, c_norm = 0.0;
+  long long max_iter = 0;
+  // Iteration state.
+  double omega = 1.0, tau = 0.0, sigma = 0.0;
+  long long k_since = 0;
+  double kkt_at_restart = kInf, kkt_previous = kInf;
+  bool fp32 = false;
+  bool active = true;
+  Measure best{kInf, kInf, kInf, kInf, 0.0, 0.0};
 };
 
-Measure Solve::finish(const Sums& s) const {
+void Lane::log(int level, const char* fmt, ...) const {
+  if (level > opt_.log_level) return;
+  std::fputs(tag_.c_str(), stdout);
+  va_list args;
+  va_start(args, fmt);
+  std::vfprintf(stdout, fmt, args);
+  va_end(args);
+  std::fputc('\n', stdout);
+  std::fflush(stdout);
+}
+
+void Lane::prepare() {
+  const int m = lp_.m, n = lp_.n, nt = n + m;
+  if (opt_.algorithm == Algorithm::kEnhanced)
+    ruiz_pock_chambolle(lp_.A, lp_.At, rs, cs, 10);
+  else
+    equilibrate(lp_.A, lp_.At, rs, cs, 10);
+  A_eq = csc_scaled(lp_.A, rs, cs);
+  At_eq = csc_scaled(lp_.At, cs, rs);
+  cost = lp_.cost;
+  lower = lp_.lower;
+  upper = lp_.upper;
+  for (int j = 0; j < n; ++j) {
+    cost[j] *= cs[j];
+    lower[j] /= cs[j];
+    upper[j] /= cs[j];
+  }
+  for (int i = 0; i < m; ++i) {
+    cost[n + i] *= rs[i];
+    lower[n + i] *= rs[i];
+    upper[n + i] *= rs[i];
+  }
+  const double L = operator_norm_bound(A_eq, At_eq);
+  res.operator_norm = L;
+  eta = 0.99 / L;
+  omega = 1.0;
+  tau = eta / omega;
+  sigma = eta * omega;
+  log(3, ": estimated ||[A -I]|| = %.3e", L);
+
+  max_iter = opt_.max_iterations;  // src/lp/pdlp.cpp:365-373
+  if (max_iter <= 0) {
+    max_iter = std::max<long long>(10000LL, 10000LL * (m + n) / 10);
+    if (opt_.time_limit >= kInf) max_iter = std::min<long long>(max_iter, 1000000LL);
+  }
+  if (m > 0) {  // src/lp/pdlp.cpp:377-394
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) s += lp_.A.value[p] * lp_.A.value[p];
+      a_colmax = std::max(a_colmax, std::sqrt(s));
+    }
+    for (int i = 0; i < m; ++i) {
+      double s = 0.0;
+      for (int64_t p = lp_.At.start[i]; p < lp_.At.start[i + 1]; ++p) s += lp_.At.value[p] * lp_.At.value[p];
+      at_rowmax = std::max(at_rowmax, std::sqrt(s));
+    }
+  }
+  for (int k = 0; k < nt; ++k) c_norm += lp_.cost[k] * lp_.cost[k];
+  c_norm = std::sqrt(c_norm);
+
+  const std::vector<double>& sx = start_x != nullptr ? *start_x : opt_.start_x;
+  const std::vector<double>& sy = start_y != nullptr ? *start_y : opt_.start_y;
+  if (!sx.empty()) {
+    // Warm start: into the equilibrated space (x' = x / cs, r' = rs r, y' = y / rs).
+    v0.assign(static_cast<size_t>(nt), 0.0);
+    y0.assign(static_cast<size_t>(m), 0.0);
+    for (int j = 0; j < n; ++j) v0[j] = sx[j] / cs[j];
+    for (int i = 0; i < m; ++i) v0[n + i] = sx[n + i] * rs[i];
+    for (int i = 0; i < m; ++i) y0[i] = sy.empty() ? 0.0 : sy[i] / rs[i];
+  }
+}
+
+Measure Lane::finish(const Sums& s) const {
   // src/lp/pdlp.cpp:493-519.
   const int m = lp_.m;
-  const double primal_scale = 1.0 + a_colmax_ * std::sqrt(s.x_sq) + std::sqrt(s.r_sq);
-  const double dual_scale = 1.0 + at_rowmax_ * std::sqrt(s.y_sq) + c_norm_;
+  const double primal_scale = 1.0 + a_colmax * std::sqrt(s.x_sq) + std::sqrt(s.r_sq);
+  const double dual_scale = 1.0 + at_rowmax * std::sqrt(s.y_sq) + c_norm;
   const double gap_scale = 1.0 + std::fabs(s.primal_obj) + std::fabs(s.dual_obj);
   const double gap = s.primal_obj - s.dual_obj;
   return Measure{m > 0 ? std::sqrt(s.primal_sq) / primal_scale : 0.0,
@@ -290,12 +399,12 @@ Measure Solve::finish(const Sums& s) const {
 }
 
 // Farkas certificate test on the average dual (src/lp/pdlp.cpp:724-842).
-bool Solve::detect_farkas(long long iter, const std::vector<double>& y_erg) {
+bool Lane::detect_farkas(long long iter, const std::vector<double>& y_erg) {
   const int m = lp_.m, n = lp_.n;
   std::vector<double> yf(static_cast<size_t>(m));
   double y_max = 0.0;
   for (int i = 0; i < m; ++i) {
-    yf[i] = y_erg[i] * rs_[i];
+    yf[i] = y_erg[i] * rs[i];
     y_max = std::max(y_max, std::fabs(yf[i]));
   }
   if (!(y_max > 1e-6)) return false;
@@ -331,18 +440,18 @@ bool Solve::detect_farkas(long long iter, const std::vector<double>& y_erg) {
   }
   if (!std::isfinite(scale)) scale = 0.0;
   const double margin = 1e-8 * (1.0 + scale);
-  logf(opt_, 2, "pdlp: infeas check iter=%lld y_max=%.2e min_f=%.2e max_f=%.2e margin=%.2e scale=%.2e",
-       iter, y_max, f_min, f_max, margin, scale);
+  log(2, ": infeas check iter=%lld y_max=%.2e min_f=%.2e max_f=%.2e margin=%.2e scale=%.2e",
+      iter, y_max, f_min, f_max, margin, scale);
   if ((std::isfinite(f_min) && f_min > margin) || (std::isfinite(f_max) && f_max < -margin)) {
-    logf(opt_, 1, "pdlp: detected primal infeasibility at iter %lld (Farkas certificate)", iter);
-    res_.dual_ray = y_erg;
+    log(1, ": detected primal infeasibility at iter %lld (Farkas certificate)", iter);
+    res.dual_ray = y_erg;
     return true;
   }
   return false;
 }
 
 // Improving-ray test on the difference of consecutive iterates (src/lp/pdlp.cpp:852-1042).
-bool Solve::detect_ray(long long iter, const std::vector<double>& v, const std::vector<double>& v_new) {
+bool Lane::detect_ray(long long iter, const std::vector<double>& v, const std::vector<double>& v_new) {
   const int m = lp_.m, n = lp_.n;
   std::vector<double> d(v_new.size());
   double d_max = 0.0;
@@ -352,25 +461,25 @@ bool Solve::detect_ray(long long iter, const std::vector<double>& v, const std::
   }
   if (!(d_max > 1e-6)) return false;
   for (double& t : d) t /= d_max;
-  std::vector<double> res(static_cast<size_t>(m), 0.0);
+  std::vector<double> r(static_cast<size_t>(m), 0.0);
   for (int j = 0; j < n; ++j) {
-    const double xj = d[j] * cs_[j];
+    const double xj = d[j] * cs[j];
     if (xj == 0.0) continue;
-    for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) res[lp_.A.index[p]] += lp_.A.value[p] * xj;
+    for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) r[lp_.A.index[p]] += lp_.A.value[p] * xj;
   }
-  for (int i = 0; i < m; ++i) res[i] -= d[n + i] / rs_[i];
+  for (int i = 0; i < m; ++i) r[i] -= d[n + i] / rs[i];
   double primal_feas = 0.0;
-  for (double t : res) primal_feas += t * t;
+  for (double t : r) primal_feas += t * t;
   primal_feas = std::sqrt(primal_feas);
   double cost_dir = 0.0;
-  for (int j = 0; j < n; ++j) cost_dir += lp_.cost[j] * (d[j] * cs_[j]);
-  for (int i = 0; i < m; ++i) cost_dir += lp_.cost[n + i] * (d[n + i] / rs_[i]);
+  for (int j = 0; j < n; ++j) cost_dir += lp_.cost[j] * (d[j] * cs[j]);
+  for (int i = 0; i < m; ++i) cost_dir += lp_.cost[n + i] * (d[n + i] / rs[i]);
   constexpr double kBoxTol = 1e-4, kFixedTol = 1e-3;
   const auto fixed = [](double lo, double up) { return up - lo <= 1e-10 * (1.0 + std::fabs(lo)); };
   bool ok = true;
   std::vector<double> ray(static_cast<size_t>(n), 0.0);
   for (int j = 0; j < n && ok; ++j) {
-    double xj = d[j] * cs_[j];
+    double xj = d[j] * cs[j];
     const double lo = lp_.lower[j], up = lp_.upper[j];
     const bool lb = lo > -kInf / 2, ub = up < kInf / 2;
     if (lb && ub) {
@@ -402,84 +511,95 @@ bool Solve::detect_ray(long long iter, const std::vector<double>& v, const std::
     }
   }
   if (primal_feas < 1e-6 && cost_dir < -1e-8 && ok) {
-    logf(opt_, 1, "pdlp: detected dual infeasibility (unbounded) at iter %lld (improving ray)", iter);
-    res_.primal_ray = std::move(ray);
+    log(1, ": detected dual infeasibility (unbounded) at iter %lld (improving ray)", iter);
+    res.primal_ray = std::move(ray);
     return true;
   }
   return false;
 }
 
-void Solve::map_back(const std::vector<double>& v, const std::vector<double>& y) {
+void Lane::map_back(const std::vector<double>& v, const std::vector<double>& y) {
   // src/lp/pdlp.cpp:1167-1189: x = cs v_x, r = v_r / rs, y = rs y'.
   const int m = lp_.m, n = lp_.n;
-  res_.x.assign(static_cast<size_t>(n + m), 0.0);
-  for (int j = 0; j < n; ++j) res_.x[j] = v[j] * cs_[j];
-  for (int i = 0; i < m; ++i) res_.x[n + i] = v[n + i] / rs_[i];
-  res_.y.assign(static_cast<size_t>(m), 0.0);
-  for (int i = 0; i < m; ++i) res_.y[i] = y[i] * rs_[i];
+  res.x.assign(static_cast<size_t>(n + m), 0.0);
+  for (int j = 0; j < n; ++j) res.x[j] = v[j] * cs[j];
+  for (int i = 0; i < m; ++i) res.x[n + i] = v[n + i] / rs[i];
+  res.y.assign(static_cast<size_t>(m), 0.0);
+  for (int i = 0; i < m; ++i) res.y[i] = y[i] * rs[i];
 }
 
-PdlpResult Solve::run() {
-  const int m = lp_.m, n = lp_.n, nt = n + m;
-  logf(opt_, 2, "pdlp: m=%d n=%d nnz=%lld tol=%.2e", m, n, static_cast<long long>(lp_.A.nnz()), opt_.tol);
-  if (!opt_.use_gpu) {
-    res_.message = "this PDLP runs its iterations on the GPU only (use_gpu = false)";
-    res_.status = Status::kNumericalError;
-    return res_;
+// Runs fn(i) for i in [0, count) on up to `threads` host threads.
+template <typename F>
+void parallel_for(int count, int threads, F&& fn) {
+  threads = std::max(1, std::min(threads, count));
+  if (threads == 1) {
+    for (int i = 0; i < count; ++i) fn(i);
+    return;
   }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> errors(static_cast<size_t>(threads));
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] {
+      try {
+        for (int i = next++; i < count; i = next++) fn(i);
+      } catch (...) {
+        errors[t] = std::current_exception();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errors)
+    if (e) std::rethrow_exception(e);
+}
 
-  // --- Setup (src/lp/pdlp.cpp:260-318) ---
-  if (opt_.algorithm == Algorithm::kEnhanced)
-    ruiz_pock_chambolle(lp_.A, lp_.At, rs_, cs_, 10);
-  else
-    equilibrate(lp_.A, lp_.At, rs_, cs_, 10);
-  const Csc A_eq = csc_scaled(lp_.A, rs_, cs_);
-  const Csc At_eq = csc_scaled(lp_.At, cs_, rs_);
-  std::vector<double> cost = lp_.cost, lower = lp_.lower, upper = lp_.upper;
-  for (int j = 0; j < n; ++j) {
-    cost[j] *= cs_[j];
-    lower[j] /= cs_[j];
-    upper[j] /= cs_[j];
-  }
-  for (int i = 0; i < m; ++i) {
-    cost[n + i] *= rs_[i];
-    lower[n + i] *= rs_[i];
-    upper[n + i] *= rs_[i];
-  }
-  const double L = operator_norm_bound(A_eq, At_eq);
-  res_.operator_norm = L;
-  const double eta = 0.99 / L;
-  omega_ = 1.0;
-  double tau = eta / omega_, sigma = eta * omega_;
-  const double theta = 1.0;
-  logf(opt_, 3, "pdlp: estimated ||[A -I]|| = %.3e", L);
+int host_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return static_cast<int>(std::min(16u, std::max(1u, hw)));
+}
 
-  long long max_iter = opt_.max_iterations;  // src/lp/pdlp.cpp:365-373
-  if (max_iter <= 0) {
-    max_iter = std::max<long long>(10000LL, 10000LL * (m + n) / 10);
-    if (opt_.time_limit >= kInf) max_iter = std::min<long long>(max_iter, 1000000LL);
-  }
-  if (m > 0) {  // src/lp/pdlp.cpp:377-394
-    for (int j = 0; j < n; ++j) {
-      double s = 0.0;
-      for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) s += lp_.A.value[p] * lp_.A.value[p];
-      a_colmax_ = std::max(a_colmax_, std::sqrt(s));
+// The PDLP loop of src/lp/pdlp.cpp:522-1150 over a set of lanes sharing one device handle. Every
+// lane steps, evaluates, restarts, switches precision and stops on its own schedule; the lanes
+// only share the kernel launches.
+std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& opt, const Clock& clock,
+                                  PdlpBatchInfo* info) {
+  const int K = static_cast<int>(lanes.size());
+  for (Lane& L : lanes)
+    L.log(2, ": m=%d n=%d nnz=%lld tol=%.2e", L.lp_.m, L.lp_.n, static_cast<long long>(L.lp_.A.nnz()), opt.tol);
+  const auto fail_all = [&](Status status, const std::string& message) {
+    std::vector<PdlpResult> out(static_cast<size_t>(K));
+    for (PdlpResult& r : out) {
+      r.status = status;
+      r.message = message;
+      r.solve_seconds = clock.seconds();
     }
-    for (int i = 0; i < m; ++i) {
-      double s = 0.0;
-      for (int64_t p = lp_.At.start[i]; p < lp_.At.start[i + 1]; ++p) s += lp_.At.value[p] * lp_.At.value[p];
-      at_rowmax_ = std::max(at_rowmax_, std::sqrt(s));
-    }
+    return out;
+  };
+  if (!opt.use_gpu) {
+    return fail_all(Status::kNumericalError,
+                    "this PDLP runs its iterations on the GPU only (use_gpu = false)");
   }
-  for (int k = 0; k < nt; ++k) c_norm_ += lp_.cost[k] * lp_.cost[k];
-  c_norm_ = std::sqrt(c_norm_);
+  // --- Setup (src/lp/pdlp.cpp:260-318), lanes in parallel on the host ---
+  parallel_for(K, K > 1 ? host_threads() : 1, [&](int i) { lanes[i].prepare(); });
 
-  const bool mixed = opt_.precision == Precision::kMixed;
+  const bool mixed = opt.precision == Precision::kMixed;
   std::unique_ptr<Device> device;
   std::string why;
   try {
-    device = Device::create(A_eq, At_eq, cost, lower, upper, mixed ? SMY_FP32 : SMY_FP64);
-    if (device) device->set_original(cs_, rs_, lp_.cost, lp_.lower, lp_.upper);
+    std::vector<const Csc*> A, At;
+    std::vector<const std::vector<double>*> c, l, u;
+    for (Lane& L : lanes) {
+      A.push_back(&L.A_eq);
+      At.push_back(&L.At_eq);
+      c.push_back(&L.cost);
+      l.push_back(&L.lower);
+      u.push_back(&L.upper);
+    }
+    device = Device::create(A, At, c, l, u, mixed ? SMY_FP32 : SMY_FP64);
+    if (device)
+      for (int i = 0; i < K; ++i) {
+        Lane& L = lanes[i];
+        device->set_original(i, L.cs, L.rs, L.lp_.cost, L.lp_.lower, L.lp_.upper);
+      }
   } catch (const std::exception& e) {
     device.reset();
     why = e.what();
@@ -487,194 +607,264 @@ PdlpResult Solve::run() {
   if (!device) {
     // src/lp/pdlp.cpp:355-358 (the reference then iterates on the CPU; this library has no CPU
     // iterations and reports the failure instead).
-    if (!why.empty()) {
-      logf(opt_, 1, "pdlp: the GPU could not start (%s)", why.c_str());
-      res_.message = "the GPU could not start: " + why;
-    } else {
-      logf(opt_, 1, "pdlp: no usable GPU");
-      res_.message = std::string("no usable GPU: ") + smy_last_error();
+    for (Lane& L : lanes) {
+      if (!why.empty())
+        L.log(1, ": the GPU could not start (%s)", why.c_str());
+      else
+        L.log(1, ": no usable GPU");
     }
-    res_.status = Status::kNumericalError;
-    res_.solve_seconds = clock_.seconds();
-    return res_;
+    return fail_all(Status::kNumericalError, !why.empty() ? "the GPU could not start: " + why
+                                                          : std::string("no usable GPU: ") + smy_last_error());
   }
   {
     char name[256];
     smy_gpu_device_name(name, sizeof name);
-    logf(opt_, 1, "pdlp: iterations on the GPU (%s)", name);
+    for (Lane& L : lanes) L.log(1, ": iterations on the GPU (%s)", name);
   }
-  if (!opt_.start_x.empty()) {
-    // Warm start: into the equilibrated space (x' = x / cs, r' = rs r, y' = y / rs).
-    std::vector<double> v0(static_cast<size_t>(nt)), y0(static_cast<size_t>(m));
-    for (int j = 0; j < n; ++j) v0[j] = opt_.start_x[j] / cs_[j];
-    for (int i = 0; i < m; ++i) v0[n + i] = opt_.start_x[n + i] * rs_[i];
-    for (int i = 0; i < m; ++i) y0[i] = opt_.start_y.empty() ? 0.0 : opt_.start_y[i] / rs_[i];
-    device->set_start(v0, y0);
+  for (int i = 0; i < K; ++i)
+    if (!lanes[i].v0.empty()) device->set_start(i, lanes[i].v0, lanes[i].y0);
+  const double setup_seconds = clock.seconds();
+  for (Lane& L : lanes) {
+    L.res.setup_seconds = setup_seconds;
+    L.fp32 = mixed;
   }
-  res_.setup_seconds = clock_.seconds();
+  if (info != nullptr) info->setup_seconds = setup_seconds;
 
-  double kkt_at_restart = kInf, kkt_previous = kInf;
-  long long k_since = 0;
-  bool fp32 = mixed;
-  bool done = false;
-  Measure best{kInf, kInf, kInf, kInf, 0.0, 0.0};
+  const double theta = 1.0;
+  std::vector<double> tau(static_cast<size_t>(K)), sigma(static_cast<size_t>(K)), sums, dist;
+  std::vector<long long> k0(static_cast<size_t>(K));
+  std::vector<int> which(static_cast<size_t>(K)), precision(static_cast<size_t>(K));
   std::vector<double> v, v_new, y, y_new, v_erg, y_erg;
+  int active = K;
+  const auto stop = [&](int i) {
+    device->freeze(i);
+    lanes[i].active = false;
+    lanes[i].res.solve_seconds = clock.seconds();
+    --active;
+  };
+  // The lane's answer at its current point (a limit stopped it: src/lp/pdlp.cpp:1153-1165).
+  const auto stop_at_current = [&](int i) {
+    Lane& L = lanes[i];
+    const int nt = L.lp_.n + L.lp_.m;
+    device->download(i, &v, nullptr, &y, nullptr, nullptr, nullptr, nt, L.lp_.m);
+    L.map_back(v, y);
+    stop(i);
+  };
   long long iter = 0;
-  while (iter < max_iter && !done) {
-    if (opt_.time_limit < kInf && clock_.seconds() > opt_.time_limit) {
-      res_.status = Status::kTimeLimit;
-      res_.iterations = iter;
-      logf(opt_, 1, "pdlp: time limit after %lld iterations, %.1f s, KKT at the last restart %.2e; "
-                    "last evaluation: primal %.2e, dual %.2e, gap %.2e (tolerance %.0e)",
-           iter, clock_.seconds(), kkt_at_restart, best.rel_primal, best.rel_dual, best.rel_gap, opt_.tol);
+  while (active > 0) {
+    if (opt.time_limit < kInf && clock.seconds() > opt.time_limit) {
+      for (int i = 0; i < K; ++i) {
+        Lane& L = lanes[i];
+        if (!L.active) continue;
+        L.res.status = Status::kTimeLimit;
+        L.res.iterations = iter;
+        L.log(1, ": time limit after %lld iterations, %.1f s, KKT at the last restart %.2e; "
+                 "last evaluation: primal %.2e, dual %.2e, gap %.2e (tolerance %.0e)",
+              iter, clock.seconds(), L.kkt_at_restart, L.best.rel_primal, L.best.rel_dual,
+              L.best.rel_gap, opt.tol);
+        stop_at_current(i);
+      }
       break;
     }
-    // Steps up to the next evaluated iteration, (iter + 1) % 64 == 0 or the last one.
-    const long long eval_iter =
-        std::min((iter / kCheckFrequency + 1) * kCheckFrequency - 1, max_iter - 1);
+    // Steps up to the next evaluated iteration of the earliest lane: (iter + 1) % 64 == 0 or the
+    // lane's last one.
+    long long eval_iter = LLONG_MAX;
+    for (const Lane& L : lanes)
+      if (L.active)
+        eval_iter = std::min(eval_iter, std::min((iter / kCheckFrequency + 1) * kCheckFrequency - 1,
+                                                 L.max_iter - 1));
     const int count = static_cast<int>(eval_iter - iter + 1);
-    device->run(tau, sigma, theta, k_since, count, opt_.use_graph);
-    res_.kernel_ms += device->last_run_ms();
-    k_since += count;
-    if (fp32) res_.fp32_iterations += count;
+    for (int i = 0; i < K; ++i) {
+      tau[i] = lanes[i].tau;
+      sigma[i] = lanes[i].sigma;
+      k0[i] = lanes[i].k_since;
+    }
+    device->run(tau, sigma, theta, k0, count, opt.use_graph);
+    const double ms = device->last_run_ms();
+    if (info != nullptr) info->kernel_ms += ms;
+    for (Lane& L : lanes) {
+      if (!L.active) continue;
+      L.res.kernel_ms += ms;
+      L.k_since += count;
+      if (L.fp32) L.res.fp32_iterations += count;
+    }
     iter = eval_iter;
 
-    double s14[14];
-    device->evaluate_both(s14);
-    const Measure current = finish(Sums{s14[0], s14[1], s14[2], s14[3], s14[4], s14[5], s14[6]});
-    const Measure average = finish(Sums{s14[7], s14[8], s14[9], s14[10], s14[11], s14[12], s14[13]});
-    res_.evals.push_back({iter,
-                          {current.rel_primal, current.rel_dual, current.rel_gap},
-                          {average.rel_primal, average.rel_dual, average.rel_gap}});
-    best = worst(average) < worst(current) ? average : current;
+    device->evaluate(sums);
+    std::fill(which.begin(), which.end(), -1);
+    std::fill(precision.begin(), precision.end(), -1);
+    std::vector<char> force(static_cast<size_t>(K), 0);
+    std::vector<double> kkt_candidate(static_cast<size_t>(K), 0.0);
+    for (int i = 0; i < K; ++i) {
+      Lane& L = lanes[i];
+      // A lane is evaluated at its own evaluation points only: every 64th iteration and its last.
+      if (!L.active || ((iter + 1) % kCheckFrequency != 0 && iter != L.max_iter - 1)) continue;
+      const double* s14 = &sums[14 * static_cast<size_t>(i)];
+      const Measure current = L.finish(Sums{s14[0], s14[1], s14[2], s14[3], s14[4], s14[5], s14[6]});
+      const Measure average = L.finish(Sums{s14[7], s14[8], s14[9], s14[10], s14[11], s14[12], s14[13]});
+      L.res.evals.push_back({iter,
+                             {current.rel_primal, current.rel_dual, current.rel_gap},
+                             {average.rel_primal, average.rel_dual, average.rel_gap}});
+      L.best = worst(average) < worst(current) ? average : current;
+      const int nt = L.lp_.n + L.lp_.m, m = L.lp_.m;
 
-    // Termination (src/lp/pdlp.cpp:684-713): strict < tol for the current, then the average.
-    const auto converged = [&](const Measure& e) {
-      return e.rel_primal < opt_.tol && e.rel_dual < opt_.tol && std::fabs(e.rel_gap) < opt_.tol;
-    };
-    if (converged(current) || converged(average)) {
-      const bool cur = converged(current);
-      const Measure& e = cur ? current : average;
-      logf(opt_, 2, cur ? "pdlp: converged at iter %lld: primal=%.2e dual=%.2e gap=%.2e"
-                        : "pdlp: average converged at iter %lld: primal=%.2e dual=%.2e gap=%.2e",
-           iter, e.rel_primal, e.rel_dual, e.rel_gap);
-      res_.status = Status::kOptimal;
-      res_.iterations = iter + 1;
-      res_.primal_residual = e.rel_primal;
-      res_.dual_residual = e.rel_dual;
-      res_.gap = e.rel_gap;
-      if (cur) {
-        device->download(nullptr, &v_new, nullptr, &y_new, nullptr, nullptr, nt, m);
-        v = v_new;
-        y = y_new;
-      } else {
-        device->download(nullptr, nullptr, nullptr, nullptr, &v_erg, &y_erg, nt, m);
-        v = v_erg;
-        y = y_erg;
-      }
-      break;
-    }
-
-    // Detection (src/lp/pdlp.cpp:715-1043), at every 8th evaluation after iteration 1000.
-    const bool detect = opt_.detect && iter > 1000 && m > 0 &&
-                        ((iter + 1) / kCheckFrequency) % kDetectionEvery == 0;
-    if (detect) {
-      device->download(&v, &v_new, nullptr, &y_new, &v_erg, &y_erg, nt, m);
-      if (detect_farkas(iter, y_erg)) {
-        res_.status = Status::kInfeasible;
-        done = true;
-      } else if (detect_ray(iter, v, v_new)) {
-        res_.status = Status::kUnbounded;
-        done = true;
-      }
-      if (done) {
-        res_.iterations = iter + 1;
-        res_.primal_residual = current.rel_primal;
-        res_.dual_residual = current.rel_dual;
-        res_.gap = current.rel_gap;
-        v = v_new;
-        y = y_new;
-        break;
-      }
-    }
-
-    // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance.
-    bool force_restart = false;
-    if (fp32 && std::min(worst(current), worst(average)) < opt_.fp32_switch_tol) {
-      logf(opt_, 2, "pdlp: FP32 steps reached %.2e at iter %lld; FP64 steps from here",
-           std::min(worst(current), worst(average)), iter);
-      device->set_precision(SMY_FP64);
-      fp32 = false;
-      force_restart = true;
-    }
-
-    // Adaptive restart and primal weight (src/lp/pdlp.cpp:1075-1132).
-    {
-      const double kkt_current = kkt(current);
-      const double kkt_average = kkt(average);
-      const bool to_average = kkt_average < kkt_current;
-      const double kkt_candidate = std::min(kkt_current, kkt_average);
-      if (!(kkt_at_restart < kInf)) kkt_at_restart = kkt_candidate;
-      const bool restart =
-          force_restart || kkt_candidate <= 0.2 * kkt_at_restart ||
-          (kkt_candidate <= 0.8 * kkt_at_restart && kkt_candidate > kkt_previous) ||
-          static_cast<double>(k_since) >= 0.36 * static_cast<double>(iter + 1);
-      kkt_previous = kkt_candidate;
-      if (restart) {
-        const int which = to_average ? SMY_AVERAGE : SMY_CURRENT;
-        double dx = 0.0, dy = 0.0;
-        device->restart_distance(which, dx, dy);
-        dx = std::sqrt(dx);
-        dy = std::sqrt(dy);
-        if (dx > kWeightDistanceFloor && dy > kWeightDistanceFloor) {
-          omega_ = std::exp(0.5 * std::log(dy / dx) + 0.5 * std::log(omega_));
-          tau = eta / omega_;
-          sigma = eta * omega_;
+      // Termination (src/lp/pdlp.cpp:684-713): strict < tol for the current, then the average.
+      const auto converged = [&](const Measure& e) {
+        return e.rel_primal < opt.tol && e.rel_dual < opt.tol && std::fabs(e.rel_gap) < opt.tol;
+      };
+      if (converged(current) || converged(average)) {
+        const bool cur = converged(current);
+        const Measure& e = cur ? current : average;
+        L.log(2, cur ? ": converged at iter %lld: primal=%.2e dual=%.2e gap=%.2e"
+                     : ": average converged at iter %lld: primal=%.2e dual=%.2e gap=%.2e",
+              iter, e.rel_primal, e.rel_dual, e.rel_gap);
+        L.res.status = Status::kOptimal;
+        L.res.iterations = iter + 1;
+        L.res.primal_residual = e.rel_primal;
+        L.res.dual_residual = e.rel_dual;
+        L.res.gap = e.rel_gap;
+        if (cur) {
+          device->download(i, nullptr, &v_new, nullptr, &y_new, nullptr, nullptr, nt, m);
+          L.map_back(v_new, y_new);
+        } else {
+          device->download(i, nullptr, nullptr, nullptr, nullptr, &v_erg, &y_erg, nt, m);
+          L.map_back(v_erg, y_erg);
         }
-        logf(opt_, 2, "pdlp: restart at iter %lld to the %s (KKT %.2e), omega %.3e", iter,
-             to_average ? "average" : "current iterate", kkt_candidate, omega_);
-        res_.restarts.push_back({iter, to_average ? 1 : 0, kkt_candidate, omega_});
-        device->restart_to(which);
-        k_since = 0;
-        kkt_at_restart = kkt_candidate;
-        kkt_previous = kInf;
+        stop(i);
+        continue;
       }
+
+      // Detection (src/lp/pdlp.cpp:715-1043), at every 8th evaluation after iteration 1000.
+      const bool detect = opt.detect && iter > 1000 && m > 0 &&
+                          ((iter + 1) / kCheckFrequency) % kDetectionEvery == 0;
+      if (detect) {
+        device->download(i, &v, &v_new, nullptr, &y_new, &v_erg, &y_erg, nt, m);
+        bool done = false;
+        if (L.detect_farkas(iter, y_erg)) {
+          L.res.status = Status::kInfeasible;
+          done = true;
+        } else if (L.detect_ray(iter, v, v_new)) {
+          L.res.status = Status::kUnbounded;
+          done = true;
+        }
+        if (done) {
+          L.res.iterations = iter + 1;
+          L.res.primal_residual = current.rel_primal;
+          L.res.dual_residual = current.rel_dual;
+          L.res.gap = current.rel_gap;
+          L.map_back(v_new, y_new);
+          stop(i);
+          continue;
+        }
+      }
+
+      // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance.
+      if (L.fp32 && std::min(worst(current), worst(average)) < opt.fp32_switch_tol) {
+        L.log(2, ": FP32 steps reached %.2e at iter %lld; FP64 steps from here",
+              std::min(worst(current), worst(average)), iter);
+        precision[i] = SMY_FP64;
+        L.fp32 = false;
+        force[i] = 1;
+      }
+
+      // Adaptive restart (src/lp/pdlp.cpp:1075-1132): the decision; the primal weight follows
+      // from the distance below.
+      const double kkt_current = L.kkt(current);
+      const double kkt_average = L.kkt(average);
+      const bool to_average = kkt_average < kkt_current;
+      const double cand = std::min(kkt_current, kkt_average);
+      if (!(L.kkt_at_restart < kInf)) L.kkt_at_restart = cand;
+      const bool restart =
+          force[i] || cand <= 0.2 * L.kkt_at_restart ||
+          (cand <= 0.8 * L.kkt_at_restart && cand > L.kkt_previous) ||
+          static_cast<double>(L.k_since) >= 0.36 * static_cast<double>(iter + 1);
+      L.kkt_previous = cand;
+      if (restart) {
+        which[i] = to_average ? SMY_AVERAGE : SMY_CURRENT;
+        kkt_candidate[i] = cand;
+      }
+    }
+    if (std::any_of(precision.begin(), precision.end(), [](int q) { return q >= 0; }))
+      device->set_precision(precision);
+    if (std::any_of(which.begin(), which.end(), [](int w) { return w >= 0; })) {
+      device->restart_distance(which, dist);
+      for (int i = 0; i < K; ++i) {
+        if (which[i] < 0) continue;
+        Lane& L = lanes[i];
+        const double dx = std::sqrt(dist[2 * i]);
+        const double dy = std::sqrt(dist[2 * i + 1]);
+        if (dx > kWeightDistanceFloor && dy > kWeightDistanceFloor) {
+          L.omega = std::exp(0.5 * std::log(dy / dx) + 0.5 * std::log(L.omega));
+          L.tau = L.eta / L.omega;
+          L.sigma = L.eta * L.omega;
+        }
+        const bool to_average = which[i] == SMY_AVERAGE;
+        L.log(2, ": restart at iter %lld to the %s (KKT %.2e), omega %.3e", iter,
+              to_average ? "average" : "current iterate", kkt_candidate[i], L.omega);
+        L.res.restarts.push_back({iter, to_average ? 1 : 0, kkt_candidate[i], L.omega});
+        L.k_since = 0;
+        L.kkt_at_restart = kkt_candidate[i];
+        L.kkt_previous = kInf;
+      }
+      device->restart_to(which);
     }
     device->advance();
     ++iter;
-  }
-  if (res_.status != Status::kOptimal && res_.status != Status::kInfeasible &&
-      res_.status != Status::kUnbounded) {
-    // A limit stopped the loop: the current point (src/lp/pdlp.cpp:1153-1165).
-    device->download(&v, nullptr, &y, nullptr, nullptr, nullptr, nt, m);
-    if (res_.status == Status::kNumericalError) {
-      res_.status = Status::kIterationLimit;
-      res_.iterations = max_iter;
-      logf(opt_, 1, "pdlp: iteration limit reached after %lld iterations", max_iter);
+    for (int i = 0; i < K; ++i) {
+      Lane& L = lanes[i];
+      if (!L.active || iter < L.max_iter) continue;
+      L.res.status = Status::kIterationLimit;
+      L.res.iterations = L.max_iter;
+      L.log(1, ": iteration limit reached after %lld iterations", L.max_iter);
+      stop_at_current(i);
     }
   }
-  map_back(v, y);
-  res_.solve_seconds = clock_.seconds();
-  return res_;
+  std::vector<PdlpResult> out;
+  out.reserve(static_cast<size_t>(K));
+  for (Lane& L : lanes) out.push_back(std::move(L.res));
+  if (info != nullptr) info->solve_seconds = clock.seconds();
+  return out;
+}
+
+std::vector<PdlpResult> solve_lanes(const std::vector<PdlpTask>& tasks, const PdlpOptions& options,
+                                    PdlpBatchInfo* info) {
+  const Clock clock;
+  const int K = static_cast<int>(tasks.size());
+  try {
+    std::vector<Lane> lanes;
+    lanes.reserve(static_cast<size_t>(K));
+    for (int i = 0; i < K; ++i) {
+      lanes.emplace_back(*tasks[i].lp, options, K == 1 ? std::string("pdlp") : "pdlp[" + std::to_string(i) + "]");
+      lanes.back().start_x = tasks[i].start_x;
+      lanes.back().start_y = tasks[i].start_y;
+    }
+    return run_lanes(lanes, options, clock, info);
+  } catch (const std::exception& e) {
+    // The reference re-solves on the CPU after a GPU failure (src/lp/pdlp.cpp:228-246); this
+    // library reports it.
+    std::vector<PdlpResult> out(static_cast<size_t>(K));
+    for (PdlpResult& r : out) {
+      r.status = Status::kNumericalError;
+      r.message = std::string("the GPU failed during the solve: ") + e.what();
+      r.solve_seconds = clock.seconds();
+    }
+    if (options.log_level >= 1) {
+      std::printf("pdlp: the GPU failed during the solve (%s)\n", e.what());
+      std::fflush(stdout);
+    }
+    return out;
+  }
 }
 
 }  // namespace
 
 PdlpResult solve_pdlp(const LpProblem& lp, const PdlpOptions& options) {
-  try {
-    return Solve(lp, options).run();
-  } catch (const std::exception& e) {
-    // The reference re-solves on the CPU after a GPU failure (src/lp/pdlp.cpp:228-246); this
-    // library reports it.
-    PdlpResult r;
-    r.status = Status::kNumericalError;
-    r.message = std::string("the GPU failed during the solve: ") + e.what();
-    if (options.log_level >= 1) {
-      std::printf("pdlp: the GPU failed during the solve (%s)\n", e.what());
-      std::fflush(stdout);
-    }
-    return r;
-  }
+  return solve_lanes({PdlpTask{&lp, nullptr, nullptr}}, options, nullptr)[0];
+}
+
+std::vector<PdlpResult> solve_pdlp_batch(const std::vector<PdlpTask>& tasks,
+                                         const PdlpOptions& options, PdlpBatchInfo* info) {
+  return solve_lanes(tasks, options, info);
 }
 
 }  // namespace smy::host

@@ -92,6 +92,25 @@ struct PdlpResult {
 
 PdlpResult solve_pdlp(const LpProblem& lp, const PdlpOptions& options);
 
+// ---- Batches (no reference counterpart) ----
+// Many LPs solved together on one device handle: their steps share the kernel launches, every
+// other decision is each LP's own, so every result equals solve_pdlp of that LP alone (same
+// iterates, iterations, restarts and status).
+struct PdlpTask {
+  const LpProblem* lp = nullptr;
+  const std::vector<double>* start_x = nullptr;  // warm start (null: options.start_x)
+  const std::vector<double>* start_y = nullptr;
+};
+
+struct PdlpBatchInfo {
+  double setup_seconds = 0.0;  // host setup of every LP and the upload
+  double solve_seconds = 0.0;  // until the last LP stopped
+  double kernel_ms = 0.0;      // device time of the step runs
+};
+
+std::vector<PdlpResult> solve_pdlp_batch(const std::vector<PdlpTask>& tasks,
+                                         const PdlpOptions& options, PdlpBatchInfo* info = nullptr);
+
 // ---- Model-level path (solve_pdlp_lp, src/core/solver.cpp:392-477) ----
 
 struct VerifyReport {
@@ -139,5 +158,11 @@ struct LpSolveResult {
 };
 
 LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options);
+
+// solve_lp of every model, the PDLP solves of each round batched (polish rounds batch the models
+// that need them).
+std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& models,
+                                          const LpSolveOptions& options,
+                                          PdlpBatchInfo* info = nullptr);
 
 }  // namespace smy::host

@@ -7,9 +7,11 @@
 // -ffp-contract=off).
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pdlp.hpp"
@@ -314,71 +316,144 @@ VerifyReport verify_unbounded_ray(const Model& md, const double* ray, double pri
   return r;
 }
 
-LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options) {
-  // solve_pdlp_lp (src/core/solver.cpp:392-477): scaling, computational form, PDLP, mapping back
-  // (unscale; y <- -sense y since PDLP's reduced costs are c + A'y; d = c - A'y; objective in
-  // double-double) and the verifier.
+namespace {
+
+// One model of solve_lp / solve_lp_batch between its PDLP rounds.
+struct LpJob {
+  const Model* model = nullptr;
+  Scaling scaled;
+  LpProblem lp;
   LpSolveResult out;
-  const Scaling scaled = compute_scaling(model.A);
-  const LpProblem lp = make_problem(model, scaled);
-  PdlpOptions popt = options.pdlp;
+  std::vector<double> start_x, start_y;  // the rejected point a polish round starts from
+  bool open = true;                      // another round is wanted
+};
+
+// Maps one PDLP result back to the model (solve_pdlp_lp, src/core/solver.cpp:392-477: unscale;
+// y <- -sense y since PDLP's reduced costs are c + A'y; d = c - A'y; objective in double-double),
+// runs the verifier and decides whether a polish round follows.
+void absorb(LpJob& job, PdlpResult&& result, int round, const LpSolveOptions& options) {
+  const Model& model = *job.model;
+  LpSolveResult& out = job.out;
+  const Scaling& scaled = job.scaled;
   const int n = model.n, m = model.m;
   const double sense = model.maximize ? -1.0 : 1.0;
-  for (int round = 0;; ++round) {
-    out.pdlp = solve_pdlp(lp, popt);
-    const PdlpResult& r = out.pdlp;
-    out.status = r.status;
-    out.iterations += r.iterations;
-    out.fp32_iterations += r.fp32_iterations;
-    out.rounds = round + 1;
-    out.message = r.message;
-    if (r.x.size() == static_cast<size_t>(n + m)) {
-      out.col_value.assign(r.x.begin(), r.x.begin() + n);
-      out.row_activity.assign(r.x.begin() + n, r.x.end());
-      for (int j = 0; j < n; ++j) out.col_value[j] *= scaled.col[j];
-      for (int i = 0; i < m; ++i) out.row_activity[i] /= scaled.row[i];
-      out.row_dual = r.y;
-      for (int i = 0; i < m; ++i) out.row_dual[i] *= scaled.row[i];
-      for (double& yv : out.row_dual) yv *= -sense;
-      out.col_dual.assign(static_cast<size_t>(n), 0.0);
-      DD obj(model.offset);
-      for (int j = 0; j < n; ++j) {
-        double s = 0.0;
-        for (int64_t p = model.A.start[j]; p < model.A.start[j + 1]; ++p)
-          s += model.A.value[p] * out.row_dual[model.A.index[p]];
-        out.col_dual[j] = model.obj[j] - s;
-        obj += DD::product(model.obj[j], out.col_value[j]);
-      }
-      out.objective = obj.value();
+  out.pdlp = std::move(result);
+  const PdlpResult& r = out.pdlp;
+  out.status = r.status;
+  out.iterations += r.iterations;
+  out.fp32_iterations += r.fp32_iterations;
+  out.rounds = round + 1;
+  out.message = r.message;
+  if (r.x.size() == static_cast<size_t>(n + m)) {
+    out.col_value.assign(r.x.begin(), r.x.begin() + n);
+    out.row_activity.assign(r.x.begin() + n, r.x.end());
+    for (int j = 0; j < n; ++j) out.col_value[j] *= scaled.col[j];
+    for (int i = 0; i < m; ++i) out.row_activity[i] /= scaled.row[i];
+    out.row_dual = r.y;
+    for (int i = 0; i < m; ++i) out.row_dual[i] *= scaled.row[i];
+    for (double& yv : out.row_dual) yv *= -sense;
+    out.col_dual.assign(static_cast<size_t>(n), 0.0);
+    DD obj(model.offset);
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int64_t p = model.A.start[j]; p < model.A.start[j + 1]; ++p)
+        s += model.A.value[p] * out.row_dual[model.A.index[p]];
+      out.col_dual[j] = model.obj[j] - s;
+      obj += DD::product(model.obj[j], out.col_value[j]);
     }
-    VerifyReport rep;
-    bool answered = true;
-    if (r.status == Status::kOptimal) {
-      if (options.verify) rep = verify_lp_optimality(model, out.col_value.data(), out.row_dual.data());
-    } else if (r.status == Status::kInfeasible && !r.dual_ray.empty()) {
-      out.infeasibility_certificate = r.dual_ray;
-      for (int i = 0; i < m; ++i) out.infeasibility_certificate[i] *= scaled.row[i];
-      for (double& t : out.infeasibility_certificate) t *= sense;
-      if (options.verify) rep = verify_infeasibility(model, out.infeasibility_certificate.data());
-    } else if (r.status == Status::kUnbounded && !r.primal_ray.empty()) {
-      out.unbounded_ray = r.primal_ray;
-      for (int j = 0; j < n; ++j) out.unbounded_ray[j] *= scaled.col[j];
-      if (options.verify) rep = verify_unbounded_ray(model, out.unbounded_ray.data());
-    } else {
-      answered = false;
-    }
-    out.verified = answered && options.verify && rep.ok;
-    out.max_primal_violation = std::max(rep.max_bound_violation, rep.max_row_violation);
-    out.max_dual_violation = rep.max_dual_violation;
-    if (answered && !rep.ok) out.message = "PDLP: " + std::string(to_string(r.status)) + " failed verification (" + rep.message + ")";
-    if (out.verified || !answered || r.status != Status::kOptimal || round >= options.polish_rounds) break;
+    out.objective = obj.value();
+  }
+  VerifyReport rep;
+  bool answered = true;
+  if (r.status == Status::kOptimal) {
+    if (options.verify) rep = verify_lp_optimality(model, out.col_value.data(), out.row_dual.data());
+  } else if (r.status == Status::kInfeasible && !r.dual_ray.empty()) {
+    out.infeasibility_certificate = r.dual_ray;
+    for (int i = 0; i < m; ++i) out.infeasibility_certificate[i] *= scaled.row[i];
+    for (double& t : out.infeasibility_certificate) t *= sense;
+    if (options.verify) rep = verify_infeasibility(model, out.infeasibility_certificate.data());
+  } else if (r.status == Status::kUnbounded && !r.primal_ray.empty()) {
+    out.unbounded_ray = r.primal_ray;
+    for (int j = 0; j < n; ++j) out.unbounded_ray[j] *= scaled.col[j];
+    if (options.verify) rep = verify_unbounded_ray(model, out.unbounded_ray.data());
+  } else {
+    answered = false;
+  }
+  out.verified = answered && options.verify && rep.ok;
+  out.max_primal_violation = std::max(rep.max_bound_violation, rep.max_row_violation);
+  out.max_dual_violation = rep.max_dual_violation;
+  if (answered && !rep.ok)
+    out.message = "PDLP: " + std::string(to_string(r.status)) + " failed verification (" + rep.message + ")";
+  job.open = !(out.verified || !answered || r.status != Status::kOptimal || round >= options.polish_rounds);
+  if (job.open) {
     // Polish: the same solve from the rejected point, in FP64, with a tighter tolerance.
+    job.start_x = r.x;
+    job.start_y = r.y;
+  }
+}
+
+template <typename F>
+void for_each_parallel(int count, F&& fn) {
+  const int threads = std::min<int>(count, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  if (threads <= 1) {
+    for (int i = 0; i < count; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&] {
+      for (int i = next++; i < count; i = next++) fn(i);
+    });
+  for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& models,
+                                          const LpSolveOptions& options, PdlpBatchInfo* info) {
+  const int K = static_cast<int>(models.size());
+  std::vector<LpJob> jobs(static_cast<size_t>(K));
+  for_each_parallel(K, [&](int i) {
+    jobs[i].model = models[i];
+    jobs[i].scaled = compute_scaling(models[i]->A);
+    jobs[i].lp = make_problem(*models[i], jobs[i].scaled);
+  });
+  PdlpOptions popt = options.pdlp;
+  for (int round = 0;; ++round) {
+    std::vector<int> open;
+    for (int i = 0; i < K; ++i)
+      if (jobs[i].open) open.push_back(i);
+    if (open.empty()) break;
+    std::vector<PdlpTask> tasks;
+    for (int i : open) {
+      PdlpTask t{&jobs[i].lp, nullptr, nullptr};
+      if (round > 0) {
+        t.start_x = &jobs[i].start_x;
+        t.start_y = &jobs[i].start_y;
+      }
+      tasks.push_back(t);
+    }
+    PdlpBatchInfo round_info;
+    std::vector<PdlpResult> results = solve_pdlp_batch(tasks, popt, &round_info);
+    if (info != nullptr) {
+      info->setup_seconds += round_info.setup_seconds;
+      info->solve_seconds += round_info.solve_seconds;
+      info->kernel_ms += round_info.kernel_ms;
+    }
+    for_each_parallel(static_cast<int>(open.size()),
+                      [&](int q) { absorb(jobs[open[q]], std::move(results[q]), round, options); });
     popt.tol /= 10.0;
     popt.precision = Precision::kFp64;
-    popt.start_x = r.x;
-    popt.start_y = r.y;
   }
+  std::vector<LpSolveResult> out;
+  out.reserve(static_cast<size_t>(K));
+  for (LpJob& j : jobs) out.push_back(std::move(j.out));
   return out;
+}
+
+LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options) {
+  return solve_lp_batch({&model}, options)[0];
 }
 
 }  // namespace smy::host

@@ -116,6 +116,97 @@ class PdhgDevice:
         N.check(self.lib.smy_pdhg_set_start(self.h, _p(_f64(v)), _p(_f64(y))), "set_start")
 
 
+class PdhgBatch:
+    """Many LPs' PDHG state on one device handle (smy_pdhg_create_batch): instance i is the LP of
+    A[i] (m_i x n_i CSC), its transpose At[i] and cost / lower / upper[i] (n_i + m_i). Per-instance
+    arguments are sequences of batch length; `which` entries are -1 (skip), CURRENT or AVERAGE."""
+
+    _PREC = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}
+
+    def __init__(self, h, sizes):
+        self.lib = N.load()
+        self.h = h
+        self.sizes = sizes  # (n_i, m_i)
+        self.size = len(sizes)
+
+    @classmethod
+    def create(cls, A, At, cost, lower, upper, precision: str = "fp64"):
+        lib = N.load()
+        k = len(A)
+        a = (N.SmyCsc * k)()
+        at = (N.SmyCsc * k)()
+        keep = []
+        for i in range(k):
+            a[i], ka = _csc(*A[i])
+            at[i], kt = _csc(*At[i])
+            keep += [ka, kt]
+        vecs = [[_f64(v) for v in vs] for vs in (cost, lower, upper)]
+        ptrs = [(C.c_void_p * k)(*[v.ctypes.data for v in vs]) for vs in vecs]
+        h = C.c_void_p()
+        rc = lib.smy_pdhg_create_batch(k, a, at, ptrs[0], ptrs[1], ptrs[2], cls._PREC[precision],
+                                       C.byref(h))
+        if rc in (N.SMY_NO_DEVICE, N.SMY_TOO_LARGE):
+            return None
+        N.check(rc, "pdhg_create_batch")
+        return cls(h, [(A[i][1], A[i][0]) for i in range(k)])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.smy_pdhg_destroy(self.h)
+            self.h = None
+
+    def set_original(self, i, col_scale, row_scale, cost, lower, upper):
+        N.check(self.lib.smy_pdhg_set_original_at(self.h, i, _p(_f64(col_scale)), _p(_f64(row_scale)),
+                                                  _p(_f64(cost)), _p(_f64(lower)), _p(_f64(upper))),
+                "set_original_at")
+
+    def set_start(self, i, v, y):
+        N.check(self.lib.smy_pdhg_set_start_at(self.h, i, _p(_f64(v)), _p(_f64(y))), "set_start_at")
+
+    def run(self, tau, sigma, theta: float, iter0, count: int):
+        t, s = _f64(tau), _f64(sigma)
+        k0 = np.ascontiguousarray(iter0, np.int64)
+        N.check(self.lib.smy_pdhg_run_batch(self.h, _p(t), _p(s), theta, _p(k0), count), "run_batch")
+
+    def advance(self):
+        N.check(self.lib.smy_pdhg_advance(self.h), "advance")
+
+    def evaluate(self) -> np.ndarray:
+        """(batch, 14): the current iterate's 7 sums, then the average's."""
+        out = np.zeros((self.size, 14))
+        N.check(self.lib.smy_pdhg_evaluate_batch(self.h, _p(out)), "evaluate_batch")
+        return out
+
+    def restart_distance(self, which) -> np.ndarray:
+        w = np.ascontiguousarray(which, np.int32)
+        out = np.zeros((self.size, 2))
+        N.check(self.lib.smy_pdhg_restart_distance_batch(self.h, _p(w), _p(out)),
+                "restart_distance_batch")
+        return out
+
+    def restart_to(self, which):
+        w = np.ascontiguousarray(which, np.int32)
+        N.check(self.lib.smy_pdhg_restart_to_batch(self.h, _p(w)), "restart_to_batch")
+
+    def set_precision(self, precision):
+        """precision[i]: "fp64", "fp32" or -1 (unchanged)."""
+        p = np.array([self._PREC[q] if isinstance(q, str) else -1 for q in precision], np.int32)
+        N.check(self.lib.smy_pdhg_set_precision_batch(self.h, _p(p)), "set_precision_batch")
+
+    def freeze(self, i: int):
+        N.check(self.lib.smy_pdhg_freeze(self.h, i), "freeze")
+
+    def download(self, i: int):
+        """dict of v, v_new, y, y_new, v_erg, y_erg of instance i (host copies)."""
+        n, m = self.sizes[i]
+        out = {k: np.zeros(n + m) for k in ("v", "v_new", "v_erg")}
+        out.update({k: np.zeros(m) for k in ("y", "y_new", "y_erg")})
+        N.check(self.lib.smy_pdhg_download_at(self.h, i, _p(out["v"]), _p(out["v_new"]), _p(out["y"]),
+                                              _p(out["y_new"]), _p(out["v_erg"]), _p(out["y_erg"])),
+                "download_at")
+        return out
+
+
 def spmv_threads_per_row(nnz: int, rows: int) -> int:
     return int(N.load().smy_spmv_threads_per_row(nnz, rows))
 

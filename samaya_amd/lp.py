@@ -306,17 +306,53 @@ def solve_lp(model: Model | NativeModel, options: PdlpOptions | None = None) -> 
     res = N.SmyLpResult()
     params = options.native()
     lib.smy_solve_lp(nm.h, C.byref(params), C.byref(res), _p(x), _p(yd), _p(d), _p(act), _p(cert))
+    return _lp_result(res, x, yd, d, act, cert)
+
+
+def _lp_result(res: "N.SmyLpResult", x, yd, d, act, cert) -> LpResult:
     st = res.status
+    m, n = len(yd), len(x)
     return LpResult(
         status=st, verified=bool(res.verified), objective=res.objective, col_value=x,
         row_dual=yd, col_dual=d, row_activity=act,
-        certificate=cert[:m] if st == INFEASIBLE else cert[:n] if st == UNBOUNDED else None,
+        certificate=(None if cert is None else cert[:m] if st == INFEASIBLE
+                     else cert[:n] if st == UNBOUNDED else None),
         max_primal_violation=res.max_primal_violation, max_dual_violation=res.max_dual_violation,
         iterations=res.iterations, fp32_iterations=res.fp32_iterations, rounds=res.rounds,
         primal_residual=res.pdlp.primal_residual, dual_residual=res.pdlp.dual_residual,
         gap=res.pdlp.gap, setup_seconds=res.pdlp.setup_seconds,
         solve_seconds=res.pdlp.solve_seconds, kernel_ms=res.pdlp.kernel_ms,
         message=res.message.decode(errors="replace"))
+
+
+@dataclass
+class BatchTimes:
+    setup_seconds: float   # host setup of every LP and the upload, summed over rounds
+    solve_seconds: float   # setup included, summed over rounds
+    kernel_ms: float       # device time of the step runs
+
+
+def solve_lp_batch(models, options: PdlpOptions | None = None
+                   ) -> tuple[list[LpResult], BatchTimes]:
+    """solve_lp of every model, their PDLP iterations batched on one device handle: each result
+    equals solve_lp of that model alone (no reference counterpart; the reference solves one LP at
+    a time). col_dual, row_activity and certificates are not returned by the batch."""
+    options = options or PdlpOptions()
+    nms = [m if isinstance(m, NativeModel) else m.native() for m in models]
+    lib = N.load()
+    k = len(nms)
+    xs = [np.zeros(nm.n) for nm in nms]
+    ys = [np.zeros(nm.m) for nm in nms]
+    hs = (C.c_void_p * k)(*[nm.h for nm in nms])
+    xp = (C.c_void_p * k)(*[x.ctypes.data for x in xs])
+    yp = (C.c_void_p * k)(*[y.ctypes.data for y in ys])
+    res = (N.SmyLpResult * k)()
+    times = np.zeros(3)
+    params = options.native()
+    rc = lib.smy_solve_lp_batch(hs, k, C.byref(params), res, xp, yp, _p(times))
+    N.check(rc, "solve_lp_batch")
+    out = [_lp_result(res[i], xs[i], ys[i], None, None, None) for i in range(k)]
+    return out, BatchTimes(float(times[0]), float(times[1]), float(times[2]))
 
 
 def verify_lp_optimality(model: Model | NativeModel, x, y, primal_tol=1e-6, dual_tol=1e-6):

@@ -151,6 +151,44 @@ def test_gpu_paths_fail_loudly_without_gpu():
                              np.zeros(2), np.zeros(2), np.ones(2)) is None
     built, dev = smy.gpu_info()
     assert built and dev == ""
+    batch, _ = smy.solve_lp_batch([smy.generators.sparse_uniform(5, 10, 20, 3)] * 2)
+    assert all(r.status == smy.NUMERICAL_ERROR and "no usable GPU" in r.message for r in batch)
+    from samaya_amd.device import PdhgBatch
+    one = (1, 1, np.array([0, 1]), np.array([0], np.int32), np.array([1.0]))
+    assert PdhgBatch.create([one, one], [one, one], [np.zeros(2)] * 2, [np.zeros(2)] * 2,
+                            [np.ones(2)] * 2) is None
+
+
+def test_blending_generator_structure():
+    # The batched workload's LPs: 20 units x 30 stream tanks per period, periods U{4..48},
+    # qualities U{1..30}; rows = periods x (600 tank balances + 20 unit balances + 20 capacities
+    # + 20 x qualities), columns = periods x (3 x 600 + 20).
+    sizes = set()
+    for i in range(12):
+        md = smy.generators.blending(4000 + i)
+        per = md.n // 1820
+        assert md.n == 1820 * per and 4 <= per <= 48
+        q = (md.m // per - 640) // 20
+        assert md.m == per * (640 + 20 * q) and 1 <= q <= 30
+        sizes.add((per, q))
+        # Tank balances and unit balances are equalities, the rest <=; 0 <= x <= 10.
+        eq = md.row_lower == md.row_upper
+        assert eq.sum() == per * 620 and np.all(np.isneginf(md.row_lower[~eq]))
+        assert np.all(md.col_lower == 0) and np.all(md.col_upper == 10)
+        # Quality and capacity coefficients in [0.5, 1.5]; balance coefficients +-1; the quality
+        # limits on the outputs in [-1.3, -0.9].
+        v = np.abs(md.value)
+        assert np.all(((v >= 0.5) & (v <= 1.5)))
+    assert len(sizes) == 12  # the sizes vary with the seed
+    a, b = smy.generators.blending(4007), smy.generators.blending(4007)
+    assert np.array_equal(a.value, b.value) and np.array_equal(a.row_upper, b.row_upper)
+    small = smy.generators.blending(4000, periods=4, qualities=1)
+    assert small.m == 4 * 660 and small.n == 4 * 1820
+    # Inventories link consecutive periods: the period-1 tank balances touch period-0 columns.
+    t1_row = 660  # first tank balance of period 1
+    cols = np.repeat(np.arange(small.n), np.diff(small.start))
+    touched = cols[small.index == t1_row]
+    assert touched.min() < 1820 <= touched.max()
 
 
 def test_cpu_iterations_are_refused():
