@@ -114,6 +114,17 @@ int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal
 int smy_pdhg_create_batch(int count, const smy_csc* A, const smy_csc* At,
                           const double* const* cost, const double* const* lower,
                           const double* const* upper, int precision, smy_pdhg** out);
+/* The shared-matrix batch: `count` LPs with one matrix A (m x n CSC, At its transpose) and one
+ * cost (n + m), each with its own bounds lower[i] / upper[i] (n + m; the m row entries must be
+ * equal for all instances: SMY_INVALID otherwise). Branch-and-bound node relaxations of one LP.
+ * Every iterate is stored instance-minor (row-major variables x instances), so each product with
+ * A or A' is one SpMM for the whole batch: the matrix is read once per step, and every gather
+ * reads the contiguous values of all instances. Driven by the batch functions below; the
+ * instances' sums are taken in another order than by smy_pdhg_create_batch (equal up to
+ * rounding, bit-identical for identical instances). */
+int smy_pdhg_create_shared(const smy_csc* A, const smy_csc* At, int count, const double* cost,
+                           const double* const* lower, const double* const* upper, int precision,
+                           smy_pdhg** out);
 /* Number of instances (0 for NULL). */
 int smy_pdhg_batch_size(smy_pdhg* h);
 /* set_original / set_start / download for instance i (vectors of that instance's sizes). */

@@ -47,7 +47,9 @@ enum {
                                    p3 (max realized), values N(0,1) x 10^U[-2,2] */
   SMY_GEN_BLENDING = 3,         /* multi-period blending LP: p0 periods (0: U{4..48}), p1 units,
                                    p2 streams, p3 qualities (0: U{1..30}); staircase structure */
-  SMY_GEN_KNAPSACK = 4          /* multi-dimensional knapsack LP, p0 rows, p1 cols, p2 nnz/col */
+  SMY_GEN_KNAPSACK = 4          /* multi-dimensional knapsack LP: maximize profit'x (U{1..100}),
+                                   p0 rows W x <= 0.4 row sums, p1 columns in [0, 1], p2 weights
+                                   U{1..100} per column at distinct uniform rows */
 };
 
 typedef struct smy_model smy_model;
@@ -91,6 +93,7 @@ typedef struct smy_pdlp_params {
   int verify;                /* model-level solve: run the verifier */
   int polish_rounds;         /* model-level solve: FP64 continuations with tol/10 if rejected */
   int algorithm;             /* SMY_ALGORITHM_REFERENCE (the reference's decisions) or _ENHANCED */
+  int share_matrix;          /* batches of one model: run them as a shared-matrix batch (1) */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);
@@ -156,6 +159,24 @@ int smy_solve_lp(const smy_model* model, const smy_pdlp_params* params, smy_lp_r
 int smy_solve_lp_batch(const smy_model* const* models, int count, const smy_pdlp_params* params,
                        smy_lp_result* results, double* const* col_values,
                        double* const* row_duals, double* times);
+
+/* Node relaxations of one model (no reference counterpart): smy_solve_lp of `model` with column
+ * bounds col_lower[i] / col_upper[i] (n each) for i < count, in one batch on one shared matrix
+ * (the shared-matrix batch unless params->share_matrix is 0). start_col_value (n) and
+ * start_row_dual (m, d = c - A'y convention) may be NULL; when given they warm-start every
+ * relaxation (typically the root LP's answer). Outputs as smy_solve_lp_batch. */
+int smy_solve_lp_bounds_batch(const smy_model* model, int count, const double* const* col_lower,
+                              const double* const* col_upper, const smy_pdlp_params* params,
+                              const double* start_col_value, const double* start_row_dual,
+                              smy_lp_result* results, double* const* col_values,
+                              double* const* row_duals, double* times);
+
+/* A branch-and-bound bound vector (deterministic, seeded): a fraction U[frac_min, frac_max] of
+ * the n columns, distinct and uniform, fixed at their lower or upper bound (even odds); the rest
+ * keep lower / upper. */
+int smy_generate_fixings(uint64_t seed, int n, double frac_min, double frac_max,
+                         const double* lower, const double* upper, double* out_lower,
+                         double* out_upper);
 
 /* samaya_solve (include/samaya_c.h:41) for this library: LP by PDLP on the GPU with default
  * parameters. Returns samaya's Status code (1 optimal, 2 infeasible, 3 unbounded, 5 time limit,

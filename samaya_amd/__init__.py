@@ -6,7 +6,8 @@ include/samaya_pdlp.h); it has no compute path of its own.
 """
 from ._native import LIB_PATH, load, last_error, NativeError
 from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, PdlpOptions,
-                 PdlpResult, LpResult, BatchTimes, solve_pdlp, solve_lp, solve_lp_batch, solve, verify_lp_optimality, verify_infeasibility,
+                 PdlpResult, LpResult, BatchTimes, solve_pdlp, solve_lp, solve_lp_batch,
+                 solve_lp_bounds_batch, solve, verify_lp_optimality, verify_infeasibility,
                  verify_unbounded_ray, gpu_available, gpu_info, STATUS_NAMES, OPTIMAL,
                  INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR)
 from . import generators
@@ -15,7 +16,7 @@ __all__ = [
     "LIB_PATH", "load", "last_error", "NativeError", "Model", "NativeModel", "LpProblem",
     "compute_scaling", "make_problem",
     "PdlpOptions", "PdlpResult", "LpResult", "BatchTimes", "solve_pdlp", "solve_lp",
-    "solve_lp_batch", "solve",
+    "solve_lp_batch", "solve_lp_bounds_batch", "solve",
     "verify_lp_optimality", "verify_infeasibility", "verify_unbounded_ray", "gpu_available",
     "gpu_info", "STATUS_NAMES", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "ITERATION_LIMIT",
     "TIME_LIMIT", "NUMERICAL_ERROR", "generators",

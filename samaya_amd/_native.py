@@ -27,13 +27,14 @@ EXPORTS = (
     "smy_pdhg_create_batch", "smy_pdhg_batch_size", "smy_pdhg_set_original_at",
     "smy_pdhg_set_start_at", "smy_pdhg_download_at", "smy_pdhg_run_batch",
     "smy_pdhg_evaluate_batch", "smy_pdhg_restart_distance_batch", "smy_pdhg_restart_to_batch",
-    "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
     "smy_model_num_cols", "smy_model_num_nonzeros", "smy_model_export", "smy_compute_scaling",
     "smy_make_problem", "smy_pdlp_params_default",
-    "smy_solve_pdlp", "smy_solve_lp", "smy_solve_lp_batch", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
+    "smy_solve_pdlp", "smy_solve_lp", "smy_solve_lp_batch", "smy_solve_lp_bounds_batch",
+    "smy_generate_fixings", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
     "smy_verify_unbounded_ray", "smy_gpu_info",
 )
 
@@ -47,7 +48,8 @@ class SmyPdlpParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iterations", C.c_longlong), ("time_limit", C.c_double),
                 ("use_gpu", C.c_int), ("precision", C.c_int), ("fp32_switch_tol", C.c_double),
                 ("use_graph", C.c_int), ("detect", C.c_int), ("log_level", C.c_int),
-                ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int)]
+                ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int),
+                ("share_matrix", C.c_int)]
 
 
 class SmyPdlpInfo(C.Structure):
@@ -129,6 +131,8 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_restart_to_batch", i32, vp, vp)
     _proto(lib, "smy_pdhg_set_precision_batch", i32, vp, vp)
     _proto(lib, "smy_pdhg_freeze", i32, vp, i32)
+    _proto(lib, "smy_pdhg_create_shared", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), i32, vp, vp,
+           vp, i32, C.POINTER(vp))
     _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)
     _proto(lib, "smy_csr_spmv_f32", i32, i32, vp, vp, vp, vp, vp, vp)
     _proto(lib, "smy_spmv_threads_per_row", i32, i64, i32)
@@ -154,6 +158,9 @@ def load() -> C.CDLL:
            vp, vp, vp)
     _proto(lib, "smy_solve_lp_batch", i32, vp, i32, C.POINTER(SmyPdlpParams),
            C.POINTER(SmyLpResult), vp, vp, vp)
+    _proto(lib, "smy_solve_lp_bounds_batch", i32, vp, i32, vp, vp, C.POINTER(SmyPdlpParams), vp,
+           vp, C.POINTER(SmyLpResult), vp, vp, vp)
+    _proto(lib, "smy_generate_fixings", i32, C.c_uint64, i32, f64, f64, vp, vp, vp, vp)
     _proto(lib, "smy_solve", i32, vp, C.POINTER(f64))
     _proto(lib, "smy_verify_lp", i32, vp, vp, vp, f64, f64, vp, C.c_char_p, C.c_size_t)
     _proto(lib, "smy_verify_infeasibility", i32, vp, vp, C.c_char_p, C.c_size_t)

@@ -567,7 +567,7 @@ struct ConvertOp {
 // Per-instance totals of block partials, in a fixed order. Workgroup (q, i * parts + part) sums
 // partial[q * stride + b] over b in ranges[i * parts + part] = [first, last) and writes
 // out[(i * parts + part) * nsums + q].
-__global__ __launch_bounds__(kBlock) void finish_ranges_kernel(int stride, int nsums,
+static __global__ __launch_bounds__(kBlock) void finish_ranges_kernel(int stride, int nsums,
                                                                const double* __restrict__ partial,
                                                                const int2* __restrict__ ranges,
                                                                double* __restrict__ out) {

@@ -25,219 +25,10 @@
 // aty vectors disappear. An instance's arithmetic does not depend on the rest of its batch: a
 // batch of K gives each LP the bits it gets alone.
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <climits>
-#include <cstring>
-#include <memory>
-#include <string>
-#include <vector>
-
-#include "../../../include/samaya_pdhg.h"
-#include "csr_kernels.hpp"
-#include "hip_common.hpp"
-#include "row_blocks.hpp"
+#include "pdhg_handle.hpp"
 
 namespace smy {
 namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string& what) {
-  g_last_error = what;
-  return code;
-}
-
-template <typename F>
-int guarded(F&& f) {
-  try {
-    g_last_error.clear();
-    return f();
-  } catch (const HipError& e) {
-    return fail(SMY_ERROR, e.what());
-  } catch (const std::exception& e) {
-    return fail(SMY_ERROR, e.what());
-  } catch (...) {
-    return fail(SMY_ERROR, "unknown error");
-  }
-}
-
-bool device_is_gfx950(int device) {
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return false;
-  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
-}
-
-int blocks_for(long long count) { return static_cast<int>((count + kBlock - 1) / kBlock); }
-
-constexpr int kFrozen = -1;
-constexpr int kMaxBatch = 16384;  // instances per handle (grid y of the per-instance reductions)
-
-struct DevMatrix {
-  int rows = 0;
-  long long nnz = 0;
-  int* ptr = nullptr;
-  int* idx = nullptr;
-  double* val64 = nullptr;
-  float* val32 = nullptr;
-  int2* meta = nullptr;   // row-block schedule: nblocks + 1 (first row, first nonzero)
-  int* binst = nullptr;   // instance of each block
-  int nblocks = 0;
-  int* list[2] = {nullptr, nullptr};  // blocks of the instances stepping in FP64 / FP32
-  int nlist[2] = {0, 0};
-};
-
-template <typename T>
-struct Iter {
-  T* v[2] = {nullptr, nullptr};
-  T* y[2] = {nullptr, nullptr};
-  T* vbar[2] = {nullptr, nullptr};
-  T* v_erg = nullptr;
-  T* y_erg = nullptr;
-  T* v_rst = nullptr;
-  T* y_rst = nullptr;
-  T* cost = nullptr;
-  T* lower = nullptr;
-  T* upper = nullptr;
-  bool ready = false;
-};
-
-struct Layout {
-  int n = 0, m = 0;
-  int ox = 0, oy = 0;    // first structural variable and first row in the stack
-  int ba0 = 0, ba1 = 0;  // its blocks in the A schedule
-  int bt0 = 0, bt1 = 0;  // its blocks in the A' schedule
-  int prec = SMY_FP64;   // precision of its steps, or kFrozen
-  int last_prec = SMY_FP64;  // precision of its state (also when frozen)
-  bool has_original = false;
-};
-
-template <typename T>
-constexpr int prec_of() {
-  return std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
-}
-
-// Returns the grid used (0: nothing launched).
-template <typename V, typename Op>
-int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, const Op& op,
-                hipStream_t s) {
-  if (nlist == 0) return 0;
-  CsrRows<V> A;
-  A.rows = M.rows;
-  A.nblocks = M.nblocks;
-  A.ptr = M.ptr;
-  A.idx = M.idx;
-  A.val = val;
-  A.meta = M.meta;
-  A.binst = M.binst;
-  A.list = list;
-  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(nlist), dim3(kBlock), 0, s, A, op);
-  return nlist;
-}
-
-}  // namespace
-}  // namespace smy
-
-using namespace smy;
-
-struct smy_pdhg {
-  int K = 1;
-  int n = 0, m = 0, nt = 0;
-  std::vector<Layout> inst;
-  DevMatrix a, at;  // A by rows, A' by rows
-  Iter<double> d;
-  Iter<float> f;
-  int p = 0;                 // parity: v = v[p], v_new = v[1 - p] (same for y, vbar)
-  double theta_vbar = 1.0;   // theta the current vbar was formed with
-  bool vbar_any = true;      // vbar == v (valid for every theta): start, restart
-  bool stepped = false;      // a step since the last advance / restart_to
-  double* cs = nullptr;
-  double* rs = nullptr;
-  double* cost_o = nullptr;
-  double* lower_o = nullptr;
-  double* upper_o = nullptr;
-  double* partial = nullptr;  // 14 x (a.nblocks + at.nblocks): block b of A', column a.nblocks + b
-  int2* ranges = nullptr;     // per instance: its A blocks, then its A' blocks (partial columns)
-  double* sums_dev = nullptr;   // K x 2 x 14
-  double* sums_host = nullptr;  // pinned
-  StepScalars* scalars = nullptr;
-  StepScalars* scalars_host = nullptr;  // pinned staging
-  int* flags = nullptr;                 // 2 x K per-instance flags (by precision)
-  int* flags_host = nullptr;            // pinned staging
-  hipEvent_t ev_staged = nullptr;       // the last copy out of the pinned staging buffers
-  bool staged = false;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  bool timed = false;
-  // Captured runs of kGraphSteps steps, by (precisions in use, parity); valid for one generation
-  // of the block lists.
-  static constexpr int kGraphSteps = 64;
-  struct Captured {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int gen = -1;
-  };
-  Captured graphs[4][2];
-  int list_gen = 0;
-  bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
-  std::vector<void*> allocations;
-
-  template <typename T>
-  T* alloc(long long count) {
-    void* ptr = nullptr;
-    SMY_HIP(hipMalloc(&ptr, static_cast<size_t>(std::max<long long>(count, 1)) * sizeof(T)));
-    allocations.push_back(ptr);
-    return static_cast<T*>(ptr);
-  }
-  template <typename T>
-  T* upload(const T* host, long long count) {
-    T* dev = alloc<T>(count);
-    if (count > 0) SMY_HIP(hipMemcpy(dev, host, static_cast<size_t>(count) * sizeof(T), hipMemcpyHostToDevice));
-    return dev;
-  }
-  template <typename T>
-  T* zeros(long long count) {
-    T* dev = alloc<T>(count);
-    SMY_HIP(hipMemsetAsync(dev, 0, static_cast<size_t>(std::max<long long>(count, 1)) * sizeof(T), stream));
-    return dev;
-  }
-  void drop_graph(Captured& c) {
-    if (c.exec) (void)hipGraphExecDestroy(c.exec);
-    if (c.graph) (void)hipGraphDestroy(c.graph);
-    c = Captured{};
-  }
-
-  ~smy_pdhg() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (auto& row : graphs)
-      for (auto& c : row) drop_graph(c);
-    for (void* ptr : allocations) (void)hipFree(ptr);
-    if (sums_host) (void)hipHostFree(sums_host);
-    if (scalars_host) (void)hipHostFree(scalars_host);
-    if (flags_host) (void)hipHostFree(flags_host);
-    if (ev_staged) (void)hipEventDestroy(ev_staged);
-    if (ev_start) (void)hipEventDestroy(ev_start);
-    if (ev_stop) (void)hipEventDestroy(ev_stop);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
-namespace smy {
-namespace {
-
-// The pinned staging buffers may be rewritten once the copy that last read them has run.
-void staging_wait(smy_pdhg* h) {
-  if (h->staged) {
-    SMY_HIP(hipEventSynchronize(h->ev_staged));
-    h->staged = false;
-  }
-}
-
-void staging_sent(smy_pdhg* h) {
-  SMY_HIP(hipEventRecord(h->ev_staged, h->stream));
-  h->staged = true;
-}
 
 // The stacked matrix by rows from each instance's CSC of the transpose (`rows_of` = the CSC whose
 // columns are the rows), with the column indices moved by the instances' offsets, and its
@@ -1014,7 +805,8 @@ int smy_pdhg_batch_size(smy_pdhg* h) { return h == nullptr ? 0 : h->K; }
 #define SMY_SINGLE(h)                                                                   \
   do {                                                                                  \
     if ((h) == nullptr) return fail(SMY_INVALID, "null handle");                        \
-    if (!single(h)) return fail(SMY_STATE, "a batch handle needs the batch functions"); \
+    if (!single(h) || (h)->shared != nullptr)                                           \
+      return fail(SMY_STATE, "a batch handle needs the batch functions");               \
   } while (0)
 
 #define SMY_INSTANCE(h, i)                                                   \
@@ -1033,6 +825,13 @@ int smy_pdhg_set_original_at(smy_pdhg* h, int instance, const double* col_scale,
                              const double* row_scale, const double* cost, const double* lower,
                              const double* upper) {
   SMY_INSTANCE(h, instance);
+  if (h->shared) {
+    const int n = h->n, m = h->m;
+    if ((n > 0 && col_scale == nullptr) || (m > 0 && row_scale == nullptr) ||
+        (n + m > 0 && (cost == nullptr || lower == nullptr || upper == nullptr)))
+      return fail(SMY_INVALID, "null argument");
+    return guarded([&]() -> int { return shared_set_original(h, instance, col_scale, row_scale, cost, lower, upper); });
+  }
   return set_original_impl(h, instance, col_scale, row_scale, cost, lower, upper);
 }
 
@@ -1050,12 +849,14 @@ int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, doub
                        const long long* iter0, int count) {
   if (h == nullptr || tau == nullptr || sigma == nullptr || iter0 == nullptr)
     return fail(SMY_INVALID, "null argument");
+  if (h->shared) return guarded([&]() -> int { return shared_run(h, tau, sigma, theta, iter0, count); });
   return guarded([&]() -> int { return run_impl(h, tau, sigma, theta, iter0, count); });
 }
 
 int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   if (h == nullptr || dual_ms == nullptr || primal_ms == nullptr || reps <= 0)
     return fail(SMY_INVALID, "bad argument");
+  if (h->shared) return guarded([&]() -> int { return shared_time_kernels(h, reps, dual_ms, primal_ms); });
   return guarded([&]() -> int {
     refresh_lists(h);
     if (uses(h, SMY_FP32)) return time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
@@ -1111,6 +912,8 @@ int smy_pdhg_download(smy_pdhg* h, double* v, double* v_new, double* y, double* 
 int smy_pdhg_download_at(smy_pdhg* h, int instance, double* v, double* v_new, double* y,
                          double* y_new, double* v_erg, double* y_erg) {
   SMY_INSTANCE(h, instance);
+  if (h->shared)
+    return guarded([&]() -> int { return shared_download(h, instance, v, v_new, y, y_new, v_erg, y_erg); });
   return guarded([&]() -> int {
     return h->inst[instance].last_prec == SMY_FP64
                ? download_impl<double>(h, instance, v, v_new, y, y_new, v_erg, y_erg)
@@ -1140,6 +943,7 @@ int smy_pdhg_evaluate_both(smy_pdhg* h, double out[14]) {
 
 int smy_pdhg_evaluate_batch(smy_pdhg* h, double* out) {
   if (h == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->shared) return guarded([&]() -> int { return shared_evaluate(h, out); });
   return guarded([&]() -> int { return evaluate_batch_impl(h, out); });
 }
 
@@ -1157,6 +961,7 @@ int smy_pdhg_restart_distance(smy_pdhg* h, int which, double* dv_sq, double* dy_
 
 int smy_pdhg_restart_distance_batch(smy_pdhg* h, const int* which, double* out) {
   if (h == nullptr || which == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->shared) return guarded([&]() -> int { return shared_distance(h, which, out); });
   return guarded([&]() -> int { return distance_batch_impl(h, which, out); });
 }
 
@@ -1167,6 +972,7 @@ int smy_pdhg_restart_to(smy_pdhg* h, int which) {
 
 int smy_pdhg_restart_to_batch(smy_pdhg* h, const int* which) {
   if (h == nullptr || which == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->shared) return guarded([&]() -> int { return shared_restart_to(h, which); });
   return guarded([&]() -> int { return restart_to_batch_impl(h, which); });
 }
 
@@ -1181,11 +987,13 @@ int smy_pdhg_set_precision_batch(smy_pdhg* h, const int* precision) {
   for (int i = 0; i < h->K; ++i)
     if (precision[i] != -1 && precision[i] != SMY_FP64 && precision[i] != SMY_FP32)
       return fail(SMY_INVALID, "bad precision");
+  if (h->shared) return guarded([&]() -> int { return shared_set_precision(h, precision); });
   return guarded([&]() -> int { return set_precision_batch_impl(h, precision); });
 }
 
 int smy_pdhg_freeze(smy_pdhg* h, int instance) {
   SMY_INSTANCE(h, instance);
+  if (h->shared) return shared_freeze(h, instance);
   if (h->inst[instance].prec == kFrozen) return SMY_OK;
   h->inst[instance].prec = kFrozen;
   h->lists_dirty = true;
@@ -1201,6 +1009,7 @@ int smy_pdhg_set_start_at(smy_pdhg* h, int instance, const double* v, const doub
   SMY_INSTANCE(h, instance);
   const Layout& L = h->inst[instance];
   if ((v == nullptr && L.n + L.m > 0) || (y == nullptr && L.m > 0)) return fail(SMY_INVALID, "null argument");
+  if (h->shared) return guarded([&]() -> int { return shared_set_start(h, instance, v, y); });
   return guarded([&]() -> int {
     if (L.last_prec == SMY_FP64) set_start_impl<double>(h, instance, v, y);
     else set_start_impl<float>(h, instance, v, y);

@@ -1,0 +1,241 @@
+// The device handle behind include/samaya_pdhg.h, shared by the two device sources: the stacked
+// batch (pdhg_device.hip: any LPs, block-diagonal) and the shared-matrix batch (pdhg_shared.hip:
+// one A and cost, per-instance column bounds, iterates stored instance-minor for SpMM products).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/samaya_pdhg.h"
+#include "csr_kernels.hpp"
+#include "hip_common.hpp"
+#include "row_blocks.hpp"
+
+namespace smy {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string& what) {
+  g_last_error = what;
+  return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    return f();
+  } catch (const HipError& e) {
+    return fail(SMY_ERROR, e.what());
+  } catch (const std::exception& e) {
+    return fail(SMY_ERROR, e.what());
+  } catch (...) {
+    return fail(SMY_ERROR, "unknown error");
+  }
+}
+
+inline bool device_is_gfx950(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return false;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+inline int blocks_for(long long count) { return static_cast<int>((count + kBlock - 1) / kBlock); }
+
+constexpr int kFrozen = -1;
+constexpr int kMaxBatch = 16384;  // instances per handle (grid y of the per-instance reductions)
+
+struct DevMatrix {
+  int rows = 0;
+  long long nnz = 0;
+  int* ptr = nullptr;
+  int* idx = nullptr;
+  double* val64 = nullptr;
+  float* val32 = nullptr;
+  int2* meta = nullptr;   // row-block schedule: nblocks + 1 (first row, first nonzero)
+  int* binst = nullptr;   // instance of each block
+  int nblocks = 0;
+  int* list[2] = {nullptr, nullptr};  // blocks of the instances stepping in FP64 / FP32
+  int nlist[2] = {0, 0};
+};
+
+template <typename T>
+struct Iter {
+  T* v[2] = {nullptr, nullptr};
+  T* y[2] = {nullptr, nullptr};
+  T* vbar[2] = {nullptr, nullptr};
+  T* v_erg = nullptr;
+  T* y_erg = nullptr;
+  T* v_rst = nullptr;
+  T* y_rst = nullptr;
+  T* cost = nullptr;
+  T* lower = nullptr;
+  T* upper = nullptr;
+  bool ready = false;
+};
+
+struct Layout {
+  int n = 0, m = 0;
+  int ox = 0, oy = 0;    // first structural variable and first row in the stack
+  int ba0 = 0, ba1 = 0;  // its blocks in the A schedule
+  int bt0 = 0, bt1 = 0;  // its blocks in the A' schedule
+  int prec = SMY_FP64;   // precision of its steps, or kFrozen
+  int last_prec = SMY_FP64;  // precision of its state (also when frozen)
+  bool has_original = false;
+};
+
+template <typename T>
+constexpr int prec_of() {
+  return std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
+}
+
+// Returns the grid used (0: nothing launched).
+template <typename V, typename Op>
+int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, const Op& op,
+                hipStream_t s) {
+  if (nlist == 0) return 0;
+  CsrRows<V> A;
+  A.rows = M.rows;
+  A.nblocks = M.nblocks;
+  A.ptr = M.ptr;
+  A.idx = M.idx;
+  A.val = val;
+  A.meta = M.meta;
+  A.binst = M.binst;
+  A.list = list;
+  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(nlist), dim3(kBlock), 0, s, A, op);
+  return nlist;
+}
+
+struct SharedBatch;
+void shared_free(SharedBatch* s);
+
+}  // namespace smy
+
+using namespace smy;
+
+struct smy_pdhg {
+  int K = 1;
+  int n = 0, m = 0, nt = 0;
+  std::vector<Layout> inst;
+  DevMatrix a, at;  // A by rows, A' by rows
+  Iter<double> d;
+  Iter<float> f;
+  int p = 0;                 // parity: v = v[p], v_new = v[1 - p] (same for y, vbar)
+  double theta_vbar = 1.0;   // theta the current vbar was formed with
+  bool vbar_any = true;      // vbar == v (valid for every theta): start, restart
+  bool stepped = false;      // a step since the last advance / restart_to
+  double* cs = nullptr;
+  double* rs = nullptr;
+  double* cost_o = nullptr;
+  double* lower_o = nullptr;
+  double* upper_o = nullptr;
+  double* partial = nullptr;  // 14 x (a.nblocks + at.nblocks): block b of A', column a.nblocks + b
+  int2* ranges = nullptr;     // per instance: its A blocks, then its A' blocks (partial columns)
+  double* sums_dev = nullptr;   // K x 2 x 14
+  double* sums_host = nullptr;  // pinned
+  StepScalars* scalars = nullptr;
+  StepScalars* scalars_host = nullptr;  // pinned staging
+  int* flags = nullptr;                 // 2 x K per-instance flags (by precision)
+  int* flags_host = nullptr;            // pinned staging
+  hipEvent_t ev_staged = nullptr;       // the last copy out of the pinned staging buffers
+  bool staged = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bool timed = false;
+  // Captured runs of kGraphSteps steps, by (precisions in use, parity); valid for one generation
+  // of the block lists.
+  static constexpr int kGraphSteps = 64;
+  struct Captured {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int gen = -1;
+  };
+  Captured graphs[4][2];
+  int list_gen = 0;
+  bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
+  SharedBatch* shared = nullptr;  // the shared-matrix batch (then the stacked fields are unused)
+  std::vector<void*> allocations;
+
+  template <typename T>
+  T* alloc(long long count) {
+    void* ptr = nullptr;
+    SMY_HIP(hipMalloc(&ptr, static_cast<size_t>(std::max<long long>(count, 1)) * sizeof(T)));
+    allocations.push_back(ptr);
+    return static_cast<T*>(ptr);
+  }
+  template <typename T>
+  T* upload(const T* host, long long count) {
+    T* dev = alloc<T>(count);
+    if (count > 0) SMY_HIP(hipMemcpy(dev, host, static_cast<size_t>(count) * sizeof(T), hipMemcpyHostToDevice));
+    return dev;
+  }
+  template <typename T>
+  T* zeros(long long count) {
+    T* dev = alloc<T>(count);
+    SMY_HIP(hipMemsetAsync(dev, 0, static_cast<size_t>(std::max<long long>(count, 1)) * sizeof(T), stream));
+    return dev;
+  }
+  void drop_graph(Captured& c) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c = Captured{};
+  }
+
+  ~smy_pdhg() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (shared) shared_free(shared);
+    for (auto& row : graphs)
+      for (auto& c : row) drop_graph(c);
+    for (void* ptr : allocations) (void)hipFree(ptr);
+    if (sums_host) (void)hipHostFree(sums_host);
+    if (scalars_host) (void)hipHostFree(scalars_host);
+    if (flags_host) (void)hipHostFree(flags_host);
+    if (ev_staged) (void)hipEventDestroy(ev_staged);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace smy {
+
+// The pinned staging buffers may be rewritten once the copy that last read them has run.
+inline void staging_wait(smy_pdhg* h) {
+  if (h->staged) {
+    SMY_HIP(hipEventSynchronize(h->ev_staged));
+    h->staged = false;
+  }
+}
+
+inline void staging_sent(smy_pdhg* h) {
+  SMY_HIP(hipEventRecord(h->ev_staged, h->stream));
+  h->staged = true;
+}
+
+// The shared-matrix batch (pdhg_shared.hip): the batch entry points of a handle whose `shared` is
+// set go here.
+int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* cost,
+                  const double* const* lower, const double* const* upper, int precision,
+                  smy_pdhg** out);
+int shared_set_original(smy_pdhg* h, int k, const double* cs, const double* rs, const double* cost,
+                        const double* lower, const double* upper);
+int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta,
+               const long long* k0, int count);
+int shared_evaluate(smy_pdhg* h, double* out);
+int shared_distance(smy_pdhg* h, const int* which, double* out);
+int shared_restart_to(smy_pdhg* h, const int* which);
+int shared_set_precision(smy_pdhg* h, const int* precision);
+int shared_freeze(smy_pdhg* h, int k);
+int shared_download(smy_pdhg* h, int k, double* v, double* v_new, double* y, double* y_new,
+                    double* v_erg, double* y_erg);
+int shared_set_start(smy_pdhg* h, int k, const double* v, const double* y);
+int shared_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
+
+}  // namespace smy

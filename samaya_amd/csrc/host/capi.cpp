@@ -48,6 +48,7 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.detect = p->detect != 0;
   o.log_level = p->log_level;
   o.algorithm = p->algorithm == 1 ? Algorithm::kEnhanced : Algorithm::kReference;
+  o.share_matrix = p->share_matrix != 0;
   return o;
 }
 
@@ -123,6 +124,7 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->verify = 1;
   p->polish_rounds = 0;
   p->algorithm = SMY_ALGORITHM_REFERENCE;
+  p->share_matrix = 1;
 }
 
 int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,
@@ -172,6 +174,10 @@ int smy_model_generate(int kind, uint64_t seed, long long p0, long long p1, long
       case SMY_GEN_SPARSE_POWERLAW:
         md->model = generate_sparse_powerlaw(seed, static_cast<int>(p0), static_cast<int>(p1),
                                              static_cast<int>(p2), p3);
+        break;
+      case SMY_GEN_KNAPSACK:
+        md->model = generate_knapsack(seed, static_cast<int>(p0), static_cast<int>(p1),
+                                      static_cast<int>(p2));
         break;
       case SMY_GEN_BLENDING:
         md->model = generate_blending(seed, static_cast<int>(p0), static_cast<int>(p1),
@@ -349,6 +355,56 @@ int smy_solve_lp_batch(const smy_model* const* models, int count, const smy_pdlp
       copy_text(results[i].message, sizeof results[i].message, e.what());
     }
     return SMY_ERROR;
+  }
+}
+
+int smy_solve_lp_bounds_batch(const smy_model* model, int count, const double* const* col_lower,
+                              const double* const* col_upper, const smy_pdlp_params* params,
+                              const double* start_col_value, const double* start_row_dual,
+                              smy_lp_result* results, double* const* col_values,
+                              double* const* row_duals, double* times) {
+  if (model == nullptr || results == nullptr || count <= 0 || col_lower == nullptr ||
+      col_upper == nullptr)
+    return SMY_INVALID;
+  for (int i = 0; i < count; ++i)
+    if (model->model.n > 0 && (col_lower[i] == nullptr || col_upper[i] == nullptr)) return SMY_INVALID;
+  std::memset(results, 0, sizeof(smy_lp_result) * static_cast<size_t>(count));
+  try {
+    std::vector<const double*> lo(col_lower, col_lower + count), up(col_upper, col_upper + count);
+    PdlpBatchInfo info;
+    const std::vector<LpSolveResult> rs = solve_lp_bounds_batch(
+        model->model, lo, up, to_lp_options(params), start_col_value, start_row_dual, &info);
+    for (int i = 0; i < count; ++i) {
+      fill_result(rs[i], &results[i]);
+      if (col_values != nullptr) put(rs[i].col_value, col_values[i]);
+      if (row_duals != nullptr) put(rs[i].row_dual, row_duals[i]);
+    }
+    if (times != nullptr) {
+      times[0] = info.setup_seconds;
+      times[1] = info.solve_seconds;
+      times[2] = info.kernel_ms;
+    }
+    return SMY_OK;
+  } catch (const std::exception& e) {
+    for (int i = 0; i < count; ++i) {
+      results[i].status = static_cast<int>(Status::kNumericalError);
+      copy_text(results[i].message, sizeof results[i].message, e.what());
+    }
+    return SMY_ERROR;
+  }
+}
+
+int smy_generate_fixings(uint64_t seed, int n, double frac_min, double frac_max,
+                         const double* lower, const double* upper, double* out_lower,
+                         double* out_upper) {
+  if (n < 0 || (n > 0 && (lower == nullptr || upper == nullptr || out_lower == nullptr ||
+                          out_upper == nullptr)))
+    return SMY_INVALID;
+  try {
+    generate_fixings(seed, n, frac_min, frac_max, lower, upper, out_lower, out_upper);
+    return SMY_OK;
+  } catch (const std::invalid_argument&) {
+    return SMY_INVALID;
   }
 }
 

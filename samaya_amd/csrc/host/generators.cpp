@@ -252,4 +252,55 @@ Model generate_blending(uint64_t seed, int periods, int units, int streams, int 
   return md;
 }
 
+Model generate_knapsack(uint64_t seed, int m, int n, int per_col) {
+  if (m <= 0 || n <= 0 || per_col <= 0 || per_col > m) throw std::invalid_argument("bad sizes");
+  Rng rng(seed);
+  Model md;
+  md.m = m;
+  md.n = n;
+  md.maximize = true;
+  md.A.rows = m;
+  md.A.cols = n;
+  md.A.start.assign(static_cast<size_t>(n) + 1, 0);
+  md.A.index.reserve(static_cast<size_t>(n) * per_col);
+  md.A.value.reserve(static_cast<size_t>(n) * per_col);
+  std::vector<int> rows;
+  std::vector<double> row_sum(static_cast<size_t>(m), 0.0);
+  for (int j = 0; j < n; ++j) {
+    distinct_sorted(rng, m, per_col, rows);
+    for (int i : rows) {
+      const double w = static_cast<double>(1 + rng.below(100));  // U{1..100}
+      md.A.index.push_back(i);
+      md.A.value.push_back(w);
+      row_sum[i] += w;
+    }
+    md.A.start[j + 1] = static_cast<int64_t>(md.A.index.size());
+  }
+  md.obj.resize(static_cast<size_t>(n));
+  for (double& p : md.obj) p = static_cast<double>(1 + rng.below(100));  // profits U{1..100}
+  md.row_lower.assign(static_cast<size_t>(m), -kInf);
+  md.row_upper.resize(static_cast<size_t>(m));
+  for (int i = 0; i < m; ++i) md.row_upper[i] = 0.4 * row_sum[i];
+  md.col_lower.assign(static_cast<size_t>(n), 0.0);
+  md.col_upper.assign(static_cast<size_t>(n), 1.0);
+  return md;
+}
+
+void generate_fixings(uint64_t seed, int n, double frac_min, double frac_max, const double* lower,
+                      const double* upper, double* out_lower, double* out_upper) {
+  if (n < 0 || frac_min < 0.0 || frac_max < frac_min || frac_max > 1.0)
+    throw std::invalid_argument("bad fixing fractions");
+  Rng rng(seed);
+  std::copy(lower, lower + n, out_lower);
+  std::copy(upper, upper + n, out_upper);
+  const double f = rng.uniform(frac_min, frac_max);
+  const int count = std::min(n, static_cast<int>(std::lround(f * n)));
+  std::vector<int> cols;
+  distinct_sorted(rng, n, count, cols);
+  for (int j : cols) {
+    const double v = rng.below(2) == 0 ? lower[j] : upper[j];
+    out_lower[j] = out_upper[j] = v;
+  }
+}
+
 }  // namespace smy::host

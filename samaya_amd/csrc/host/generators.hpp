@@ -25,4 +25,14 @@ Model generate_sparse_powerlaw(uint64_t seed, int m, int n, int kmax, double mea
 // With 20 units and 30 streams: 660..1240 rows per period, 2.6k..60k rows.
 Model generate_blending(uint64_t seed, int periods, int units, int streams, int qualities);
 
+// Multi-dimensional knapsack LP: maximize profit'x (profits U{1..100}) subject to W x <= 0.4 x
+// (row sums of W), 0 <= x <= 1, with per_col weights U{1..100} per column at distinct uniform rows.
+Model generate_knapsack(uint64_t seed, int m, int n, int per_col);
+
+// Branch-and-bound fixings of a bound vector: a fraction U[frac_min, frac_max] of the n columns
+// (distinct, uniform) fixed at their lower or their upper bound (even odds); the other columns
+// keep lower / upper.
+void generate_fixings(uint64_t seed, int n, double frac_min, double frac_max, const double* lower,
+                      const double* upper, double* out_lower, double* out_upper);
+
 }  // namespace smy::host

@@ -79,6 +79,26 @@ class Device {
     check(rc);
     return std::unique_ptr<Device>(new Device(h, static_cast<int>(K)));
   }
+  // One matrix and cost for every instance, per-instance bounds (the logical ones equal).
+  static std::unique_ptr<Device> create_shared(const Csc* A, const Csc* At, const std::vector<double>* cost,
+                                               const std::vector<const std::vector<double>*>& lower,
+                                               const std::vector<const std::vector<double>*>& upper,
+                                               int precision) {
+    const size_t K = lower.size();
+    const smy_csc a{A->rows, A->cols, A->start.data(), A->index.data(), A->value.data()};
+    const smy_csc at{At->rows, At->cols, At->start.data(), At->index.data(), At->value.data()};
+    std::vector<const double*> l(K), u(K);
+    for (size_t i = 0; i < K; ++i) {
+      l[i] = lower[i]->data();
+      u[i] = upper[i]->data();
+    }
+    smy_pdhg* h = nullptr;
+    const int rc = smy_pdhg_create_shared(&a, &at, static_cast<int>(K), cost->data(), l.data(), u.data(),
+                                          precision, &h);
+    if (rc == SMY_NO_DEVICE || rc == SMY_TOO_LARGE) return nullptr;
+    check(rc);
+    return std::unique_ptr<Device>(new Device(h, static_cast<int>(K)));
+  }
   ~Device() { smy_pdhg_destroy(h_); }
   Device(const Device&) = delete;
   Device& operator=(const Device&) = delete;
@@ -271,6 +291,18 @@ double worst(const Measure& e) {
   return std::max({e.rel_primal, e.rel_dual, std::fabs(e.rel_gap)});
 }
 
+// The equilibrated matrix of a problem (src/lp/pdlp.cpp:260-318, 377-394): the row and column
+// factors, A_eq = D_r A D_c and its transpose, the bound L on ||[A_eq -I]|| and the norms of the
+// termination test. Lanes of one problem (node relaxations differing in bounds) share it.
+struct Equilibrated {
+  std::vector<double> rs, cs;
+  Csc A_eq, At_eq;
+  double L = 1.0;
+  double a_colmax = 0.0, at_rowmax = 0.0;
+};
+
+std::shared_ptr<const Equilibrated> equilibrate_problem(const LpProblem& lp, Algorithm algorithm);
+
 // One LP of a solve: its setup, its decisions, its result. A single solve has one lane; a batch
 // has one per LP, each taking exactly the decisions it would take alone.
 class Lane {
@@ -278,9 +310,9 @@ class Lane {
   Lane(const LpProblem& lp, const PdlpOptions& opt, std::string tag)
       : lp_(lp), opt_(opt), tag_(std::move(tag)) {}
 
-  // Host setup (src/lp/pdlp.cpp:260-318, 365-394): equilibration, the scaled matrix, cost and
-  // bounds, the step size, the iteration limit and the norms of the termination test.
-  void prepare();
+  // Host setup (src/lp/pdlp.cpp:260-318, 365-394) given the matrix's equilibration: the scaled
+  // cost and bounds, the step size, the iteration limit and the norms of the termination test.
+  void prepare(std::shared_ptr<const Equilibrated> e);
   void log(int level, const char* fmt, ...) const;
   Measure finish(const Sums& s) const;
   double kkt(const Measure& e) const {
@@ -291,25 +323,33 @@ class Lane {
   bool detect_ray(long long iter, const std::vector<double>& v, const std::vector<double>& v_new);
   void map_back(const std::vector<double>& v, const std::vector<double>& y);
 
+  // The bounds of this LP: the task's own when given, else the problem's.
+  const std::vector<double>& lo() const { return lower_ != nullptr ? *lower_ : lp_.lower; }
+  const std::vector<double>& up() const { return upper_ != nullptr ? *upper_ : lp_.upper; }
+
   const LpProblem& lp_;
   const PdlpOptions& opt_;
   std::string tag_;  // "pdlp" alone, "pdlp[i]" in a batch
   // Warm start of this LP (null: the options' start_x / start_y).
   const std::vector<double>* start_x = nullptr;
   const std::vector<double>* start_y = nullptr;
+  const std::vector<double>* lower_ = nullptr;
+  const std::vector<double>* upper_ = nullptr;
   PdlpResult res;
-  std::vector<double> rs, cs;
-  Csc A_eq, At_eq;
+  // Equilibration of the matrix (shared by the lanes of one problem).
+  std::shared_ptr<const Equilibrated> eq;
   std::vector<double> cost, lower, upper;  // equilibrated
   std::vector<double> v0, y0;              // warm start, equilibrated (empty: cold start)
   double eta = 0.0;
-  double a_colmax = 0.0, at_rowmax = 0.0, c_norm = 0.0;
+  double c_norm = 0.0;
   long long max_iter = 0;
   // Iteration state.
   double omega = 1.0, tau = 0.0, sigma = 0.0;
   long long k_since = 0;
   double kkt_at_restart = kInf, kkt_previous = kInf;
   bool fp32 = false;
+  double fp32_best = kInf;       // best distance to termination in FP32 (by 20% steps)
+  long long fp32_best_iter = 0;  // ... and where it was reached
   bool active = true;
   Measure best{kInf, kInf, kInf, kInf, 0.0, 0.0};
 };
@@ -325,28 +365,50 @@ void Lane::log(int level, const char* fmt, ...) const {
   std::fflush(stdout);
 }
 
-void Lane::prepare() {
-  const int m = lp_.m, n = lp_.n, nt = n + m;
-  if (opt_.algorithm == Algorithm::kEnhanced)
-    ruiz_pock_chambolle(lp_.A, lp_.At, rs, cs, 10);
+std::shared_ptr<const Equilibrated> equilibrate_problem(const LpProblem& lp, Algorithm algorithm) {
+  auto e = std::make_shared<Equilibrated>();
+  const int m = lp.m, n = lp.n;
+  if (algorithm == Algorithm::kEnhanced)
+    ruiz_pock_chambolle(lp.A, lp.At, e->rs, e->cs, 10);
   else
-    equilibrate(lp_.A, lp_.At, rs, cs, 10);
-  A_eq = csc_scaled(lp_.A, rs, cs);
-  At_eq = csc_scaled(lp_.At, cs, rs);
+    equilibrate(lp.A, lp.At, e->rs, e->cs, 10);
+  e->A_eq = csc_scaled(lp.A, e->rs, e->cs);
+  e->At_eq = csc_scaled(lp.At, e->cs, e->rs);
+  e->L = operator_norm_bound(e->A_eq, e->At_eq);
+  if (m > 0) {  // src/lp/pdlp.cpp:377-394
+    for (int j = 0; j < n; ++j) {
+      double t = 0.0;
+      for (int64_t p = lp.A.start[j]; p < lp.A.start[j + 1]; ++p) t += lp.A.value[p] * lp.A.value[p];
+      e->a_colmax = std::max(e->a_colmax, std::sqrt(t));
+    }
+    for (int i = 0; i < m; ++i) {
+      double t = 0.0;
+      for (int64_t p = lp.At.start[i]; p < lp.At.start[i + 1]; ++p) t += lp.At.value[p] * lp.At.value[p];
+      e->at_rowmax = std::max(e->at_rowmax, std::sqrt(t));
+    }
+  }
+  return e;
+}
+
+void Lane::prepare(std::shared_ptr<const Equilibrated> e) {
+  const int m = lp_.m, n = lp_.n, nt = n + m;
+  eq = std::move(e);
+  const std::vector<double>& c_s = eq->cs;
+  const std::vector<double>& r_s = eq->rs;
   cost = lp_.cost;
-  lower = lp_.lower;
-  upper = lp_.upper;
+  lower = lo();
+  upper = up();
   for (int j = 0; j < n; ++j) {
-    cost[j] *= cs[j];
-    lower[j] /= cs[j];
-    upper[j] /= cs[j];
+    cost[j] *= c_s[j];
+    lower[j] /= c_s[j];
+    upper[j] /= c_s[j];
   }
   for (int i = 0; i < m; ++i) {
-    cost[n + i] *= rs[i];
-    lower[n + i] *= rs[i];
-    upper[n + i] *= rs[i];
+    cost[n + i] *= r_s[i];
+    lower[n + i] *= r_s[i];
+    upper[n + i] *= r_s[i];
   }
-  const double L = operator_norm_bound(A_eq, At_eq);
+  const double L = eq->L;
   res.operator_norm = L;
   eta = 0.99 / L;
   omega = 1.0;
@@ -359,18 +421,6 @@ void Lane::prepare() {
     max_iter = std::max<long long>(10000LL, 10000LL * (m + n) / 10);
     if (opt_.time_limit >= kInf) max_iter = std::min<long long>(max_iter, 1000000LL);
   }
-  if (m > 0) {  // src/lp/pdlp.cpp:377-394
-    for (int j = 0; j < n; ++j) {
-      double s = 0.0;
-      for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) s += lp_.A.value[p] * lp_.A.value[p];
-      a_colmax = std::max(a_colmax, std::sqrt(s));
-    }
-    for (int i = 0; i < m; ++i) {
-      double s = 0.0;
-      for (int64_t p = lp_.At.start[i]; p < lp_.At.start[i + 1]; ++p) s += lp_.At.value[p] * lp_.At.value[p];
-      at_rowmax = std::max(at_rowmax, std::sqrt(s));
-    }
-  }
   for (int k = 0; k < nt; ++k) c_norm += lp_.cost[k] * lp_.cost[k];
   c_norm = std::sqrt(c_norm);
 
@@ -380,17 +430,17 @@ void Lane::prepare() {
     // Warm start: into the equilibrated space (x' = x / cs, r' = rs r, y' = y / rs).
     v0.assign(static_cast<size_t>(nt), 0.0);
     y0.assign(static_cast<size_t>(m), 0.0);
-    for (int j = 0; j < n; ++j) v0[j] = sx[j] / cs[j];
-    for (int i = 0; i < m; ++i) v0[n + i] = sx[n + i] * rs[i];
-    for (int i = 0; i < m; ++i) y0[i] = sy.empty() ? 0.0 : sy[i] / rs[i];
+    for (int j = 0; j < n; ++j) v0[j] = sx[j] / c_s[j];
+    for (int i = 0; i < m; ++i) v0[n + i] = sx[n + i] * r_s[i];
+    for (int i = 0; i < m; ++i) y0[i] = sy.empty() ? 0.0 : sy[i] / r_s[i];
   }
 }
 
 Measure Lane::finish(const Sums& s) const {
   // src/lp/pdlp.cpp:493-519.
   const int m = lp_.m;
-  const double primal_scale = 1.0 + a_colmax * std::sqrt(s.x_sq) + std::sqrt(s.r_sq);
-  const double dual_scale = 1.0 + at_rowmax * std::sqrt(s.y_sq) + c_norm;
+  const double primal_scale = 1.0 + eq->a_colmax * std::sqrt(s.x_sq) + std::sqrt(s.r_sq);
+  const double dual_scale = 1.0 + eq->at_rowmax * std::sqrt(s.y_sq) + c_norm;
   const double gap_scale = 1.0 + std::fabs(s.primal_obj) + std::fabs(s.dual_obj);
   const double gap = s.primal_obj - s.dual_obj;
   return Measure{m > 0 ? std::sqrt(s.primal_sq) / primal_scale : 0.0,
@@ -404,7 +454,7 @@ bool Lane::detect_farkas(long long iter, const std::vector<double>& y_erg) {
   std::vector<double> yf(static_cast<size_t>(m));
   double y_max = 0.0;
   for (int i = 0; i < m; ++i) {
-    yf[i] = y_erg[i] * rs[i];
+    yf[i] = y_erg[i] * eq->rs[i];
     y_max = std::max(y_max, std::fabs(yf[i]));
   }
   if (!(y_max > 1e-6)) return false;
@@ -421,7 +471,7 @@ bool Lane::detect_farkas(long long iter, const std::vector<double>& y_erg) {
   double f_min = 0.0, f_max = 0.0;
   for (int k = 0; k < n + m; ++k) {
     const double c = std::fabs(w[k]) <= drop ? 0.0 : w[k];
-    const double lo = lp_.lower[k], up = lp_.upper[k];
+    const double lo = this->lo()[k], up = this->up()[k];
     if (c > 0) {
       f_min += lo > -kInf ? c * lo : -INFINITY;
       f_max += up < kInf ? c * up : INFINITY;
@@ -434,8 +484,8 @@ bool Lane::detect_farkas(long long iter, const std::vector<double>& y_erg) {
   for (int k = 0; k < n + m; ++k) {
     const double t = (k < n ? w[k] : -w[k]) / y_max;
     if (std::fabs(t) > 1e-9) {
-      if (lp_.lower[k] > -kInf) scale = std::max(scale, std::fabs(t * lp_.lower[k]));
-      if (lp_.upper[k] < kInf) scale = std::max(scale, std::fabs(t * lp_.upper[k]));
+      if (lo()[k] > -kInf) scale = std::max(scale, std::fabs(t * lo()[k]));
+      if (up()[k] < kInf) scale = std::max(scale, std::fabs(t * up()[k]));
     }
   }
   if (!std::isfinite(scale)) scale = 0.0;
@@ -463,24 +513,24 @@ bool Lane::detect_ray(long long iter, const std::vector<double>& v, const std::v
   for (double& t : d) t /= d_max;
   std::vector<double> r(static_cast<size_t>(m), 0.0);
   for (int j = 0; j < n; ++j) {
-    const double xj = d[j] * cs[j];
+    const double xj = d[j] * eq->cs[j];
     if (xj == 0.0) continue;
     for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) r[lp_.A.index[p]] += lp_.A.value[p] * xj;
   }
-  for (int i = 0; i < m; ++i) r[i] -= d[n + i] / rs[i];
+  for (int i = 0; i < m; ++i) r[i] -= d[n + i] / eq->rs[i];
   double primal_feas = 0.0;
   for (double t : r) primal_feas += t * t;
   primal_feas = std::sqrt(primal_feas);
   double cost_dir = 0.0;
-  for (int j = 0; j < n; ++j) cost_dir += lp_.cost[j] * (d[j] * cs[j]);
-  for (int i = 0; i < m; ++i) cost_dir += lp_.cost[n + i] * (d[n + i] / rs[i]);
+  for (int j = 0; j < n; ++j) cost_dir += lp_.cost[j] * (d[j] * eq->cs[j]);
+  for (int i = 0; i < m; ++i) cost_dir += lp_.cost[n + i] * (d[n + i] / eq->rs[i]);
   constexpr double kBoxTol = 1e-4, kFixedTol = 1e-3;
   const auto fixed = [](double lo, double up) { return up - lo <= 1e-10 * (1.0 + std::fabs(lo)); };
   bool ok = true;
   std::vector<double> ray(static_cast<size_t>(n), 0.0);
   for (int j = 0; j < n && ok; ++j) {
-    double xj = d[j] * cs[j];
-    const double lo = lp_.lower[j], up = lp_.upper[j];
+    double xj = d[j] * eq->cs[j];
+    const double lo = this->lo()[j], up = this->up()[j];
     const bool lb = lo > -kInf / 2, ub = up < kInf / 2;
     if (lb && ub) {
       if (std::fabs(xj) > (fixed(lo, up) ? kFixedTol : kBoxTol)) ok = false;
@@ -500,7 +550,7 @@ bool Lane::detect_ray(long long iter, const std::vector<double>& v, const std::v
     for (int64_t p = lp_.A.start[j]; p < lp_.A.start[j + 1]; ++p) av[lp_.A.index[p]] += lp_.A.value[p] * ray[j];
   }
   for (int i = 0; i < m && ok; ++i) {
-    const double lo = lp_.lower[n + i], up = lp_.upper[n + i];
+    const double lo = this->lo()[n + i], up = this->up()[n + i];
     const bool lb = lo > -kInf / 2, ub = up < kInf / 2;
     if (lb && ub) {
       if (std::fabs(av[i]) > (fixed(lo, up) ? kFixedTol : kBoxTol)) ok = false;
@@ -522,10 +572,10 @@ void Lane::map_back(const std::vector<double>& v, const std::vector<double>& y) 
   // src/lp/pdlp.cpp:1167-1189: x = cs v_x, r = v_r / rs, y = rs y'.
   const int m = lp_.m, n = lp_.n;
   res.x.assign(static_cast<size_t>(n + m), 0.0);
-  for (int j = 0; j < n; ++j) res.x[j] = v[j] * cs[j];
-  for (int i = 0; i < m; ++i) res.x[n + i] = v[n + i] / rs[i];
+  for (int j = 0; j < n; ++j) res.x[j] = v[j] * eq->cs[j];
+  for (int i = 0; i < m; ++i) res.x[n + i] = v[n + i] / eq->rs[i];
   res.y.assign(static_cast<size_t>(m), 0.0);
-  for (int i = 0; i < m; ++i) res.y[i] = y[i] * rs[i];
+  for (int i = 0; i < m; ++i) res.y[i] = y[i] * eq->rs[i];
 }
 
 // Runs fn(i) for i in [0, count) on up to `threads` host threads.
@@ -578,8 +628,31 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     return fail_all(Status::kNumericalError,
                     "this PDLP runs its iterations on the GPU only (use_gpu = false)");
   }
-  // --- Setup (src/lp/pdlp.cpp:260-318), lanes in parallel on the host ---
-  parallel_for(K, K > 1 ? host_threads() : 1, [&](int i) { lanes[i].prepare(); });
+  // --- Setup (src/lp/pdlp.cpp:260-318), lanes in parallel on the host; lanes of one problem
+  // share its equilibration ---
+  std::vector<int> first(static_cast<size_t>(K));
+  std::vector<int> owners;
+  for (int i = 0; i < K; ++i) {
+    first[i] = i;
+    for (int o : owners)
+      if (&lanes[o].lp_ == &lanes[i].lp_) first[i] = o;
+    if (first[i] == i) owners.push_back(i);
+  }
+  std::vector<std::shared_ptr<const Equilibrated>> eqs(static_cast<size_t>(K));
+  const int threads = K > 1 ? host_threads() : 1;
+  parallel_for(static_cast<int>(owners.size()), threads, [&](int q) {
+    eqs[owners[q]] = equilibrate_problem(lanes[owners[q]].lp_, opt.algorithm);
+  });
+  parallel_for(K, threads, [&](int i) { lanes[i].prepare(eqs[first[i]]); });
+
+  // One problem for all lanes, with one set of row bounds: the shared-matrix batch (iterates
+  // stored instance-minor, SpMM products); otherwise the stacked batch.
+  bool share = opt.share_matrix && K > 1 && owners.size() == 1;
+  for (int i = 1; share && i < K; ++i) {
+    const int n = lanes[0].lp_.n;
+    share = std::equal(lanes[i].lower.begin() + n, lanes[i].lower.end(), lanes[0].lower.begin() + n) &&
+            std::equal(lanes[i].upper.begin() + n, lanes[i].upper.end(), lanes[0].upper.begin() + n);
+  }
 
   const bool mixed = opt.precision == Precision::kMixed;
   std::unique_ptr<Device> device;
@@ -588,18 +661,21 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     std::vector<const Csc*> A, At;
     std::vector<const std::vector<double>*> c, l, u;
     for (Lane& L : lanes) {
-      A.push_back(&L.A_eq);
-      At.push_back(&L.At_eq);
+      A.push_back(&L.eq->A_eq);
+      At.push_back(&L.eq->At_eq);
       c.push_back(&L.cost);
       l.push_back(&L.lower);
       u.push_back(&L.upper);
     }
-    device = Device::create(A, At, c, l, u, mixed ? SMY_FP32 : SMY_FP64);
-    if (device)
+    device = share ? Device::create_shared(A[0], At[0], c[0], l, u, mixed ? SMY_FP32 : SMY_FP64)
+                   : Device::create(A, At, c, l, u, mixed ? SMY_FP32 : SMY_FP64);
+    if (device) {
+      if (share) lanes[0].log(2, ": %d LPs share one matrix: shared-matrix batch", K);
       for (int i = 0; i < K; ++i) {
         Lane& L = lanes[i];
-        device->set_original(i, L.cs, L.rs, L.lp_.cost, L.lp_.lower, L.lp_.upper);
+        device->set_original(i, L.eq->cs, L.eq->rs, L.lp_.cost, L.lo(), L.up());
       }
+    }
   } catch (const std::exception& e) {
     device.reset();
     why = e.what();
@@ -651,7 +727,15 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     stop(i);
   };
   long long iter = 0;
+  double next_progress = clock.seconds() + 10.0;
   while (active > 0) {
+    if (K > 1 && opt.log_level >= 1 && clock.seconds() >= next_progress) {
+      // A long batch reports itself every 10 seconds.
+      std::printf("pdlp batch: iteration %lld, %d of %d LPs still running, %.1f s\n", iter, active, K,
+                  clock.seconds());
+      std::fflush(stdout);
+      next_progress = clock.seconds() + 10.0;
+    }
     if (opt.time_limit < kInf && clock.seconds() > opt.time_limit) {
       for (int i = 0; i < K; ++i) {
         Lane& L = lanes[i];
@@ -758,13 +842,25 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         }
       }
 
-      // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance.
-      if (L.fp32 && std::min(worst(current), worst(average)) < opt.fp32_switch_tol) {
-        L.log(2, ": FP32 steps reached %.2e at iter %lld; FP64 steps from here",
-              std::min(worst(current), worst(average)), iter);
-        precision[i] = SMY_FP64;
-        L.fp32 = false;
-        force[i] = 1;
+      // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance, or once
+      // FP32 stopped improving (its rounding floor lies above the switch point): no 20% gain on
+      // its best distance to termination within max(4096, iter / 4) iterations.
+      if (L.fp32) {
+        const double w = std::min(worst(current), worst(average));
+        if (w < 0.8 * L.fp32_best) {
+          L.fp32_best = w;
+          L.fp32_best_iter = iter;
+        }
+        const bool reached = w < opt.fp32_switch_tol;
+        const bool stalled = iter - L.fp32_best_iter >= std::max<long long>(4096, iter / 4);
+        if (reached || stalled) {
+          L.log(2, reached ? ": FP32 steps reached %.2e at iter %lld; FP64 steps from here"
+                           : ": FP32 steps stalled at %.2e at iter %lld; FP64 steps from here",
+                w, iter);
+          precision[i] = SMY_FP64;
+          L.fp32 = false;
+          force[i] = 1;
+        }
       }
 
       // Adaptive restart (src/lp/pdlp.cpp:1075-1132): the decision; the primal weight follows
@@ -837,6 +933,8 @@ std::vector<PdlpResult> solve_lanes(const std::vector<PdlpTask>& tasks, const Pd
       lanes.emplace_back(*tasks[i].lp, options, K == 1 ? std::string("pdlp") : "pdlp[" + std::to_string(i) + "]");
       lanes.back().start_x = tasks[i].start_x;
       lanes.back().start_y = tasks[i].start_y;
+      lanes.back().lower_ = tasks[i].lower;
+      lanes.back().upper_ = tasks[i].upper;
     }
     return run_lanes(lanes, options, clock, info);
   } catch (const std::exception& e) {

@@ -48,10 +48,12 @@ struct PdlpOptions {
   Precision precision = Precision::kFp64;
   Algorithm algorithm = Algorithm::kReference;
   // Mixed precision: FP32 steps until the better of current / average reaches this (FP64)
-  // termination distance, or FP32 stops improving; then FP64 steps to `tol`.
+  // termination distance, or FP32 stops improving (no 20% gain within max(4096, iter / 4)
+  // iterations); then FP64 steps to `tol`.
   double fp32_switch_tol = 1e-5;
   bool use_graph = true;          // replay each 64-step run from a HIP graph
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
+  bool share_matrix = true;       // batches of one problem: the shared-matrix (SpMM) batch
   int log_level = 0;
   // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m).
   std::vector<double> start_x, start_y;
@@ -100,6 +102,11 @@ struct PdlpTask {
   const LpProblem* lp = nullptr;
   const std::vector<double>* start_x = nullptr;  // warm start (null: options.start_x)
   const std::vector<double>* start_y = nullptr;
+  // Bounds of this task (n + m, the problem's space; null: the problem's own). Tasks of one
+  // problem with equal row bounds and their own column bounds (node relaxations) run as a
+  // shared-matrix batch.
+  const std::vector<double>* lower = nullptr;
+  const std::vector<double>* upper = nullptr;
 };
 
 struct PdlpBatchInfo {
@@ -164,5 +171,17 @@ LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options);
 std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& models,
                                           const LpSolveOptions& options,
                                           PdlpBatchInfo* info = nullptr);
+
+// Node relaxations of one model: solve_lp of the model with column bounds col_lower[k] /
+// col_upper[k] (n each) for every k, batched on one shared matrix (the shared-matrix batch when
+// the options allow it). An optional model-space start (column values, and row duals in the
+// d = c - A'y convention) warm-starts every relaxation.
+std::vector<LpSolveResult> solve_lp_bounds_batch(const Model& model,
+                                                 const std::vector<const double*>& col_lower,
+                                                 const std::vector<const double*>& col_upper,
+                                                 const LpSolveOptions& options,
+                                                 const double* start_col_value = nullptr,
+                                                 const double* start_row_dual = nullptr,
+                                                 PdlpBatchInfo* info = nullptr);
 
 }  // namespace smy::host

@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -321,20 +322,37 @@ namespace {
 // One model of solve_lp / solve_lp_batch between its PDLP rounds.
 struct LpJob {
   const Model* model = nullptr;
-  Scaling scaled;
-  LpProblem lp;
+  std::shared_ptr<const Scaling> scaled;
+  std::shared_ptr<const LpProblem> lp;
+  // Node relaxations: this job's column bounds (model space) and their computational-form
+  // bounds (n + m); empty: the model's own.
+  const double* col_lower = nullptr;
+  const double* col_upper = nullptr;
+  std::vector<double> lower, upper;
   LpSolveResult out;
-  std::vector<double> start_x, start_y;  // the rejected point a polish round starts from
+  std::vector<double> start_x, start_y;  // the point a round starts from (warm or polish)
+  bool warm = false;
   bool open = true;                      // another round is wanted
 };
+
+// The model a job's answer is verified against: the model itself, or a copy with the job's
+// column bounds.
+std::unique_ptr<Model> job_model(const LpJob& job) {
+  if (job.col_lower == nullptr) return nullptr;
+  auto md = std::make_unique<Model>(*job.model);
+  md->col_lower.assign(job.col_lower, job.col_lower + md->n);
+  md->col_upper.assign(job.col_upper, job.col_upper + md->n);
+  return md;
+}
 
 // Maps one PDLP result back to the model (solve_pdlp_lp, src/core/solver.cpp:392-477: unscale;
 // y <- -sense y since PDLP's reduced costs are c + A'y; d = c - A'y; objective in double-double),
 // runs the verifier and decides whether a polish round follows.
 void absorb(LpJob& job, PdlpResult&& result, int round, const LpSolveOptions& options) {
-  const Model& model = *job.model;
+  const std::unique_ptr<Model> own = job_model(job);
+  const Model& model = own ? *own : *job.model;
   LpSolveResult& out = job.out;
-  const Scaling& scaled = job.scaled;
+  const Scaling& scaled = *job.scaled;
   const int n = model.n, m = model.m;
   const double sense = model.maximize ? -1.0 : 1.0;
   out.pdlp = std::move(result);
@@ -408,17 +426,11 @@ void for_each_parallel(int count, F&& fn) {
   for (auto& th : pool) th.join();
 }
 
-}  // namespace
 
-std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& models,
-                                          const LpSolveOptions& options, PdlpBatchInfo* info) {
-  const int K = static_cast<int>(models.size());
-  std::vector<LpJob> jobs(static_cast<size_t>(K));
-  for_each_parallel(K, [&](int i) {
-    jobs[i].model = models[i];
-    jobs[i].scaled = compute_scaling(models[i]->A);
-    jobs[i].lp = make_problem(*models[i], jobs[i].scaled);
-  });
+// The PDLP rounds of a set of jobs: every open job in one batch per round, then mapping,
+// verification and the polish decision per job.
+void run_jobs(std::vector<LpJob>& jobs, const LpSolveOptions& options, PdlpBatchInfo* info) {
+  const int K = static_cast<int>(jobs.size());
   PdlpOptions popt = options.pdlp;
   for (int round = 0;; ++round) {
     std::vector<int> open;
@@ -427,10 +439,14 @@ std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& model
     if (open.empty()) break;
     std::vector<PdlpTask> tasks;
     for (int i : open) {
-      PdlpTask t{&jobs[i].lp, nullptr, nullptr};
-      if (round > 0) {
+      PdlpTask t{jobs[i].lp.get(), nullptr, nullptr, nullptr, nullptr};
+      if (round > 0 || jobs[i].warm) {
         t.start_x = &jobs[i].start_x;
         t.start_y = &jobs[i].start_y;
+      }
+      if (!jobs[i].lower.empty()) {
+        t.lower = &jobs[i].lower;
+        t.upper = &jobs[i].upper;
       }
       tasks.push_back(t);
     }
@@ -446,10 +462,80 @@ std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& model
     popt.tol /= 10.0;
     popt.precision = Precision::kFp64;
   }
+}
+
+std::vector<LpSolveResult> collect(std::vector<LpJob>& jobs) {
   std::vector<LpSolveResult> out;
-  out.reserve(static_cast<size_t>(K));
+  out.reserve(jobs.size());
   for (LpJob& j : jobs) out.push_back(std::move(j.out));
   return out;
+}
+
+}  // namespace
+
+std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& models,
+                                          const LpSolveOptions& options, PdlpBatchInfo* info) {
+  const int K = static_cast<int>(models.size());
+  std::vector<LpJob> jobs(static_cast<size_t>(K));
+  for_each_parallel(K, [&](int i) {
+    jobs[i].model = models[i];
+    auto sc = std::make_shared<Scaling>(compute_scaling(models[i]->A));
+    jobs[i].lp = std::make_shared<LpProblem>(make_problem(*models[i], *sc));
+    jobs[i].scaled = std::move(sc);
+  });
+  run_jobs(jobs, options, info);
+  return collect(jobs);
+}
+
+std::vector<LpSolveResult> solve_lp_bounds_batch(const Model& model,
+                                                 const std::vector<const double*>& col_lower,
+                                                 const std::vector<const double*>& col_upper,
+                                                 const LpSolveOptions& options,
+                                                 const double* start_col_value,
+                                                 const double* start_row_dual,
+                                                 PdlpBatchInfo* info) {
+  const int K = static_cast<int>(col_lower.size());
+  const int n = model.n, m = model.m;
+  auto scaled = std::make_shared<const Scaling>(compute_scaling(model.A));
+  auto lp = std::make_shared<const LpProblem>(make_problem(model, *scaled));
+  // Warm start in the computational form (make_problem's space): x / col scale, the row
+  // activities A x times the row scale, and y = -sense row dual / row scale.
+  std::vector<double> sx, sy;
+  if (start_col_value != nullptr) {
+    const double sense = model.maximize ? -1.0 : 1.0;
+    sx.assign(static_cast<size_t>(n + m), 0.0);
+    sy.assign(static_cast<size_t>(m), 0.0);
+    std::vector<double> act(static_cast<size_t>(m), 0.0);
+    for (int j = 0; j < n; ++j)
+      for (int64_t p = model.A.start[j]; p < model.A.start[j + 1]; ++p)
+        act[model.A.index[p]] += model.A.value[p] * start_col_value[j];
+    for (int j = 0; j < n; ++j) sx[j] = start_col_value[j] / scaled->col[j];
+    for (int i = 0; i < m; ++i) sx[n + i] = act[i] * scaled->row[i];
+    if (start_row_dual != nullptr)
+      for (int i = 0; i < m; ++i) sy[i] = -sense * start_row_dual[i] / scaled->row[i];
+  }
+  std::vector<LpJob> jobs(static_cast<size_t>(K));
+  for_each_parallel(K, [&](int k) {
+    LpJob& job = jobs[k];
+    job.model = &model;
+    job.scaled = scaled;
+    job.lp = lp;
+    job.col_lower = col_lower[k];
+    job.col_upper = col_upper[k];
+    job.lower = lp->lower;
+    job.upper = lp->upper;
+    for (int j = 0; j < n; ++j) {
+      job.lower[j] = col_lower[k][j] / scaled->col[j];
+      job.upper[j] = col_upper[k][j] / scaled->col[j];
+    }
+    if (!sx.empty()) {
+      job.start_x = sx;
+      job.start_y = sy;
+      job.warm = true;
+    }
+  });
+  run_jobs(jobs, options, info);
+  return collect(jobs);
 }
 
 LpSolveResult solve_lp(const Model& model, const LpSolveOptions& options) {

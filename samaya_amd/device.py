@@ -150,6 +150,26 @@ class PdhgBatch:
         N.check(rc, "pdhg_create_batch")
         return cls(h, [(A[i][1], A[i][0]) for i in range(k)])
 
+    @classmethod
+    def create_shared(cls, A: tuple, At: tuple, cost, lower, upper, precision: str = "fp64"):
+        """One matrix and cost for every instance, instance i with bounds lower[i] / upper[i]
+        (n + m; equal row parts): the shared-matrix batch (instance-minor iterates, SpMM)."""
+        lib = N.load()
+        a, keep_a = _csc(*A)
+        at, keep_at = _csc(*At)
+        k = len(lower)
+        lo = [_f64(v) for v in lower]
+        up = [_f64(v) for v in upper]
+        lop = (C.c_void_p * k)(*[v.ctypes.data for v in lo])
+        upp = (C.c_void_p * k)(*[v.ctypes.data for v in up])
+        h = C.c_void_p()
+        rc = lib.smy_pdhg_create_shared(C.byref(a), C.byref(at), k, _p(_f64(cost)), lop, upp,
+                                        cls._PREC[precision], C.byref(h))
+        if rc in (N.SMY_NO_DEVICE, N.SMY_TOO_LARGE):
+            return None
+        N.check(rc, "pdhg_create_shared")
+        return cls(h, [(A[1], A[0])] * k)
+
     def __del__(self):
         if getattr(self, "h", None):
             self.lib.smy_pdhg_destroy(self.h)
@@ -170,6 +190,17 @@ class PdhgBatch:
 
     def advance(self):
         N.check(self.lib.smy_pdhg_advance(self.h), "advance")
+
+    def last_run_ms(self) -> float:
+        ms = C.c_double()
+        N.check(self.lib.smy_pdhg_last_run_ms(self.h, C.byref(ms)), "last_run_ms")
+        return ms.value
+
+    def time_kernels(self, reps: int = 20) -> tuple[float, float]:
+        """Average ms of the dual and primal step kernels (changes the iterates)."""
+        d, p = C.c_double(), C.c_double()
+        N.check(self.lib.smy_pdhg_time_kernels(self.h, reps, C.byref(d), C.byref(p)), "time_kernels")
+        return d.value, p.value
 
     def evaluate(self) -> np.ndarray:
         """(batch, 14): the current iterate's 7 sums, then the average's."""

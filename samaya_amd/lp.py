@@ -183,6 +183,7 @@ class PdlpOptions:
     verify: bool = True
     polish_rounds: int = 0
     algorithm: str = "reference"    # "reference" or "enhanced"
+    share_matrix: bool = True       # batches of one model: the shared-matrix (SpMM) batch
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -199,6 +200,7 @@ class PdlpOptions:
         p.verify = int(self.verify)
         p.polish_rounds = self.polish_rounds
         p.algorithm = {"reference": 0, "enhanced": 1}[self.algorithm]
+        p.share_matrix = int(self.share_matrix)
         return p
 
 
@@ -351,6 +353,36 @@ def solve_lp_batch(models, options: PdlpOptions | None = None
     params = options.native()
     rc = lib.smy_solve_lp_batch(hs, k, C.byref(params), res, xp, yp, _p(times))
     N.check(rc, "solve_lp_batch")
+    out = [_lp_result(res[i], xs[i], ys[i], None, None, None) for i in range(k)]
+    return out, BatchTimes(float(times[0]), float(times[1]), float(times[2]))
+
+
+def solve_lp_bounds_batch(model, col_lower, col_upper, options: PdlpOptions | None = None,
+                          start=None) -> tuple[list[LpResult], BatchTimes]:
+    """Node relaxations of one model: solve_lp with column bounds col_lower[k] / col_upper[k] for
+    every k, batched on one shared matrix (no reference counterpart). `start` = (column values,
+    row duals) of a model-space point (e.g. the root LP's answer) warm-starts every relaxation."""
+    options = options or PdlpOptions()
+    nm = model if isinstance(model, NativeModel) else model.native()
+    lib = N.load()
+    k = len(col_lower)
+    lo = [_f64(v) for v in col_lower]
+    up = [_f64(v) for v in col_upper]
+    lop = (C.c_void_p * k)(*[v.ctypes.data for v in lo])
+    upp = (C.c_void_p * k)(*[v.ctypes.data for v in up])
+    xs = [np.zeros(nm.n) for _ in range(k)]
+    ys = [np.zeros(nm.m) for _ in range(k)]
+    xp = (C.c_void_p * k)(*[x.ctypes.data for x in xs])
+    yp = (C.c_void_p * k)(*[y.ctypes.data for y in ys])
+    sx = sy = None
+    if start is not None:
+        sx, sy = _f64(start[0]), _f64(start[1])
+    res = (N.SmyLpResult * k)()
+    times = np.zeros(3)
+    params = options.native()
+    rc = lib.smy_solve_lp_bounds_batch(nm.h, k, lop, upp, C.byref(params), _p(sx), _p(sy), res, xp,
+                                       yp, _p(times))
+    N.check(rc, "solve_lp_bounds_batch")
     out = [_lp_result(res[i], xs[i], ys[i], None, None, None) for i in range(k)]
     return out, BatchTimes(float(times[0]), float(times[1]), float(times[2]))
 

@@ -1,0 +1,134 @@
+"""The shared-matrix batch on the GPU: K LPs with one matrix and cost and their own column bounds
+(branch-and-bound node relaxations), iterates stored instance-minor and products as SpMMs.
+
+It sums in another order than the stacked batch, so the two agree to rounding, not to the bit:
+device sums after the same steps agree to 1e-9 relative (FP64), answers agree in status and
+objective to the solve tolerance. Identical instances give identical bits, and node relaxations
+match solve_lp of the model with those bounds."""
+import numpy as np
+import pytest
+
+import samaya_amd as smy
+import samaya_amd.device as D
+import samaya_amd.generators as G
+from oracle_lib import M
+from test_gpu_pdhg import equilibrated, identity_lp, transpose_csc
+
+pytestmark = pytest.mark.gpu
+
+
+def node_bounds(md, count, seed0=900, lo_frac=0.01, hi_frac=0.05):
+    out = [G.fixings(seed0 + k, md.col_lower, md.col_upper, lo_frac, hi_frac) for k in range(count)]
+    return [o[0] for o in out], [o[1] for o in out]
+
+
+def rel_close(a, b, tol):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.all(np.abs(a - b) <= tol * (1.0 + np.abs(b)))
+
+
+def test_shared_device_matches_stacked_device(gpu):
+    md = G.knapsack(71, 60, 150, 4)
+    lp = identity_lp(M.from_product(md))
+    rng = np.random.default_rng(3)
+    rs, cs = rng.uniform(0.5, 2.0, lp.m), rng.uniform(0.5, 2.0, lp.n)
+    eq = equilibrated(lp, rs, cs)
+    K = 7
+    lows, ups = [], []
+    for k in range(K):
+        lo, up = G.fixings(800 + k, eq.lower[:lp.n], eq.upper[:lp.n], 0.05, 0.2)
+        lows.append(np.concatenate([lo, eq.lower[lp.n:]]))
+        ups.append(np.concatenate([up, eq.upper[lp.n:]]))
+    a = (eq.m, eq.n, eq.start, eq.index, eq.value)
+    at = transpose_csc(eq)
+    shared = D.PdhgBatch.create_shared(a, at, eq.cost, lows, ups)
+    stacked = D.PdhgBatch.create([a] * K, [at] * K, [eq.cost] * K, lows, ups)
+    ol = [np.concatenate([lows[k][:lp.n] * cs, lp.lower[lp.n:]]) for k in range(K)]
+    ou = [np.concatenate([ups[k][:lp.n] * cs, lp.upper[lp.n:]]) for k in range(K)]
+    for b in (shared, stacked):
+        for k in range(K):
+            b.set_original(k, cs, rs, lp.cost, ol[k], ou[k])
+    tau = np.linspace(0.2, 0.4, K)
+    sigma = np.linspace(0.4, 0.2, K)
+    for b in (shared, stacked):
+        b.run(tau, sigma, 1.0, [0] * K, 64)
+    assert rel_close(shared.evaluate(), stacked.evaluate(), 1e-9)
+    which = [0, -1, 1, -1, 0, 1, -1]
+    flagged = [k for k in range(K) if which[k] >= 0]  # the others' distances are not computed
+    assert rel_close(shared.restart_distance(which)[flagged], stacked.restart_distance(which)[flagged], 1e-9)
+    for b in (shared, stacked):
+        b.restart_to(which)
+        b.advance()
+        b.freeze(3)
+        b.set_precision([-1, "fp32", -1, -1, -1, -1, -1])
+        b.run(tau, sigma, 1.0, [0, 64, 0, 64, 0, 0, 64], 5)
+    es, et = shared.evaluate(), stacked.evaluate()
+    for k in (0, 2, 4, 5, 6):
+        assert rel_close(es[k], et[k], 1e-9)
+    assert rel_close(es[1], et[1], 1e-4)  # FP32 steps: rounding of FP32 sums in another order
+    for k in (0, 2, 5):
+        ds, dt = shared.download(k), stacked.download(k)
+        for key in ds:
+            assert rel_close(ds[key], dt[key], 1e-9), key
+
+
+def test_identical_bound_vectors_give_identical_answers(gpu):
+    md = G.knapsack(72, 300, 800, 6)
+    lo, up = G.fixings(901, md.col_lower, md.col_upper)
+    res, _ = smy.solve_lp_bounds_batch(md, [lo] * 6, [up] * 6,
+                                       smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced"))
+    for r in res[1:]:
+        assert r.status == res[0].status and r.iterations == res[0].iterations
+        assert r.objective == res[0].objective
+        np.testing.assert_array_equal(r.col_value, res[0].col_value)
+    assert res[0].status == smy.OPTIMAL
+
+
+@pytest.mark.parametrize("precision,algorithm", [("fp64", "reference"), ("mixed", "enhanced")])
+def test_node_relaxations_match_single_solves(gpu, precision, algorithm):
+    md = G.knapsack(73, 400, 1000, 6)
+    lows, ups = node_bounds(md, 9)
+    opts = smy.PdlpOptions(tol=1e-7, precision=precision, algorithm=algorithm, polish_rounds=2)
+    batch, times = smy.solve_lp_bounds_batch(md, lows, ups, opts)
+    assert times.kernel_ms > 0
+    for k in range(len(lows)):
+        mk = smy.Model(md.m, md.n, md.start, md.index, md.value, md.obj, lows[k], ups[k],
+                       md.row_lower, md.row_upper, md.maximize, md.offset)
+        single = smy.solve_lp(mk, opts)
+        assert batch[k].status == single.status == smy.OPTIMAL
+        assert abs(batch[k].objective - single.objective) <= 1e-5 * (1 + abs(single.objective))
+        assert batch[k].verified == single.verified
+        # The answer respects the node's fixings.
+        fixed = lows[k] == ups[k]
+        assert np.max(np.abs(batch[k].col_value[fixed] - lows[k][fixed])) <= 1e-6
+
+
+def test_shared_and_stacked_batches_agree(gpu):
+    md = G.knapsack(74, 300, 700, 5)
+    lows, ups = node_bounds(md, 5)
+    opts = smy.PdlpOptions(tol=1e-7, precision="mixed", algorithm="enhanced")
+    shared, _ = smy.solve_lp_bounds_batch(md, lows, ups, opts)
+    opts.share_matrix = False
+    stacked, _ = smy.solve_lp_bounds_batch(md, lows, ups, opts)
+    for a, b in zip(shared, stacked):
+        assert a.status == b.status == smy.OPTIMAL
+        assert abs(a.objective - b.objective) <= 1e-5 * (1 + abs(b.objective))
+
+
+def test_warm_start_is_used(gpu):
+    # Nodes started from their own answers stop at the first evaluations; started from the root's
+    # answer they reach the same objectives as cold starts.
+    md = G.knapsack(75, 400, 1000, 6)
+    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced")
+    lows, ups = node_bounds(md, 6)
+    cold, _ = smy.solve_lp_bounds_batch(md, lows, ups, opts)
+    for k, c in enumerate(cold):
+        again, _ = smy.solve_lp_bounds_batch(md, [lows[k]], [ups[k]], opts,
+                                             start=(c.col_value, c.row_dual))
+        assert again[0].status == smy.OPTIMAL and again[0].iterations <= 3 * 64
+        assert abs(again[0].objective - c.objective) <= 1e-5 * (1 + abs(c.objective))
+    root = smy.solve_lp(md, opts)
+    warm, _ = smy.solve_lp_bounds_batch(md, lows, ups, opts, start=(root.col_value, root.row_dual))
+    for c, w in zip(cold, warm):
+        assert c.status == w.status == smy.OPTIMAL
+        assert abs(c.objective - w.objective) <= 1e-4 * (1 + abs(c.objective))

@@ -157,6 +157,11 @@ def test_gpu_paths_fail_loudly_without_gpu():
     one = (1, 1, np.array([0, 1]), np.array([0], np.int32), np.array([1.0]))
     assert PdhgBatch.create([one, one], [one, one], [np.zeros(2)] * 2, [np.zeros(2)] * 2,
                             [np.ones(2)] * 2) is None
+    assert PdhgBatch.create_shared(one, one, np.zeros(2), [np.zeros(2)] * 2, [np.ones(2)] * 2) is None
+    kp = smy.generators.knapsack(7, 20, 50, 3)
+    lo, up = smy.generators.fixings(8, kp.col_lower, kp.col_upper)
+    nodes, _ = smy.solve_lp_bounds_batch(kp, [lo, lo], [up, up])
+    assert all(r.status == smy.NUMERICAL_ERROR and "no usable GPU" in r.message for r in nodes)
 
 
 def test_blending_generator_structure():
@@ -189,6 +194,28 @@ def test_blending_generator_structure():
     cols = np.repeat(np.arange(small.n), np.diff(small.start))
     touched = cols[small.index == t1_row]
     assert touched.min() < 1820 <= touched.max()
+
+
+def test_knapsack_and_fixings_generators():
+    md = smy.generators.knapsack(5000, 200, 500, 8)
+    assert md.maximize and md.m == 200 and md.n == 500 and md.nnz == 4000
+    assert np.all(np.diff(md.start) == 8)
+    assert np.all((md.value >= 1) & (md.value <= 100) & (md.value == np.round(md.value)))
+    assert np.all((md.obj >= 1) & (md.obj <= 100))
+    rows = np.zeros(md.m)
+    np.add.at(rows, md.index, md.value)
+    assert np.allclose(md.row_upper, 0.4 * rows) and np.all(np.isneginf(md.row_lower))
+    assert np.all(md.col_lower == 0) and np.all(md.col_upper == 1)
+    fr = []
+    for k in range(20):
+        lo, up = smy.generators.fixings(6000 + k, md.col_lower, md.col_upper)
+        fixed = lo == up
+        fr.append(fixed.mean())
+        assert np.all(np.isin(lo[fixed], (0.0, 1.0)))
+        assert np.all(lo[~fixed] == 0) and np.all(up[~fixed] == 1)
+        lo2, up2 = smy.generators.fixings(6000 + k, md.col_lower, md.col_upper)
+        assert np.array_equal(lo, lo2) and np.array_equal(up, up2)
+    assert 0.009 <= min(fr) and max(fr) <= 0.051 and len(set(fr)) > 5
 
 
 def test_cpu_iterations_are_refused():
