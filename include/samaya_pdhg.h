@@ -150,6 +150,18 @@ int smy_pdhg_set_precision_batch(smy_pdhg* h, const int* precision);
 /* Instance i takes no further steps and is not evaluated; its state stays downloadable. */
 int smy_pdhg_freeze(smy_pdhg* h, int instance);
 
+/* Device-side infeasibility and unboundedness tests (the reference downloads the iterates and
+ * runs them on the host: src/lp/pdlp.cpp:715-1043) for the instances with flags[i] > 0, in two
+ * device phases. out[11 i + q] for those instances: 0 sum |w| and 1 max |yf| (yf = row-scaled
+ * average dual, w = (A' yf, -yf)); 2 f_min and 3 f_max of the Farkas bound test, finite parts, with
+ * 4 and 5 the counts of their infinite terms; 6 scale (max |t bound|, |t| > 1e-9, t = +-w/y_max);
+ * 7 d_max = max |v_new - v|; 8 ||A x - r||^2 of the normalized difference; 9 its cost slope; 10 the
+ * number of bound failures of the projected ray and of A times it. The host decides as the
+ * reference does from these. Synchronizes. */
+int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out);
+/* The projected ray (n_i structural entries) of instance i's last detection. */
+int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray);
+
 /* Replaces gpu::csr_spmv (spmv.hpp:16-21): y = A x for a CSR matrix in device memory (rows x ?,
  * ptr[rows + 1], idx[nnz], val[nnz]), x and y device arrays. threads_per_row is accepted for
  * the reference's signature; the gfx950 kernel picks its lane groups per row block. Runs on

@@ -94,6 +94,8 @@ typedef struct smy_pdlp_params {
   int polish_rounds;         /* model-level solve: FP64 continuations with tol/10 if rejected */
   int algorithm;             /* SMY_ALGORITHM_REFERENCE (the reference's decisions) or _ENHANCED */
   int share_matrix;          /* batches of one model: run them as a shared-matrix batch (1) */
+  int device_detect;         /* infeasibility / unboundedness tests on the device (1) or host */
+  double start_omega;        /* primal weight to start with (<= 0: 1); with a warm start */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);
@@ -107,6 +109,7 @@ typedef struct smy_pdlp_info {
   double dual_residual;
   double gap;
   double operator_norm;
+  double omega;         /* final primal weight */
   double setup_seconds;
   double solve_seconds;
   double kernel_ms;

@@ -27,7 +27,8 @@ EXPORTS = (
     "smy_pdhg_create_batch", "smy_pdhg_batch_size", "smy_pdhg_set_original_at",
     "smy_pdhg_set_start_at", "smy_pdhg_download_at", "smy_pdhg_run_batch",
     "smy_pdhg_evaluate_batch", "smy_pdhg_restart_distance_batch", "smy_pdhg_restart_to_batch",
-    "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared",
+    "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
@@ -49,14 +50,14 @@ class SmyPdlpParams(C.Structure):
                 ("use_gpu", C.c_int), ("precision", C.c_int), ("fp32_switch_tol", C.c_double),
                 ("use_graph", C.c_int), ("detect", C.c_int), ("log_level", C.c_int),
                 ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int),
-                ("share_matrix", C.c_int)]
+                ("share_matrix", C.c_int), ("device_detect", C.c_int), ("start_omega", C.c_double)]
 
 
 class SmyPdlpInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_longlong),
                 ("fp32_iterations", C.c_longlong), ("primal_residual", C.c_double),
                 ("dual_residual", C.c_double), ("gap", C.c_double),
-                ("operator_norm", C.c_double), ("setup_seconds", C.c_double),
+                ("operator_norm", C.c_double), ("omega", C.c_double), ("setup_seconds", C.c_double),
                 ("solve_seconds", C.c_double), ("kernel_ms", C.c_double), ("restarts", C.c_int),
                 ("evaluations", C.c_int), ("message", C.c_char * 256)]
 
@@ -131,6 +132,8 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_restart_to_batch", i32, vp, vp)
     _proto(lib, "smy_pdhg_set_precision_batch", i32, vp, vp)
     _proto(lib, "smy_pdhg_freeze", i32, vp, i32)
+    _proto(lib, "smy_pdhg_detect_batch", i32, vp, vp, vp)
+    _proto(lib, "smy_pdhg_download_ray", i32, vp, i32, vp)
     _proto(lib, "smy_pdhg_create_shared", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), i32, vp, vp,
            vp, i32, C.POINTER(vp))
     _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)

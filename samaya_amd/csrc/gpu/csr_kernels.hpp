@@ -37,6 +37,9 @@ struct CsrRows {
   const int2* __restrict__ meta = nullptr;   // nblocks + 1 entries (first row, first nonzero)
   const int* __restrict__ binst = nullptr;   // instance of each block (null: all instance 0)
   const int* __restrict__ list = nullptr;    // blocks of this launch (null: block = workgroup)
+  // The instance is blockIdx.y (one matrix shared by the instances, each launched over all
+  // blocks) instead of binst[b].
+  bool inst_from_y = false;
 };
 
 // Scalars of one PDHG step for one instance, read by the step kernels from device memory so that
@@ -56,20 +59,22 @@ __device__ inline T clamp_to(T v, T lo, T up) {
 
 // Sums of K per-thread values over the workgroup (wavefront butterfly, then the four wavefront
 // totals in index order); thread q < K writes total q to partial[q * stride].
-template <int K>
+// Slots q >= MaxFrom are maxima instead of sums.
+template <int K, int MaxFrom = K>
 __device__ inline void block_partials(const double (&local)[K], double* __restrict__ partial,
-                                      int stride) {
+                                      size_t stride) {
   __shared__ double wsum[K][kBlock / kWave];
   const int tid = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < K; ++q) {
-    const double t = group_sum<kWave>(local[q]);
+    const double t = q < MaxFrom ? group_sum<kWave>(local[q]) : group_max<kWave>(local[q]);
     if ((tid & (kWave - 1)) == 0) wsum[q][tid / kWave] = t;
   }
   __syncthreads();
   if (tid < K) {
     partial[static_cast<size_t>(tid) * stride] =
-        ((wsum[tid][0] + wsum[tid][1]) + wsum[tid][2]) + wsum[tid][3];
+        tid < MaxFrom ? ((wsum[tid][0] + wsum[tid][1]) + wsum[tid][2]) + wsum[tid][3]
+                      : fmax(fmax(wsum[tid][0], wsum[tid][1]), fmax(wsum[tid][2], wsum[tid][3]));
   }
 }
 
@@ -87,7 +92,8 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
   const int b = A.list != nullptr ? A.list[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int2 m0 = A.meta[b];
   const int2 m1 = A.meta[b + 1];
-  const int inst = A.binst != nullptr ? A.binst[b] : 0;
+  const int inst = A.inst_from_y ? static_cast<int>(blockIdx.y) : A.binst != nullptr ? A.binst[b] : 0;
+  if (op.skip(inst)) return;  // uniform over the workgroup, before any barrier
   const int r0 = m0.x, p0 = m0.y;
   const int nrows = m1.x - m0.x;
   const int nnz = m1.y - m0.y;
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
     for (int u = 0; u < kPer; ++u) {
       const bool live = tid + u * kBlock < nnz;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, c[u]) : Acc(0);
+      for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, c[u], inst) : Acc(0);
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -178,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) gv[v][u] = p + u * kBlock < end ? op.gather(v, cl[u]) : Acc(0);
+        for (int v = 0; v < NV; ++v) gv[v][u] = p + u * kBlock < end ? op.gather(v, cl[u], inst) : Acc(0);
       }
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
@@ -205,7 +211,9 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
     for (int v = 0; v < NV; ++v) acc[v] = rowsum[v][tid];
     op.row(r0 + tid, inst, acc, pre, sums);
   }
-  if constexpr (Op::kSums > 0) block_partials<Op::kSums>(sums, op.partial + b, op.stride);
+  if constexpr (Op::kSums > 0)
+    block_partials<Op::kSums, Op::kMaxFrom>(sums, op.partial + static_cast<size_t>(inst) * op.inst_stride + b,
+                                           op.stride);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -223,8 +231,9 @@ struct SpmvOp {
   static constexpr int kSums = 0;
   const X* __restrict__ x;
   X* __restrict__ y;
+  __device__ bool skip(int) const { return false; }
   __device__ Pre pre(int, int) const { return {}; }
-  __device__ Acc gather(int, int c) const { return x[c]; }
+  __device__ Acc gather(int, int c, int) const { return x[c]; }
   __device__ void row(int r, int, const Acc* acc, const Pre&, double*) const { y[r] = acc[0]; }
 };
 
@@ -257,11 +266,12 @@ struct DualRowsOp {
   const T* __restrict__ cost;
   const T* __restrict__ lower;
   const T* __restrict__ upper;
+  __device__ bool skip(int) const { return false; }
   __device__ Pre pre(int i, int) const {
     const int k = n + i;
     return Pre{y[i], y_erg[i], vbar[k], v[k], v_erg[k], cost[k], lower[k], upper[k]};
   }
-  __device__ Acc gather(int, int c) const { return vbar[c]; }
+  __device__ Acc gather(int, int c, int) const { return vbar[c]; }
   __device__ void row(int i, int inst, const Acc* acc, const Pre& p, double*) const {
     const StepScalars& s = sc[inst];
     const T sigma = static_cast<T>(s.sigma);
@@ -303,10 +313,11 @@ struct PrimalColsOp {
   const T* __restrict__ cost;
   const T* __restrict__ lower;
   const T* __restrict__ upper;
+  __device__ bool skip(int) const { return false; }
   __device__ Pre pre(int col, int) const {
     return Pre{v[col], v_erg[col], cost[col], lower[col], upper[col]};
   }
-  __device__ Acc gather(int, int c) const { return y_new[c]; }
+  __device__ Acc gather(int, int c, int) const { return y_new[c]; }
   __device__ void row(int col, int inst, const Acc* acc, const Pre& p, double*) const {
     const StepScalars& s = sc[inst];
     const T tau = static_cast<T>(s.tau);
@@ -346,6 +357,7 @@ struct MeasureRowsOp {
   using Pre = MeasurePre<NV>;
   static constexpr int kVec = NV;
   static constexpr int kSums = 7 * NV;
+  static constexpr int kMaxFrom = kSums;
   int n;
   const T* __restrict__ vv[NV];
   const T* __restrict__ yy[NV];
@@ -354,7 +366,9 @@ struct MeasureRowsOp {
   const double* __restrict__ lower;
   const double* __restrict__ upper;
   double* __restrict__ partial;
-  int stride;
+  size_t stride;
+  int inst_stride = 0;
+  __device__ bool skip(int) const { return false; }
   __device__ Pre pre(int i, int) const {
     const int k = n + i;
     Pre p;
@@ -369,7 +383,7 @@ struct MeasureRowsOp {
     }
     return p;
   }
-  __device__ Acc gather(int v, int c) const { return static_cast<double>(vv[v][c]); }
+  __device__ Acc gather(int v, int c, int) const { return static_cast<double>(vv[v][c]); }
   __device__ void row(int, int, const Acc* acc, const Pre& p, double* s) const {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -396,6 +410,7 @@ struct MeasureColsOp {
   using Pre = MeasurePre<NV>;
   static constexpr int kVec = NV;
   static constexpr int kSums = 7 * NV;
+  static constexpr int kMaxFrom = kSums;
   const T* __restrict__ vv[NV];
   const T* __restrict__ yy[NV];
   const double* __restrict__ cs;
@@ -403,7 +418,9 @@ struct MeasureColsOp {
   const double* __restrict__ lower;
   const double* __restrict__ upper;
   double* __restrict__ partial;
-  int stride;
+  size_t stride;
+  int inst_stride = 0;
+  __device__ bool skip(int) const { return false; }
   __device__ Pre pre(int col, int) const {
     Pre p;
     p.scale = cs[col];
@@ -417,7 +434,7 @@ struct MeasureColsOp {
     }
     return p;
   }
-  __device__ Acc gather(int v, int c) const { return static_cast<double>(yy[v][c]); }
+  __device__ Acc gather(int v, int c, int) const { return static_cast<double>(yy[v][c]); }
   __device__ void row(int, int, const Acc* acc, const Pre& p, double* s) const {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -447,16 +464,18 @@ struct BlockRows {
   const int2* __restrict__ meta_t;
   const int* __restrict__ binst_t;
   int n;  // structural variables of the stack: v_{n+i} is row i's logical variable
+  bool inst_from_y = false;  // the instance is blockIdx.y (a shared matrix's schedules)
+  int first = 0;             // workgroup w covers block first + w of the combined schedule
 };
 
 template <typename Op>
 __global__ __launch_bounds__(kBlock) void block_rows_kernel(BlockRows R, Op op) {
   constexpr int NS = Op::kSums > 0 ? Op::kSums : 1;
-  const int w = blockIdx.x;
+  const int w = R.first + static_cast<int>(blockIdx.x);
   const bool is_a = w < R.nba;
   const int b = is_a ? w : w - R.nba;
   const int2* meta = is_a ? R.meta_a : R.meta_t;
-  const int inst = (is_a ? R.binst_a : R.binst_t)[b];
+  const int inst = R.inst_from_y ? static_cast<int>(blockIdx.y) : (is_a ? R.binst_a : R.binst_t)[b];
   const int r0 = meta[b].x;
   const int nrows = meta[b + 1].x - r0;
   const int tid = threadIdx.x;
@@ -469,11 +488,13 @@ __global__ __launch_bounds__(kBlock) void block_rows_kernel(BlockRows R, Op op) 
   if (flag < 0) return;
   if (tid < nrows) {
     if (is_a)
-      op.row_a(r0 + tid, R.n + r0 + tid, flag, local);
+      op.row_a(r0 + tid, R.n + r0 + tid, flag, local, inst);
     else
-      op.col(r0 + tid, flag, local);
+      op.col(r0 + tid, flag, local, inst);
   }
-  if constexpr (Op::kSums > 0) block_partials<Op::kSums>(local, op.partial + w, op.stride);
+  if constexpr (Op::kSums > 0)
+    block_partials<Op::kSums, Op::kMaxFrom>(local, op.partial + static_cast<size_t>(inst) * op.inst_stride + w,
+                                           op.stride);
 }
 
 // Squared distances of the candidate (current or average, by the instance's flag) from the
@@ -481,6 +502,7 @@ __global__ __launch_bounds__(kBlock) void block_rows_kernel(BlockRows R, Op op) 
 template <typename T>
 struct DistanceOp {
   static constexpr int kSums = 2;
+  static constexpr int kMaxFrom = 2;
   const int* __restrict__ which;  // per instance: -1 none, 0 current (v_new, y_new), 1 average
   const T* __restrict__ v_new;
   const T* __restrict__ y_new;
@@ -489,15 +511,16 @@ struct DistanceOp {
   const T* __restrict__ v_rst;
   const T* __restrict__ y_rst;
   double* __restrict__ partial;
-  int stride;
+  size_t stride;
+  int inst_stride = 0;
   __device__ int flag(int inst) const { return which[inst]; }
-  __device__ void row_a(int i, int k, int f, double* s) const {
+  __device__ void row_a(int i, int k, int f, double* s, int) const {
     const double dv = static_cast<double>(f == 0 ? v_new[k] : v_erg[k]) - static_cast<double>(v_rst[k]);
     const double dy = static_cast<double>(f == 0 ? y_new[i] : y_erg[i]) - static_cast<double>(y_rst[i]);
     s[0] += dv * dv;
     s[1] += dy * dy;
   }
-  __device__ void col(int k, int f, double* s) const {
+  __device__ void col(int k, int f, double* s, int) const {
     const double dv = static_cast<double>(f == 0 ? v_new[k] : v_erg[k]) - static_cast<double>(v_rst[k]);
     s[0] += dv * dv;
   }
@@ -529,14 +552,14 @@ struct RestartOp {
     vbar0[k] = c;
     vbar1[k] = c;
   }
-  __device__ void row_a(int i, int k, int f, double*) const {
+  __device__ void row_a(int i, int k, int f, double*, int) const {
     set_v(k, f);
     const T c = f == 0 ? y_new[i] : y_erg[i];
     y_rst[i] = c;
     y[i] = c;
     y_new[i] = c;
   }
-  __device__ void col(int k, int f, double*) const { set_v(k, f); }
+  __device__ void col(int k, int f, double*, int) const { set_v(k, f); }
 };
 
 // The iterate state of the flagged instances from one precision to the other.
@@ -549,12 +572,12 @@ struct ConvertOp {
   const From* yin[4];  // y[2], y_erg, y_rst
   To* yout[4];
   __device__ int flag(int inst) const { return which[inst]; }
-  __device__ void col(int k, int, double*) const {
+  __device__ void col(int k, int, double*, int) const {
 #pragma unroll
     for (int q = 0; q < 6; ++q) vout[q][k] = static_cast<To>(vin[q][k]);
   }
-  __device__ void row_a(int i, int k, int f, double* s) const {
-    col(k, f, s);
+  __device__ void row_a(int i, int k, int f, double* s, int inst) const {
+    col(k, f, s, inst);
 #pragma unroll
     for (int q = 0; q < 4; ++q) yout[q][i] = static_cast<To>(yin[q][i]);
   }

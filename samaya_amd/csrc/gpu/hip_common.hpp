@@ -41,6 +41,13 @@ __device__ inline T group_sum(T v) {
   return v;
 }
 
+template <int G, typename T>
+__device__ inline T group_max(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, G));
+  return v;
+}
+
 template <typename T>
 __device__ inline T group_sum_dyn(T v, int g) {
   for (int off = g >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

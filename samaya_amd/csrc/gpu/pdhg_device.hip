@@ -744,6 +744,138 @@ int set_original_impl(smy_pdhg* h, int i, const double* col_scale, const double*
   });
 }
 
+// ---- Device-side infeasibility / unboundedness tests (stacked layout) ----
+
+void ensure_detect(smy_pdhg* h) {
+  if (h->det_partial != nullptr) return;
+  h->det_partial = h->alloc<double>(static_cast<long long>(kDetectSlots) * std::max(1, stride(h)));
+  h->det_scratch = h->alloc<double>(std::max(1, h->n));
+  h->det_ray = h->alloc<double>(std::max(1, h->n));
+  h->det_prm = h->zeros<double>(3LL * h->K);
+  h->det_out = h->alloc<double>(static_cast<long long>(kDetectSlots) * h->K);
+  SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->det_prm_host), 3 * sizeof(double) * h->K,
+                        hipHostMallocDefault));
+  SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->det_out_host), kDetectSlots * sizeof(double) * h->K,
+                        hipHostMallocDefault));
+}
+
+template <typename T, template <typename, bool> class Op>
+Op<T, false> stacked_detect_op(smy_pdhg* h, int slot) {
+  Iter<T>& S = state<T>(h);
+  Op<T, false> op{};
+  op.n = h->n;
+  op.kp = 1;
+  op.v = S.v[h->p];
+  op.v_new = S.v[1 - h->p];
+  op.y_erg = S.y_erg;
+  op.rs = h->rs;
+  op.cs = h->cs;
+  op.cost = h->cost_o;
+  op.lower = h->lower_o;
+  op.upper = h->upper_o;
+  op.row_lower = nullptr;
+  op.row_upper = nullptr;
+  op.scratch = h->det_scratch;
+  op.ray = h->det_ray;
+  op.prm = h->det_prm;
+  op.flags = h->flags + prec_of<T>() * h->K;
+  op.partial = h->det_partial + static_cast<size_t>(slot) * stride(h);
+  op.stride = static_cast<size_t>(stride(h));
+  op.inst_stride = 0;
+  return op;
+}
+
+template <typename Op>
+void detect_csr(smy_pdhg* h, const DevMatrix& M, const Op& op) {
+  if (M.nblocks == 0) return;
+  CsrRows<double> A;
+  A.rows = M.rows;
+  A.nblocks = M.nblocks;
+  A.ptr = M.ptr;
+  A.idx = M.idx;
+  A.val = M.val64;
+  A.meta = M.meta;
+  A.binst = M.binst;
+  hipLaunchKernelGGL((csr_rows_kernel<double, Op>), dim3(M.nblocks), dim3(kBlock), 0, h->stream, A, op);
+}
+
+template <typename Op>
+void detect_rows(smy_pdhg* h, const Op& op, int first, int count) {
+  if (count == 0) return;
+  BlockRows R = block_rows(h);
+  R.first = first;
+  hipLaunchKernelGGL((block_rows_kernel<Op>), dim3(count), dim3(kBlock), 0, h->stream, R, op);
+}
+
+template <typename T>
+void stacked_detect_phase(smy_pdhg* h, int phase) {
+  const int nba = h->a.nblocks, nbt = h->at.nblocks;
+  if (phase == 1) {
+    // Block partials of A' rows follow those of A (the instance ranges' layout).
+    auto cols = stacked_detect_op<T, FarkasColsOp>(h, 0);
+    cols.partial += nba;
+    detect_csr(h, h->at, cols);
+    detect_rows(h, stacked_detect_op<T, FarkasRowsOp>(h, 0), 0, nba);
+    detect_rows(h, stacked_detect_op<T, RayMaxOp>(h, 7), 0, nba + nbt);
+  } else {
+    detect_rows(h, stacked_detect_op<T, FarkasBoundsOp>(h, 2), 0, nba + nbt);
+    detect_csr(h, h->a, stacked_detect_op<T, RayRowsOp>(h, 8));
+    detect_rows(h, stacked_detect_op<T, RayColsOp>(h, 9), nba, nbt);
+    detect_csr(h, h->a, stacked_detect_op<T, RayCheckOp>(h, 10));
+  }
+}
+
+void zero_slots(smy_pdhg* h, double* partial, size_t slot_stride, int first, int count) {
+  SMY_HIP(hipMemsetAsync(partial + static_cast<size_t>(first) * slot_stride, 0,
+                         static_cast<size_t>(count) * slot_stride * sizeof(double), h->stream));
+}
+
+// Both phases; the host forms drop, y_max and d_max between them.
+int stacked_detect(smy_pdhg* h, const int* flags, double* out) {
+  refresh_lists(h);
+  ensure_detect(h);
+  stage_flags(h, flags);
+  const bool f32 = any_flag(h, flags, SMY_FP32), f64 = any_flag(h, flags, SMY_FP64);
+  const size_t st = static_cast<size_t>(stride(h));
+  const auto finish_det = [&]() {
+    hipLaunchKernelGGL(finish_detect_kernel, dim3(kDetectSlots, h->K), dim3(kBlock), 0, h->stream, st,
+                       h->det_partial, h->ranges, h->det_out);
+    SMY_HIP(hipGetLastError());
+    SMY_HIP(hipMemcpyAsync(h->det_out_host, h->det_out, kDetectSlots * sizeof(double) * h->K,
+                           hipMemcpyDeviceToHost, h->stream));
+    SMY_HIP(hipStreamSynchronize(h->stream));
+  };
+  if (st > 0) {
+    zero_slots(h, h->det_partial, st, 0, 2);
+    zero_slots(h, h->det_partial, st, 7, 1);
+    if (f32) stacked_detect_phase<float>(h, 1);
+    if (f64) stacked_detect_phase<double>(h, 1);
+  }
+  finish_det();
+  staging_wait(h);
+  for (int i = 0; i < h->K; ++i) {
+    const double* o = h->det_out_host + static_cast<size_t>(i) * kDetectSlots;
+    h->det_prm_host[3 * i] = 1e-9 * o[0];
+    h->det_prm_host[3 * i + 1] = o[1];
+    h->det_prm_host[3 * i + 2] = o[7];
+    for (int q = 0; q < kDetectSlots; ++q) out[static_cast<size_t>(i) * kDetectSlots + q] = o[q];
+  }
+  SMY_HIP(hipMemcpyAsync(h->det_prm, h->det_prm_host, 3 * sizeof(double) * h->K, hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+  if (st > 0) {
+    zero_slots(h, h->det_partial, st, 2, 5);
+    zero_slots(h, h->det_partial, st, 8, 3);
+    if (f32) stacked_detect_phase<float>(h, 2);
+    if (f64) stacked_detect_phase<double>(h, 2);
+  }
+  finish_det();
+  for (int i = 0; i < h->K; ++i)
+    for (int q : {2, 3, 4, 5, 6, 8, 9, 10})
+      out[static_cast<size_t>(i) * kDetectSlots + q] = h->det_out_host[static_cast<size_t>(i) * kDetectSlots + q];
+  return SMY_OK;
+}
+
 bool single(smy_pdhg* h) { return h->K == 1; }
 
 }  // namespace
@@ -1013,6 +1145,28 @@ int smy_pdhg_set_start_at(smy_pdhg* h, int instance, const double* v, const doub
   return guarded([&]() -> int {
     if (L.last_prec == SMY_FP64) set_start_impl<double>(h, instance, v, y);
     else set_start_impl<float>(h, instance, v, y);
+    return SMY_OK;
+  });
+}
+
+int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out) {
+  if (h == nullptr || flags == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  for (int i = 0; i < h->K; ++i)
+    if (flags[i] > 0 && (h->inst[i].prec == kFrozen || !h->inst[i].has_original))
+      return fail(SMY_STATE, "detection needs a running instance with set_original()");
+  if (h->shared) return guarded([&]() -> int { return shared_detect(h, flags, out); });
+  return guarded([&]() -> int { return stacked_detect(h, flags, out); });
+}
+
+int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray) {
+  SMY_INSTANCE(h, instance);
+  if (ray == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->det_ray == nullptr) return fail(SMY_STATE, "no detection has run");
+  if (h->shared) return guarded([&]() -> int { return shared_download_ray(h, instance, ray); });
+  return guarded([&]() -> int {
+    const Layout& L = h->inst[instance];
+    if (L.n > 0)
+      SMY_HIP(hipMemcpy(ray, h->det_ray + L.ox, L.n * sizeof(double), hipMemcpyDeviceToHost));
     return SMY_OK;
   });
 }

@@ -14,6 +14,7 @@
 
 #include "../../../include/samaya_pdhg.h"
 #include "csr_kernels.hpp"
+#include "detect_kernels.hpp"
 #include "hip_common.hpp"
 #include "row_blocks.hpp"
 
@@ -161,6 +162,15 @@ struct smy_pdhg {
   int list_gen = 0;
   bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
   SharedBatch* shared = nullptr;  // the shared-matrix batch (then the stacked fields are unused)
+  // Device-side infeasibility / unboundedness tests (allocated at their first use).
+  double* det_partial = nullptr;   // detection slots x blocks
+  double* det_scratch = nullptr;   // per structural variable and instance
+  double* det_ray = nullptr;
+  double* det_prm = nullptr;       // per instance: drop, y_max, d_max
+  double* det_prm_host = nullptr;  // pinned
+  double* det_out = nullptr;       // per instance: the detection slots
+  double* det_out_host = nullptr;  // pinned
+  int2* det_ranges = nullptr;      // per instance: its blocks in the detection partials
   std::vector<void*> allocations;
 
   template <typename T>
@@ -195,6 +205,8 @@ struct smy_pdhg {
       for (auto& c : row) drop_graph(c);
     for (void* ptr : allocations) (void)hipFree(ptr);
     if (sums_host) (void)hipHostFree(sums_host);
+    if (det_prm_host) (void)hipHostFree(det_prm_host);
+    if (det_out_host) (void)hipHostFree(det_out_host);
     if (scalars_host) (void)hipHostFree(scalars_host);
     if (flags_host) (void)hipHostFree(flags_host);
     if (ev_staged) (void)hipEventDestroy(ev_staged);
@@ -237,5 +249,7 @@ int shared_download(smy_pdhg* h, int k, double* v, double* v_new, double* y, dou
                     double* v_erg, double* y_erg);
 int shared_set_start(smy_pdhg* h, int k, const double* v, const double* y);
 int shared_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
+int shared_detect(smy_pdhg* h, const int* flags, double* out);
+int shared_download_ray(smy_pdhg* h, int k, double* ray);
 
 }  // namespace smy

@@ -49,6 +49,10 @@ struct SharedBatch {
   double* t_val64 = nullptr;
   float* t_val32 = nullptr;
   long long nnz = 0;
+  // Row-block schedules of both matrices (the detection products run per instance over them).
+  int2* a_meta = nullptr;
+  int2* t_meta = nullptr;
+  int nba = 0, nbt = 0;
   SharedIter<double> d;
   SharedIter<float> f;
   std::vector<int> mode_host;  // kp: SMY_FP64 / SMY_FP32 / kFrozen (padding: frozen)
@@ -461,7 +465,159 @@ int time_part(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   return SMY_OK;
 }
 
+// ---- Device-side infeasibility / unboundedness tests (shared layout): the stacked passes with
+// the instance from the grid's y dimension and instance-minor indexing ----
+
+size_t det_stride(const SharedBatch* s) { return static_cast<size_t>(s->kp) * (s->nba + s->nbt); }
+
+void ensure_detect_shared(smy_pdhg* h) {
+  if (h->det_partial != nullptr) return;
+  SharedBatch* s = h->shared;
+  h->det_partial = h->alloc<double>(static_cast<long long>(kDetectSlots) * std::max<size_t>(1, det_stride(s)));
+  h->det_scratch = h->alloc<double>(std::max<long long>(1, static_cast<long long>(s->n) * s->kp));
+  h->det_ray = h->alloc<double>(std::max<long long>(1, static_cast<long long>(s->n) * s->kp));
+  h->det_prm = h->zeros<double>(3LL * s->kp);
+  h->det_out = h->alloc<double>(static_cast<long long>(kDetectSlots) * s->kp);
+  std::vector<int2> ranges(2 * static_cast<size_t>(s->kp));
+  const int S = s->nba + s->nbt;
+  for (int k = 0; k < s->kp; ++k) {
+    ranges[2 * k] = make_int2(k * S, k * S + s->nba);
+    ranges[2 * k + 1] = make_int2(k * S + s->nba, (k + 1) * S);
+  }
+  h->det_ranges = h->upload(ranges.data(), static_cast<long long>(ranges.size()));
+  SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->det_prm_host), 3 * sizeof(double) * s->kp,
+                        hipHostMallocDefault));
+  SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->det_out_host), kDetectSlots * sizeof(double) * s->kp,
+                        hipHostMallocDefault));
+}
+
+template <typename T, template <typename, bool> class Op>
+Op<T, true> shared_detect_op(smy_pdhg* h, int slot) {
+  SharedBatch* s = h->shared;
+  SharedIter<T>& S = sstate<T>(s);
+  Op<T, true> op{};
+  op.n = s->n;
+  op.kp = s->kp;
+  op.v = S.v[h->p];
+  op.v_new = S.v[1 - h->p];
+  op.y_erg = S.y_erg;
+  op.rs = s->rs;
+  op.cs = s->cs;
+  op.cost = s->cost_o;
+  op.lower = s->col_lower_o;
+  op.upper = s->col_upper_o;
+  op.row_lower = s->row_lower_o;
+  op.row_upper = s->row_upper_o;
+  op.scratch = h->det_scratch;
+  op.ray = h->det_ray;
+  op.prm = h->det_prm;
+  op.flags = s->which + (std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32) * s->kp;
+  op.partial = h->det_partial + static_cast<size_t>(slot) * det_stride(s);
+  op.stride = det_stride(s);
+  op.inst_stride = s->nba + s->nbt;
+  return op;
+}
+
+template <typename Op>
+void shared_detect_csr(smy_pdhg* h, bool rows_of_a, const Op& op) {
+  SharedBatch* s = h->shared;
+  const int nb = rows_of_a ? s->nba : s->nbt;
+  if (nb == 0) return;
+  CsrRows<double> A;
+  A.rows = rows_of_a ? s->m : s->n;
+  A.nblocks = nb;
+  A.ptr = rows_of_a ? s->a_ptr : s->t_ptr;
+  A.idx = rows_of_a ? s->a_idx : s->t_idx;
+  A.val = rows_of_a ? s->a_val64 : s->t_val64;
+  A.meta = rows_of_a ? s->a_meta : s->t_meta;
+  A.inst_from_y = true;
+  hipLaunchKernelGGL((csr_rows_kernel<double, Op>), dim3(nb, s->K), dim3(kBlock), 0, h->stream, A, op);
+}
+
+template <typename Op>
+void shared_detect_rows(smy_pdhg* h, const Op& op, int first, int count) {
+  SharedBatch* s = h->shared;
+  if (count == 0) return;
+  BlockRows R{s->a_meta, nullptr, s->nba, s->t_meta, nullptr, s->n};
+  R.inst_from_y = true;
+  R.first = first;
+  hipLaunchKernelGGL((block_rows_kernel<Op>), dim3(count, s->K), dim3(kBlock), 0, h->stream, R, op);
+}
+
+template <typename T>
+void shared_detect_phase(smy_pdhg* h, int phase) {
+  SharedBatch* s = h->shared;
+  const int nba = s->nba, nbt = s->nbt;
+  if (phase == 1) {
+    // Block partials of A' rows follow those of A within each instance's range.
+    auto cols = shared_detect_op<T, FarkasColsOp>(h, 0);
+    cols.partial += nba;
+    shared_detect_csr(h, false, cols);
+    shared_detect_rows(h, shared_detect_op<T, FarkasRowsOp>(h, 0), 0, nba);
+    shared_detect_rows(h, shared_detect_op<T, RayMaxOp>(h, 7), 0, nba + nbt);
+  } else {
+    shared_detect_rows(h, shared_detect_op<T, FarkasBoundsOp>(h, 2), 0, nba + nbt);
+    shared_detect_csr(h, true, shared_detect_op<T, RayRowsOp>(h, 8));
+    shared_detect_rows(h, shared_detect_op<T, RayColsOp>(h, 9), nba, nbt);
+    shared_detect_csr(h, true, shared_detect_op<T, RayCheckOp>(h, 10));
+  }
+}
+
 }  // namespace
+
+int shared_detect(smy_pdhg* h, const int* flags, double* out) {
+  SharedBatch* s = h->shared;
+  ensure_detect_shared(h);
+  stage_which(h, flags);
+  const bool f32 = any_which(s, flags, SMY_FP32), f64 = any_which(s, flags, SMY_FP64);
+  const size_t st = det_stride(s);
+  const auto zero = [&](int first, int count) {
+    SMY_HIP(hipMemsetAsync(h->det_partial + static_cast<size_t>(first) * st, 0,
+                           static_cast<size_t>(count) * st * sizeof(double), h->stream));
+  };
+  const auto finish_det = [&]() {
+    hipLaunchKernelGGL(finish_detect_kernel, dim3(kDetectSlots, s->K), dim3(kBlock), 0, h->stream, st,
+                       h->det_partial, h->det_ranges, h->det_out);
+    SMY_HIP(hipGetLastError());
+    SMY_HIP(hipMemcpyAsync(h->det_out_host, h->det_out, kDetectSlots * sizeof(double) * s->K,
+                           hipMemcpyDeviceToHost, h->stream));
+    SMY_HIP(hipStreamSynchronize(h->stream));
+  };
+  if (st > 0) {
+    zero(0, 2);
+    zero(7, 1);
+    if (f32) shared_detect_phase<float>(h, 1);
+    if (f64) shared_detect_phase<double>(h, 1);
+  }
+  finish_det();
+  staging_wait(h);
+  for (int k = 0; k < s->K; ++k) {
+    const double* o = h->det_out_host + static_cast<size_t>(k) * kDetectSlots;
+    h->det_prm_host[3 * k] = 1e-9 * o[0];
+    h->det_prm_host[3 * k + 1] = o[1];
+    h->det_prm_host[3 * k + 2] = o[7];
+    for (int q = 0; q < kDetectSlots; ++q) out[static_cast<size_t>(k) * kDetectSlots + q] = o[q];
+  }
+  SMY_HIP(hipMemcpyAsync(h->det_prm, h->det_prm_host, 3 * sizeof(double) * s->K, hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+  if (st > 0) {
+    zero(2, 5);
+    zero(8, 3);
+    if (f32) shared_detect_phase<float>(h, 2);
+    if (f64) shared_detect_phase<double>(h, 2);
+  }
+  finish_det();
+  for (int k = 0; k < s->K; ++k)
+    for (int q : {2, 3, 4, 5, 6, 8, 9, 10})
+      out[static_cast<size_t>(k) * kDetectSlots + q] = h->det_out_host[static_cast<size_t>(k) * kDetectSlots + q];
+  return SMY_OK;
+}
+
+int shared_download_ray(smy_pdhg* h, int k, double* ray) {
+  column_to_host<double>(h, h->det_ray, h->shared->n, k, ray);
+  return SMY_OK;
+}
 
 int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* cost,
                   const double* const* lower, const double* const* upper, int precision,
@@ -504,15 +660,18 @@ int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* 
     SMY_HIP(hipEventCreate(&h->ev_stop));
     SMY_HIP(hipEventCreateWithFlags(&h->ev_staged, hipEventDisableTiming));
     // A by rows = the CSC of A' read as rows; A' by rows = the CSC of A.
-    const auto rows_of = [&](const smy_csc* c, int*& ptr, int*& idx, double*& val) {
+    const auto rows_of = [&](const smy_csc* c, int*& ptr, int*& idx, double*& val, int2*& meta, int& nb) {
       std::vector<int> p(static_cast<size_t>(c->cols) + 1);
       for (int r = 0; r <= c->cols; ++r) p[r] = static_cast<int>(c->start[r]);
       ptr = h->upload(p.data(), static_cast<long long>(p.size()));
       idx = h->upload(c->index, nnz);
       val = h->upload(c->value, nnz);
+      const std::vector<RowBlockMeta> bl = build_row_blocks(p.data(), c->cols);
+      nb = static_cast<int>(bl.size()) - 1;
+      meta = h->upload(reinterpret_cast<const int2*>(bl.data()), static_cast<long long>(bl.size()));
     };
-    rows_of(At, s->a_ptr, s->a_idx, s->a_val64);
-    rows_of(A, s->t_ptr, s->t_idx, s->t_val64);
+    rows_of(At, s->a_ptr, s->a_idx, s->a_val64, s->a_meta, s->nba);
+    rows_of(A, s->t_ptr, s->t_idx, s->t_val64, s->t_meta, s->nbt);
     alloc_state<double>(h.get(), s->d);
     s->d.cost = h->upload(cost, nt);
     SMY_HIP(hipMemcpy(s->d.row_lower, lower[0] + n, m * sizeof(double), hipMemcpyHostToDevice));

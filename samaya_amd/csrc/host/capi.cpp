@@ -49,6 +49,8 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.log_level = p->log_level;
   o.algorithm = p->algorithm == 1 ? Algorithm::kEnhanced : Algorithm::kReference;
   o.share_matrix = p->share_matrix != 0;
+  o.device_detect = p->device_detect != 0;
+  o.start_omega = p->start_omega;
   return o;
 }
 
@@ -61,6 +63,7 @@ void fill_info(const PdlpResult& r, smy_pdlp_info* info) {
   info->dual_residual = r.dual_residual;
   info->gap = r.gap;
   info->operator_norm = r.operator_norm;
+  info->omega = r.omega;
   info->setup_seconds = r.setup_seconds;
   info->solve_seconds = r.solve_seconds;
   info->kernel_ms = r.kernel_ms;
@@ -125,6 +128,8 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->polish_rounds = 0;
   p->algorithm = SMY_ALGORITHM_REFERENCE;
   p->share_matrix = 1;
+  p->device_detect = 1;
+  p->start_omega = 0.0;
 }
 
 int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,

@@ -37,6 +37,10 @@ namespace {
 constexpr long long kCheckFrequency = 64;
 constexpr long long kDetectionEvery = 8;
 constexpr double kWeightDistanceFloor = 1e-10;
+// Per-instance numbers of the device-side detection (include/samaya_pdhg.h,
+// smy_pdhg_detect_batch): sum |w|, max |yf|, f_min and f_max (finite parts) with their counts of
+// infinite terms, scale, d_max, ||A x - r||^2, cost slope, bound failures.
+constexpr int kDetectSlotsHost = 11;
 
 class Clock {
  public:
@@ -140,6 +144,14 @@ class Device {
     check(smy_pdhg_set_precision_batch(h_, precision.data()));
   }
   void freeze(int i) { check(smy_pdhg_freeze(h_, i)); }
+  void detect(const std::vector<int>& flags, std::vector<double>& out) {
+    out.resize(static_cast<size_t>(kDetectSlotsHost) * k_);
+    check(smy_pdhg_detect_batch(h_, flags.data(), out.data()));
+  }
+  void download_ray(int i, std::vector<double>& ray, int n) {
+    ray.assign(static_cast<size_t>(n), 0.0);
+    check(smy_pdhg_download_ray(h_, i, ray.data()));
+  }
   void advance() { check(smy_pdhg_advance(h_)); }
   void download(int i, std::vector<double>* v, std::vector<double>* v_new, std::vector<double>* y,
                 std::vector<double>* y_new, std::vector<double>* v_erg, std::vector<double>* y_erg,
@@ -321,6 +333,9 @@ class Lane {
   }
   bool detect_farkas(long long iter, const std::vector<double>& y_erg);
   bool detect_ray(long long iter, const std::vector<double>& v, const std::vector<double>& v_new);
+  // The same decisions from the device-side totals (11 numbers, smy_pdhg_detect_batch).
+  bool farkas_from(long long iter, const double* o) const;
+  bool ray_from(long long iter, const double* o) const;
   void map_back(const std::vector<double>& v, const std::vector<double>& y);
 
   // The bounds of this LP: the task's own when given, else the problem's.
@@ -411,7 +426,7 @@ void Lane::prepare(std::shared_ptr<const Equilibrated> e) {
   const double L = eq->L;
   res.operator_norm = L;
   eta = 0.99 / L;
-  omega = 1.0;
+  omega = opt_.start_omega > 0.0 ? opt_.start_omega : 1.0;
   tau = eta / omega;
   sigma = eta * omega;
   log(3, ": estimated ||[A -I]|| = %.3e", L);
@@ -568,6 +583,34 @@ bool Lane::detect_ray(long long iter, const std::vector<double>& v, const std::v
   return false;
 }
 
+bool Lane::farkas_from(long long iter, const double* o) const {
+  // src/lp/pdlp.cpp:724-842 from the device totals.
+  const double y_max = o[1];
+  if (!(y_max > 1e-6)) return false;
+  const double f_min = o[4] > 0.0 ? -INFINITY : o[2];
+  const double f_max = o[5] > 0.0 ? INFINITY : o[3];
+  double scale = o[6];
+  if (!std::isfinite(scale)) scale = 0.0;
+  const double margin = 1e-8 * (1.0 + scale);
+  log(2, ": infeas check iter=%lld y_max=%.2e min_f=%.2e max_f=%.2e margin=%.2e scale=%.2e", iter,
+      y_max, f_min, f_max, margin, scale);
+  if ((std::isfinite(f_min) && f_min > margin) || (std::isfinite(f_max) && f_max < -margin)) {
+    log(1, ": detected primal infeasibility at iter %lld (Farkas certificate)", iter);
+    return true;
+  }
+  return false;
+}
+
+bool Lane::ray_from(long long iter, const double* o) const {
+  // src/lp/pdlp.cpp:852-1042 from the device totals.
+  if (!(o[7] > 1e-6)) return false;
+  if (std::sqrt(o[8]) < 1e-6 && o[9] < -1e-8 && o[10] == 0.0) {
+    log(1, ": detected dual infeasibility (unbounded) at iter %lld (improving ray)", iter);
+    return true;
+  }
+  return false;
+}
+
 void Lane::map_back(const std::vector<double>& v, const std::vector<double>& y) {
   // src/lp/pdlp.cpp:1167-1189: x = cs v_x, r = v_r / rs, y = rs y'.
   const int m = lp_.m, n = lp_.n;
@@ -707,7 +750,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   if (info != nullptr) info->setup_seconds = setup_seconds;
 
   const double theta = 1.0;
-  std::vector<double> tau(static_cast<size_t>(K)), sigma(static_cast<size_t>(K)), sums, dist;
+  std::vector<double> tau(static_cast<size_t>(K)), sigma(static_cast<size_t>(K)), sums, dist, dout;
   std::vector<long long> k0(static_cast<size_t>(K));
   std::vector<int> which(static_cast<size_t>(K)), precision(static_cast<size_t>(K));
   std::vector<double> v, v_new, y, y_new, v_erg, y_erg;
@@ -777,15 +820,19 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     device->evaluate(sums);
     std::fill(which.begin(), which.end(), -1);
     std::fill(precision.begin(), precision.end(), -1);
-    std::vector<char> force(static_cast<size_t>(K), 0);
+    std::vector<char> force(static_cast<size_t>(K), 0), due(static_cast<size_t>(K), 0);
     std::vector<double> kkt_candidate(static_cast<size_t>(K), 0.0);
+    std::vector<Measure> cur_m(static_cast<size_t>(K)), avg_m(static_cast<size_t>(K));
+    // Measures and termination of the lanes at an evaluation point (every 64th iteration and
+    // their last).
     for (int i = 0; i < K; ++i) {
       Lane& L = lanes[i];
-      // A lane is evaluated at its own evaluation points only: every 64th iteration and its last.
       if (!L.active || ((iter + 1) % kCheckFrequency != 0 && iter != L.max_iter - 1)) continue;
       const double* s14 = &sums[14 * static_cast<size_t>(i)];
       const Measure current = L.finish(Sums{s14[0], s14[1], s14[2], s14[3], s14[4], s14[5], s14[6]});
       const Measure average = L.finish(Sums{s14[7], s14[8], s14[9], s14[10], s14[11], s14[12], s14[13]});
+      cur_m[i] = current;
+      avg_m[i] = average;
       L.res.evals.push_back({iter,
                              {current.rel_primal, current.rel_dual, current.rel_gap},
                              {average.rel_primal, average.rel_dual, average.rel_gap}});
@@ -817,13 +864,41 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         stop(i);
         continue;
       }
+      due[i] = 1;
+    }
 
-      // Detection (src/lp/pdlp.cpp:715-1043), at every 8th evaluation after iteration 1000.
-      const bool detect = opt.detect && iter > 1000 && m > 0 &&
-                          ((iter + 1) / kCheckFrequency) % kDetectionEvery == 0;
-      if (detect) {
+    // Detection (src/lp/pdlp.cpp:715-1043), at every 8th evaluation after iteration 1000: the
+    // Farkas test, then the ray test. On the device for all lanes at once (only per-instance
+    // totals come back), or on the host from downloaded iterates (the reference's arithmetic).
+    std::vector<int> dflag(static_cast<size_t>(K), 0);
+    bool any_detect = false;
+    for (int i = 0; i < K; ++i)
+      if (due[i] && opt.detect && iter > 1000 && lanes[i].lp_.m > 0 &&
+          ((iter + 1) / kCheckFrequency) % kDetectionEvery == 0) {
+        dflag[i] = 1;
+        any_detect = true;
+      }
+    if (any_detect && opt.device_detect) device->detect(dflag, dout);
+    for (int i = 0; any_detect && i < K; ++i) {
+      if (!dflag[i]) continue;
+      Lane& L = lanes[i];
+      const int nt = L.lp_.n + L.lp_.m, m = L.lp_.m;
+      bool done = false;
+      if (opt.device_detect) {
+        const double* o = &dout[static_cast<size_t>(i) * kDetectSlotsHost];
+        if (L.farkas_from(iter, o)) {
+          device->download(i, nullptr, nullptr, nullptr, nullptr, nullptr, &y_erg, nt, m);
+          L.res.dual_ray = y_erg;
+          L.res.status = Status::kInfeasible;
+          done = true;
+        } else if (L.ray_from(iter, o)) {
+          device->download_ray(i, L.res.primal_ray, L.lp_.n);
+          L.res.status = Status::kUnbounded;
+          done = true;
+        }
+        if (done) device->download(i, nullptr, &v_new, nullptr, &y_new, nullptr, nullptr, nt, m);
+      } else {
         device->download(i, &v, &v_new, nullptr, &y_new, &v_erg, &y_erg, nt, m);
-        bool done = false;
         if (L.detect_farkas(iter, y_erg)) {
           L.res.status = Status::kInfeasible;
           done = true;
@@ -831,17 +906,23 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
           L.res.status = Status::kUnbounded;
           done = true;
         }
-        if (done) {
-          L.res.iterations = iter + 1;
-          L.res.primal_residual = current.rel_primal;
-          L.res.dual_residual = current.rel_dual;
-          L.res.gap = current.rel_gap;
-          L.map_back(v_new, y_new);
-          stop(i);
-          continue;
-        }
       }
+      if (done) {
+        L.res.iterations = iter + 1;
+        L.res.primal_residual = cur_m[i].rel_primal;
+        L.res.dual_residual = cur_m[i].rel_dual;
+        L.res.gap = cur_m[i].rel_gap;
+        L.map_back(v_new, y_new);
+        stop(i);
+        due[i] = 0;
+      }
+    }
 
+    for (int i = 0; i < K; ++i) {
+      if (!due[i]) continue;
+      Lane& L = lanes[i];
+      const Measure& current = cur_m[i];
+      const Measure& average = avg_m[i];
       // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance, or once
       // FP32 stopped improving (its rounding floor lies above the switch point): no 20% gain on
       // its best distance to termination within max(4096, iter / 4) iterations.
@@ -917,7 +998,10 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   }
   std::vector<PdlpResult> out;
   out.reserve(static_cast<size_t>(K));
-  for (Lane& L : lanes) out.push_back(std::move(L.res));
+  for (Lane& L : lanes) {
+    L.res.omega = L.omega;
+    out.push_back(std::move(L.res));
+  }
   if (info != nullptr) info->solve_seconds = clock.seconds();
   return out;
 }

@@ -53,10 +53,13 @@ struct PdlpOptions {
   double fp32_switch_tol = 1e-5;
   bool use_graph = true;          // replay each 64-step run from a HIP graph
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
+  bool device_detect = true;      // ... computed on the device (false: host, from downloads)
   bool share_matrix = true;       // batches of one problem: the shared-matrix (SpMM) batch
   int log_level = 0;
-  // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m).
+  // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m),
+  // and the primal weight to start with (<= 0: 1, the reference's start).
   std::vector<double> start_x, start_y;
+  double start_omega = 0.0;
 };
 
 struct RestartEvent {
@@ -84,6 +87,7 @@ struct PdlpResult {
   double dual_residual = 0.0;
   double gap = 0.0;
   double operator_norm = 0.0;      // L, the bound on ||[A_eq -I]||
+  double omega = 1.0;              // the primal weight at the end (a warm start's start_omega)
   double setup_seconds = 0.0;      // equilibration, norm estimate, upload
   double solve_seconds = 0.0;      // everything
   double kernel_ms = 0.0;          // device time of the step runs (HIP events)

@@ -227,6 +227,19 @@ class PdhgBatch:
     def freeze(self, i: int):
         N.check(self.lib.smy_pdhg_freeze(self.h, i), "freeze")
 
+    def detect(self, flags) -> np.ndarray:
+        """(batch, 11): the device-side infeasibility / unboundedness numbers of the flagged
+        instances (smy_pdhg_detect_batch)."""
+        f = np.ascontiguousarray(flags, np.int32)
+        out = np.zeros((self.size, 11))
+        N.check(self.lib.smy_pdhg_detect_batch(self.h, _p(f), _p(out)), "detect_batch")
+        return out
+
+    def download_ray(self, i: int) -> np.ndarray:
+        ray = np.zeros(self.sizes[i][0])
+        N.check(self.lib.smy_pdhg_download_ray(self.h, i, _p(ray)), "download_ray")
+        return ray
+
     def download(self, i: int):
         """dict of v, v_new, y, y_new, v_erg, y_erg of instance i (host copies)."""
         n, m = self.sizes[i]

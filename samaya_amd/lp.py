@@ -184,6 +184,8 @@ class PdlpOptions:
     polish_rounds: int = 0
     algorithm: str = "reference"    # "reference" or "enhanced"
     share_matrix: bool = True       # batches of one model: the shared-matrix (SpMM) batch
+    device_detect: bool = True      # infeasibility / unboundedness tests on the GPU (else host)
+    start_omega: float = 0.0        # primal weight to start with (<= 0: 1), with a warm start
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -201,6 +203,8 @@ class PdlpOptions:
         p.polish_rounds = self.polish_rounds
         p.algorithm = {"reference": 0, "enhanced": 1}[self.algorithm]
         p.share_matrix = int(self.share_matrix)
+        p.device_detect = int(self.device_detect)
+        p.start_omega = self.start_omega
         return p
 
 
@@ -223,6 +227,7 @@ class PdlpResult:
     message: str
     restarts: list = field(default_factory=list)   # (iter, to_average, kkt, omega)
     evals: list = field(default_factory=list)      # (iter, (cur p, d, g), (avg p, d, g))
+    omega: float = 1.0                             # the final primal weight
 
     @property
     def status_name(self) -> str:
@@ -260,7 +265,7 @@ def solve_pdlp(lp: LpProblem, options: PdlpOptions | None = None, start_x=None,
         primal_ray=pray if info.status == UNBOUNDED else None,
         iterations=info.iterations, fp32_iterations=info.fp32_iterations,
         primal_residual=info.primal_residual, dual_residual=info.dual_residual, gap=info.gap,
-        operator_norm=info.operator_norm, setup_seconds=info.setup_seconds,
+        operator_norm=info.operator_norm, omega=info.omega, setup_seconds=info.setup_seconds,
         solve_seconds=info.solve_seconds, kernel_ms=info.kernel_ms,
         message=info.message.decode(errors="replace"),
         restarts=[(int(r_it[k]), int(r_av[k]), float(r_kkt[k]), float(r_om[k])) for k in range(nr)],
@@ -291,6 +296,7 @@ class LpResult:
     solve_seconds: float
     kernel_ms: float
     message: str
+    omega: float = 1.0  # the last solve's final primal weight (start_omega of a warm start)
 
     @property
     def status_name(self) -> str:
@@ -324,7 +330,7 @@ def _lp_result(res: "N.SmyLpResult", x, yd, d, act, cert) -> LpResult:
         primal_residual=res.pdlp.primal_residual, dual_residual=res.pdlp.dual_residual,
         gap=res.pdlp.gap, setup_seconds=res.pdlp.setup_seconds,
         solve_seconds=res.pdlp.solve_seconds, kernel_ms=res.pdlp.kernel_ms,
-        message=res.message.decode(errors="replace"))
+        message=res.message.decode(errors="replace"), omega=res.pdlp.omega)
 
 
 @dataclass
