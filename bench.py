@@ -18,8 +18,17 @@ restarts and termination, FP64 polish). One step = one such solve.
                bounded sample of the same LP on one host core, extrapolated to the GPU's
                iteration count
 
-Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--scale 1.0] [--no-fp64]
-       [--no-cpu-baseline] [--no-spmv]
+Batched workloads (--workload):
+  c4  256 multi-period blending LPs (2.9k-59.5k rows each, 91M nonzeros in all) solved as one
+      stacked batch, mixed precision, to 1e-6 each; value = seconds until the last LP stopped;
+      compared with plain FP64 PDLP solving them one at a time (a timed sample of 16, scaled)
+  c5  1,024 node relaxations of one knapsack LP (m 20,000, n 50,000, nnz 400k; 1-5% of the
+      columns fixed at 0 or 1 per node) solved as one shared-matrix batch (SpMM steps), warm-started
+      from the root LP's answer and primal weight; infeasible nodes detected on the device;
+      compared with plain FP64 PDLP solving the nodes one at a time (a sample of 8, scaled)
+
+Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload c3|c4|c5] [--scale 1.0]
+       [--no-fp64] [--no-cpu-baseline] [--no-spmv]
 """
 from __future__ import annotations
 
@@ -29,6 +38,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -50,6 +61,15 @@ def step_kernel_bytes(nnz: int, m: int, n: int, s: int) -> dict:
     the gathered vector once, the per-row and per-column update vectors once)."""
     dual = nnz * (s + 4) + 4 * (m + 1) + s * n + 13 * s * m
     primal = nnz * (s + 4) + 4 * (n + 1) + s * m + 8 * s * n
+    return {"dual": dual, "primal": primal}
+
+
+def shared_step_bytes(nnz: int, m: int, n: int, k: int, s: int) -> dict:
+    """Algorithmic HBM bytes of the shared-matrix step kernels for k instances: the matrix once,
+    the gathered iterate block once (k values per row), and the per-row update streams of all
+    instances (rows of A: 5 read, 5 written; rows of A': v, average and both bounds read, 3 written)."""
+    dual = nnz * (s + 4) + 4 * (m + 1) + s * k * n + 10 * s * k * m
+    primal = nnz * (s + 4) + 4 * (n + 1) + s * k * m + 7 * s * k * n
     return {"dual": dual, "primal": primal}
 
 
@@ -152,6 +172,7 @@ def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 128)) -> dict | None
 
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c5"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
@@ -161,6 +182,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
+    ap.add_argument("--verbose", action="store_true", help="progress lines of long batches (stdout)")
+    ap.add_argument("--fp32-switch-tol", type=float, default=1e-5,
+                    help="mixed precision: FP32 steps until this distance to termination (or a stall)")
     ap.add_argument("--max-iterations", type=int, default=-1,
                     help="cap each solve (profiling runs); the line then reports no speedup")
     args = ap.parse_args()
@@ -176,12 +200,17 @@ def main() -> int:
 
     assert torch.cuda.is_available() and smy.gpu_available(), "no usable MI355X"
     torch.cuda.set_device(0)
+    if args.workload == "c4":
+        return run_c4(args)
+    if args.workload == "c5":
+        return run_c5(args)
     t0 = time.perf_counter()
     md = smy.generators.c2(args.scale)
     log(f"LP m={md.m} n={md.n} nnz={md.nnz} generated in {time.perf_counter() - t0:.1f} s")
     nm = md.native()
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, detect=False,
-                           verify=True, polish_rounds=0, max_iterations=args.max_iterations)
+                           verify=True, polish_rounds=0, max_iterations=args.max_iterations,
+                           fp32_switch_tol=args.fp32_switch_tol)
 
     for w in range(args.warmup):
         r = smy.solve_lp(nm, opts)
@@ -264,6 +293,266 @@ def main() -> int:
         line["rel_objective_error_vs_fp64"] = abs(last.objective - fp64.objective) / (1 + abs(fp64.objective))
     if spmv is not None:
         line["spmv"] = spmv
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict) -> dict:
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 1),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "fp32 steps + fp64 evaluation/polish",
+        "algorithm": args.algorithm,
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": workload},
+    }
+    line.update(extra)
+    return line
+
+
+def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None) -> dict:
+    """Live timing of the FP32 step kernels of a batch handle; k: instances of a shared-matrix
+    batch (None: stacked batch)."""
+    dual_ms, primal_ms = dev.time_kernels(20)
+    b = shared_step_bytes(nnz, m, n, k, 4) if k else step_kernel_bytes(nnz, m, n, 4)
+    dom = "primal" if primal_ms >= dual_ms else "dual"
+    ms = primal_ms if dom == "primal" else dual_ms
+    achieved = b[dom] / (ms * 1e-3) / 1e9
+    kernel = (f"shared_{dom}_kernel<float>" if k else
+              f"csr_rows_kernel<float, {'PrimalColsOp' if dom == 'primal' else 'DualRowsOp'}<float>>")
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
+            "avg_us": round(ms * 1e3, 2), "bytes_per_launch": b[dom],
+            "other_kernel_us": round((dual_ms if dom == "primal" else primal_ms) * 1e3, 2)}
+
+
+def cpu_rate(models, its=(64, 128)) -> tuple[float, str]:
+    """Seconds per nonzero-iteration of the reference's CPU PDLP (oracle/_ref, one core), from
+    two short runs on each model."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    kind = "reference" if O.ref_available() else "port"
+    rates = []
+    for md in models:
+        mm = O.M.from_product(md)
+        times = []
+        for k in its:
+            t0 = time.perf_counter()
+            if kind == "reference":
+                O.ref_solve_lp(mm, 1e-6, k, capture_log=False)
+            else:
+                O.oracle_solve_lp(mm, 1e-6, k)
+            times.append(time.perf_counter() - t0)
+        rates.append((times[1] - times[0]) / (its[1] - its[0]) / md.nnz)
+    return float(np.mean(rates)), kind
+
+
+def run_c4(args) -> int:
+    import numpy as np
+    import samaya_amd as smy
+    import samaya_amd.device as D
+    import scipy.sparse as sp
+    t0 = time.perf_counter()
+    K = int(256 * args.scale) if args.scale < 1.0 else 256
+    models = [smy.generators.blending(4000 + i) for i in range(K)]
+    nms = [md.native() for md in models]
+    nnz = sum(md.nnz for md in models)
+    log(f"{K} blending LPs, rows {min(md.m for md in models)}..{max(md.m for md in models)}, "
+        f"nnz {nnz}, generated in {time.perf_counter() - t0:.1f} s")
+    opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
+                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0)
+    for w in range(args.warmup):
+        smy.solve_lp_batch(nms[:8], opts)
+    runs = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        res, times = smy.solve_lp_batch(nms, opts)
+        runs.append((res, times))
+        its = np.array([r.iterations for r in res])
+        log(f"step {k}: batch {times.solve_seconds - times.setup_seconds:.2f} s after setup "
+            f"{times.setup_seconds:.2f} s; iterations median {int(np.median(its))} max {its.max()}; "
+            f"statuses {sorted(set(r.status_name for r in res))}")
+    wall = time.perf_counter() - t_start
+    res, times = runs[-1]
+    value = float(np.mean([t.solve_seconds - t.setup_seconds for _, t in runs]))
+    assert all(r.status == smy.OPTIMAL for r in res), [r.message for r in res if r.status != smy.OPTIMAL][:3]
+
+    fp64 = None
+    if not args.no_fp64:
+        # Plain FP64 PDLP (the reference's algorithm), one LP at a time: every 16th LP, scaled.
+        sample = list(range(0, K, max(1, K // 16)))
+        o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=False)
+        secs, its64, objs = [], [], []
+        for i in sample:
+            r = smy.solve_lp(nms[i], o64)
+            secs.append(r.solve_seconds - r.setup_seconds)
+            its64.append(r.iterations)
+            objs.append(abs(res[i].objective - r.objective) / (1 + abs(r.objective)))
+        fp64 = {"seconds": sum(secs) * K / len(sample), "sample": sample, "its": its64,
+                "obj_err": max(objs)}
+        log(f"fp64 one at a time: {sum(secs):.2f} s for {len(sample)} LPs -> {fp64['seconds']:.1f} s")
+
+    # Roofline: the FP32 step kernels over the whole stacked batch.
+    As, Ats, costs, los, ups = [], [], [], [], []
+    for md in models:
+        A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+        T = A.T.tocsc()
+        T.sort_indices()
+        As.append((md.m, md.n, md.start, md.index, md.value))
+        Ats.append((md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data))
+        costs.append(np.concatenate([md.obj, np.zeros(md.m)]))
+        los.append(np.concatenate([md.col_lower, md.row_lower]))
+        ups.append(np.concatenate([md.col_upper, md.row_upper]))
+    dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
+    M = sum(md.m for md in models)
+    N = sum(md.n for md in models)
+    rl = batch_roofline(dev, nnz, M, N, None)
+    del dev
+
+    cpu = None
+    if not args.no_cpu_baseline and fp64 is not None:
+        probe = [models[i] for i in fp64["sample"][::4]]
+        rate, kind = cpu_rate(probe)
+        est = sum(rate * models[i].nnz * it for i, it in zip(fp64["sample"], fp64["its"])) * K / len(fp64["sample"])
+        cpu = {"value": round(est, 1), "unit": "s", "cores": 1, "kind": kind,
+               "sample": (f"reference CPU PDLP, 64 and 128 FP64 iterations on {len(probe)} of the LPs "
+                          f"({rate * 1e9:.2f} ns per nonzero-iteration), times the FP64 iteration "
+                          f"counts of the 16 sampled LPs, scaled to {K}; extrapolated")}
+    its = np.array([r.iterations for r in res])
+    extra = {
+        "status": "optimal", "iterations_median": int(np.median(its)), "iterations_max": int(its.max()),
+        "fp32_iterations_sum": int(sum(r.fp32_iterations for r in res)),
+        "fp32_switch_tol": args.fp32_switch_tol,
+        "setup_s": round(times.setup_seconds, 3), "verified": int(sum(r.verified for r in res)),
+        "roofline": rl, "cpu_baseline": cpu,
+    }
+    extra["config"] = {"workload": f"{K} multi-period blending LPs (periods U{{4..48}}, 20 units x "
+                                   f"30 streams), {M} rows, {N} columns, {nnz} nonzeros; stacked "
+                                   f"batch, mixed precision, rel KKT {args.tol:g} each",
+                       "lps": K, "rows": M, "cols": N, "nnz": nnz, "tol": args.tol}
+    if fp64 is not None:
+        extra["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
+        extra["fp64_sample"] = f"{len(fp64['sample'])} of {K} LPs solved one at a time, scaled"
+        extra["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
+        extra["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
+    line = base_line(args, value, wall / args.steps * 1e3, extra["config"]["workload"], extra)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_c5(args) -> int:
+    import numpy as np
+    import samaya_amd as smy
+    import samaya_amd.device as D
+    import scipy.sparse as sp
+    t0 = time.perf_counter()
+    K = int(1024 * args.scale) if args.scale < 1.0 else 1024
+    md, lows, ups = smy.generators.c5(K)
+    nm = md.native()
+    log(f"knapsack m={md.m} n={md.n} nnz={md.nnz}, {K} bound vectors, {time.perf_counter() - t0:.1f} s")
+    opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
+                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0)
+    root = smy.solve_lp(nm, opts)
+    log(f"root: {root.status_name} {root.iterations} it, omega {root.omega:.3e}")
+    start = (root.col_value, root.row_dual)
+    opts.start_omega = root.omega
+    for w in range(args.warmup):
+        smy.solve_lp_bounds_batch(nm, lows[:16], ups[:16], opts, start=start)
+    runs = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        res, times = smy.solve_lp_bounds_batch(nm, lows, ups, opts, start=start)
+        runs.append((res, times))
+        its = np.array([r.iterations for r in res])
+        log(f"step {k}: batch {times.solve_seconds - times.setup_seconds:.2f} s after setup "
+            f"{times.setup_seconds:.2f} s; iterations median {int(np.median(its))} max {its.max()}; "
+            f"statuses {sorted(set(r.status_name for r in res))}")
+    wall = time.perf_counter() - t_start
+    res, times = runs[-1]
+    value = float(np.mean([t.solve_seconds - t.setup_seconds for _, t in runs]))
+    A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+    infeasible = [bool(np.any(A @ lo > md.row_upper + 1e-9)) for lo in lows]
+    statuses_ok = all(r.status == (smy.INFEASIBLE if inf else smy.OPTIMAL) for r, inf in zip(res, infeasible))
+
+    fp64 = None
+    if not args.no_fp64:
+        # Plain FP64 PDLP node by node, warm-started alike; a sample stratified by the nodes'
+        # classes (feasible nodes take far fewer iterations than the infeasible ones, which end
+        # when the Farkas test succeeds), each class scaled by its count.
+        feas = [i for i in range(K) if not infeasible[i]]
+        infs = [i for i in range(K) if infeasible[i]]
+        strata = [(feas, feas[::max(1, len(feas) // 8)][:8]), (infs, infs[::max(1, len(infs) // 4)][:4])]
+        o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=False,
+                              start_omega=root.omega)
+        total, its64, objs, sample = 0.0, [], [], []
+        for members, chosen in strata:
+            if not chosen:
+                continue
+            secs = []
+            for i in chosen:
+                r = smy.solve_lp_bounds_batch(nm, [lows[i]], [ups[i]], o64, start=start)[0][0]
+                secs.append(r.solve_seconds - r.setup_seconds)
+                its64.append(r.iterations)
+                sample.append(i)
+                if r.status == smy.OPTIMAL and res[i].status == smy.OPTIMAL:
+                    objs.append(abs(res[i].objective - r.objective) / (1 + abs(r.objective)))
+            total += float(np.mean(secs)) * len(members)
+            log(f"fp64 one at a time: {len(chosen)} of {len(members)} nodes, mean {np.mean(secs):.3f} s")
+        fp64 = {"seconds": total, "sample": sample, "its": its64, "obj_err": max(objs) if objs else None}
+        log(f"fp64 one at a time: {fp64['seconds']:.1f} s estimated for {K} nodes")
+
+    T = A.T.tocsc()
+    T.sort_indices()
+    a = (md.m, md.n, md.start, md.index, md.value)
+    at = (md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data)
+    cost = np.concatenate([-md.obj, np.zeros(md.m)])
+    lo_full = [np.concatenate([l, md.row_lower]) for l in lows]
+    up_full = [np.concatenate([u, md.row_upper]) for u in ups]
+    dev = D.PdhgBatch.create_shared(a, at, cost, lo_full, up_full, "fp32")
+    rl = batch_roofline(dev, md.nnz, md.m, md.n, K)
+    del dev
+
+    cpu = None
+    if not args.no_cpu_baseline and fp64 is not None:
+        node = smy.Model(md.m, md.n, md.start, md.index, md.value, md.obj, lows[0], ups[0],
+                         md.row_lower, md.row_upper, md.maximize, md.offset)
+        rate, kind = cpu_rate([node])
+        est = rate * md.nnz * float(np.mean(fp64["its"])) * K
+        cpu = {"value": round(est, 1), "unit": "s", "cores": 1, "kind": kind,
+               "sample": (f"reference CPU PDLP, 64 and 128 FP64 iterations on one node "
+                          f"({rate * 1e9:.2f} ns per nonzero-iteration), times the mean FP64 "
+                          f"iteration count of the sampled nodes, times {K}; extrapolated")}
+    its = np.array([r.iterations for r in res])
+    extra = {
+        "status": "expected statuses" if statuses_ok else "unexpected statuses",
+        "optimal": int(sum(r.status == smy.OPTIMAL for r in res)),
+        "infeasible": int(sum(r.status == smy.INFEASIBLE for r in res)),
+        "infeasible_by_construction": int(sum(infeasible)),
+        "iterations_median": int(np.median(its)), "iterations_max": int(its.max()),
+        "root_iterations": root.iterations, "setup_s": round(times.setup_seconds, 3),
+        "fp32_switch_tol": args.fp32_switch_tol,
+        "fp32_iterations_sum": int(sum(r.fp32_iterations for r in res)),
+        "verified": int(sum(r.verified for r in res)), "roofline": rl, "cpu_baseline": cpu,
+    }
+    extra["config"] = {"workload": f"{K} node relaxations (1-5% of columns fixed) of a knapsack LP "
+                                   f"m={md.m} n={md.n} nnz={md.nnz}; shared-matrix batch (SpMM), "
+                                   f"mixed precision, warm-started from the root, rel KKT {args.tol:g}",
+                       "nodes": K, "m": md.m, "n": md.n, "nnz": md.nnz, "tol": args.tol}
+    if fp64 is not None:
+        extra["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
+        extra["fp64_sample"] = (f"{len(fp64['sample'])} of {K} nodes solved one at a time, warm-started "
+                                f"alike: 8 feasible and 4 infeasible, each mean scaled by its class count")
+        extra["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
+        extra["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
+    line = base_line(args, value, wall / args.steps * 1e3, extra["config"]["workload"], extra)
     print(json.dumps(line), flush=True)
     return 0
 

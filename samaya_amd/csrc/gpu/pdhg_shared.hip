@@ -59,6 +59,10 @@ struct SharedBatch {
   std::vector<int> state_prec;  // kp: precision of the instance's state (also when frozen)
   int* mode = nullptr;
   bool mode_dirty = true;
+  // Instances stepping in each precision (FP64, FP32) when there are few of them: the sparse
+  // step kernels take these lists.
+  int* active[2] = {nullptr, nullptr};
+  int nactive[2] = {0, 0};
   int* which = nullptr;            // 2 x kp, by precision
   int* which_host = nullptr;       // pinned
   SharedScalars* scalars = nullptr;
@@ -123,8 +127,18 @@ void upload_modes(smy_pdhg* h) {
   if (!s->mode_dirty) return;
   SMY_HIP(hipStreamSynchronize(h->stream));
   SMY_HIP(hipMemcpy(s->mode, s->mode_host.data(), s->kp * sizeof(int), hipMemcpyHostToDevice));
+  for (int q = 0; q < 2; ++q) {
+    std::vector<int> list;
+    for (int k = 0; k < s->K; ++k)
+      if (s->mode_host[k] == q) list.push_back(k);
+    s->nactive[q] = static_cast<int>(list.size());
+    if (!list.empty() && s->nactive[q] <= kSparseInstances)
+      SMY_HIP(hipMemcpy(s->active[q], list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   s->mode_dirty = false;
 }
+
+bool sparse_steps(const SharedBatch* s, int q) { return s->nactive[q] > 0 && s->nactive[q] <= kSparseInstances; }
 
 template <typename T>
 void alloc_state(smy_pdhg* h, SharedIter<T>& S) {
@@ -208,12 +222,31 @@ SharedStep<T> step_args(smy_pdhg* h, int j, bool dual) {
 template <typename T>
 void shared_step_kernels(smy_pdhg* h, int j) {
   SharedBatch* s = h->shared;
+  const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
+  if (sparse_steps(s, q)) {
+    const int na = s->nactive[q];
+    if (s->m > 0)
+      hipLaunchKernelGGL((shared_dual_sparse_kernel<T>), dim3(blocks_for(static_cast<long long>(s->m) * na)),
+                         dim3(kBlock), 0, h->stream, step_args<T>(h, j, true), s->active[q], na);
+    if (s->n > 0)
+      hipLaunchKernelGGL((shared_primal_sparse_kernel<T>), dim3(blocks_for(static_cast<long long>(s->n) * na)),
+                         dim3(kBlock), 0, h->stream, step_args<T>(h, j, false), s->active[q], na);
+    return;
+  }
   if (s->m > 0)
     hipLaunchKernelGGL((shared_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
                        step_args<T>(h, j, true));
   if (s->n > 0)
     hipLaunchKernelGGL((shared_primal_kernel<T>), dim3(s->n, chunks(s)), dim3(kBlock), 0, h->stream,
                        step_args<T>(h, j, false));
+}
+
+// The launch shape the captured steps depend on: per precision, the sparse instance count or
+// dense (a graph is captured again when it changes).
+int step_signature(const SharedBatch* s) {
+  const int a = sparse_steps(s, SMY_FP64) ? s->nactive[SMY_FP64] : 0;
+  const int b = sparse_steps(s, SMY_FP32) ? s->nactive[SMY_FP32] : 0;
+  return a * 1024 + b;
 }
 
 void shared_step_all(smy_pdhg* h, int j) {
@@ -689,6 +722,8 @@ int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* 
     }
     if (precision == SMY_FP32) ensure_fp32_shared(h.get());
     s->mode = h->alloc<int>(s->kp);
+    s->active[0] = h->alloc<int>(kSparseInstances);
+    s->active[1] = h->alloc<int>(kSparseInstances);
     s->which = h->alloc<int>(2LL * s->kp);
     s->scalars = h->zeros<SharedScalars>(s->kp);
     s->cs = h->alloc<double>(n);
@@ -773,7 +808,9 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
   if (count == smy_pdhg::kGraphSteps && mask != 0) {
     const int p0 = h->p;
     smy_pdhg::Captured& c = h->graphs[mask][p0];
-    if (c.exec == nullptr) {
+    const int sig = step_signature(s);
+    if (c.exec == nullptr || c.gen != sig) {
+      h->drop_graph(c);
       SMY_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       for (int j = 0; j < count; ++j) {
         shared_step_all(h, j);
@@ -782,7 +819,7 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
       SMY_HIP(hipStreamEndCapture(h->stream, &c.graph));
       h->p = p0;
       SMY_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
-      c.gen = 0;
+      c.gen = sig;
     }
     SMY_HIP(hipGraphLaunch(c.exec, h->stream));
     if ((count - 1) & 1) h->p ^= 1;

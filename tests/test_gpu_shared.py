@@ -27,13 +27,13 @@ def rel_close(a, b, tol):
     return np.all(np.abs(a - b) <= tol * (1.0 + np.abs(b)))
 
 
-def test_shared_device_matches_stacked_device(gpu):
+@pytest.mark.parametrize("K", [7, 72])  # few instances: per-(row, instance) steps; many: SpMM rows
+def test_shared_device_matches_stacked_device(gpu, K):
     md = G.knapsack(71, 60, 150, 4)
     lp = identity_lp(M.from_product(md))
     rng = np.random.default_rng(3)
     rs, cs = rng.uniform(0.5, 2.0, lp.m), rng.uniform(0.5, 2.0, lp.n)
     eq = equilibrated(lp, rs, cs)
-    K = 7
     lows, ups = [], []
     for k in range(K):
         lo, up = G.fixings(800 + k, eq.lower[:lp.n], eq.upper[:lp.n], 0.05, 0.2)
@@ -48,23 +48,25 @@ def test_shared_device_matches_stacked_device(gpu):
     for b in (shared, stacked):
         for k in range(K):
             b.set_original(k, cs, rs, lp.cost, ol[k], ou[k])
-    tau = np.linspace(0.2, 0.4, K)
-    sigma = np.linspace(0.4, 0.2, K)
+    tau = 0.2 + 0.2 * (np.arange(K) % 7) / 6
+    sigma = 0.4 - 0.2 * (np.arange(K) % 5) / 4
     for b in (shared, stacked):
         b.run(tau, sigma, 1.0, [0] * K, 64)
     assert rel_close(shared.evaluate(), stacked.evaluate(), 1e-9)
-    which = [0, -1, 1, -1, 0, 1, -1]
+    which = [(0, -1, 1, -1, 0, 1, -1)[k % 7] for k in range(K)]
     flagged = [k for k in range(K) if which[k] >= 0]  # the others' distances are not computed
     assert rel_close(shared.restart_distance(which)[flagged], stacked.restart_distance(which)[flagged], 1e-9)
+    k0 = [(0 if which[k] >= 0 else 64) for k in range(K)]
     for b in (shared, stacked):
         b.restart_to(which)
         b.advance()
         b.freeze(3)
-        b.set_precision([-1, "fp32", -1, -1, -1, -1, -1])
-        b.run(tau, sigma, 1.0, [0, 64, 0, 64, 0, 0, 64], 5)
+        b.set_precision(["fp32" if k == 1 else -1 for k in range(K)])
+        b.run(tau, sigma, 1.0, k0, 5)
     es, et = shared.evaluate(), stacked.evaluate()
-    for k in (0, 2, 4, 5, 6):
-        assert rel_close(es[k], et[k], 1e-9)
+    for k in range(K):
+        if k not in (1, 3):
+            assert rel_close(es[k], et[k], 1e-9), k
     assert rel_close(es[1], et[1], 1e-4)  # FP32 steps: rounding of FP32 sums in another order
     for k in (0, 2, 5):
         ds, dt = shared.download(k), stacked.download(k)
