@@ -38,6 +38,21 @@ struct SharedIter {
   bool ready = false;
 };
 
+// The compact block of a precision's few remaining instances (rows x kc, kc <= 64).
+template <typename T>
+struct CompactState {
+  T* v[2] = {nullptr, nullptr};
+  T* vbar[2] = {nullptr, nullptr};
+  T* v_erg = nullptr;
+  T* y[2] = {nullptr, nullptr};
+  T* y_erg = nullptr;
+  T* col_lower = nullptr;  // n x kc
+  T* col_upper = nullptr;
+  T* tau = nullptr;        // kc
+  T* sigma = nullptr;
+  T* alpha = nullptr;      // kMaxRunSteps x kc
+};
+
 struct SharedBatch {
   int K = 0, kp = 0, n = 0, m = 0, nt = 0;
   int* a_ptr = nullptr;
@@ -59,10 +74,12 @@ struct SharedBatch {
   std::vector<int> state_prec;  // kp: precision of the instance's state (also when frozen)
   int* mode = nullptr;
   bool mode_dirty = true;
-  // Instances stepping in each precision (FP64, FP32) when there are few of them: the sparse
-  // step kernels take these lists.
+  // Instances stepping in each precision (FP64, FP32); when there are few of them their columns
+  // step in a compact block (the list gives each slot's instance).
   int* active[2] = {nullptr, nullptr};
   int nactive[2] = {0, 0};
+  CompactState<double> cd;
+  CompactState<float> cf;
   int* which = nullptr;            // 2 x kp, by precision
   int* which_host = nullptr;       // pinned
   SharedScalars* scalars = nullptr;
@@ -132,13 +149,16 @@ void upload_modes(smy_pdhg* h) {
     for (int k = 0; k < s->K; ++k)
       if (s->mode_host[k] == q) list.push_back(k);
     s->nactive[q] = static_cast<int>(list.size());
-    if (!list.empty() && s->nactive[q] <= kSparseInstances)
+    if (!list.empty() && s->nactive[q] <= kCompactInstances)
       SMY_HIP(hipMemcpy(s->active[q], list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   s->mode_dirty = false;
 }
 
-bool sparse_steps(const SharedBatch* s, int q) { return s->nactive[q] > 0 && s->nactive[q] <= kSparseInstances; }
+bool compact_steps(const SharedBatch* s, int q) {
+  return s->nactive[q] > 0 && s->nactive[q] <= kCompactInstances;
+}
+int compact_width(const SharedBatch* s, int q) { return (s->nactive[q] + 3) / 4 * 4; }
 
 template <typename T>
 void alloc_state(smy_pdhg* h, SharedIter<T>& S) {
@@ -220,17 +240,110 @@ SharedStep<T> step_args(smy_pdhg* h, int j, bool dual) {
 }
 
 template <typename T>
+CompactState<T>& cstate(SharedBatch* s);
+template <>
+CompactState<double>& cstate<double>(SharedBatch* s) { return s->cd; }
+template <>
+CompactState<float>& cstate<float>(SharedBatch* s) { return s->cf; }
+
+template <typename T>
+void ensure_compact(smy_pdhg* h) {
+  SharedBatch* s = h->shared;
+  CompactState<T>& C = cstate<T>(s);
+  if (C.v[0] != nullptr) return;
+  const long long w = kCompactInstances;
+  for (int q = 0; q < 2; ++q) {
+    C.v[q] = h->alloc<T>(s->nt * w);
+    C.vbar[q] = h->alloc<T>(s->nt * w);
+    C.y[q] = h->alloc<T>(s->m * w);
+  }
+  C.v_erg = h->alloc<T>(s->nt * w);
+  C.y_erg = h->alloc<T>(s->m * w);
+  C.col_lower = h->alloc<T>(s->n * w);
+  C.col_upper = h->alloc<T>(s->n * w);
+  C.tau = h->alloc<T>(w);
+  C.sigma = h->alloc<T>(w);
+  C.alpha = h->alloc<T>(kMaxRunSteps * w);
+}
+
+// The run's arguments on the compact block.
+template <typename T>
+SharedStep<T> compact_args(smy_pdhg* h, int j, bool dual) {
+  SharedBatch* s = h->shared;
+  CompactState<T>& C = cstate<T>(s);
+  SharedStep<T> a = step_args<T>(h, j, dual);
+  const int p = h->p;
+  const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
+  a.kp = compact_width(s, q);
+  a.tab = StepTable<T>{C.tau, C.sigma, C.alpha, static_cast<T>(h->theta_vbar)};
+  a.vbar = C.vbar[p];
+  a.v = C.v[p];
+  a.v_new = C.v[1 - p];
+  a.v_erg = C.v_erg;
+  a.vbar_next = C.vbar[1 - p];
+  a.y = C.y[p];
+  a.y_new = C.y[1 - p];
+  a.y_erg = C.y_erg;
+  a.col_lower = C.col_lower;
+  a.col_upper = C.col_upper;
+  return a;
+}
+
+// Copies the stepping columns of precision T between the big layout and the compact block
+// (in: before a run, with the column bounds; out: after it, the iterates only).
+template <typename T>
+void compact_copy(smy_pdhg* h, bool in) {
+  SharedBatch* s = h->shared;
+  SharedIter<T>& S = sstate<T>(s);
+  CompactState<T>& C = cstate<T>(s);
+  const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
+  CompactCopy<T> c{};
+  c.kp = s->kp;
+  c.kc = compact_width(s, q);
+  c.na = s->nactive[q];
+  c.list = s->active[q];
+  c.in = in;
+  int a = 0;
+  const auto add = [&](T* big, T* small, int rows) {
+    c.big[a] = big;
+    c.small[a] = small;
+    c.rows[a] = rows;
+    ++a;
+  };
+  for (int k = 0; k < 2; ++k) {
+    add(S.v[k], C.v[k], s->nt);
+    add(S.vbar[k], C.vbar[k], s->nt);
+    add(S.y[k], C.y[k], s->m);
+  }
+  add(S.v_erg, C.v_erg, s->nt);
+  add(S.y_erg, C.y_erg, s->m);
+  if (in) {
+    add(S.col_lower, C.col_lower, s->n);
+    add(S.col_upper, C.col_upper, s->n);
+  }
+  c.count = a;
+  const long long threads = static_cast<long long>(s->nt) * c.kc;
+  if (threads > 0)
+    hipLaunchKernelGGL((compact_copy_kernel<T>), dim3(blocks_for(threads)), dim3(kBlock), 0, h->stream, c,
+                       s->nt);
+  if (in)
+    hipLaunchKernelGGL((compact_table_kernel<T>), dim3(1), dim3(kCompactInstances), 0, h->stream, c.kc,
+                       c.na, c.list, s->scalars, C.tau, C.sigma, C.alpha);
+}
+
+template <typename T>
 void shared_step_kernels(smy_pdhg* h, int j) {
   SharedBatch* s = h->shared;
   const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
-  if (sparse_steps(s, q)) {
+  if (compact_steps(s, q)) {
     const int na = s->nactive[q];
+    const int rows_per_wg = kBlock / (compact_width(s, q) / kInstPerThread);
     if (s->m > 0)
-      hipLaunchKernelGGL((shared_dual_sparse_kernel<T>), dim3(blocks_for(static_cast<long long>(s->m) * na)),
-                         dim3(kBlock), 0, h->stream, step_args<T>(h, j, true), s->active[q], na);
+      hipLaunchKernelGGL((compact_dual_kernel<T>), dim3((s->m + rows_per_wg - 1) / rows_per_wg), dim3(kBlock), 0,
+                         h->stream, compact_args<T>(h, j, true), na);
     if (s->n > 0)
-      hipLaunchKernelGGL((shared_primal_sparse_kernel<T>), dim3(blocks_for(static_cast<long long>(s->n) * na)),
-                         dim3(kBlock), 0, h->stream, step_args<T>(h, j, false), s->active[q], na);
+      hipLaunchKernelGGL((compact_primal_kernel<T>), dim3((s->n + rows_per_wg - 1) / rows_per_wg), dim3(kBlock),
+                         0, h->stream, compact_args<T>(h, j, false), na);
     return;
   }
   if (s->m > 0)
@@ -241,11 +354,11 @@ void shared_step_kernels(smy_pdhg* h, int j) {
                        step_args<T>(h, j, false));
 }
 
-// The launch shape the captured steps depend on: per precision, the sparse instance count or
-// dense (a graph is captured again when it changes).
+// The launch shape the captured steps depend on: per precision, the compact instance count or
+// 0 for the full layout (a graph is captured again when it changes).
 int step_signature(const SharedBatch* s) {
-  const int a = sparse_steps(s, SMY_FP64) ? s->nactive[SMY_FP64] : 0;
-  const int b = sparse_steps(s, SMY_FP32) ? s->nactive[SMY_FP32] : 0;
+  const int a = compact_steps(s, SMY_FP64) ? s->nactive[SMY_FP64] : 0;
+  const int b = compact_steps(s, SMY_FP32) ? s->nactive[SMY_FP32] : 0;
   return a * 1024 + b;
 }
 
@@ -722,8 +835,8 @@ int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* 
     }
     if (precision == SMY_FP32) ensure_fp32_shared(h.get());
     s->mode = h->alloc<int>(s->kp);
-    s->active[0] = h->alloc<int>(kSparseInstances);
-    s->active[1] = h->alloc<int>(kSparseInstances);
+    s->active[0] = h->alloc<int>(kCompactInstances);
+    s->active[1] = h->alloc<int>(kCompactInstances);
     s->which = h->alloc<int>(2LL * s->kp);
     s->scalars = h->zeros<SharedScalars>(s->kp);
     s->cs = h->alloc<double>(n);
@@ -804,6 +917,15 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
   const bool u32 = s_uses(s, SMY_FP32), u64 = s_uses(s, SMY_FP64);
   if (u32) step_table<float>(h);
   if (u64) step_table<double>(h);
+  const bool c32 = compact_steps(s, SMY_FP32), c64 = compact_steps(s, SMY_FP64);
+  if (c32) {
+    ensure_compact<float>(h);
+    compact_copy<float>(h, true);
+  }
+  if (c64) {
+    ensure_compact<double>(h);
+    compact_copy<double>(h, true);
+  }
   const int mask = (u64 ? 1 : 0) | (u32 ? 2 : 0);
   if (count == smy_pdhg::kGraphSteps && mask != 0) {
     const int p0 = h->p;
@@ -829,6 +951,8 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
       if (j + 1 < count) h->p ^= 1;
     }
   }
+  if (c32) compact_copy<float>(h, false);
+  if (c64) compact_copy<double>(h, false);
   SMY_HIP(hipGetLastError());
   SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
   h->timed = true;
