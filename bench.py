@@ -25,7 +25,8 @@ Batched workloads (--workload):
   c5  1,024 node relaxations of one knapsack LP (m 20,000, n 50,000, nnz 400k; 1-5% of the
       columns fixed at 0 or 1 per node) solved as one shared-matrix batch (SpMM steps), warm-started
       from the root LP's answer and primal weight; infeasible nodes detected on the device;
-      compared with plain FP64 PDLP solving the nodes one at a time (a sample of 8, scaled)
+      compared with plain FP64 PDLP solving the nodes one at a time (a stratified sample of 32
+      feasible and 16 infeasible nodes, scaled)
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--workload c3|c4|c5] [--scale 1.0]
        [--no-fp64] [--no-cpu-baseline] [--no-spmv]
@@ -122,8 +123,9 @@ def spmv_sweep(md) -> dict:
     return out
 
 
-def kernel_roofline(md, precision: str) -> dict:
-    """Live per-kernel timing of the two step kernels on the workload's matrix."""
+def kernel_roofline(md, precision: str, adaptive: bool = False) -> dict:
+    """Live per-kernel timing of the two step kernels on the workload's matrix (the adaptive
+    attempt's kernels when `adaptive`: same byte model, plus the per-attempt decision kernel)."""
     import numpy as np
     import scipy.sparse as sp
 
@@ -139,10 +141,16 @@ def kernel_roofline(md, precision: str) -> dict:
     dev = PdhgDevice.create((md.m, md.n, md.start, md.index, md.value),
                             (md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data),
                             cost, lower, upper, "fp32" if precision == "fp32" else "fp64")
-    dual_ms, primal_ms = dev.time_kernels(50)
+    decide_ms = None
+    if adaptive:
+        dual_ms, primal_ms, decide_ms = dev.time_adaptive_kernels(50)
+    else:
+        dual_ms, primal_ms = dev.time_kernels(50)
     s = 4 if precision == "fp32" else 8
     b = step_kernel_bytes(md.nnz, md.m, md.n, s)
-    return {"dual_ms": dual_ms, "primal_ms": primal_ms, "bytes": b}
+    ops = ("AdaptDualOp", "AdaptPrimalOp") if adaptive else ("DualRowsOp", "PrimalColsOp")
+    return {"dual_ms": dual_ms, "primal_ms": primal_ms, "decide_ms": decide_ms, "bytes": b,
+            "ops": {"dual": ops[0], "primal": ops[1]}}
 
 
 def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 128)) -> dict | None:
@@ -182,12 +190,19 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
+    ap.add_argument("--adaptive-steps", default="auto", choices=["auto", "on", "off"],
+                    help="PDLP's adaptive step sizes for the mixed-precision solves (auto: on for "
+                         "the single LP; the shared-matrix batch has none)")
     ap.add_argument("--verbose", action="store_true", help="progress lines of long batches (stdout)")
-    ap.add_argument("--fp32-switch-tol", type=float, default=1e-5,
-                    help="mixed precision: FP32 steps until this distance to termination (or a stall)")
+    ap.add_argument("--fp32-switch-tol", type=float, default=None,
+                    help="mixed precision: FP32 steps until this distance to termination (or a "
+                         "stall); default 1e-6 for the node relaxations, 1e-5 otherwise")
     ap.add_argument("--max-iterations", type=int, default=-1,
                     help="cap each solve (profiling runs); the line then reports no speedup")
     args = ap.parse_args()
+    if args.fp32_switch_tol is None:
+        args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
+    args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload == "c3")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the batched and "
               "single-LP shardings are described in DESIGN.md); run with --gpus 1",
@@ -210,7 +225,7 @@ def main() -> int:
     nm = md.native()
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, detect=False,
                            verify=True, polish_rounds=0, max_iterations=args.max_iterations,
-                           fp32_switch_tol=args.fp32_switch_tol)
+                           fp32_switch_tol=args.fp32_switch_tol, adaptive_steps=args.adaptive)
 
     for w in range(args.warmup):
         r = smy.solve_lp(nm, opts)
@@ -241,11 +256,13 @@ def main() -> int:
         fp64 = f
         log(f"fp64: {f.status_name} {f.iterations} it, iterate {f.solve_seconds - f.setup_seconds:.3f} s")
 
-    rl = kernel_roofline(md, "fp32")
+    rl = kernel_roofline(md, "fp32", args.adaptive)
     dom = "primal" if rl["primal_ms"] >= rl["dual_ms"] else "dual"
     dom_ms = rl[f"{dom}_ms"]
+    dom_op = rl["ops"][dom]
     achieved = rl["bytes"][dom] / (dom_ms * 1e-3) / 1e9
-    log(f"step kernels fp32: dual {rl['dual_ms'] * 1e3:.1f} us, primal {rl['primal_ms'] * 1e3:.1f} us")
+    log(f"step kernels fp32: dual {rl['dual_ms'] * 1e3:.1f} us, primal {rl['primal_ms'] * 1e3:.1f} us"
+        + (f", decision {rl['decide_ms'] * 1e3:.1f} us" if rl["decide_ms"] is not None else ""))
 
     spmv = None if args.no_spmv else spmv_sweep(md)
     cpu = None
@@ -266,6 +283,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32 steps + fp64 evaluation/polish",
         "algorithm": args.algorithm,
+        "adaptive_steps": args.adaptive,
         "status": last.status_name,
         "data": "synthetic (seeded generator)",
         "config": {"workload": "sparse LP m=500000 n=1000000 nnz~12M power-law columns, "
@@ -279,9 +297,8 @@ def main() -> int:
         "objective": last.objective,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-                     "traffic": (profiled_traffic("PrimalColsOp<float>" if dom == "primal" else "DualRowsOp<float>")
-                                 if args.scale == 1.0 else None),
-                     "kernel": f"csr_rows_kernel<float, {'PrimalColsOp' if dom == 'primal' else 'DualRowsOp'}<float>>",
+                     "traffic": profiled_traffic(f"{dom_op}<float>") if args.scale == 1.0 else None,
+                     "kernel": f"csr_rows_kernel<float, {dom_op}<float>>",
                      "avg_us": round(dom_ms * 1e3, 2), "bytes_per_launch": rl["bytes"][dom]},
         "cpu_baseline": cpu,
     }
@@ -311,6 +328,7 @@ def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict
         "vs_baseline": None,
         "dtype": "fp32 steps + fp64 evaluation/polish",
         "algorithm": args.algorithm,
+        "adaptive_steps": args.adaptive,
         "data": "synthetic (seeded generator)",
         "config": {"workload": workload},
     }
@@ -368,7 +386,8 @@ def run_c4(args) -> int:
     log(f"{K} blending LPs, rows {min(md.m for md in models)}..{max(md.m for md in models)}, "
         f"nnz {nnz}, generated in {time.perf_counter() - t0:.1f} s")
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
-                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0)
+                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
+                           adaptive_steps=args.adaptive)
     for w in range(args.warmup):
         smy.solve_lp_batch(nms[:8], opts)
     runs = []
@@ -459,7 +478,8 @@ def run_c5(args) -> int:
     nm = md.native()
     log(f"knapsack m={md.m} n={md.n} nnz={md.nnz}, {K} bound vectors, {time.perf_counter() - t0:.1f} s")
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
-                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0)
+                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
+                           adaptive_steps=args.adaptive)
     root = smy.solve_lp(nm, opts)
     log(f"root: {root.status_name} {root.iterations} it, omega {root.omega:.3e}")
     start = (root.col_value, root.row_dual)
@@ -489,7 +509,7 @@ def run_c5(args) -> int:
         # when the Farkas test succeeds), each class scaled by its count.
         feas = [i for i in range(K) if not infeasible[i]]
         infs = [i for i in range(K) if infeasible[i]]
-        strata = [(feas, feas[::max(1, len(feas) // 8)][:8]), (infs, infs[::max(1, len(infs) // 4)][:4])]
+        strata = [(feas, feas[::max(1, len(feas) // 32)][:32]), (infs, infs[::max(1, len(infs) // 16)][:16])]
         o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=False,
                               start_omega=root.omega)
         total, its64, objs, sample = 0.0, [], [], []
@@ -549,7 +569,7 @@ def run_c5(args) -> int:
     if fp64 is not None:
         extra["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
         extra["fp64_sample"] = (f"{len(fp64['sample'])} of {K} nodes solved one at a time, warm-started "
-                                f"alike: 8 feasible and 4 infeasible, each mean scaled by its class count")
+                                f"alike: 32 feasible and 16 infeasible, each mean scaled by its class count")
         extra["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
         extra["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
     line = base_line(args, value, wall / args.steps * 1e3, extra["config"]["workload"], extra)

@@ -162,6 +162,43 @@ int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out);
 /* The projected ray (n_i structural entries) of instance i's last detection. */
 int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray);
 
+/* Adaptive step sizes (no reference counterpart: the reference's adaptive step is disabled code,
+ * src/lp/pdlp.cpp:1051-1067). PDLP's rule (Applegate et al., NeurIPS 2021, sec. 3.1): an attempt
+ * is one PDHG step with tau = eta / omega, sigma = eta * omega in the primal-first order; with the
+ * moves dv, dy it is accepted when eta <= (omega ||dv||^2 + ||dy||^2 / omega) / (2 |dy' [A -I] dv|),
+ * else retried from the same point; either way eta becomes
+ * min((1 - (k+1)^-0.3) * that limit, (1 + (k+1)^-0.6) * eta), k = attempts so far. The averages
+ * weight each accepted iterate by its eta. The decisions are taken on the device. */
+typedef struct smy_pdhg_adaptive {
+  double eta;          /* step size of the next attempt */
+  double omega;        /* primal weight */
+  double weight;       /* sum of the accepted step sizes since the restart (0 after a restart:
+                          the next accepted iterate becomes the average) */
+  long long attempts;  /* attempts since the start of the solve */
+  long long rejected;  /* of which rejected */
+} smy_pdhg_adaptive;
+
+/* `count` attempts of every instance that is not frozen, from state[i] (batch size), which is
+ * updated. Afterwards the handle is laid out as after `count` smy_pdhg_run_batch steps: v_new /
+ * y_new hold the current (last accepted) iterate, v / y the point before it (or the last
+ * rejected attempt), the averages their step-size weighted means; smy_pdhg_advance and the other
+ * calls apply unchanged. Not available for the shared-matrix batch. Synchronizes. */
+int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count);
+/* Measurement helper: average milliseconds of the adaptive attempt's dual (rows of A), primal
+ * (rows of A') and decision kernels over `reps` launches each. Changes the state. */
+int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms,
+                                   double* decide_ms);
+
+/* Product tiles (no reference counterpart; the reference picks threads per row from nnz / rows,
+ * src/gpu/spmv.cu:71-77). A tile maps the row blocks of the stacked batch's A (matrix 0) or A'
+ * (matrix 1) with values of one precision to workgroups: per_wg (1, 2, 4) consecutive blocks per
+ * workgroup, and with xcd = 1 one contiguous range of blocks per XCD. The first run in a
+ * precision times every tile on that matrix's product and keeps the fastest; since each row is
+ * summed inside its own block the same way under every tile, the choice never changes a result.
+ * get returns SMY_STATE before the tuner ran; set fixes a tile (the tuner then leaves it). */
+int smy_pdhg_get_tile(smy_pdhg* h, int matrix, int precision, int* per_wg, int* xcd);
+int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xcd);
+
 /* Replaces gpu::csr_spmv (spmv.hpp:16-21): y = A x for a CSR matrix in device memory (rows x ?,
  * ptr[rows + 1], idx[nnz], val[nnz]), x and y device arrays. threads_per_row is accepted for
  * the reference's signature; the gfx950 kernel picks its lane groups per row block. Runs on

@@ -96,6 +96,11 @@ typedef struct smy_pdlp_params {
   int share_matrix;          /* batches of one model: run them as a shared-matrix batch (1) */
   int device_detect;         /* infeasibility / unboundedness tests on the device (1) or host */
   double start_omega;        /* primal weight to start with (<= 0: 1); with a warm start */
+  int adaptive_steps;        /* PDLP's adaptive step sizes, decided on the device (0: constant
+                                0.99 / L as the reference); batches then use the stacked layout */
+  double fp32_stall_level;   /* mixed: a plateau of the FP32 steps (no 20% gain within
+                                max(4096, iter / 4) iterations) below this distance also
+                                switches to FP64 (1e-4; 0: never) */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);
@@ -116,6 +121,7 @@ typedef struct smy_pdlp_info {
   int restarts;
   int evaluations;
   char message[256];
+  long long rejected_steps;  /* adaptive steps: rejected attempts (included in iterations) */
 } smy_pdlp_info;
 
 /* solve_pdlp on an LpProblem in computational form (min cost'v, [A -I] v = 0,

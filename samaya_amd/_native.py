@@ -28,7 +28,8 @@ EXPORTS = (
     "smy_pdhg_set_start_at", "smy_pdhg_download_at", "smy_pdhg_run_batch",
     "smy_pdhg_evaluate_batch", "smy_pdhg_restart_distance_batch", "smy_pdhg_restart_to_batch",
     "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared",
-    "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_run_adaptive",
+    "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_time_adaptive_kernels", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
@@ -50,7 +51,8 @@ class SmyPdlpParams(C.Structure):
                 ("use_gpu", C.c_int), ("precision", C.c_int), ("fp32_switch_tol", C.c_double),
                 ("use_graph", C.c_int), ("detect", C.c_int), ("log_level", C.c_int),
                 ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int),
-                ("share_matrix", C.c_int), ("device_detect", C.c_int), ("start_omega", C.c_double)]
+                ("share_matrix", C.c_int), ("device_detect", C.c_int), ("start_omega", C.c_double),
+                ("adaptive_steps", C.c_int), ("fp32_stall_level", C.c_double)]
 
 
 class SmyPdlpInfo(C.Structure):
@@ -59,7 +61,13 @@ class SmyPdlpInfo(C.Structure):
                 ("dual_residual", C.c_double), ("gap", C.c_double),
                 ("operator_norm", C.c_double), ("omega", C.c_double), ("setup_seconds", C.c_double),
                 ("solve_seconds", C.c_double), ("kernel_ms", C.c_double), ("restarts", C.c_int),
-                ("evaluations", C.c_int), ("message", C.c_char * 256)]
+                ("evaluations", C.c_int), ("message", C.c_char * 256),
+                ("rejected_steps", C.c_longlong)]
+
+
+class SmyPdhgAdaptive(C.Structure):
+    _fields_ = [("eta", C.c_double), ("omega", C.c_double), ("weight", C.c_double),
+                ("attempts", C.c_longlong), ("rejected", C.c_longlong)]
 
 
 class SmyLpResult(C.Structure):
@@ -134,6 +142,11 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_freeze", i32, vp, i32)
     _proto(lib, "smy_pdhg_detect_batch", i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_download_ray", i32, vp, i32, vp)
+    _proto(lib, "smy_pdhg_run_adaptive", i32, vp, vp, i32)
+    _proto(lib, "smy_pdhg_get_tile", i32, vp, i32, i32, C.POINTER(i32), C.POINTER(i32))
+    _proto(lib, "smy_pdhg_set_tile", i32, vp, i32, i32, i32, i32)
+    _proto(lib, "smy_pdhg_time_adaptive_kernels", i32, vp, i32, C.POINTER(f64), C.POINTER(f64),
+           C.POINTER(f64))
     _proto(lib, "smy_pdhg_create_shared", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), i32, vp, vp,
            vp, i32, C.POINTER(vp))
     _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)

@@ -13,6 +13,10 @@
 //   4. applies the epilogue functor (Op::row) to each row total: the PDHG update of that row's
 //      variable, or the evaluation sums, so no product makes a round trip through HBM.
 //
+// Tiles. A launch may give each workgroup 1, 2 or 4 consecutive blocks of its list, and may
+// order the workgroups so that each XCD takes one contiguous range of blocks; a row's sum depends
+// only on its block, so every tile gives the same bits and a timing-based tuner can pick freely.
+//
 // Batches. Several LPs stacked block-diagonally share the kernels: every block belongs to one
 // LP (its instance, binst[b]), whose step scalars the epilogue reads, and a launch covers a list
 // of blocks (the blocks of the instances that step in that precision). A single LP is the batch
@@ -40,7 +44,22 @@ struct CsrRows {
   // The instance is blockIdx.y (one matrix shared by the instances, each launched over all
   // blocks) instead of binst[b].
   bool inst_from_y = false;
+  // The launch's tile: `count` blocks (list entries, or blocks 0..count-1), `per_wg` consecutive
+  // ones per workgroup; with `xcd`, each of the 8 XCDs (which take workgroups round-robin) gets
+  // one contiguous range of the blocks, so rows that gather nearby columns share that XCD's L2.
+  // No tile changes a single sum: every row is summed within its own block.
+  int count = 0;
+  int per_wg = 1;
+  bool xcd = false;
 };
+
+// Workgroup w's place when the 8 XCDs each take one contiguous range of the n workgroups
+// (hardware dispatches workgroup w to XCD w % 8).
+__device__ inline int xcd_order(int w, int n) {
+  constexpr int X = 8;
+  const int x = w % X, i = w / X, q = n / X, r = n % X;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
 
 // Scalars of one PDHG step for one instance, read by the step kernels from device memory so that
 // the same kernel nodes can be replayed from a HIP graph with new values. alpha = 1 / (k0 + j + 1)
@@ -78,8 +97,20 @@ __device__ inline void block_partials(const double (&local)[K], double* __restri
   }
 }
 
+// What an epilogue's pre / gather / row receive about the workgroup's instance: the instance
+// index, or, for an op that defines context(inst), whatever it derives from it once per
+// workgroup (step sizes and the buffers of the instance's current slot).
+template <typename Op>
+__device__ inline auto op_context(const Op& op, int inst) {
+  if constexpr (requires { op.context(inst); })
+    return op.context(inst);
+  else
+    return inst;
+}
+
+// One row block b of instance inst (see the file comment).
 template <typename V, typename Op>
-__global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
+__device__ inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b, int inst) {
   using Acc = typename Op::Acc;
   using Pre = typename Op::Pre;
   constexpr int NV = Op::kVec;
@@ -89,11 +120,10 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
   __shared__ Acc wsum[NV][kBlock / kWave];
   __shared__ int rp[kBlockRows + 1];
   const int tid = threadIdx.x;
-  const int b = A.list != nullptr ? A.list[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int2 m0 = A.meta[b];
   const int2 m1 = A.meta[b + 1];
-  const int inst = A.inst_from_y ? static_cast<int>(blockIdx.y) : A.binst != nullptr ? A.binst[b] : 0;
   if (op.skip(inst)) return;  // uniform over the workgroup, before any barrier
+  const auto ctx = op_context(op, inst);
   const int r0 = m0.x, p0 = m0.y;
   const int nrows = m1.x - m0.x;
   const int nnz = m1.y - m0.y;
@@ -118,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
   Pre pre{};
   if (tid < nrows) {
     rp[tid] = A.ptr[r0 + tid] - p0;
-    pre = op.pre(r0 + tid, inst);
+    pre = op.pre(r0 + tid, ctx);
   }
   if (tid == 0) rp[nrows] = nnz;
 
@@ -132,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
     for (int u = 0; u < kPer; ++u) {
       const bool live = tid + u * kBlock < nnz;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, c[u], inst) : Acc(0);
+      for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, c[u], ctx) : Acc(0);
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -184,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) gv[v][u] = p + u * kBlock < end ? op.gather(v, cl[u], inst) : Acc(0);
+        for (int v = 0; v < NV; ++v) gv[v][u] = p + u * kBlock < end ? op.gather(v, cl[u], ctx) : Acc(0);
       }
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
@@ -209,11 +239,25 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
     Acc acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = rowsum[v][tid];
-    op.row(r0 + tid, inst, acc, pre, sums);
+    op.row(r0 + tid, ctx, acc, pre, sums);
   }
   if constexpr (Op::kSums > 0)
     block_partials<Op::kSums, Op::kMaxFrom>(sums, op.partial + static_cast<size_t>(inst) * op.inst_stride + b,
                                            op.stride);
+}
+
+template <typename V, typename Op>
+__global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
+  const int w = A.xcd ? xcd_order(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))
+                      : static_cast<int>(blockIdx.x);
+  for (int t = 0; t < A.per_wg; ++t) {
+    const int slot = w * A.per_wg + t;
+    if (slot >= A.count) return;  // uniform over the workgroup
+    if (t > 0) __syncthreads();   // the last block's LDS reads are done
+    const int b = A.list != nullptr ? A.list[slot] : slot;
+    const int inst = A.inst_from_y ? static_cast<int>(blockIdx.y) : A.binst != nullptr ? A.binst[b] : 0;
+    csr_rows_block(A, op, b, inst);
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -328,6 +372,322 @@ struct PrimalColsOp {
     v_new[col] = vn;
     v_erg[col] = (T(1) - alpha) * p.v_erg + alpha * vn;
     vbar_next[col] = vn + theta * (vn - p.v);
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// Adaptive step sizes: PDLP's rule (Applegate et al., "Practical large-scale linear programming
+// using primal-dual hybrid gradient", NeurIPS 2021, sec. 3.1). The reference has no counterpart:
+// its adaptive step is disabled code (src/lp/pdlp.cpp:1051-1067). An attempt takes one PDHG step
+// with step size eta from z = (v, y) in the primal-first order
+//   v+ = clamp(v - tau (c + K'y)),  y+ = y + sigma K (2 v+ - v),   K = [A -I],
+// tau = eta / omega, sigma = eta omega. With dv = v+ - v, dy = y+ - y,
+//   movement = omega ||dv||^2 / 2 + ||dy||^2 / (2 omega),  interaction = |dy' K dv|,
+//   limit = movement / interaction (infinite when the interaction is 0).
+// The attempt is accepted when eta <= limit; either way the next attempt uses
+//   eta' = min((1 - (k + 1)^-0.3) limit, (1 + (k + 1)^-0.6) eta),   k = attempts so far.
+// A rejected attempt is retried from the same z. The averages weight each accepted iterate by its
+// step size. Every quantity lives on the device, so a run of attempts replays from one graph:
+// each instance's current iterate sits in slot `cur` of the two buffers of v, y and A x (and of
+// the averages in slot `erg`), an attempt writes the other slot, and acceptance flips the slot.
+// K dv comes without another product: the dual kernel keeps h = A x of the current iterate, and
+// A (2 x+ - x) = 2 A x+ - h gives A x+ and A dx from the one product it forms anyway.
+// ------------------------------------------------------------------------------------------
+
+struct AdaptState {
+  double eta;        // step size of the next attempt
+  double omega;      // primal weight
+  double weight;     // sum of the accepted step sizes since the restart (the averages' weight)
+  long long total;   // attempts since the start of the solve (k of the rule)
+  long long rejected;
+  int cur;           // slot of the current v, y, h
+  int erg;           // slot of the current averages
+};
+
+// A x of the current iterate, for the instances flagged (after a start, restart or change of
+// precision; the others keep the A x their own steps formed, so a batch stays bit-equal to
+// single handles).
+template <typename T>
+struct AxRefreshOp {
+  using Acc = T;
+  using Pre = NoPre;
+  static constexpr int kVec = 1;
+  static constexpr int kSums = 0;
+  const int* __restrict__ flags;
+  const T* __restrict__ x;
+  T* __restrict__ y;
+  __device__ bool skip(int inst) const { return flags[inst] == 0; }
+  __device__ Pre pre(int, int) const { return {}; }
+  __device__ Acc gather(int, int c, int) const { return x[c]; }
+  __device__ void row(int r, int, const Acc* acc, const Pre&, double*) const { y[r] = acc[0]; }
+};
+
+template <typename T>
+struct AdaptPrimalCtx {
+  T tau, alpha;
+  const T* y;   // gathered: the current dual
+  const T* v;
+  T* v_out;
+  const T* e;
+  T* e_out;
+};
+
+template <typename T>
+struct AdaptPrimalPre {
+  T v, v_erg, cost, lower, upper;
+};
+
+// Rows of A' (structural variables): v+_j = clamp(v_j - tau (c_j + (A'y)_j)), the point
+// 2 v+ - v the dual kernel multiplies, the weighted average, and ||dx||^2 (partial slot 0).
+template <typename T>
+struct AdaptPrimalOp {
+  using Acc = T;
+  using Pre = AdaptPrimalPre<T>;
+  using Ctx = AdaptPrimalCtx<T>;
+  static constexpr int kVec = 1;
+  static constexpr int kSums = 1;
+  static constexpr int kMaxFrom = 1;
+  const AdaptState* __restrict__ st;
+  const T* y0;
+  const T* y1;
+  T* v0;
+  T* v1;
+  T* e0;
+  T* e1;
+  T* __restrict__ xbar;
+  const T* __restrict__ cost;
+  const T* __restrict__ lower;
+  const T* __restrict__ upper;
+  double* __restrict__ partial;
+  size_t stride;
+  int inst_stride = 0;
+  __device__ bool skip(int) const { return false; }
+  __device__ Ctx context(int inst) const {
+    const AdaptState s = st[inst];
+    const bool c = s.cur != 0, g = s.erg != 0;
+    return Ctx{static_cast<T>(s.eta / s.omega), static_cast<T>(s.eta / (s.weight + s.eta)),
+               c ? y1 : y0, c ? v1 : v0, c ? v0 : v1, g ? e1 : e0, g ? e0 : e1};
+  }
+  __device__ Pre pre(int col, const Ctx& x) const {
+    return Pre{x.v[col], x.e[col], cost[col], lower[col], upper[col]};
+  }
+  __device__ Acc gather(int, int c, const Ctx& x) const { return x.y[c]; }
+  __device__ void row(int col, const Ctx& x, const Acc* acc, const Pre& p, double* s) const {
+    const T vn = clamp_to(p.v - x.tau * (p.cost + acc[0]), p.lower, p.upper);
+    x.v_out[col] = vn;
+    xbar[col] = vn + (vn - p.v);
+    x.e_out[col] = (T(1) - x.alpha) * p.v_erg + x.alpha * vn;
+    const double d = static_cast<double>(vn) - static_cast<double>(p.v);
+    s[0] += d * d;
+  }
+};
+
+template <typename T>
+struct AdaptDualCtx {
+  T tau, sigma, alpha;
+  const T* y;
+  T* y_out;
+  const T* ye;
+  T* ye_out;
+  const T* v;
+  T* v_out;
+  const T* ve;
+  T* ve_out;
+  const T* h;
+  T* h_out;
+};
+
+template <typename T>
+struct AdaptDualPre {
+  T y, y_erg, r, r_erg, cost, lower, upper, h;
+};
+
+// Rows of A: row i's logical variable r = v_{n+i} steps first (r+ = clamp(r - tau (c_r - y_i))),
+// then y+_i = y_i + sigma ((A xbar)_i - (2 r+ - r)), with xbar = 2 x+ - x from the primal kernel.
+// A x+ = ((A xbar)_i + h_i) / 2 replaces h, and (A dx)_i = ((A xbar)_i - h_i) / 2 gives row i's
+// share of dy' K dv. Partial slots: 0 ||dr||^2, 1 ||dy||^2, 2 dy' K dv.
+template <typename T>
+struct AdaptDualOp {
+  using Acc = T;
+  using Pre = AdaptDualPre<T>;
+  using Ctx = AdaptDualCtx<T>;
+  static constexpr int kVec = 1;
+  static constexpr int kSums = 3;
+  static constexpr int kMaxFrom = 3;
+  int n;
+  const AdaptState* __restrict__ st;
+  const T* __restrict__ xbar;
+  T* y0;
+  T* y1;
+  T* ye0;
+  T* ye1;
+  T* v0;
+  T* v1;
+  T* ve0;
+  T* ve1;
+  T* h0;
+  T* h1;
+  const T* __restrict__ cost;
+  const T* __restrict__ lower;
+  const T* __restrict__ upper;
+  double* __restrict__ partial;
+  size_t stride;
+  int inst_stride = 0;
+  __device__ bool skip(int) const { return false; }
+  __device__ Ctx context(int inst) const {
+    const AdaptState s = st[inst];
+    const bool c = s.cur != 0, g = s.erg != 0;
+    return Ctx{static_cast<T>(s.eta / s.omega), static_cast<T>(s.eta * s.omega),
+               static_cast<T>(s.eta / (s.weight + s.eta)),
+               c ? y1 : y0, c ? y0 : y1, g ? ye1 : ye0, g ? ye0 : ye1,
+               c ? v1 : v0, c ? v0 : v1, g ? ve1 : ve0, g ? ve0 : ve1,
+               c ? h1 : h0, c ? h0 : h1};
+  }
+  __device__ Pre pre(int i, const Ctx& x) const {
+    const int k = n + i;
+    return Pre{x.y[i], x.ye[i], x.v[k], x.ve[k], cost[k], lower[k], upper[k], x.h[i]};
+  }
+  __device__ Acc gather(int, int c, const Ctx&) const { return xbar[c]; }
+  __device__ void row(int i, const Ctx& x, const Acc* acc, const Pre& p, double* s) const {
+    const int k = n + i;
+    const T rn = clamp_to(p.r - x.tau * (p.cost - p.y), p.lower, p.upper);
+    const T rbar = rn + (rn - p.r);
+    const T yn = p.y + x.sigma * (acc[0] - rbar);
+    const T adx = (acc[0] - p.h) * T(0.5);
+    x.y_out[i] = yn;
+    x.v_out[k] = rn;
+    x.h_out[i] = (acc[0] + p.h) * T(0.5);
+    x.ye_out[i] = (T(1) - x.alpha) * p.y_erg + x.alpha * yn;
+    x.ve_out[k] = (T(1) - x.alpha) * p.r_erg + x.alpha * rn;
+    const double dr = static_cast<double>(rn) - static_cast<double>(p.r);
+    const double dy = static_cast<double>(yn) - static_cast<double>(p.y);
+    s[0] += dr * dr;
+    s[1] += dy * dy;
+    s[2] += dy * (static_cast<double>(adx) - dr);
+  }
+};
+
+// One 1024-thread workgroup per instance: the attempt's totals from the block partials (the
+// instance's A blocks hold slots 0-2, its A' blocks slot 0), then the rule. It runs after every
+// attempt, so the partials (tens of thousands for a large LP) are read with several loads in
+// flight per thread; each thread still adds its entries in index order, so the totals are fixed.
+// Instances that do not step (frozen) keep their state.
+constexpr int kDecideThreads = 1024;
+constexpr int kDecideUnroll = 4;
+
+static __global__ __launch_bounds__(kDecideThreads) void adapt_decide_kernel(
+    int stride, const double* __restrict__ partial, const int2* __restrict__ ranges,
+    const int* __restrict__ prec, AdaptState* __restrict__ st) {
+  constexpr int W = kDecideThreads / kWave;
+  __shared__ double wsum[3][W];
+  const int i = blockIdx.x;
+  if (prec[i] < 0) return;  // frozen
+  const int tid = threadIdx.x;
+  const int2 ra = ranges[2 * i];
+  const int2 rt = ranges[2 * i + 1];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int b0 = ra.x + tid; b0 < ra.y; b0 += kDecideUnroll * kDecideThreads) {
+    double v[kDecideUnroll][3];
+#pragma unroll
+    for (int u = 0; u < kDecideUnroll; ++u) {
+      const int b = b0 + u * kDecideThreads;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[u][q] = b < ra.y ? partial[static_cast<size_t>(q) * stride + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kDecideUnroll; ++u) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc[q] += v[u][q];
+    }
+  }
+  double at = 0.0;
+  for (int b0 = rt.x + tid; b0 < rt.y; b0 += kDecideUnroll * kDecideThreads) {
+    double v[kDecideUnroll];
+#pragma unroll
+    for (int u = 0; u < kDecideUnroll; ++u) {
+      const int b = b0 + u * kDecideThreads;
+      v[u] = b < rt.y ? partial[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kDecideUnroll; ++u) at += v[u];
+  }
+  acc[0] = acc[0] + at;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double t = group_sum<kWave>(acc[q]);
+    if ((tid & (kWave - 1)) == 0) wsum[q][tid / kWave] = t;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  double tot[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    tot[q] = 0.0;
+    for (int w = 0; w < W; ++w) tot[q] += wsum[q][w];
+  }
+  AdaptState s = st[i];
+  const double movement = 0.5 * s.omega * tot[0] + 0.5 / s.omega * tot[1];
+  const double interaction = fabs(tot[2]);
+  const double limit = interaction > 0.0 ? movement / interaction : INFINITY;
+  const bool ok = s.eta <= limit;
+  s.total += 1;
+  const double k1 = static_cast<double>(s.total + 1);
+  const double next = fmin((1.0 - pow(k1, -0.3)) * limit, (1.0 + pow(k1, -0.6)) * s.eta);
+  if (ok) {
+    s.cur ^= 1;
+    s.erg ^= 1;
+    s.weight += s.eta;
+  } else {
+    s.rejected += 1;
+  }
+  s.eta = next;
+  st[i] = s;
+}
+
+// After a run of attempts: every stepping instance's current iterate moves to the slot the
+// caller expects (`want`; the previous point, accepted or tried, to the other one), and its
+// averages to slot 0, so that the rest of the handle sees the layout of constant steps.
+template <typename T>
+struct AdaptSettleOp {
+  static constexpr int kSums = 0;
+  const AdaptState* __restrict__ st;
+  const int* __restrict__ prec;  // per instance: precision of its steps (frozen < 0)
+  int q;                         // the precision of this launch's buffers
+  int want;
+  T* v0;
+  T* v1;
+  T* y0;
+  T* y1;
+  T* h0;
+  T* h1;
+  T* ve0;
+  const T* ve1;
+  T* ye0;
+  const T* ye1;
+  __device__ int flag(int inst) const {
+    if (prec[inst] != q) return -1;
+    const int f = (st[inst].cur != want ? 1 : 0) | (st[inst].erg != 0 ? 2 : 0);
+    return f == 0 ? -1 : f;
+  }
+  __device__ void col(int k, int f, double*, int) const {
+    if (f & 1) {
+      const T a = v0[k];
+      v0[k] = v1[k];
+      v1[k] = a;
+    }
+    if (f & 2) ve0[k] = ve1[k];
+  }
+  __device__ void row_a(int i, int k, int f, double* s, int inst) const {
+    col(k, f, s, inst);
+    if (f & 1) {
+      const T a = y0[i];
+      y0[i] = y1[i];
+      y1[i] = a;
+      const T b = h0[i];
+      h0[i] = h1[i];
+      h1[i] = b;
+    }
+    if (f & 2) ye0[i] = ye1[i];
   }
 };
 

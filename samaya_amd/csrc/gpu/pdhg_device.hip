@@ -109,6 +109,9 @@ void rebuild_lists(smy_pdhg* h) {
         SMY_HIP(hipMemcpy(M->list[q], list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
     }
   }
+  std::vector<int> prec(static_cast<size_t>(h->K));
+  for (int i = 0; i < h->K; ++i) prec[i] = h->inst[i].prec;
+  SMY_HIP(hipMemcpy(h->step_prec, prec.data(), prec.size() * sizeof(int), hipMemcpyHostToDevice));
   ++h->list_gen;
   h->lists_dirty = false;
 }
@@ -216,6 +219,57 @@ void step_kernels(smy_pdhg* h, int j) {
   launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
 }
 
+// ---- Tile tuner ----------------------------------------------------------------------------
+// The first run in a precision times the candidate tiles of both matrices on their product with
+// the values of that precision (over the blocks of the instances stepping in it) and keeps the
+// fastest. Every tile sums every row in the same order, so the choice changes speed only: answers
+// do not depend on the timing.
+constexpr Tile kTiles[] = {{1, false}, {1, true}, {2, false}, {2, true}, {4, false}, {4, true}};
+
+template <typename T>
+void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x) {
+  constexpr int q = prec_of<T>();
+  if (M.tuned[q] || M.nlist[q] == 0) return;
+  if (h->tune_out == nullptr) h->tune_out = h->alloc<double>(std::max(h->m, h->n));
+  SpmvOp<T, T> op{x, reinterpret_cast<T*>(h->tune_out)};
+  constexpr int kReps = 3;
+  float best = 0.0f;
+  Tile pick;
+  for (const Tile& t : kTiles) {
+    launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream, &t);  // warm
+    SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+    for (int r = 0; r < kReps; ++r) launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream, &t);
+    SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+    SMY_HIP(hipEventSynchronize(h->ev_stop));
+    float ms = 0.0f;
+    SMY_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+    if (&t == kTiles || ms < best) {
+      best = ms;
+      pick = t;
+    }
+  }
+  M.tile[q] = pick;
+  M.tuned[q] = true;
+}
+
+// Before any capture: tiles of the precisions in use that have none yet.
+void tune(smy_pdhg* h) {
+  if (!h->autotune) return;
+  const bool any = (uses(h, SMY_FP32) && !(h->a.tuned[SMY_FP32] && h->at.tuned[SMY_FP32])) ||
+                   (uses(h, SMY_FP64) && !(h->a.tuned[SMY_FP64] && h->at.tuned[SMY_FP64]));
+  if (!any) return;
+  if (uses(h, SMY_FP32)) {
+    tune_matrix<float>(h, h->a, h->f.v[h->p]);
+    tune_matrix<float>(h, h->at, h->f.y[h->p]);
+  }
+  if (uses(h, SMY_FP64)) {
+    tune_matrix<double>(h, h->a, h->d.v[h->p]);
+    tune_matrix<double>(h, h->at, h->d.y[h->p]);
+  }
+  ++h->list_gen;  // graphs captured before hold the old launch shapes
+  h->timed = false;
+}
+
 void step_all(smy_pdhg* h, int j) {
   if (uses(h, SMY_FP32)) step_kernels<float>(h, j);
   if (uses(h, SMY_FP64)) step_kernels<double>(h, j);
@@ -256,6 +310,7 @@ int step_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
   step_all(h, 0);
   SMY_HIP(hipGetLastError());
   h->theta_vbar = theta;
+  std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   h->vbar_any = false;
   h->stepped = true;
   return SMY_OK;
@@ -265,6 +320,7 @@ int run_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
              const long long* k0, int count) {
   if (count <= 0) return SMY_OK;
   refresh_lists(h);
+  tune(h);
   const int rc = refresh_vbar(h, theta);
   if (rc != SMY_OK) return rc;
   SMY_HIP(hipEventRecord(h->ev_start, h->stream));
@@ -297,7 +353,234 @@ int run_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
   SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
   h->timed = true;
   h->theta_vbar = theta;
+  std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   h->vbar_any = false;
+  h->stepped = true;
+  return SMY_OK;
+}
+
+// ---- Adaptive steps ----------------------------------------------------------------------
+
+template <typename T>
+void ensure_adaptive_state(smy_pdhg* h, Iter<T>& S) {
+  if (!S.ready || S.ax[0] != nullptr) return;
+  S.ax[0] = h->zeros<T>(h->m);
+  S.ax[1] = h->zeros<T>(h->m);
+  S.v_erg2 = h->zeros<T>(h->nt);
+  S.y_erg2 = h->zeros<T>(h->m);
+}
+
+void ensure_adaptive(smy_pdhg* h) {
+  if (h->adapt == nullptr) {
+    h->adapt = h->alloc<AdaptState>(h->K);
+    h->ax_flags = h->alloc<int>(h->K);
+    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->adapt_host), sizeof(AdaptState) * h->K,
+                          hipHostMallocDefault));
+  }
+  ensure_adaptive_state(h, h->d);
+  ensure_adaptive_state(h, h->f);
+}
+
+// ax[p] = A x of the current iterates of the flagged instances (after a start, a restart or a
+// change of precision).
+template <typename T>
+void refresh_ax(smy_pdhg* h) {
+  constexpr int q = prec_of<T>();
+  Iter<T>& S = state<T>(h);
+  AxRefreshOp<T> op{h->ax_flags, S.v[h->p], S.ax[h->p]};
+  launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], op, h->stream);
+}
+
+template <typename T>
+void adapt_primal(smy_pdhg* h) {
+  constexpr int q = prec_of<T>();
+  Iter<T>& S = state<T>(h);
+  AdaptPrimalOp<T> primal;
+  primal.st = h->adapt;
+  primal.y0 = S.y[0];
+  primal.y1 = S.y[1];
+  primal.v0 = S.v[0];
+  primal.v1 = S.v[1];
+  primal.e0 = S.v_erg;
+  primal.e1 = S.v_erg2;
+  primal.xbar = S.vbar[0];
+  primal.cost = S.cost;
+  primal.lower = S.lower;
+  primal.upper = S.upper;
+  primal.partial = h->partial + h->a.nblocks;
+  primal.stride = stride(h);
+  launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
+}
+
+template <typename T>
+void adapt_dual(smy_pdhg* h) {
+  constexpr int q = prec_of<T>();
+  Iter<T>& S = state<T>(h);
+  AdaptDualOp<T> dual;
+  dual.n = h->n;
+  dual.st = h->adapt;
+  dual.xbar = S.vbar[0];
+  dual.y0 = S.y[0];
+  dual.y1 = S.y[1];
+  dual.ye0 = S.y_erg;
+  dual.ye1 = S.y_erg2;
+  dual.v0 = S.v[0];
+  dual.v1 = S.v[1];
+  dual.ve0 = S.v_erg;
+  dual.ve1 = S.v_erg2;
+  dual.h0 = S.ax[0];
+  dual.h1 = S.ax[1];
+  dual.cost = S.cost;
+  dual.lower = S.lower;
+  dual.upper = S.upper;
+  dual.partial = h->partial;
+  dual.stride = stride(h);
+  launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], dual, h->stream);
+}
+
+void adapt_decide(smy_pdhg* h) {
+  hipLaunchKernelGGL(adapt_decide_kernel, dim3(h->K), dim3(kDecideThreads), 0, h->stream, stride(h),
+                     h->partial, h->ranges, h->step_prec, h->adapt);
+}
+
+void adapt_attempt(smy_pdhg* h) {
+  if (uses(h, SMY_FP32)) {
+    adapt_primal<float>(h);
+    adapt_dual<float>(h);
+  }
+  if (uses(h, SMY_FP64)) {
+    adapt_primal<double>(h);
+    adapt_dual<double>(h);
+  }
+  adapt_decide(h);
+}
+
+// A x of the instances whose iterate changed outside the attempts.
+void refresh_stale_ax(smy_pdhg* h, int mask) {
+  if (std::none_of(h->ax_stale.begin(), h->ax_stale.end(), [](int f) { return f != 0; })) return;
+  SMY_HIP(hipMemcpyAsync(h->ax_flags, h->ax_stale.data(), h->K * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  if (mask & 2) refresh_ax<float>(h);
+  if (mask & 1) refresh_ax<double>(h);
+  SMY_HIP(hipStreamSynchronize(h->stream));  // the flags' host copy may change after this
+  std::fill(h->ax_stale.begin(), h->ax_stale.end(), 0);
+}
+
+// Measurement helper: average milliseconds of the adaptive primal (rows of A'), dual (rows of A)
+// and decision kernels of the first precision in use, from a small step size. Changes the state.
+template <typename T>
+int time_adaptive_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms, double* decide_ms) {
+  ensure_adaptive(h);
+  staging_wait(h);
+  for (int i = 0; i < h->K; ++i) h->adapt_host[i] = AdaptState{1e-3, 1.0, 0.0, 0, 0, h->p, 0};
+  SMY_HIP(hipMemcpyAsync(h->adapt, h->adapt_host, h->K * sizeof(AdaptState), hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+  std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
+  refresh_stale_ax(h, prec_of<T>() == SMY_FP32 ? 2 : 1);
+  const auto timed = [&](auto go) {
+    go();  // warm
+    SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+    for (int r = 0; r < reps; ++r) go();
+    SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+    SMY_HIP(hipEventSynchronize(h->ev_stop));
+    float t = 0.0f;
+    SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
+    return static_cast<double>(t) / reps;
+  };
+  *primal_ms = timed([&] { adapt_primal<T>(h); });
+  *dual_ms = timed([&] { adapt_dual<T>(h); });
+  *decide_ms = timed([&] { adapt_decide(h); });
+  SMY_HIP(hipGetLastError());
+  std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
+  h->timed = false;
+  return SMY_OK;
+}
+
+template <typename T>
+void adapt_settle(smy_pdhg* h, int want) {
+  Iter<T>& S = state<T>(h);
+  AdaptSettleOp<T> op;
+  op.st = h->adapt;
+  op.prec = h->step_prec;
+  op.q = prec_of<T>();
+  op.want = want;
+  op.v0 = S.v[0];
+  op.v1 = S.v[1];
+  op.y0 = S.y[0];
+  op.y1 = S.y[1];
+  op.h0 = S.ax[0];
+  op.h1 = S.ax[1];
+  op.ve0 = S.v_erg;
+  op.ve1 = S.v_erg2;
+  op.ye0 = S.y_erg;
+  op.ye1 = S.y_erg2;
+  hipLaunchKernelGGL((block_rows_kernel<AdaptSettleOp<T>>), dim3(block_rows_grid(h)), dim3(kBlock), 0,
+                     h->stream, block_rows(h), op);
+}
+
+// `count` attempts, then every instance's current iterate moved to slot 1 - p of the final
+// parity (as after `count` constant steps). The state goes up before and comes back after.
+int run_adaptive_impl(smy_pdhg* h, smy_pdhg_adaptive* st, int count) {
+  if (count <= 0) return SMY_OK;
+  for (int i = 0; i < h->K; ++i)
+    if (h->inst[i].prec != kFrozen && !(st[i].eta > 0.0 && st[i].omega > 0.0))
+      return fail(SMY_INVALID, "adaptive steps need eta > 0 and omega > 0");
+  refresh_lists(h);
+  ensure_adaptive(h);
+  tune(h);
+  const int mask = (uses(h, SMY_FP64) ? 1 : 0) | (uses(h, SMY_FP32) ? 2 : 0);
+  refresh_stale_ax(h, mask);
+  staging_wait(h);
+  const int p0 = h->p;
+  for (int i = 0; i < h->K; ++i)
+    h->adapt_host[i] = AdaptState{st[i].eta, st[i].omega, st[i].weight, st[i].attempts, st[i].rejected, p0, 0};
+  SMY_HIP(hipMemcpyAsync(h->adapt, h->adapt_host, h->K * sizeof(AdaptState), hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+  const int p_end = p0 ^ ((count - 1) & 1);
+  const auto all = [&] {
+    if (h->nt > 0)
+      for (int j = 0; j < count; ++j) adapt_attempt(h);
+    if (block_rows_grid(h) > 0) {
+      if (mask & 2) adapt_settle<float>(h, 1 - p_end);
+      if (mask & 1) adapt_settle<double>(h, 1 - p_end);
+    }
+  };
+  if (count == smy_pdhg::kGraphSteps && mask != 0) {
+    smy_pdhg::Captured& c = h->agraphs[mask][p0];
+    if (c.exec == nullptr || c.gen != h->list_gen) {
+      h->drop_graph(c);
+      SMY_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      all();
+      SMY_HIP(hipStreamEndCapture(h->stream, &c.graph));
+      SMY_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+      c.gen = h->list_gen;
+    }
+    SMY_HIP(hipGraphLaunch(c.exec, h->stream));
+  } else if (mask != 0) {
+    all();
+  }
+  SMY_HIP(hipGetLastError());
+  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+  SMY_HIP(hipMemcpyAsync(h->adapt_host, h->adapt, h->K * sizeof(AdaptState), hipMemcpyDeviceToHost,
+                         h->stream));
+  SMY_HIP(hipStreamSynchronize(h->stream));
+  h->staged = false;
+  for (int i = 0; i < h->K; ++i) {
+    if (h->inst[i].prec == kFrozen) continue;
+    const AdaptState& s = h->adapt_host[i];
+    st[i].eta = s.eta;
+    st[i].weight = s.weight;
+    st[i].attempts = s.total;
+    st[i].rejected = s.rejected;
+  }
+  h->p = p_end;
+  h->timed = true;
+  // vbar holds 2 v+ - v of the last attempt, not the constant steps' extrapolation.
+  h->vbar_any = false;
+  h->theta_vbar = std::nan("");
   h->stepped = true;
   return SMY_OK;
 }
@@ -515,6 +798,8 @@ int restart_to_batch_impl(smy_pdhg* h, const int* which) {
   if (any_flag(h, which, SMY_FP64)) restart_launch<double>(h);
   SMY_HIP(hipGetLastError());
   h->vbar_any = true;
+  for (int i = 0; i < h->K; ++i)
+    if (which[i] >= 0) h->ax_stale[i] = 1;
   h->stepped = false;
   return SMY_OK;
 }
@@ -563,7 +848,10 @@ int set_precision_batch_impl(smy_pdhg* h, const int* precision) {
     SMY_HIP(hipGetLastError());
   }
   for (int i = 0; i < h->K; ++i)
-    if (flag[i] > 0) h->inst[i].prec = h->inst[i].last_prec = precision[i];
+    if (flag[i] > 0) {
+      h->inst[i].prec = h->inst[i].last_prec = precision[i];
+      h->ax_stale[i] = 1;
+    }
   h->lists_dirty = true;
   return SMY_OK;
 }
@@ -635,6 +923,7 @@ void set_start_impl(smy_pdhg* h, int i, const double* v, const double* y) {
   up(y, S.y_erg + L.oy, L.m);
   up(y, S.y_rst + L.oy, L.m);
   h->vbar_any = true;
+  h->ax_stale[i] = 1;
   h->stepped = false;
 }
 
@@ -707,6 +996,8 @@ int create_impl(int count, const smy_csc* A, const smy_csc* At, const double* co
     h->sums_dev = h->alloc<double>(28LL * count);
     h->scalars = h->alloc<StepScalars>(count);
     h->flags = h->alloc<int>(2LL * count);
+    h->step_prec = h->alloc<int>(count);
+    h->ax_stale.assign(static_cast<size_t>(count), 1);
     h->cs = h->alloc<double>(h->n);
     h->rs = h->alloc<double>(h->m);
     h->cost_o = h->alloc<double>(h->nt);
@@ -796,6 +1087,7 @@ void detect_csr(smy_pdhg* h, const DevMatrix& M, const Op& op) {
   A.val = M.val64;
   A.meta = M.meta;
   A.binst = M.binst;
+  A.count = M.nblocks;
   hipLaunchKernelGGL((csr_rows_kernel<double, Op>), dim3(M.nblocks), dim3(kBlock), 0, h->stream, A, op);
 }
 
@@ -985,16 +1277,60 @@ int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, doub
   return guarded([&]() -> int { return run_impl(h, tau, sigma, theta, iter0, count); });
 }
 
+int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count) {
+  if (h == nullptr || state == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->shared) return fail(SMY_INVALID, "adaptive steps need the stacked batch layout");
+  return guarded([&]() -> int { return run_adaptive_impl(h, state, count); });
+}
+
 int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   if (h == nullptr || dual_ms == nullptr || primal_ms == nullptr || reps <= 0)
     return fail(SMY_INVALID, "bad argument");
   if (h->shared) return guarded([&]() -> int { return shared_time_kernels(h, reps, dual_ms, primal_ms); });
   return guarded([&]() -> int {
     refresh_lists(h);
+    tune(h);
     if (uses(h, SMY_FP32)) return time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
     if (uses(h, SMY_FP64)) return time_kernels_impl<double>(h, reps, dual_ms, primal_ms);
     return fail(SMY_STATE, "no instance steps");
   });
+}
+
+int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms,
+                                   double* decide_ms) {
+  if (h == nullptr || dual_ms == nullptr || primal_ms == nullptr || decide_ms == nullptr || reps <= 0)
+    return fail(SMY_INVALID, "bad argument");
+  if (h->shared) return fail(SMY_INVALID, "adaptive steps need the stacked batch layout");
+  return guarded([&]() -> int {
+    refresh_lists(h);
+    tune(h);
+    if (uses(h, SMY_FP32)) return time_adaptive_impl<float>(h, reps, dual_ms, primal_ms, decide_ms);
+    if (uses(h, SMY_FP64)) return time_adaptive_impl<double>(h, reps, dual_ms, primal_ms, decide_ms);
+    return fail(SMY_STATE, "no instance steps");
+  });
+}
+
+int smy_pdhg_get_tile(smy_pdhg* h, int matrix, int precision, int* per_wg, int* xcd) {
+  if (h == nullptr || per_wg == nullptr || xcd == nullptr || (matrix != 0 && matrix != 1) ||
+      (precision != SMY_FP64 && precision != SMY_FP32))
+    return fail(SMY_INVALID, "bad argument");
+  if (h->shared) return fail(SMY_INVALID, "the shared-matrix batch has no product tiles");
+  const DevMatrix& M = matrix == 0 ? h->a : h->at;
+  *per_wg = M.tile[precision].per_wg;
+  *xcd = M.tile[precision].xcd ? 1 : 0;
+  return M.tuned[precision] ? SMY_OK : SMY_STATE;
+}
+
+int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xcd) {
+  if (h == nullptr || (matrix != 0 && matrix != 1) || (precision != SMY_FP64 && precision != SMY_FP32))
+    return fail(SMY_INVALID, "bad argument");
+  if (h->shared) return fail(SMY_INVALID, "the shared-matrix batch has no product tiles");
+  if (per_wg != 1 && per_wg != 2 && per_wg != 4) return fail(SMY_INVALID, "per_wg must be 1, 2 or 4");
+  DevMatrix& M = matrix == 0 ? h->a : h->at;
+  M.tile[precision] = Tile{per_wg, xcd != 0};
+  M.tuned[precision] = true;
+  ++h->list_gen;
+  return SMY_OK;
 }
 
 int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms) {

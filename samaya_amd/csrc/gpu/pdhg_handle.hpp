@@ -52,6 +52,12 @@ inline int blocks_for(long long count) { return static_cast<int>((count + kBlock
 constexpr int kFrozen = -1;
 constexpr int kMaxBatch = 16384;  // instances per handle (grid y of the per-instance reductions)
 
+// How a product launch maps row blocks to workgroups (CsrRows::per_wg, CsrRows::xcd).
+struct Tile {
+  int per_wg = 1;
+  bool xcd = false;
+};
+
 struct DevMatrix {
   int rows = 0;
   long long nnz = 0;
@@ -64,6 +70,8 @@ struct DevMatrix {
   int nblocks = 0;
   int* list[2] = {nullptr, nullptr};  // blocks of the instances stepping in FP64 / FP32
   int nlist[2] = {0, 0};
+  Tile tile[2];                 // by value precision (SMY_FP64, SMY_FP32): chosen by the tuner
+  bool tuned[2] = {false, false};
 };
 
 template <typename T>
@@ -79,6 +87,11 @@ struct Iter {
   T* lower = nullptr;
   T* upper = nullptr;
   bool ready = false;
+  // Adaptive steps only (allocated at their first run): A x of both slots of v, and the spare
+  // slot of the averages.
+  T* ax[2] = {nullptr, nullptr};
+  T* v_erg2 = nullptr;
+  T* y_erg2 = nullptr;
 };
 
 struct Layout {
@@ -96,11 +109,12 @@ constexpr int prec_of() {
   return std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
 }
 
-// Returns the grid used (0: nothing launched).
+// Returns the grid used (0: nothing launched). The tile is the matrix's for values of type V.
 template <typename V, typename Op>
 int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, const Op& op,
-                hipStream_t s) {
+                hipStream_t s, const Tile* force = nullptr) {
   if (nlist == 0) return 0;
+  const Tile t = force != nullptr ? *force : M.tile[prec_of<V>()];
   CsrRows<V> A;
   A.rows = M.rows;
   A.nblocks = M.nblocks;
@@ -110,8 +124,12 @@ int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, co
   A.meta = M.meta;
   A.binst = M.binst;
   A.list = list;
-  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(nlist), dim3(kBlock), 0, s, A, op);
-  return nlist;
+  A.count = nlist;
+  A.per_wg = t.per_wg;
+  A.xcd = t.xcd;
+  const int grid = (nlist + t.per_wg - 1) / t.per_wg;
+  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(grid), dim3(kBlock), 0, s, A, op);
+  return grid;
 }
 
 struct SharedBatch;
@@ -159,8 +177,19 @@ struct smy_pdhg {
     int gen = -1;
   };
   Captured graphs[4][2];
+  Captured agraphs[4][2];    // the same for runs of adaptive attempts
   int list_gen = 0;
   bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
+  int* step_prec = nullptr;  // per instance: precision of its steps, or kFrozen (device copy)
+  // Adaptive steps: per-instance rule state (device, pinned staging), and whether ax holds A x of
+  // the current iterates.
+  AdaptState* adapt = nullptr;
+  AdaptState* adapt_host = nullptr;
+  std::vector<int> ax_stale;  // per instance: A x must be formed again before its next attempt
+  int* ax_flags = nullptr;    // device copy for the refresh launch
+  // Tile tuner: on by default; its scratch output (max(m, n) doubles, at first use).
+  bool autotune = true;
+  double* tune_out = nullptr;
   SharedBatch* shared = nullptr;  // the shared-matrix batch (then the stacked fields are unused)
   // Device-side infeasibility / unboundedness tests (allocated at their first use).
   double* det_partial = nullptr;   // detection slots x blocks
@@ -203,8 +232,11 @@ struct smy_pdhg {
     if (shared) shared_free(shared);
     for (auto& row : graphs)
       for (auto& c : row) drop_graph(c);
+    for (auto& row : agraphs)
+      for (auto& c : row) drop_graph(c);
     for (void* ptr : allocations) (void)hipFree(ptr);
     if (sums_host) (void)hipHostFree(sums_host);
+    if (adapt_host) (void)hipHostFree(adapt_host);
     if (det_prm_host) (void)hipHostFree(det_prm_host);
     if (det_out_host) (void)hipHostFree(det_out_host);
     if (scalars_host) (void)hipHostFree(scalars_host);

@@ -51,6 +51,8 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.share_matrix = p->share_matrix != 0;
   o.device_detect = p->device_detect != 0;
   o.start_omega = p->start_omega;
+  o.adaptive_steps = p->adaptive_steps != 0;
+  o.fp32_stall_level = p->fp32_stall_level;
   return o;
 }
 
@@ -64,6 +66,7 @@ void fill_info(const PdlpResult& r, smy_pdlp_info* info) {
   info->gap = r.gap;
   info->operator_norm = r.operator_norm;
   info->omega = r.omega;
+  info->rejected_steps = r.rejected_steps;
   info->setup_seconds = r.setup_seconds;
   info->solve_seconds = r.solve_seconds;
   info->kernel_ms = r.kernel_ms;
@@ -130,6 +133,8 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->share_matrix = 1;
   p->device_detect = 1;
   p->start_omega = 0.0;
+  p->adaptive_steps = 0;
+  p->fp32_stall_level = 1e-4;
 }
 
 int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,

@@ -131,6 +131,10 @@ class Device {
       }
     }
   }
+  // `count` attempts of the adaptive step-size rule; the state is read and updated.
+  void run_adaptive(std::vector<smy_pdhg_adaptive>& state, int count) {
+    check(smy_pdhg_run_adaptive(h_, state.data(), count));
+  }
   void evaluate(std::vector<double>& out) {
     out.resize(14 * static_cast<size_t>(k_));
     check(smy_pdhg_evaluate_batch(h_, out.data()));
@@ -361,6 +365,8 @@ class Lane {
   // Iteration state.
   double omega = 1.0, tau = 0.0, sigma = 0.0;
   long long k_since = 0;
+  // Adaptive steps: the rule's state (eta above is then the next attempt's step size).
+  smy_pdhg_adaptive adapt{0.0, 1.0, 0.0, 0, 0};
   double kkt_at_restart = kInf, kkt_previous = kInf;
   bool fp32 = false;
   double fp32_best = kInf;       // best distance to termination in FP32 (by 20% steps)
@@ -429,6 +435,7 @@ void Lane::prepare(std::shared_ptr<const Equilibrated> e) {
   omega = opt_.start_omega > 0.0 ? opt_.start_omega : 1.0;
   tau = eta / omega;
   sigma = eta * omega;
+  adapt = smy_pdhg_adaptive{eta, omega, 0.0, 0, 0};
   log(3, ": estimated ||[A -I]|| = %.3e", L);
 
   max_iter = opt_.max_iterations;  // src/lp/pdlp.cpp:365-373
@@ -690,7 +697,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
 
   // One problem for all lanes, with one set of row bounds: the shared-matrix batch (iterates
   // stored instance-minor, SpMM products); otherwise the stacked batch.
-  bool share = opt.share_matrix && K > 1 && owners.size() == 1;
+  bool share = opt.share_matrix && !opt.adaptive_steps && K > 1 && owners.size() == 1;
   for (int i = 1; share && i < K; ++i) {
     const int n = lanes[0].lp_.n;
     share = std::equal(lanes[i].lower.begin() + n, lanes[i].lower.end(), lanes[0].lower.begin() + n) &&
@@ -753,6 +760,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   std::vector<double> tau(static_cast<size_t>(K)), sigma(static_cast<size_t>(K)), sums, dist, dout;
   std::vector<long long> k0(static_cast<size_t>(K));
   std::vector<int> which(static_cast<size_t>(K)), precision(static_cast<size_t>(K));
+  std::vector<smy_pdhg_adaptive> adapt(static_cast<size_t>(K));
   std::vector<double> v, v_new, y, y_new, v_erg, y_erg;
   int active = K;
   const auto stop = [&](int i) {
@@ -801,12 +809,23 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         eval_iter = std::min(eval_iter, std::min((iter / kCheckFrequency + 1) * kCheckFrequency - 1,
                                                  L.max_iter - 1));
     const int count = static_cast<int>(eval_iter - iter + 1);
-    for (int i = 0; i < K; ++i) {
-      tau[i] = lanes[i].tau;
-      sigma[i] = lanes[i].sigma;
-      k0[i] = lanes[i].k_since;
+    if (opt.adaptive_steps) {
+      for (int i = 0; i < K; ++i) adapt[i] = lanes[i].adapt;
+      device->run_adaptive(adapt, count);
+      for (int i = 0; i < K; ++i) {
+        Lane& L = lanes[i];
+        if (!L.active) continue;
+        L.adapt = adapt[i];
+        L.res.rejected_steps = adapt[i].rejected;
+      }
+    } else {
+      for (int i = 0; i < K; ++i) {
+        tau[i] = lanes[i].tau;
+        sigma[i] = lanes[i].sigma;
+        k0[i] = lanes[i].k_since;
+      }
+      device->run(tau, sigma, theta, k0, count, opt.use_graph);
     }
-    device->run(tau, sigma, theta, k0, count, opt.use_graph);
     const double ms = device->last_run_ms();
     if (info != nullptr) info->kernel_ms += ms;
     for (Lane& L : lanes) {
@@ -924,16 +943,20 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       const Measure& current = cur_m[i];
       const Measure& average = avg_m[i];
       // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance, or once
-      // FP32 stopped improving (its rounding floor lies above the switch point): no 20% gain on
-      // its best distance to termination within max(4096, iter / 4) iterations.
+      // FP32 stopped improving near its rounding floor (which lies above the switch point): no
+      // 20% gain on its best distance to termination within max(4096, iter / 4) iterations,
+      // counted from the first evaluation below fp32_stall_level (a plateau above it belongs to
+      // the algorithm, which FP64 steps would only make slower).
       if (L.fp32) {
         const double w = std::min(worst(current), worst(average));
-        if (w < 0.8 * L.fp32_best) {
+        const double level = opt.fp32_stall_level;
+        if (w < 0.8 * L.fp32_best || (w < level && !(L.fp32_best < level))) {
           L.fp32_best = w;
           L.fp32_best_iter = iter;
         }
         const bool reached = w < opt.fp32_switch_tol;
-        const bool stalled = iter - L.fp32_best_iter >= std::max<long long>(4096, iter / 4);
+        const bool stalled = L.fp32_best < level &&
+                             iter - L.fp32_best_iter >= std::max<long long>(4096, iter / 4);
         if (reached || stalled) {
           L.log(2, reached ? ": FP32 steps reached %.2e at iter %lld; FP64 steps from here"
                            : ": FP32 steps stalled at %.2e at iter %lld; FP64 steps from here",
@@ -975,6 +998,8 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
           L.tau = L.eta / L.omega;
           L.sigma = L.eta * L.omega;
         }
+        L.adapt.omega = L.omega;
+        L.adapt.weight = 0.0;  // the average starts again from the next accepted iterate
         const bool to_average = which[i] == SMY_AVERAGE;
         L.log(2, ": restart at iter %lld to the %s (KKT %.2e), omega %.3e", iter,
               to_average ? "average" : "current iterate", kkt_candidate[i], L.omega);

@@ -48,9 +48,13 @@ struct PdlpOptions {
   Precision precision = Precision::kFp64;
   Algorithm algorithm = Algorithm::kReference;
   // Mixed precision: FP32 steps until the better of current / average reaches this (FP64)
-  // termination distance, or FP32 stops improving (no 20% gain within max(4096, iter / 4)
-  // iterations); then FP64 steps to `tol`.
+  // termination distance, or FP32 stops improving below fp32_stall_level (no 20% gain within
+  // max(4096, iter / 4) iterations; 0: never); then FP64 steps to `tol`.
   double fp32_switch_tol = 1e-5;
+  double fp32_stall_level = 1e-4;
+  // PDLP's adaptive step sizes (decided on the device) instead of the constant 0.99 / L; the
+  // averages are then weighted by step size. Uses the stacked layout for every batch.
+  bool adaptive_steps = false;
   bool use_graph = true;          // replay each 64-step run from a HIP graph
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
   bool device_detect = true;      // ... computed on the device (false: host, from downloads)
@@ -83,6 +87,7 @@ struct PdlpResult {
   std::vector<double> primal_ray;  // n, when unbounded
   long long iterations = 0;
   long long fp32_iterations = 0;   // of which in FP32 (mixed precision)
+  long long rejected_steps = 0;    // adaptive steps: attempts rejected (counted in iterations)
   double primal_residual = 0.0;
   double dual_residual = 0.0;
   double gap = 0.0;

@@ -76,6 +76,13 @@ class PdhgDevice:
     def advance(self):
         N.check(self.lib.smy_pdhg_advance(self.h), "advance")
 
+    def time_adaptive_kernels(self, reps: int = 20) -> tuple[float, float, float]:
+        """Average ms of the adaptive dual, primal and decision kernels (changes the state)."""
+        d, p, q = C.c_double(), C.c_double(), C.c_double()
+        N.check(self.lib.smy_pdhg_time_adaptive_kernels(self.h, reps, C.byref(d), C.byref(p), C.byref(q)),
+                "time_adaptive_kernels")
+        return d.value, p.value, q.value
+
     def restart(self):
         N.check(self.lib.smy_pdhg_restart(self.h), "restart")
 
@@ -187,6 +194,39 @@ class PdhgBatch:
         t, s = _f64(tau), _f64(sigma)
         k0 = np.ascontiguousarray(iter0, np.int64)
         N.check(self.lib.smy_pdhg_run_batch(self.h, _p(t), _p(s), theta, _p(k0), count), "run_batch")
+
+    def run_adaptive(self, state, count: int):
+        """`count` attempts of the adaptive step-size rule. `state`: a ctypes array of
+        SmyPdhgAdaptive (batch size), read and updated."""
+        N.check(self.lib.smy_pdhg_run_adaptive(self.h, state, count), "run_adaptive")
+
+    def tile(self, matrix: int, precision: str = "fp64"):
+        """(per_wg, xcd) of A (matrix 0) or A' (1) for that value precision; None before tuning."""
+        pw, x = C.c_int(), C.c_int()
+        q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
+        rc = self.lib.smy_pdhg_get_tile(self.h, matrix, q, C.byref(pw), C.byref(x))
+        if rc == N.SMY_STATE:
+            return None
+        N.check(rc, "get_tile")
+        return pw.value, bool(x.value)
+
+    def set_tile(self, matrix: int, precision: str, per_wg: int, xcd: bool):
+        q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
+        N.check(self.lib.smy_pdhg_set_tile(self.h, matrix, q, per_wg, int(xcd)), "set_tile")
+
+    @staticmethod
+    def adaptive_state(eta, omega, weight=0.0, attempts=0, rejected=0):
+        """A state array for run_adaptive (per-instance sequences or scalars)."""
+        eta = np.atleast_1d(np.asarray(eta, float))
+        k = len(eta)
+        om = np.broadcast_to(np.asarray(omega, float), (k,))
+        w = np.broadcast_to(np.asarray(weight, float), (k,))
+        at = np.broadcast_to(np.asarray(attempts, np.int64), (k,))
+        rj = np.broadcast_to(np.asarray(rejected, np.int64), (k,))
+        arr = (N.SmyPdhgAdaptive * k)()
+        for i in range(k):
+            arr[i] = N.SmyPdhgAdaptive(float(eta[i]), float(om[i]), float(w[i]), int(at[i]), int(rj[i]))
+        return arr
 
     def advance(self):
         N.check(self.lib.smy_pdhg_advance(self.h), "advance")

@@ -186,6 +186,8 @@ class PdlpOptions:
     share_matrix: bool = True       # batches of one model: the shared-matrix (SpMM) batch
     device_detect: bool = True      # infeasibility / unboundedness tests on the GPU (else host)
     start_omega: float = 0.0        # primal weight to start with (<= 0: 1), with a warm start
+    adaptive_steps: bool = False    # PDLP's adaptive step sizes (decided on the GPU)
+    fp32_stall_level: float = 1e-4  # mixed: FP32 plateaus below this switch to FP64 (0: never)
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -205,6 +207,8 @@ class PdlpOptions:
         p.share_matrix = int(self.share_matrix)
         p.device_detect = int(self.device_detect)
         p.start_omega = self.start_omega
+        p.adaptive_steps = int(self.adaptive_steps)
+        p.fp32_stall_level = self.fp32_stall_level
         return p
 
 
@@ -228,6 +232,7 @@ class PdlpResult:
     restarts: list = field(default_factory=list)   # (iter, to_average, kkt, omega)
     evals: list = field(default_factory=list)      # (iter, (cur p, d, g), (avg p, d, g))
     omega: float = 1.0                             # the final primal weight
+    rejected_steps: int = 0                        # adaptive steps: rejected attempts
 
     @property
     def status_name(self) -> str:
@@ -265,7 +270,8 @@ def solve_pdlp(lp: LpProblem, options: PdlpOptions | None = None, start_x=None,
         primal_ray=pray if info.status == UNBOUNDED else None,
         iterations=info.iterations, fp32_iterations=info.fp32_iterations,
         primal_residual=info.primal_residual, dual_residual=info.dual_residual, gap=info.gap,
-        operator_norm=info.operator_norm, omega=info.omega, setup_seconds=info.setup_seconds,
+        operator_norm=info.operator_norm, omega=info.omega, rejected_steps=info.rejected_steps,
+        setup_seconds=info.setup_seconds,
         solve_seconds=info.solve_seconds, kernel_ms=info.kernel_ms,
         message=info.message.decode(errors="replace"),
         restarts=[(int(r_it[k]), int(r_av[k]), float(r_kkt[k]), float(r_om[k])) for k in range(nr)],
@@ -297,6 +303,7 @@ class LpResult:
     kernel_ms: float
     message: str
     omega: float = 1.0  # the last solve's final primal weight (start_omega of a warm start)
+    rejected_steps: int = 0  # adaptive steps: rejected attempts of the last solve
 
     @property
     def status_name(self) -> str:
@@ -330,7 +337,8 @@ def _lp_result(res: "N.SmyLpResult", x, yd, d, act, cert) -> LpResult:
         primal_residual=res.pdlp.primal_residual, dual_residual=res.pdlp.dual_residual,
         gap=res.pdlp.gap, setup_seconds=res.pdlp.setup_seconds,
         solve_seconds=res.pdlp.solve_seconds, kernel_ms=res.pdlp.kernel_ms,
-        message=res.message.decode(errors="replace"), omega=res.pdlp.omega)
+        message=res.message.decode(errors="replace"), omega=res.pdlp.omega,
+        rejected_steps=res.pdlp.rejected_steps)
 
 
 @dataclass

@@ -1,0 +1,239 @@
+"""Adaptive step sizes on the GPU (PDLP's rule, Applegate et al., NeurIPS 2021, sec. 3.1; the
+reference's own adaptive step is disabled code, src/lp/pdlp.cpp:1051-1067, so nothing of the
+reference pins it).
+
+Device level: runs of attempts against a NumPy restatement of the rule on the equilibrated LPs of
+the device tests, started with a step size several times 1 / ||[A -I]|| so that attempts are
+rejected and the step size first shrinks, then grows again; the rule's invariants (every accepted
+step within its limit; steps no larger than 1 / ||K|| are never rejected, by Cauchy-Schwarz; the
+average is the step-size weighted mean of the accepted iterates, summed directly); and a batch
+equal to single handles bit for bit. Solver level: with adaptive steps the optimal models of the
+reference's random family keep the simplex's objectives, and no infeasible or unbounded model is
+ever reported optimal (whether the detection finds its certificate within the iteration cap is
+printed: a diverging iterate drives the primal weight without bound, which the rule's large
+steps follow); batches equal single solves."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import samaya_amd as smy
+import samaya_amd.device as D
+import samaya_amd.generators as G
+from conftest import load_golden
+from oracle_lib import M
+from test_gpu_batch import _device_lp
+from test_gpu_pdhg import transpose_csc
+
+pytestmark = pytest.mark.gpu
+
+
+def k_matrix(eq):
+    A = sp.csc_matrix((eq.value, eq.index, eq.start), shape=(eq.m, eq.n))
+    return sp.hstack([A, -sp.identity(eq.m)]).tocsr()
+
+
+def norm_k(eq):
+    return np.linalg.norm(k_matrix(eq).toarray(), 2)
+
+
+class Rule:
+    """The adaptive rule restated with NumPy (FP64), keeping every accepted iterate."""
+
+    def __init__(self, eq, eta, omega):
+        self.K = k_matrix(eq)
+        self.c, self.lo, self.up = eq.cost, eq.lower, eq.upper
+        self.v = np.zeros(eq.n + eq.m)
+        self.y = np.zeros(eq.m)
+        self.eta, self.omega, self.weight, self.total, self.rejected = eta, omega, 0.0, 0, 0
+        self.v_erg, self.y_erg = self.v.copy(), self.y.copy()
+        self.accepted = []  # (eta, v, y, limit)
+
+    def attempt(self):
+        tau, sigma = self.eta / self.omega, self.eta * self.omega
+        vn = np.clip(self.v - tau * (self.c + self.K.T @ self.y), self.lo, self.up)
+        yn = self.y + sigma * (self.K @ (2 * vn - self.v))
+        dv, dy = vn - self.v, yn - self.y
+        movement = 0.5 * self.omega * dv @ dv + 0.5 / self.omega * dy @ dy
+        inter = abs(dy @ (self.K @ dv))
+        limit = movement / inter if inter > 0 else np.inf
+        ok = self.eta <= limit
+        self.total += 1
+        nxt = min((1 - (self.total + 1) ** -0.3) * limit, (1 + (self.total + 1) ** -0.6) * self.eta)
+        if ok:
+            a = self.eta / (self.weight + self.eta)
+            self.v_erg = (1 - a) * self.v_erg + a * vn
+            self.y_erg = (1 - a) * self.y_erg + a * yn
+            self.weight += self.eta
+            self.accepted.append((self.eta, vn, yn, limit))
+            self.v, self.y = vn, yn
+        else:
+            self.rejected += 1
+        self.eta = nxt
+        return ok
+
+
+def models():
+    return [G.sparse_uniform(71, 60, 150, 4), G.sparse_powerlaw(72, 200, 450, 30, 4.0),
+            G.blending(73, periods=2, units=2, streams=3, qualities=2)]
+
+
+def make_batch(inputs):
+    b = D.PdhgBatch.create([(eq.m, eq.n, eq.start, eq.index, eq.value) for _, eq, _, _ in inputs],
+                           [transpose_csc(eq) for _, eq, _, _ in inputs],
+                           [eq.cost for _, eq, _, _ in inputs], [eq.lower for _, eq, _, _ in inputs],
+                           [eq.upper for _, eq, _, _ in inputs])
+    for i, (lp, eq, rs, cs) in enumerate(inputs):
+        b.set_original(i, cs, rs, lp.cost, lp.lower, lp.upper)
+    return b
+
+
+def close(a, b, tol):
+    # Relative to the vector's scale: the device sums rows in another order than scipy, and the
+    # steps carry those last-bit differences along (entries near 0 of a vector whose entries
+    # reach 1e3 differ by ~1e-8 relative to themselves).
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b), initial=0.0) <= tol * (1.0 + np.max(np.abs(b), initial=0.0))
+
+
+def test_attempts_match_the_rule_restated(gpu):
+    inputs = [_device_lp(md, 80 + i) for i, md in enumerate(models())]
+    batch = make_batch(inputs)
+    rules, eta0, omega = [], [], [0.7, 1.0, 1.6]
+    for i, (_, eq, _, _) in enumerate(inputs):
+        eta0.append(6.0 / norm_k(eq))  # well above the worst-case bound: attempts fail
+        rules.append(Rule(eq, eta0[i], omega[i]))
+    st = D.PdhgBatch.adaptive_state(eta0, omega)
+    for count in (64, 5):  # a captured run, then single launches
+        batch.run_adaptive(st, count)
+        for _ in range(count):
+            for r in rules:
+                r.attempt()
+        for i, r in enumerate(rules):
+            got = batch.download(i)
+            assert st[i].attempts == r.total and st[i].rejected == r.rejected, (i, st[i].rejected, r.rejected)
+            assert abs(st[i].eta - r.eta) <= 1e-9 * r.eta
+            assert abs(st[i].weight - r.weight) <= 1e-9 * r.weight
+            assert close(got["v_new"], r.v, 1e-9) and close(got["y_new"], r.y, 1e-9)
+            assert close(got["v_erg"], r.v_erg, 1e-9) and close(got["y_erg"], r.y_erg, 1e-9)
+        batch.advance()
+    assert sum(r.rejected for r in rules) >= 1  # the rejection path ran (and matched)
+    for r in rules:
+        assert len(r.accepted) >= 40
+        # Every accepted step lies within its limit; its iterates' step-size weighted mean is the
+        # average the kernels formed incrementally.
+        assert all(e <= lim for e, _, _, lim in r.accepted)
+        w = np.array([e for e, _, _, _ in r.accepted])
+        vs = np.array([v for _, v, _, _ in r.accepted])
+        assert close(r.v_erg, w @ vs / w.sum(), 1e-12)
+
+
+def test_steps_within_the_norm_bound_are_never_rejected(gpu):
+    # |dy' K dv| <= ||K|| ||dv|| ||dy|| <= ||K|| movement, so the limit is >= 1 / ||K||: with the
+    # growth factor made negligible by a huge attempt count, eta stays below 1 / ||K|| and every
+    # attempt is accepted.
+    inputs = [_device_lp(md, 90 + i) for i, md in enumerate(models())]
+    batch = make_batch(inputs)
+    eta0 = [0.999 / norm_k(eq) for _, eq, _, _ in inputs]
+    st = D.PdhgBatch.adaptive_state(eta0, [0.5, 1.0, 2.0], attempts=10**15)
+    batch.run_adaptive(st, 64)
+    for i in range(len(inputs)):
+        assert st[i].rejected == 0 and st[i].attempts == 10**15 + 64
+        # Growth over 64 attempts at k ~ 1e15: (1 + 1e-9)^64 at most, so eta stayed below 1 / ||K||.
+        assert st[i].eta <= eta0[i] * (1 + 1e-6)
+
+
+def test_adaptive_batch_equals_single_handles(gpu):
+    inputs = [_device_lp(md, 100 + i) for i, md in enumerate(models())]
+    batch = make_batch(inputs)
+    eta0 = [3.0 / norm_k(eq) for _, eq, _, _ in inputs]
+    st = D.PdhgBatch.adaptive_state(eta0, 1.3)
+    singles, sst = [], []
+    for i, inp in enumerate(inputs):
+        singles.append(make_batch([inp]))
+        sst.append(D.PdhgBatch.adaptive_state([eta0[i]], 1.3))
+    for rnd in range(3):
+        batch.run_adaptive(st, 64)
+        got = batch.evaluate()
+        for i, s in enumerate(singles):
+            s.run_adaptive(sst[i], 64)
+            assert (st[i].eta, st[i].weight, st[i].attempts, st[i].rejected) == \
+                (sst[i][0].eta, sst[i][0].weight, sst[i][0].attempts, sst[i][0].rejected)
+            np.testing.assert_array_equal(got[i], s.evaluate()[0])
+            a, b = batch.download(i), s.download(0)
+            for key in ("v", "v_new", "y", "y_new", "v_erg", "y_erg"):
+                np.testing.assert_array_equal(a[key], b[key])
+        if rnd == 1:
+            # A restart of two instances to their averages (A x is formed again from the new
+            # point) and a freeze of the third.
+            which = [D.AVERAGE, D.AVERAGE, -1]
+            batch.restart_to(which)
+            st[0].weight = st[1].weight = 0.0
+            for i in (0, 1):
+                singles[i].restart_to([D.AVERAGE])
+                sst[i][0].weight = 0.0
+            batch.freeze(2)
+            singles.pop()
+        batch.advance()
+        for s in singles:
+            s.advance()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adaptive_solver_on_the_reference_families(gpu, precision):
+    # The reference's random feasible family (tests/test_pdlp.cpp:85-232) at 1e-8 and its
+    # infeasible / unbounded models at 1e-4: statuses as the reference's PDLP, objectives as the
+    # simplex's.
+    opts = smy.PdlpOptions(tol=1e-8, max_iterations=100000, adaptive_steps=True, precision=precision,
+                           fp32_switch_tol=1e-5)
+    rejected = optimal = 0
+    for c in load_golden("pdlp_random_feasible")["cases"]:
+        r = smy.solve_lp(M.from_json(c["model"]).product_model(), opts)
+        if c["status"] == smy.OPTIMAL:
+            assert r.status == smy.OPTIMAL, (c["k"], r.status, r.message)
+            assert abs(r.objective - c["simplex_objective"]) <= 1e-6 * (1 + abs(c["simplex_objective"]))
+            optimal += 1
+        else:  # the family's few unbounded draws: see below
+            assert r.status != smy.OPTIMAL, (c["k"], r.status)
+        rejected += r.rejected_steps
+    assert optimal >= 30 and rejected > 0
+    # Infeasible and unbounded models: never reported optimal; how many the detection finds within
+    # the iteration cap is printed (the primal weight of a diverging dual grows without bound, and
+    # with adaptive steps the tests may not catch the certificate in time).
+    fx = load_golden("pdlp_infeasible_unbounded")
+    found = {}
+    for key in ("infeasible", "unbounded"):
+        hits = 0
+        for c in fx[key]:
+            r = smy.solve_lp(M.from_json(c["model"]).product_model(),
+                             smy.PdlpOptions(tol=1e-4, max_iterations=50000, adaptive_steps=True,
+                                             precision=precision))
+            assert r.status != smy.OPTIMAL, (key, c["k"])
+            hits += r.status == c["status"]
+        found[key] = (hits, len(fx[key]))
+    print("adaptive steps, detected:", found)
+
+
+def test_adaptive_batch_equals_single_solves(gpu):
+    mods = [G.sparse_uniform(111, 60, 150, 4), G.sparse_powerlaw(112, 300, 700, 40, 5.0),
+            G.blending(113, periods=2, units=3, streams=4, qualities=2)]
+    opts = smy.PdlpOptions(tol=1e-6, max_iterations=50000, precision="mixed", algorithm="enhanced",
+                           adaptive_steps=True)
+    singles = [smy.solve_lp(md, opts) for md in mods]
+    batch, _ = smy.solve_lp_batch(mods, opts)
+    for a, b in zip(batch, singles):
+        assert a.status == b.status == smy.OPTIMAL
+        assert a.iterations == b.iterations and a.rejected_steps == b.rejected_steps
+        assert a.objective == b.objective
+        np.testing.assert_array_equal(a.col_value, b.col_value)
+    # Node relaxations with adaptive steps run in the stacked layout and match single solves.
+    md = G.knapsack(114, 60, 150, 4)
+    n = md.n
+    lo = [np.zeros(n)] * 3
+    up = [np.ones(n), np.where(np.arange(n) < 5, 0.0, 1.0), np.ones(n)]
+    res, _ = smy.solve_lp_bounds_batch(md, lo, up, opts)
+    for k in range(3):
+        mk = smy.Model(md.m, md.n, md.start, md.index, md.value, md.obj, lo[k], up[k],
+                       md.row_lower, md.row_upper, md.maximize, md.offset)
+        one = smy.solve_lp(mk, opts)
+        assert res[k].status == one.status == smy.OPTIMAL
+        assert abs(res[k].objective - one.objective) <= 1e-5 * (1 + abs(one.objective))
