@@ -76,14 +76,12 @@ def shared_step_bytes(nnz: int, m: int, n: int, k: int, s: int) -> dict:
 
 def profiled_traffic(op: str) -> int | None:
     """Calibrated HBM bytes per launch of a step kernel, from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (profiles/step_kernels_c2_mixed.md), if present."""
-    path = ROOT / "profiles" / "step_kernels_c2_mixed.md"
-    if not path.exists():
-        return None
-    for line in path.read_text().splitlines():
-        parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == op:
-            return int(parts[2])
+    WRITE_SIZE summaries of this workload (profiles/*_c2_mixed.md), if present."""
+    for path in sorted((ROOT / "profiles").glob("*_c2_mixed.md")):
+        for line in path.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == op:
+                return int(parts[2])
     return None
 
 

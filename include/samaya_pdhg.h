@@ -189,6 +189,37 @@ int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count);
 int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms,
                                    double* decide_ms);
 
+/* The restart rule on the device (the reference decides on the host from downloaded sums:
+ * src/lp/pdlp.cpp:1075-1132). smy_pdhg_rule_init sets each instance's normalizers (the measures
+ * of src/lp/pdlp.cpp:493-519: rel primal = sqrt(||Ax-r||^2) / (1 + a_colmax ||x|| + ||r||), rel
+ * dual = sqrt(dual residual^2) / (1 + at_rowmax ||y|| + c_norm), both 0 without rows) and its
+ * starting primal weight. After smy_pdhg_evaluate_batch, smy_pdhg_rule_decide takes the decision
+ * of every instance with in[i].due from the device-resident sums (candidate = the current or
+ * average iterate with the smaller weighted KKT error; restart on 0.2 / 0.8-and-rising / 36% of
+ * iter1, or in[i].force), measures the distances, updates the primal weight and restarts, all on
+ * the device; out[i] reports the decision (which: -1 none, SMY_CURRENT, SMY_AVERAGE), the
+ * candidate's error and the primal weight. Both batch layouts. Synchronizes once at the end. */
+typedef struct smy_pdhg_rule_setup {
+  double a_colmax;
+  double at_rowmax;
+  double c_norm;
+  int has_rows;
+  double omega;
+} smy_pdhg_rule_setup;
+typedef struct smy_pdhg_rule_in {
+  int due;
+  int force;
+  long long since; /* iterations since the instance's last restart */
+} smy_pdhg_rule_in;
+typedef struct smy_pdhg_rule_out {
+  int which;
+  double kkt_candidate;
+  double omega;
+} smy_pdhg_rule_out;
+int smy_pdhg_rule_init(smy_pdhg* h, const smy_pdhg_rule_setup* init);
+int smy_pdhg_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1,
+                         smy_pdhg_rule_out* out);
+
 /* Product tiles (no reference counterpart; the reference picks threads per row from nnz / rows,
  * src/gpu/spmv.cu:71-77). A tile maps the row blocks of the stacked batch's A (matrix 0) or A'
  * (matrix 1) with values of one precision to workgroups: per_wg (1, 2, 4) consecutive blocks per

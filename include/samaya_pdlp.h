@@ -101,6 +101,8 @@ typedef struct smy_pdlp_params {
   double fp32_stall_level;   /* mixed: a plateau of the FP32 steps (no 20% gain within
                                 max(4096, iter / 4) iterations) below this distance also
                                 switches to FP64 (1e-4; 0: never) */
+  int device_restart;        /* restart decisions, distances and primal weights on the device (1)
+                                or on the host from downloaded sums (0) */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);

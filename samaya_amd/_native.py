@@ -29,7 +29,8 @@ EXPORTS = (
     "smy_pdhg_evaluate_batch", "smy_pdhg_restart_distance_batch", "smy_pdhg_restart_to_batch",
     "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared",
     "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_run_adaptive",
-    "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_time_adaptive_kernels", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_time_adaptive_kernels",
+    "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
@@ -52,7 +53,8 @@ class SmyPdlpParams(C.Structure):
                 ("use_graph", C.c_int), ("detect", C.c_int), ("log_level", C.c_int),
                 ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int),
                 ("share_matrix", C.c_int), ("device_detect", C.c_int), ("start_omega", C.c_double),
-                ("adaptive_steps", C.c_int), ("fp32_stall_level", C.c_double)]
+                ("adaptive_steps", C.c_int), ("fp32_stall_level", C.c_double),
+                ("device_restart", C.c_int)]
 
 
 class SmyPdlpInfo(C.Structure):
@@ -145,6 +147,8 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_run_adaptive", i32, vp, vp, i32)
     _proto(lib, "smy_pdhg_get_tile", i32, vp, i32, i32, C.POINTER(i32), C.POINTER(i32))
     _proto(lib, "smy_pdhg_set_tile", i32, vp, i32, i32, i32, i32)
+    _proto(lib, "smy_pdhg_rule_init", i32, vp, vp)
+    _proto(lib, "smy_pdhg_rule_decide", i32, vp, vp, C.c_longlong, vp)
     _proto(lib, "smy_pdhg_time_adaptive_kernels", i32, vp, i32, C.POINTER(f64), C.POINTER(f64),
            C.POINTER(f64))
     _proto(lib, "smy_pdhg_create_shared", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), i32, vp, vp,

@@ -266,6 +266,31 @@ __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
 
 struct NoPre {};
 
+// The tile tuner's stand-in for a step kernel: the same product, plus per row what a step's
+// epilogue moves (five vectors read, three written), into scratch so that timing it leaves the
+// iterates alone.
+template <typename T>
+struct TuneOp {
+  using Acc = T;
+  struct Pre {
+    T a, b, c, d, e;
+  };
+  static constexpr int kVec = 1;
+  static constexpr int kSums = 0;
+  const T* __restrict__ x;
+  const T* __restrict__ in[5];
+  T* __restrict__ out[3];
+  __device__ bool skip(int) const { return false; }
+  __device__ Pre pre(int r, int) const { return Pre{in[0][r], in[1][r], in[2][r], in[3][r], in[4][r]}; }
+  __device__ Acc gather(int, int c, int) const { return x[c]; }
+  __device__ void row(int r, int, const Acc* acc, const Pre& p, double*) const {
+    const T v = clamp_to(p.a - p.b * (p.c + acc[0]), p.d, p.e);
+    out[0][r] = v;
+    out[1][r] = v + p.a;
+    out[2][r] = v - p.b;
+  }
+};
+
 // Plain product y = A x (the standalone SpMV entry point).
 template <typename V, typename X>
 struct SpmvOp {

@@ -220,20 +220,26 @@ void step_kernels(smy_pdhg* h, int j) {
 }
 
 // ---- Tile tuner ----------------------------------------------------------------------------
-// The first run in a precision times the candidate tiles of both matrices on their product with
-// the values of that precision (over the blocks of the instances stepping in it) and keeps the
-// fastest. Every tile sums every row in the same order, so the choice changes speed only: answers
-// do not depend on the timing.
+// The first run in a precision times the candidate tiles of both matrices with the values of that
+// precision (over the blocks of the instances stepping in it) on a stand-in for the step kernels
+// (TuneOp: the product plus an epilogue's reads and writes, into scratch), and keeps the fastest;
+// the plain one-block-per-workgroup tile unless another is at least 3% faster. Every tile sums
+// every row in the same order, so the choice changes speed only: answers do not depend on the
+// timing.
 constexpr Tile kTiles[] = {{1, false}, {1, true}, {2, false}, {2, true}, {4, false}, {4, true}};
 
 template <typename T>
-void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x) {
+void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5]) {
   constexpr int q = prec_of<T>();
   if (M.tuned[q] || M.nlist[q] == 0) return;
-  if (h->tune_out == nullptr) h->tune_out = h->alloc<double>(std::max(h->m, h->n));
-  SpmvOp<T, T> op{x, reinterpret_cast<T*>(h->tune_out)};
+  const long long rows = std::max(h->m, h->n);
+  if (h->tune_out == nullptr) h->tune_out = h->alloc<double>(3 * std::max(rows, 1LL));
+  TuneOp<T> op;
+  op.x = x;
+  for (int k = 0; k < 5; ++k) op.in[k] = in[k];
+  for (int k = 0; k < 3; ++k) op.out[k] = reinterpret_cast<T*>(h->tune_out) + k * rows;
   constexpr int kReps = 3;
-  float best = 0.0f;
+  float base = 0.0f, best = 0.0f;
   Tile pick;
   for (const Tile& t : kTiles) {
     launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream, &t);  // warm
@@ -243,7 +249,9 @@ void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x) {
     SMY_HIP(hipEventSynchronize(h->ev_stop));
     float ms = 0.0f;
     SMY_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
-    if (&t == kTiles || ms < best) {
+    if (&t == kTiles) {
+      base = best = ms;
+    } else if (ms < best && ms < 0.97f * base) {
       best = ms;
       pick = t;
     }
@@ -258,14 +266,15 @@ void tune(smy_pdhg* h) {
   const bool any = (uses(h, SMY_FP32) && !(h->a.tuned[SMY_FP32] && h->at.tuned[SMY_FP32])) ||
                    (uses(h, SMY_FP64) && !(h->a.tuned[SMY_FP64] && h->at.tuned[SMY_FP64]));
   if (!any) return;
-  if (uses(h, SMY_FP32)) {
-    tune_matrix<float>(h, h->a, h->f.v[h->p]);
-    tune_matrix<float>(h, h->at, h->f.y[h->p]);
-  }
-  if (uses(h, SMY_FP64)) {
-    tune_matrix<double>(h, h->a, h->d.v[h->p]);
-    tune_matrix<double>(h, h->at, h->d.y[h->p]);
-  }
+  // Epilogue inputs: any five vectors of at least max(m, n) entries of the state.
+  const auto go = [&](auto& S) {
+    using T = std::remove_pointer_t<decltype(S.cost)>;
+    const T* const in[5] = {S.v[0], S.v[1], S.cost, S.lower, S.upper};
+    tune_matrix<T>(h, h->a, S.v[h->p], in);
+    tune_matrix<T>(h, h->at, S.y[h->p], in);
+  };
+  if (uses(h, SMY_FP32)) go(h->f);
+  if (uses(h, SMY_FP64)) go(h->d);
   ++h->list_gen;  // graphs captured before hold the old launch shapes
   h->timed = false;
 }
@@ -804,6 +813,75 @@ int restart_to_batch_impl(smy_pdhg* h, const int* which) {
   return SMY_OK;
 }
 
+// ---- Restart rule on the device ------------------------------------------------------------
+
+int rule_init_impl(smy_pdhg* h, const smy_pdhg_rule_setup* init) {
+  if (h->rule == nullptr) {
+    h->rule = h->alloc<RuleState>(h->K);
+    h->rule_in = h->alloc<RuleInput>(h->K);
+    h->rule_dist = h->alloc<double>(4LL * h->K);
+    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->rule_host), sizeof(RuleState) * h->K,
+                          hipHostMallocDefault));
+    SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->rule_in_host), sizeof(RuleInput) * h->K,
+                          hipHostMallocDefault));
+  }
+  SMY_HIP(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < h->K; ++i) {
+    const smy_pdhg_rule_setup& r = init[i];
+    if (!(r.omega > 0.0)) return fail(SMY_INVALID, "the primal weight must be positive");
+    h->rule_host[i] = RuleState{r.a_colmax, r.at_rowmax, r.c_norm, r.has_rows, -1, r.omega,
+                                INFINITY, INFINITY, 0.0, 0};
+  }
+  SMY_HIP(hipMemcpy(h->rule, h->rule_host, sizeof(RuleState) * h->K, hipMemcpyHostToDevice));
+  return SMY_OK;
+}
+
+// Decision, distances, primal weight and restart of every due instance, on the device, from the
+// totals the last evaluation left in sums_dev (the A part, then the A' part of each instance).
+int rule_decide_impl(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1, smy_pdhg_rule_out* out) {
+  if (h->rule == nullptr) return fail(SMY_STATE, "smy_pdhg_rule_init() must come first");
+  refresh_lists(h);  // the instances' step precisions, for the per-precision flags
+  staging_wait(h);
+  for (int i = 0; i < h->K; ++i) h->rule_in_host[i] = RuleInput{in[i].due, in[i].force, in[i].since};
+  SMY_HIP(hipMemcpyAsync(h->rule_in, h->rule_in_host, sizeof(RuleInput) * h->K, hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+  const int grid = (h->K + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(rule_decide_kernel, dim3(grid), dim3(kBlock), 0, h->stream, h->K, h->K, h->sums_dev,
+                     iter1, h->rule_in, h->step_prec, h->rule, h->flags);
+  const bool rows = block_rows_grid(h) > 0;
+  if (rows) {
+    if (uses(h, SMY_FP32)) distance_launch<float>(h);
+    if (uses(h, SMY_FP64)) distance_launch<double>(h);
+  }
+  hipLaunchKernelGGL(finish_ranges_kernel, dim3(2, 2 * h->K), dim3(kBlock), 0, h->stream, stride(h), 2,
+                     h->partial, h->ranges, h->rule_dist);
+  hipLaunchKernelGGL(rule_weight_kernel, dim3(grid), dim3(kBlock), 0, h->stream, h->K, 2, h->rule_dist,
+                     h->rule);
+  if (rows) {
+    if (uses(h, SMY_FP32)) restart_launch<float>(h);
+    if (uses(h, SMY_FP64)) restart_launch<double>(h);
+  }
+  SMY_HIP(hipGetLastError());
+  SMY_HIP(hipMemcpyAsync(h->rule_host, h->rule, sizeof(RuleState) * h->K, hipMemcpyDeviceToHost, h->stream));
+  SMY_HIP(hipStreamSynchronize(h->stream));
+  h->staged = false;
+  bool any = false;
+  for (int i = 0; i < h->K; ++i) {
+    const RuleState& s = h->rule_host[i];
+    out[i] = smy_pdhg_rule_out{s.which, s.kkt_candidate, s.omega};
+    if (s.which >= 0) {
+      any = true;
+      h->ax_stale[i] = 1;
+    }
+  }
+  if (any) {
+    h->vbar_any = true;
+    h->stepped = false;
+  }
+  return SMY_OK;
+}
+
 template <typename From, typename To>
 void convert_launch(smy_pdhg* h) {
   Iter<From>& A = state<From>(h);
@@ -1308,6 +1386,18 @@ int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, doubl
     if (uses(h, SMY_FP64)) return time_adaptive_impl<double>(h, reps, dual_ms, primal_ms, decide_ms);
     return fail(SMY_STATE, "no instance steps");
   });
+}
+
+int smy_pdhg_rule_init(smy_pdhg* h, const smy_pdhg_rule_setup* init) {
+  if (h == nullptr || init == nullptr) return fail(SMY_INVALID, "null argument");
+  return guarded([&]() -> int { return rule_init_impl(h, init); });
+}
+
+int smy_pdhg_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1, smy_pdhg_rule_out* out) {
+  if (h == nullptr || in == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->rule == nullptr) return fail(SMY_STATE, "smy_pdhg_rule_init() must come first");
+  if (h->shared) return guarded([&]() -> int { return shared_rule_decide(h, in, iter1, out); });
+  return guarded([&]() -> int { return rule_decide_impl(h, in, iter1, out); });
 }
 
 int smy_pdhg_get_tile(smy_pdhg* h, int matrix, int precision, int* per_wg, int* xcd) {

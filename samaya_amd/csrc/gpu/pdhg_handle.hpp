@@ -17,6 +17,7 @@
 #include "detect_kernels.hpp"
 #include "hip_common.hpp"
 #include "row_blocks.hpp"
+#include "rule_kernels.hpp"
 
 namespace smy {
 
@@ -187,6 +188,12 @@ struct smy_pdhg {
   AdaptState* adapt_host = nullptr;
   std::vector<int> ax_stale;  // per instance: A x must be formed again before its next attempt
   int* ax_flags = nullptr;    // device copy for the refresh launch
+  // Restart rule on the device (allocated by smy_pdhg_rule_init).
+  RuleState* rule = nullptr;
+  RuleState* rule_host = nullptr;    // pinned
+  RuleInput* rule_in = nullptr;
+  RuleInput* rule_in_host = nullptr;  // pinned staging
+  double* rule_dist = nullptr;        // per instance and segment: ||dv||^2, ||dy||^2
   // Tile tuner: on by default; its scratch output (max(m, n) doubles, at first use).
   bool autotune = true;
   double* tune_out = nullptr;
@@ -237,6 +244,8 @@ struct smy_pdhg {
     for (void* ptr : allocations) (void)hipFree(ptr);
     if (sums_host) (void)hipHostFree(sums_host);
     if (adapt_host) (void)hipHostFree(adapt_host);
+    if (rule_host) (void)hipHostFree(rule_host);
+    if (rule_in_host) (void)hipHostFree(rule_in_host);
     if (det_prm_host) (void)hipHostFree(det_prm_host);
     if (det_out_host) (void)hipHostFree(det_out_host);
     if (scalars_host) (void)hipHostFree(scalars_host);
@@ -283,5 +292,6 @@ int shared_set_start(smy_pdhg* h, int k, const double* v, const double* y);
 int shared_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
 int shared_detect(smy_pdhg* h, const int* flags, double* out);
 int shared_download_ray(smy_pdhg* h, int k, double* ray);
+int shared_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1, smy_pdhg_rule_out* out);
 
 }  // namespace smy

@@ -766,6 +766,50 @@ int shared_download_ray(smy_pdhg* h, int k, double* ray) {
   return SMY_OK;
 }
 
+// The restart rule on the device (rule_kernels.hpp) for the shared-matrix batch: decision from
+// the totals the last evaluation left in sums_dev (instance k's A part at k * 28, A' part at
+// k * 28 + 14), distances, primal weights and restarts, without a host round trip.
+int shared_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1, smy_pdhg_rule_out* out) {
+  SharedBatch* s = h->shared;
+  upload_modes(h);
+  staging_wait(h);
+  for (int k = 0; k < s->K; ++k) h->rule_in_host[k] = RuleInput{in[k].due, in[k].force, in[k].since};
+  SMY_HIP(hipMemcpyAsync(h->rule_in, h->rule_in_host, sizeof(RuleInput) * s->K, hipMemcpyHostToDevice,
+                         h->stream));
+  staging_sent(h);
+  hipLaunchKernelGGL(rule_decide_kernel, dim3(chunks1(s)), dim3(kBlock), 0, h->stream, s->K, s->kp,
+                     s->sums_dev, iter1, h->rule_in, s->mode, h->rule, s->which);
+  const bool u32 = s_uses(s, SMY_FP32), u64 = s_uses(s, SMY_FP64);
+  if (s->nt > 0) {
+    if (u32) distance_part<float>(h);
+    if (u64) distance_part<double>(h);
+  }
+  // Per-instance distance totals at k * 4 (zero without variables: no groups).
+  hipLaunchKernelGGL(shared_finish_kernel, dim3(blocks_for(s->kp), 2), dim3(kBlock), 0, h->stream,
+                     s->nt > 0 ? groups_of(s->nt) : 0, 2, s->kp, s->partial, s->sums_dev, 4, 0);
+  hipLaunchKernelGGL(rule_weight_kernel, dim3(chunks1(s)), dim3(kBlock), 0, h->stream, s->K, 1, s->sums_dev,
+                     h->rule);
+  if (s->nt > 0) {
+    if (u32) restart_part<float>(h);
+    if (u64) restart_part<double>(h);
+  }
+  SMY_HIP(hipGetLastError());
+  SMY_HIP(hipMemcpyAsync(h->rule_host, h->rule, sizeof(RuleState) * s->K, hipMemcpyDeviceToHost, h->stream));
+  SMY_HIP(hipStreamSynchronize(h->stream));
+  h->staged = false;
+  bool any = false;
+  for (int k = 0; k < s->K; ++k) {
+    const RuleState& r = h->rule_host[k];
+    out[k] = smy_pdhg_rule_out{r.which, r.kkt_candidate, r.omega};
+    any = any || r.which >= 0;
+  }
+  if (any) {
+    h->vbar_any = true;
+    h->stepped = false;
+  }
+  return SMY_OK;
+}
+
 int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* cost,
                   const double* const* lower, const double* const* upper, int precision,
                   smy_pdhg** out) {

@@ -131,6 +131,10 @@ class Device {
       }
     }
   }
+  void rule_init(const std::vector<smy_pdhg_rule_setup>& setup) { check(smy_pdhg_rule_init(h_, setup.data())); }
+  void rule_decide(const std::vector<smy_pdhg_rule_in>& in, long long iter1, std::vector<smy_pdhg_rule_out>& out) {
+    check(smy_pdhg_rule_decide(h_, in.data(), iter1, out.data()));
+  }
   // `count` attempts of the adaptive step-size rule; the state is read and updated.
   void run_adaptive(std::vector<smy_pdhg_adaptive>& state, int count) {
     check(smy_pdhg_run_adaptive(h_, state.data(), count));
@@ -761,6 +765,19 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   std::vector<long long> k0(static_cast<size_t>(K));
   std::vector<int> which(static_cast<size_t>(K)), precision(static_cast<size_t>(K));
   std::vector<smy_pdhg_adaptive> adapt(static_cast<size_t>(K));
+  // The restart rule on the device: decisions, distances, primal weights and restarts there; the
+  // host keeps termination, detection and precision switches.
+  const bool device_rule = opt.device_restart;
+  std::vector<smy_pdhg_rule_in> rule_in(static_cast<size_t>(K));
+  std::vector<smy_pdhg_rule_out> rule_out(static_cast<size_t>(K));
+  if (device_rule) {
+    std::vector<smy_pdhg_rule_setup> setup(static_cast<size_t>(K));
+    for (int i = 0; i < K; ++i) {
+      const Lane& L = lanes[i];
+      setup[i] = smy_pdhg_rule_setup{L.eq->a_colmax, L.eq->at_rowmax, L.c_norm, L.lp_.m > 0 ? 1 : 0, L.omega};
+    }
+    device->rule_init(setup);
+  }
   std::vector<double> v, v_new, y, y_new, v_erg, y_erg;
   int active = K;
   const auto stop = [&](int i) {
@@ -968,7 +985,8 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       }
 
       // Adaptive restart (src/lp/pdlp.cpp:1075-1132): the decision; the primal weight follows
-      // from the distance below.
+      // from the distance below. With the device rule, both happen on the device (below).
+      if (device_rule) continue;
       const double kkt_current = L.kkt(current);
       const double kkt_average = L.kkt(average);
       const bool to_average = kkt_average < kkt_current;
@@ -986,6 +1004,25 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     }
     if (std::any_of(precision.begin(), precision.end(), [](int q) { return q >= 0; }))
       device->set_precision(precision);
+    if (device_rule && std::any_of(due.begin(), due.end(), [](char d) { return d != 0; })) {
+      for (int i = 0; i < K; ++i)
+        rule_in[i] = smy_pdhg_rule_in{due[i] ? 1 : 0, force[i] ? 1 : 0, lanes[i].k_since};
+      device->rule_decide(rule_in, iter + 1, rule_out);
+      for (int i = 0; i < K; ++i) {
+        if (!due[i] || rule_out[i].which < 0) continue;
+        Lane& L = lanes[i];
+        L.omega = rule_out[i].omega;
+        L.tau = L.eta / L.omega;
+        L.sigma = L.eta * L.omega;
+        L.adapt.omega = L.omega;
+        L.adapt.weight = 0.0;
+        const bool to_average = rule_out[i].which == SMY_AVERAGE;
+        L.log(2, ": restart at iter %lld to the %s (KKT %.2e), omega %.3e", iter,
+              to_average ? "average" : "current iterate", rule_out[i].kkt_candidate, L.omega);
+        L.res.restarts.push_back({iter, to_average ? 1 : 0, rule_out[i].kkt_candidate, L.omega});
+        L.k_since = 0;
+      }
+    }
     if (std::any_of(which.begin(), which.end(), [](int w) { return w >= 0; })) {
       device->restart_distance(which, dist);
       for (int i = 0; i < K; ++i) {

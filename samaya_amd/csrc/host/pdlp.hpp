@@ -59,6 +59,7 @@ struct PdlpOptions {
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
   bool device_detect = true;      // ... computed on the device (false: host, from downloads)
   bool share_matrix = true;       // batches of one problem: the shared-matrix (SpMM) batch
+  bool device_restart = true;     // restart decisions, distances and primal weights on the device
   int log_level = 0;
   // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m),
   // and the primal weight to start with (<= 0: 1, the reference's start).

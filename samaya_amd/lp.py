@@ -188,6 +188,7 @@ class PdlpOptions:
     start_omega: float = 0.0        # primal weight to start with (<= 0: 1), with a warm start
     adaptive_steps: bool = False    # PDLP's adaptive step sizes (decided on the GPU)
     fp32_stall_level: float = 1e-4  # mixed: FP32 plateaus below this switch to FP64 (0: never)
+    device_restart: bool = True     # restart decisions, distances and primal weights on the GPU
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -209,6 +210,7 @@ class PdlpOptions:
         p.start_omega = self.start_omega
         p.adaptive_steps = int(self.adaptive_steps)
         p.fp32_stall_level = self.fp32_stall_level
+        p.device_restart = int(self.device_restart)
         return p
 
 
