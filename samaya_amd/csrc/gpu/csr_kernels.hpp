@@ -110,7 +110,8 @@ __device__ inline auto op_context(const Op& op, int inst) {
 
 // One row block b of instance inst (see the file comment).
 template <typename V, typename Op>
-__device__ inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b, int inst) {
+__device__ __attribute__((always_inline)) inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b,
+                                                                     int inst) {
   using Acc = typename Op::Acc;
   using Pre = typename Op::Pre;
   constexpr int NV = Op::kVec;
@@ -130,16 +131,22 @@ __device__ inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b, 
   const bool long_row = nrows == 1 && nnz > kLongRow;
 
   // Streams of values and indices, row offsets and epilogue inputs: all issued before any use.
+  // No branch guards the loads: a lane past the block's end reads the block's last entry again
+  // (a valid column to gather; its product is never stored), so the loads and, below, the
+  // gathers go out back to back instead of one conditional block at a time.
   V a[kPer];
   int c[kPer];
   if (!long_row) {
+    if (nnz > 0) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int k = tid + u * kBlock;
-      if (k < nnz) {
+      for (int u = 0; u < kPer; ++u) {
+        const int k = min(tid + u * kBlock, nnz - 1);
         a[u] = __builtin_nontemporal_load(A.val + p0 + k);
         c[u] = __builtin_nontemporal_load(A.idx + p0 + k);
-      } else {
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
         a[u] = V(0);
         c[u] = 0;
       }
@@ -160,9 +167,8 @@ __device__ inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b, 
     Acc xs[NV][kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const bool live = tid + u * kBlock < nnz;
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xs[v][u] = live ? op.gather(v, c[u], ctx) : Acc(0);
+      for (int v = 0; v < NV; ++v) xs[v][u] = op.gather(v, c[u], ctx);
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -206,15 +212,19 @@ __device__ inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b, 
       int cl[kPer];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
-        const int q = p + u * kBlock;
-        al[u] = q < end ? __builtin_nontemporal_load(A.val + q) : V(0);
-        cl[u] = q < end ? __builtin_nontemporal_load(A.idx + q) : 0;
+        const int q = min(p + u * kBlock, end - 1);  // as above: loads without branches
+        const V av = __builtin_nontemporal_load(A.val + q);
+        cl[u] = __builtin_nontemporal_load(A.idx + q);
+        al[u] = p + u * kBlock < end ? av : V(0);
       }
       Acc gv[NV][kPer];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) gv[v][u] = p + u * kBlock < end ? op.gather(v, cl[u], ctx) : Acc(0);
+        for (int v = 0; v < NV; ++v) {
+          const Acc g = op.gather(v, cl[u], ctx);
+          gv[v][u] = p + u * kBlock < end ? g : Acc(0);
+        }
       }
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
@@ -246,17 +256,26 @@ __device__ inline void csr_rows_block(const CsrRows<V>& A, const Op& op, int b, 
                                            op.stride);
 }
 
-template <typename V, typename Op>
+// Tiled: the launch's tile (CsrRows::per_wg, xcd) maps workgroups to blocks; otherwise workgroup
+// w takes block w of the launch (the plain tile, compiled apart so that its block's schedule
+// entries stay scalar loads and its value and index loads all go out before the first wait).
+template <typename V, typename Op, bool Tiled>
 __global__ __launch_bounds__(kBlock) void csr_rows_kernel(CsrRows<V> A, Op op) {
-  const int w = A.xcd ? xcd_order(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))
-                      : static_cast<int>(blockIdx.x);
-  for (int t = 0; t < A.per_wg; ++t) {
-    const int slot = w * A.per_wg + t;
-    if (slot >= A.count) return;  // uniform over the workgroup
-    if (t > 0) __syncthreads();   // the last block's LDS reads are done
-    const int b = A.list != nullptr ? A.list[slot] : slot;
+  if constexpr (!Tiled) {
+    const int b = A.list != nullptr ? A.list[blockIdx.x] : static_cast<int>(blockIdx.x);
     const int inst = A.inst_from_y ? static_cast<int>(blockIdx.y) : A.binst != nullptr ? A.binst[b] : 0;
     csr_rows_block(A, op, b, inst);
+  } else {
+    const int w = A.xcd ? xcd_order(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))
+                        : static_cast<int>(blockIdx.x);
+    for (int t = 0; t < A.per_wg; ++t) {
+      const int slot = w * A.per_wg + t;
+      if (slot >= A.count) return;  // uniform over the workgroup
+      if (t > 0) __syncthreads();   // the last block's LDS reads are done
+      const int b = A.list != nullptr ? A.list[slot] : slot;
+      const int inst = A.inst_from_y ? static_cast<int>(blockIdx.y) : A.binst != nullptr ? A.binst[b] : 0;
+      csr_rows_block(A, op, b, inst);
+    }
   }
 }
 

@@ -88,6 +88,7 @@ void upload_stack(smy_pdhg* h, const smy_csc* parts, bool rows_are_a, DevMatrix&
   static_assert(sizeof(RowBlockMeta) == sizeof(int2), "schedule entries are read as int2");
   M.meta = h->upload(reinterpret_cast<const int2*>(meta.data()), static_cast<long long>(meta.size()));
   M.binst = h->upload(binst.data(), static_cast<long long>(binst.size()));
+  M.single = K == 1;
   M.list[0] = h->alloc<int>(M.nblocks);
   M.list[1] = h->alloc<int>(M.nblocks);
 }
@@ -1166,7 +1167,7 @@ void detect_csr(smy_pdhg* h, const DevMatrix& M, const Op& op) {
   A.meta = M.meta;
   A.binst = M.binst;
   A.count = M.nblocks;
-  hipLaunchKernelGGL((csr_rows_kernel<double, Op>), dim3(M.nblocks), dim3(kBlock), 0, h->stream, A, op);
+  hipLaunchKernelGGL((csr_rows_kernel<double, Op, false>), dim3(M.nblocks), dim3(kBlock), 0, h->stream, A, op);
 }
 
 template <typename Op>

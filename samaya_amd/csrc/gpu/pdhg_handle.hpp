@@ -73,6 +73,9 @@ struct DevMatrix {
   int nlist[2] = {0, 0};
   Tile tile[2];                 // by value precision (SMY_FP64, SMY_FP32): chosen by the tuner
   bool tuned[2] = {false, false};
+  // One instance: launches pass no block-instance table, so a workgroup knows its instance (0)
+  // without a dependent load and reads the instance's step state in parallel with its schedule.
+  bool single = false;
 };
 
 template <typename T>
@@ -123,13 +126,16 @@ int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, co
   A.idx = M.idx;
   A.val = val;
   A.meta = M.meta;
-  A.binst = M.binst;
+  A.binst = M.single ? nullptr : M.binst;
   A.list = list;
   A.count = nlist;
   A.per_wg = t.per_wg;
   A.xcd = t.xcd;
   const int grid = (nlist + t.per_wg - 1) / t.per_wg;
-  hipLaunchKernelGGL((csr_rows_kernel<V, Op>), dim3(grid), dim3(kBlock), 0, s, A, op);
+  if (t.per_wg == 1 && !t.xcd)
+    hipLaunchKernelGGL((csr_rows_kernel<V, Op, false>), dim3(grid), dim3(kBlock), 0, s, A, op);
+  else
+    hipLaunchKernelGGL((csr_rows_kernel<V, Op, true>), dim3(grid), dim3(kBlock), 0, s, A, op);
   return grid;
 }
 

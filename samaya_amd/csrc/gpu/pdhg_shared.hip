@@ -678,7 +678,7 @@ void shared_detect_csr(smy_pdhg* h, bool rows_of_a, const Op& op) {
   A.meta = rows_of_a ? s->a_meta : s->t_meta;
   A.inst_from_y = true;
   A.count = nb;
-  hipLaunchKernelGGL((csr_rows_kernel<double, Op>), dim3(nb, s->K), dim3(kBlock), 0, h->stream, A, op);
+  hipLaunchKernelGGL((csr_rows_kernel<double, Op, false>), dim3(nb, s->K), dim3(kBlock), 0, h->stream, A, op);
 }
 
 template <typename Op>
