@@ -94,6 +94,10 @@ struct SharedBatch {
   bool shared_original = false;
   std::vector<char> has_original;
   double* partial = nullptr;
+  // The compact block of a precision holds the current iterates since the last run stepped there
+  // (nothing has changed the state since): its evaluation may then read it (compact_measure).
+  bool compact_valid[2] = {false, false};
+  double* cpartial = nullptr;   // compact evaluation partials: 14 x kCompactInstances per row group
   double* sums_dev = nullptr;   // kp x 28
   double* sums_host = nullptr;  // pinned
   double* column = nullptr;     // nt scratch
@@ -337,12 +341,16 @@ void shared_step_kernels(smy_pdhg* h, int j) {
   const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
   if (compact_steps(s, q)) {
     const int na = s->nactive[q];
-    const int rows_per_wg = kBlock / (compact_width(s, q) / kInstPerThread);
+    // 16 threads per row-quad group: few live instances give small workgroups of 16-64 rows, so
+    // the tail's short steps still spread over every CU.
+    const int lanes = compact_width(s, q) / kInstPerThread;
+    const int threads = std::min(kBlock, std::max(64, 16 * lanes));
+    const int rows_per_wg = threads / lanes;
     if (s->m > 0)
-      hipLaunchKernelGGL((compact_dual_kernel<T>), dim3((s->m + rows_per_wg - 1) / rows_per_wg), dim3(kBlock), 0,
-                         h->stream, compact_args<T>(h, j, true), na);
+      hipLaunchKernelGGL((compact_dual_kernel<T>), dim3((s->m + rows_per_wg - 1) / rows_per_wg), dim3(threads),
+                         0, h->stream, compact_args<T>(h, j, true), na);
     if (s->n > 0)
-      hipLaunchKernelGGL((compact_primal_kernel<T>), dim3((s->n + rows_per_wg - 1) / rows_per_wg), dim3(kBlock),
+      hipLaunchKernelGGL((compact_primal_kernel<T>), dim3((s->n + rows_per_wg - 1) / rows_per_wg), dim3(threads),
                          0, h->stream, compact_args<T>(h, j, false), na);
     return;
   }
@@ -425,6 +433,53 @@ void measure_part(smy_pdhg* h, bool rows_of_a) {
                        h->stream, a);
 }
 
+int cgroups_of(int rows) { return (rows + kCompactGroupRows - 1) / kCompactGroupRows; }
+
+void shared_download_sums(smy_pdhg* h, int nsums, bool two_parts, double* out);
+
+// The evaluation sums of the instances of precision T from their compact block, into sums_dev
+// (instance k's rows-of-A part at k * 28, rows-of-A' part at k * 28 + 14).
+template <typename T>
+void compact_measure(smy_pdhg* h) {
+  SharedBatch* s = h->shared;
+  CompactState<T>& C = cstate<T>(s);
+  const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
+  const int kc = compact_width(s, q), na = s->nactive[q];
+  const int ga = cgroups_of(s->m), gt = cgroups_of(s->n);
+  if (s->cpartial == nullptr) s->cpartial = h->alloc<double>(14LL * (ga + gt) * kCompactInstances);
+  const int lanes = kc / kInstPerThread;
+  const int per_wg = kBlock / lanes;
+  for (int part = 0; part < 2; ++part) {
+    const bool rows_of_a = part == 0;
+    const int groups = rows_of_a ? ga : gt;
+    if (groups == 0) continue;
+    CompactMeasure<T> a;
+    a.n = s->n;
+    a.m = s->m;
+    a.kp = s->kp;
+    a.kc = kc;
+    a.na = na;
+    a.rows_of_a = rows_of_a;
+    a.list = s->active[q];
+    a.ptr = rows_of_a ? s->a_ptr : s->t_ptr;
+    a.idx = rows_of_a ? s->a_idx : s->t_idx;
+    a.val = rows_of_a ? s->a_val64 : s->t_val64;
+    a.v[0] = C.v[1 - h->p];
+    a.v[1] = C.v_erg;
+    a.y[0] = C.y[1 - h->p];
+    a.y[1] = C.y_erg;
+    a.scale = rows_of_a ? s->rs : s->cs;
+    a.cost = s->cost_o;
+    a.lower = rows_of_a ? s->row_lower_o : s->col_lower_o;
+    a.upper = rows_of_a ? s->row_upper_o : s->col_upper_o;
+    a.partial = s->cpartial + (rows_of_a ? 0 : 14LL * ga * kc);
+    hipLaunchKernelGGL((compact_measure_kernel<T>), dim3((groups + per_wg - 1) / per_wg), dim3(kBlock), 0,
+                       h->stream, a);
+    hipLaunchKernelGGL(compact_finish_kernel, dim3(na, 14), dim3(kBlock), 0, h->stream, groups, 14, kc,
+                       s->active[q], a.partial, s->sums_dev, 28, rows_of_a ? 0 : 14);
+  }
+}
+
 // out[k * nsums + q] for k < K: the part sums (rows of A, then of A') added in that order.
 void shared_finish(smy_pdhg* h, int nsums, int groups_a, int groups_t, double* out) {
   SharedBatch* s = h->shared;
@@ -442,14 +497,21 @@ void shared_finish(smy_pdhg* h, int nsums, int groups_a, int groups_t, double* o
                        s->sums_dev, stride, nsums);
   }
   SMY_HIP(hipGetLastError());
+  if (out != nullptr) shared_download_sums(h, nsums, groups_t >= 0, out);
+}
+
+// Synchronizes and adds each instance's parts (one, or rows of A then of A') into out[k * nsums + q].
+void shared_download_sums(smy_pdhg* h, int nsums, bool two_parts, double* out) {
+  SharedBatch* s = h->shared;
+  const int stride = 2 * nsums;
   SMY_HIP(hipMemcpyAsync(s->sums_host, s->sums_dev, static_cast<size_t>(s->kp) * stride * sizeof(double),
                          hipMemcpyDeviceToHost, h->stream));
   SMY_HIP(hipStreamSynchronize(h->stream));
   for (int k = 0; k < s->K; ++k)
     for (int q = 0; q < nsums; ++q) {
       const double a = s->sums_host[static_cast<size_t>(k) * stride + q];
-      const double t = groups_t < 0 ? 0.0 : s->sums_host[static_cast<size_t>(k) * stride + nsums + q];
-      out[static_cast<size_t>(k) * nsums + q] = groups_t < 0 ? a : a + t;
+      const double t = two_parts ? s->sums_host[static_cast<size_t>(k) * stride + nsums + q] : 0.0;
+      out[static_cast<size_t>(k) * nsums + q] = two_parts ? a + t : a;
     }
 }
 
@@ -793,6 +855,7 @@ int shared_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1,
     if (u32) restart_part<float>(h);
     if (u64) restart_part<double>(h);
   }
+  s->compact_valid[0] = s->compact_valid[1] = false;
   SMY_HIP(hipGetLastError());
   SMY_HIP(hipMemcpyAsync(h->rule_host, h->rule, sizeof(RuleState) * s->K, hipMemcpyDeviceToHost, h->stream));
   SMY_HIP(hipStreamSynchronize(h->stream));
@@ -998,6 +1061,8 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
   }
   if (c32) compact_copy<float>(h, false);
   if (c64) compact_copy<double>(h, false);
+  s->compact_valid[SMY_FP32] = c32;
+  s->compact_valid[SMY_FP64] = c64;
   SMY_HIP(hipGetLastError());
   SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
   h->timed = true;
@@ -1012,12 +1077,20 @@ int shared_evaluate(smy_pdhg* h, double* out) {
     if (s->mode_host[k] != kFrozen && !s->has_original[k])
       return fail(SMY_STATE, "set_original() must be called before evaluate()");
   upload_modes(h);
+  // A precision whose last run stepped on its compact block is evaluated there (after the full
+  // finish, which would overwrite its instances' totals with stale partials).
+  const bool c32 = s_uses(s, SMY_FP32) && s->compact_valid[SMY_FP32];
+  const bool c64 = s_uses(s, SMY_FP64) && s->compact_valid[SMY_FP64];
   for (int part = 0; part < 2; ++part) {
-    if (s_uses(s, SMY_FP32)) measure_part<float>(h, part == 0);
-    if (s_uses(s, SMY_FP64)) measure_part<double>(h, part == 0);
+    if (s_uses(s, SMY_FP32) && !c32) measure_part<float>(h, part == 0);
+    if (s_uses(s, SMY_FP64) && !c64) measure_part<double>(h, part == 0);
   }
   SMY_HIP(hipGetLastError());
-  shared_finish(h, 14, groups_of(s->m), groups_of(s->n), out);
+  shared_finish(h, 14, groups_of(s->m), groups_of(s->n), nullptr);
+  if (c32) compact_measure<float>(h);
+  if (c64) compact_measure<double>(h);
+  SMY_HIP(hipGetLastError());
+  shared_download_sums(h, 14, true, out);
   return SMY_OK;
 }
 
@@ -1037,6 +1110,7 @@ int shared_distance(smy_pdhg* h, const int* which, double* out) {
 
 int shared_restart_to(smy_pdhg* h, const int* which) {
   SharedBatch* s = h->shared;
+  s->compact_valid[0] = s->compact_valid[1] = false;
   if (s->nt == 0) return SMY_OK;
   stage_which(h, which);
   if (any_which(s, which, SMY_FP32)) restart_part<float>(h);
@@ -1060,6 +1134,7 @@ int shared_set_precision(smy_pdhg* h, const int* precision) {
     to32 = to32 || want == SMY_FP32;
   }
   if (!any) return SMY_OK;
+  s->compact_valid[0] = s->compact_valid[1] = false;
   if (to32) ensure_fp32_shared(h);
   if (s->nt > 0) {
     stage_which(h, flag.data());
@@ -1078,6 +1153,7 @@ int shared_set_precision(smy_pdhg* h, const int* precision) {
 
 int shared_freeze(smy_pdhg* h, int k) {
   SharedBatch* s = h->shared;
+  s->compact_valid[0] = s->compact_valid[1] = false;
   s->mode_host[k] = kFrozen;
   h->inst[k].prec = kFrozen;
   s->mode_dirty = true;
@@ -1096,6 +1172,7 @@ int shared_download(smy_pdhg* h, int k, double* v, double* v_new, double* y, dou
 
 int shared_set_start(smy_pdhg* h, int k, const double* v, const double* y) {
   SharedBatch* s = h->shared;
+  s->compact_valid[0] = s->compact_valid[1] = false;
   if (s->state_prec[k] == SMY_FP64)
     set_start_part<double>(h, k, v, y);
   else
