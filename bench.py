@@ -190,7 +190,7 @@ def main() -> int:
     ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
     ap.add_argument("--adaptive-steps", default="auto", choices=["auto", "on", "off"],
                     help="PDLP's adaptive step sizes for the mixed-precision solves (auto: on for "
-                         "the single LP; the shared-matrix batch has none)")
+                         "the single LP and the stacked batch; the shared-matrix batch has none)")
     ap.add_argument("--verbose", action="store_true", help="progress lines of long batches (stdout)")
     ap.add_argument("--fp32-switch-tol", type=float, default=None,
                     help="mixed precision: FP32 steps until this distance to termination (or a "
@@ -200,7 +200,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.fp32_switch_tol is None:
         args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
-    args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload == "c3")
+    args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload in ("c3", "c4"))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the batched and "
               "single-LP shardings are described in DESIGN.md); run with --gpus 1",
@@ -334,16 +334,22 @@ def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict
     return line
 
 
-def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None) -> dict:
+def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool = False) -> dict:
     """Live timing of the FP32 step kernels of a batch handle; k: instances of a shared-matrix
-    batch (None: stacked batch)."""
-    dual_ms, primal_ms = dev.time_kernels(20)
+    batch (None: stacked batch); adaptive: the adaptive attempt's kernels (stacked batch; same
+    byte model)."""
+    if adaptive:
+        dual_ms, primal_ms, _ = dev.time_adaptive_kernels(20)
+        ops = ("AdaptDualOp", "AdaptPrimalOp")
+    else:
+        dual_ms, primal_ms = dev.time_kernels(20)
+        ops = ("DualRowsOp", "PrimalColsOp")
     b = shared_step_bytes(nnz, m, n, k, 4) if k else step_kernel_bytes(nnz, m, n, 4)
     dom = "primal" if primal_ms >= dual_ms else "dual"
     ms = primal_ms if dom == "primal" else dual_ms
     achieved = b[dom] / (ms * 1e-3) / 1e9
     kernel = (f"shared_{dom}_kernel<float>" if k else
-              f"csr_rows_kernel<float, {'PrimalColsOp' if dom == 'primal' else 'DualRowsOp'}<float>>")
+              f"csr_rows_kernel<float, {ops[1] if dom == 'primal' else ops[0]}<float>>")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
             "avg_us": round(ms * 1e3, 2), "bytes_per_launch": b[dom],
@@ -431,7 +437,7 @@ def run_c4(args) -> int:
     dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
     M = sum(md.m for md in models)
     N = sum(md.n for md in models)
-    rl = batch_roofline(dev, nnz, M, N, None)
+    rl = batch_roofline(dev, nnz, M, N, None, args.adaptive)
     del dev
 
     cpu = None

@@ -433,7 +433,6 @@ void measure_part(smy_pdhg* h, bool rows_of_a) {
                        h->stream, a);
 }
 
-int cgroups_of(int rows) { return (rows + kCompactGroupRows - 1) / kCompactGroupRows; }
 
 void shared_download_sums(smy_pdhg* h, int nsums, bool two_parts, double* out);
 
@@ -445,9 +444,12 @@ void compact_measure(smy_pdhg* h) {
   CompactState<T>& C = cstate<T>(s);
   const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
   const int kc = compact_width(s, q), na = s->nactive[q];
-  const int ga = cgroups_of(s->m), gt = cgroups_of(s->n);
-  if (s->cpartial == nullptr) s->cpartial = h->alloc<double>(14LL * (ga + gt) * kCompactInstances);
   const int lanes = kc / kInstPerThread;
+  const int group_rows = lanes;  // one thread per row
+  const int ga = (s->m + group_rows - 1) / group_rows, gt = (s->n + group_rows - 1) / group_rows;
+  // (groups of both parts) x 14 x kc <= 14 (4 (m + n) + 2 kc): sized once for the widest block.
+  if (s->cpartial == nullptr)
+    s->cpartial = h->alloc<double>(14LL * (4LL * (s->m + s->n) + 2 * kCompactInstances));
   const int per_wg = kBlock / lanes;
   for (int part = 0; part < 2; ++part) {
     const bool rows_of_a = part == 0;
@@ -459,6 +461,7 @@ void compact_measure(smy_pdhg* h) {
     a.kp = s->kp;
     a.kc = kc;
     a.na = na;
+    a.group_rows = group_rows;
     a.rows_of_a = rows_of_a;
     a.list = s->active[q];
     a.ptr = rows_of_a ? s->a_ptr : s->t_ptr;
@@ -1081,12 +1084,13 @@ int shared_evaluate(smy_pdhg* h, double* out) {
   // finish, which would overwrite its instances' totals with stale partials).
   const bool c32 = s_uses(s, SMY_FP32) && s->compact_valid[SMY_FP32];
   const bool c64 = s_uses(s, SMY_FP64) && s->compact_valid[SMY_FP64];
+  const bool full = (s_uses(s, SMY_FP32) && !c32) || (s_uses(s, SMY_FP64) && !c64);
   for (int part = 0; part < 2; ++part) {
     if (s_uses(s, SMY_FP32) && !c32) measure_part<float>(h, part == 0);
     if (s_uses(s, SMY_FP64) && !c64) measure_part<double>(h, part == 0);
   }
   SMY_HIP(hipGetLastError());
-  shared_finish(h, 14, groups_of(s->m), groups_of(s->n), nullptr);
+  if (full) shared_finish(h, 14, groups_of(s->m), groups_of(s->n), nullptr);
   if (c32) compact_measure<float>(h);
   if (c64) compact_measure<double>(h);
   SMY_HIP(hipGetLastError());

@@ -242,6 +242,14 @@ class PdhgBatch:
         N.check(self.lib.smy_pdhg_time_kernels(self.h, reps, C.byref(d), C.byref(p)), "time_kernels")
         return d.value, p.value
 
+    def time_adaptive_kernels(self, reps: int = 20) -> tuple[float, float, float]:
+        """Average ms of the adaptive dual, primal and decision kernels over every instance
+        (stacked layout only; changes the state)."""
+        d, p, q = C.c_double(), C.c_double(), C.c_double()
+        N.check(self.lib.smy_pdhg_time_adaptive_kernels(self.h, reps, C.byref(d), C.byref(p), C.byref(q)),
+                "time_adaptive_kernels")
+        return d.value, p.value, q.value
+
     def evaluate(self) -> np.ndarray:
         """(batch, 14): the current iterate's 7 sums, then the average's."""
         out = np.zeros((self.size, 14))
