@@ -617,7 +617,7 @@ struct AdaptDualOp {
 // flight per thread; each thread still adds its entries in index order, so the totals are fixed.
 // Instances that do not step (frozen) keep their state.
 constexpr int kDecideThreads = 1024;
-constexpr int kDecideUnroll = 4;
+constexpr int kDecideUnroll = 8;
 
 static __global__ __launch_bounds__(kDecideThreads) void adapt_decide_kernel(
     int stride, const double* __restrict__ partial, const int2* __restrict__ ranges,
@@ -629,31 +629,29 @@ static __global__ __launch_bounds__(kDecideThreads) void adapt_decide_kernel(
   const int tid = threadIdx.x;
   const int2 ra = ranges[2 * i];
   const int2 rt = ranges[2 * i + 1];
+  // Both ranges in one loop, so every round has the loads of both in flight; each thread still
+  // adds entries tid, tid + 1024, ... of each range in that order.
+  const int len_a = ra.y - ra.x, len_t = rt.y - rt.x;
+  const int span = max(len_a, len_t);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int b0 = ra.x + tid; b0 < ra.y; b0 += kDecideUnroll * kDecideThreads) {
+  double at = 0.0;
+  for (int o = tid; o < span; o += kDecideUnroll * kDecideThreads) {
     double v[kDecideUnroll][3];
+    double w[kDecideUnroll];
 #pragma unroll
     for (int u = 0; u < kDecideUnroll; ++u) {
-      const int b = b0 + u * kDecideThreads;
+      const int b = o + u * kDecideThreads;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) v[u][q] = b < ra.y ? partial[static_cast<size_t>(q) * stride + b] : 0.0;
+      for (int q = 0; q < 3; ++q)
+        v[u][q] = b < len_a ? partial[static_cast<size_t>(q) * stride + ra.x + b] : 0.0;
+      w[u] = b < len_t ? partial[rt.x + b] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kDecideUnroll; ++u) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) acc[q] += v[u][q];
+      at += w[u];
     }
-  }
-  double at = 0.0;
-  for (int b0 = rt.x + tid; b0 < rt.y; b0 += kDecideUnroll * kDecideThreads) {
-    double v[kDecideUnroll];
-#pragma unroll
-    for (int u = 0; u < kDecideUnroll; ++u) {
-      const int b = b0 + u * kDecideThreads;
-      v[u] = b < rt.y ? partial[b] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kDecideUnroll; ++u) at += v[u];
   }
   acc[0] = acc[0] + at;
 #pragma unroll

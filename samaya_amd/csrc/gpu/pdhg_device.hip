@@ -488,19 +488,16 @@ int time_adaptive_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms
   staging_sent(h);
   std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   refresh_stale_ax(h, prec_of<T>() == SMY_FP32 ? 2 : 1);
-  const auto timed = [&](auto go) {
-    go();  // warm
-    SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-    for (int r = 0; r < reps; ++r) go();
-    SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
-    SMY_HIP(hipEventSynchronize(h->ev_stop));
-    float t = 0.0f;
-    SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
-    return static_cast<double>(t) / reps;
-  };
-  *primal_ms = timed([&] { adapt_primal<T>(h); });
-  *dual_ms = timed([&] { adapt_dual<T>(h); });
-  *decide_ms = timed([&] { adapt_decide(h); });
+  // One attempt is the primal, dual and decision kernels, in that order.
+  double ms[3];
+  time_in_sequence(h, reps, 3, [&](int k) {
+    if (k == 0) adapt_primal<T>(h);
+    else if (k == 1) adapt_dual<T>(h);
+    else adapt_decide(h);
+  }, ms);
+  *primal_ms = ms[0];
+  *dual_ms = ms[1];
+  *decide_ms = ms[2];
   SMY_HIP(hipGetLastError());
   std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   h->timed = false;
@@ -596,9 +593,9 @@ int run_adaptive_impl(smy_pdhg* h, smy_pdhg_adaptive* st, int count) {
 }
 
 // Average duration of each of the two step kernels of the first precision in use (FP32 before
-// FP64) over `reps` back-to-back launches (HIP events on the handle's stream). The launches repeat
-// one step from the current state, so the iterates and averages change: a measurement helper,
-// not part of a solve.
+// FP64) over `reps` steps, dual then primal as a solve issues them (HIP events between the
+// launches on the handle's stream). The launches repeat one step from the current state, so the
+// iterates and averages change: a measurement helper, not part of a solve.
 template <typename T>
 int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   constexpr int q = prec_of<T>();
@@ -633,25 +630,14 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   primal.cost = S.cost;
   primal.lower = S.lower;
   primal.upper = S.upper;
-  float t = 0.0f;
   const auto dual_go = [&] { launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], dual, h->stream); };
   const auto primal_go = [&] {
     launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
   };
-  dual_go();  // warm
-  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-  for (int r = 0; r < reps; ++r) dual_go();
-  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
-  SMY_HIP(hipEventSynchronize(h->ev_stop));
-  SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
-  *dual_ms = static_cast<double>(t) / reps;
-  primal_go();
-  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-  for (int r = 0; r < reps; ++r) primal_go();
-  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
-  SMY_HIP(hipEventSynchronize(h->ev_stop));
-  SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
-  *primal_ms = static_cast<double>(t) / reps;
+  double ms[2];
+  time_in_sequence(h, reps, 2, [&](int k) { k == 0 ? dual_go() : primal_go(); }, ms);
+  *dual_ms = ms[0];
+  *primal_ms = ms[1];
   SMY_HIP(hipGetLastError());
   h->timed = false;
   return SMY_OK;

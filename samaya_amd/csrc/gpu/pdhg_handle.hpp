@@ -278,6 +278,39 @@ inline void staging_sent(smy_pdhg* h) {
   h->staged = true;
 }
 
+// Average duration of each of `count` launch steps, issued in order `reps` times as a solve issues
+// them (each step reads what the previous one wrote, so none finds its inputs left in cache by a
+// repeat of itself), from events recorded between the steps on the handle's stream. One untimed
+// round first.
+template <typename Step>
+void time_in_sequence(smy_pdhg* h, int reps, int count, Step&& step, double* ms) {
+  for (int k = 0; k < count; ++k) step(k);
+  std::vector<hipEvent_t> ev(static_cast<size_t>(reps) * count + 1, nullptr);
+  struct Release {
+    std::vector<hipEvent_t>& ev;
+    ~Release() {
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } release{ev};
+  for (hipEvent_t& e : ev) SMY_HIP(hipEventCreate(&e));
+  SMY_HIP(hipEventRecord(ev[0], h->stream));
+  for (int r = 0; r < reps; ++r)
+    for (int k = 0; k < count; ++k) {
+      step(k);
+      SMY_HIP(hipEventRecord(ev[static_cast<size_t>(r) * count + k + 1], h->stream));
+    }
+  SMY_HIP(hipEventSynchronize(ev.back()));
+  for (int k = 0; k < count; ++k) ms[k] = 0.0;
+  for (int r = 0; r < reps; ++r)
+    for (int k = 0; k < count; ++k) {
+      const size_t e = static_cast<size_t>(r) * count + k;
+      float t = 0.0f;
+      SMY_HIP(hipEventElapsedTime(&t, ev[e], ev[e + 1]));
+      ms[k] += static_cast<double>(t) / reps;
+    }
+}
+
 // The shared-matrix batch (pdhg_shared.hip): the batch entry points of a handle whose `shared` is
 // set go here.
 int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* cost,

@@ -445,12 +445,18 @@ void compact_measure(smy_pdhg* h) {
   const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
   const int kc = compact_width(s, q), na = s->nactive[q];
   const int lanes = kc / kInstPerThread;
-  const int group_rows = lanes;  // one thread per row
-  const int ga = (s->m + group_rows - 1) / group_rows, gt = (s->n + group_rows - 1) / group_rows;
-  // (groups of both parts) x 14 x kc <= 14 (4 (m + n) + 2 kc): sized once for the widest block.
+  const int places = kBlock / lanes;  // row places per workgroup
+  // Rows per place: about 1,024 workgroups per part (4 per CU), at most 8 rows each.
+  const auto rows_per_place = [&](int rows) {
+    return std::max(1, std::min(8, rows / (places * 1024)));
+  };
+  const int rpt_a = rows_per_place(s->m), rpt_t = rows_per_place(s->n);
+  const int ga = (s->m + places * rpt_a - 1) / (places * rpt_a);
+  const int gt = (s->n + places * rpt_t - 1) / (places * rpt_t);
+  // (workgroups of both parts) x 14 x kc <= 14 (4 (m + n) + 2 kc), as places >= kc / 4: sized
+  // once for the widest block.
   if (s->cpartial == nullptr)
     s->cpartial = h->alloc<double>(14LL * (4LL * (s->m + s->n) + 2 * kCompactInstances));
-  const int per_wg = kBlock / lanes;
   for (int part = 0; part < 2; ++part) {
     const bool rows_of_a = part == 0;
     const int groups = rows_of_a ? ga : gt;
@@ -461,7 +467,7 @@ void compact_measure(smy_pdhg* h) {
     a.kp = s->kp;
     a.kc = kc;
     a.na = na;
-    a.group_rows = group_rows;
+    a.rpt = rows_of_a ? rpt_a : rpt_t;
     a.rows_of_a = rows_of_a;
     a.list = s->active[q];
     a.ptr = rows_of_a ? s->a_ptr : s->t_ptr;
@@ -476,8 +482,7 @@ void compact_measure(smy_pdhg* h) {
     a.lower = rows_of_a ? s->row_lower_o : s->col_lower_o;
     a.upper = rows_of_a ? s->row_upper_o : s->col_upper_o;
     a.partial = s->cpartial + (rows_of_a ? 0 : 14LL * ga * kc);
-    hipLaunchKernelGGL((compact_measure_kernel<T>), dim3((groups + per_wg - 1) / per_wg), dim3(kBlock), 0,
-                       h->stream, a);
+    hipLaunchKernelGGL((compact_measure_kernel<T>), dim3(groups), dim3(kBlock), 0, h->stream, a);
     hipLaunchKernelGGL(compact_finish_kernel, dim3(na, 14), dim3(kBlock), 0, h->stream, groups, 14, kc,
                        s->active[q], a.partial, s->sums_dev, 28, rows_of_a ? 0 : 14);
   }
@@ -487,15 +492,15 @@ void compact_measure(smy_pdhg* h) {
 void shared_finish(smy_pdhg* h, int nsums, int groups_a, int groups_t, double* out) {
   SharedBatch* s = h->shared;
   const int stride = 2 * nsums;
-  const dim3 grid(blocks_for(s->kp), nsums);
+  const dim3 grid((s->kp + kWave - 1) / kWave, nsums);
   if (groups_t < 0) {
     // One part only (distances).
-    hipLaunchKernelGGL(shared_finish_kernel, grid, dim3(kBlock), 0, h->stream, groups_a, nsums,
+    hipLaunchKernelGGL(shared_finish_kernel, grid, dim3(kFinishThreads), 0, h->stream, groups_a, nsums,
                        s->kp, s->partial, s->sums_dev, stride, 0);
   } else {
-    hipLaunchKernelGGL(shared_finish_kernel, grid, dim3(kBlock), 0, h->stream, groups_a, nsums,
+    hipLaunchKernelGGL(shared_finish_kernel, grid, dim3(kFinishThreads), 0, h->stream, groups_a, nsums,
                        s->kp, s->partial, s->sums_dev, stride, 0);
-    hipLaunchKernelGGL(shared_finish_kernel, grid, dim3(kBlock), 0, h->stream, groups_t, nsums,
+    hipLaunchKernelGGL(shared_finish_kernel, grid, dim3(kFinishThreads), 0, h->stream, groups_t, nsums,
                        s->kp, s->partial + static_cast<size_t>(groups_a) * nsums * s->kp,
                        s->sums_dev, stride, nsums);
   }
@@ -651,26 +656,23 @@ void set_start_part(smy_pdhg* h, int k, const double* v, const double* y) {
 template <typename T>
 int time_part(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   SharedBatch* s = h->shared;
-  float t = 0.0f;
-  const auto timed = [&](bool dual) {
-    const auto go = [&] {
-      if (dual)
-        hipLaunchKernelGGL((shared_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
-                           step_args<T>(h, 0, true));
-      else
-        hipLaunchKernelGGL((shared_primal_kernel<T>), dim3(s->n, chunks(s)), dim3(kBlock), 0,
-                           h->stream, step_args<T>(h, 0, false));
-    };
-    go();
-    SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-    for (int r = 0; r < reps; ++r) go();
-    SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
-    SMY_HIP(hipEventSynchronize(h->ev_stop));
-    SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
-    return static_cast<double>(t) / reps;
+  // The dual then the primal kernel, as a step issues them; a side with no rows or columns has none.
+  const auto go = [&](bool dual) {
+    if (dual)
+      hipLaunchKernelGGL((shared_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
+                         step_args<T>(h, 0, true));
+    else
+      hipLaunchKernelGGL((shared_primal_kernel<T>), dim3(s->n, chunks(s)), dim3(kBlock), 0,
+                         h->stream, step_args<T>(h, 0, false));
   };
-  *dual_ms = s->m > 0 ? timed(true) : 0.0;
-  *primal_ms = s->n > 0 ? timed(false) : 0.0;
+  bool sides[2];
+  int count = 0;
+  if (s->m > 0) sides[count++] = true;
+  if (s->n > 0) sides[count++] = false;
+  double ms[2] = {0.0, 0.0};
+  if (count > 0) time_in_sequence(h, reps, count, [&](int k) { go(sides[k]); }, ms);
+  *dual_ms = s->m > 0 ? ms[0] : 0.0;
+  *primal_ms = s->n > 0 ? ms[count - 1] : 0.0;
   SMY_HIP(hipGetLastError());
   h->timed = false;
   return SMY_OK;
@@ -850,8 +852,8 @@ int shared_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1,
     if (u64) distance_part<double>(h);
   }
   // Per-instance distance totals at k * 4 (zero without variables: no groups).
-  hipLaunchKernelGGL(shared_finish_kernel, dim3(blocks_for(s->kp), 2), dim3(kBlock), 0, h->stream,
-                     s->nt > 0 ? groups_of(s->nt) : 0, 2, s->kp, s->partial, s->sums_dev, 4, 0);
+  hipLaunchKernelGGL(shared_finish_kernel, dim3((s->kp + kWave - 1) / kWave, 2), dim3(kFinishThreads), 0,
+                     h->stream, s->nt > 0 ? groups_of(s->nt) : 0, 2, s->kp, s->partial, s->sums_dev, 4, 0);
   hipLaunchKernelGGL(rule_weight_kernel, dim3(chunks1(s)), dim3(kBlock), 0, h->stream, s->K, 1, s->sums_dev,
                      h->rule);
   if (s->nt > 0) {
