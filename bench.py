@@ -65,13 +65,34 @@ def step_kernel_bytes(nnz: int, m: int, n: int, s: int) -> dict:
     return {"dual": dual, "primal": primal}
 
 
-def shared_step_bytes(nnz: int, m: int, n: int, k: int, s: int) -> dict:
+def shared_step_bytes(nnz: int, m: int, n: int, k: int, s: int, coded: bool = False) -> dict:
     """Algorithmic HBM bytes of the shared-matrix step kernels for k instances: the matrix once,
     the gathered iterate block once (k values per row), and the per-row update streams of all
-    instances (rows of A: 5 read, 5 written; rows of A': v, average and both bounds read, 3 written)."""
+    instances (rows of A: 5 read, 5 written; rows of A': v, average and both bounds read, 3 written;
+    with coded bounds one byte per (column, instance) and 4 pairs per column replace the bounds)."""
     dual = nnz * (s + 4) + 4 * (m + 1) + s * k * n + 10 * s * k * m
-    primal = nnz * (s + 4) + 4 * (n + 1) + s * k * m + 7 * s * k * n
+    bounds = k * n + 8 * s * n if coded else 2 * s * k * n
+    primal = nnz * (s + 4) + 4 * (n + 1) + s * k * m + 5 * s * k * n + bounds
     return {"dual": dual, "primal": primal}
+
+
+def shared_bounds_coded(lows, ups) -> bool:
+    """Whether the shared-matrix batch keeps these column bounds as codes: at most 4 distinct
+    (lower, upper) pairs per column over the instances, padding instances (up to a multiple of 4)
+    counting as (0, 0). Pairs are compared by a hash of their bits (a byte model, not a solve)."""
+    import numpy as np
+    k = len(lows)
+    if k == 0 or len(lows[0]) == 0:
+        return False
+    kp = (k + 3) // 4 * 4
+    lo = np.stack([np.asarray(x, dtype=np.float64) for x in lows], axis=1).view(np.int64)
+    up = np.stack([np.asarray(x, dtype=np.float64) for x in ups], axis=1).view(np.int64)
+    key = lo * np.int64(1000003) ^ up
+    if kp > k:
+        key = np.concatenate([key, np.zeros((key.shape[0], kp - k), np.int64)], axis=1)
+    key = np.sort(key, axis=1)
+    distinct = 1 + np.count_nonzero(key[:, 1:] != key[:, :-1], axis=1)
+    return bool(distinct.max() <= 4)
 
 
 def profiled_traffic(op: str) -> int | None:
@@ -334,17 +355,18 @@ def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict
     return line
 
 
-def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool = False) -> dict:
+def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool = False,
+                   coded: bool = False) -> dict:
     """Live timing of the FP32 step kernels of a batch handle; k: instances of a shared-matrix
     batch (None: stacked batch); adaptive: the adaptive attempt's kernels (stacked batch; same
-    byte model)."""
+    byte model); coded: the shared batch keeps its column bounds as codes."""
     if adaptive:
         dual_ms, primal_ms, _ = dev.time_adaptive_kernels(20)
         ops = ("AdaptDualOp", "AdaptPrimalOp")
     else:
         dual_ms, primal_ms = dev.time_kernels(20)
         ops = ("DualRowsOp", "PrimalColsOp")
-    b = shared_step_bytes(nnz, m, n, k, 4) if k else step_kernel_bytes(nnz, m, n, 4)
+    b = shared_step_bytes(nnz, m, n, k, 4, coded) if k else step_kernel_bytes(nnz, m, n, 4)
     dom = "primal" if primal_ms >= dual_ms else "dual"
     ms = primal_ms if dom == "primal" else dual_ms
     achieved = b[dom] / (ms * 1e-3) / 1e9
@@ -541,7 +563,8 @@ def run_c5(args) -> int:
     lo_full = [np.concatenate([l, md.row_lower]) for l in lows]
     up_full = [np.concatenate([u, md.row_upper]) for u in ups]
     dev = D.PdhgBatch.create_shared(a, at, cost, lo_full, up_full, "fp32")
-    rl = batch_roofline(dev, md.nnz, md.m, md.n, K)
+    rl = batch_roofline(dev, md.nnz, md.m, md.n, K, coded=shared_bounds_coded(lows, ups))
+    rl["bounds_coded"] = shared_bounds_coded(lows, ups)
     del dev
 
     cpu = None

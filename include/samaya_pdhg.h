@@ -99,8 +99,7 @@ int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y);
 int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
 
 /* Measurement helper: average milliseconds of the dual (rows of A) and primal (rows of A') step
- * kernels over `reps` steps, launched in turn as a solve launches them, from HIP events between
- * the launches on the handle's stream. Changes the iterates. */
+ * kernels over `reps` back-to-back launches on the handle's stream. Changes the iterates. */
 int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
 
 /* ---- Batches: many LPs in one handle (no reference counterpart; the reference solves one LP
@@ -186,8 +185,7 @@ typedef struct smy_pdhg_adaptive {
  * calls apply unchanged. Not available for the shared-matrix batch. Synchronizes. */
 int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count);
 /* Measurement helper: average milliseconds of the adaptive attempt's dual (rows of A), primal
- * (rows of A') and decision kernels over `reps` attempts, launched in the attempt's order (primal,
- * dual, decision). Changes the state. */
+ * (rows of A') and decision kernels over `reps` launches each. Changes the state. */
 int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms,
                                    double* decide_ms);
 

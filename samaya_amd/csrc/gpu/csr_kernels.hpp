@@ -617,7 +617,7 @@ struct AdaptDualOp {
 // flight per thread; each thread still adds its entries in index order, so the totals are fixed.
 // Instances that do not step (frozen) keep their state.
 constexpr int kDecideThreads = 1024;
-constexpr int kDecideUnroll = 8;
+constexpr int kDecideUnroll = 12;
 
 static __global__ __launch_bounds__(kDecideThreads) void adapt_decide_kernel(
     int stride, const double* __restrict__ partial, const int2* __restrict__ ranges,

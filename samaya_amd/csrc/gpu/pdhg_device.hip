@@ -233,6 +233,14 @@ template <typename T>
 void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5]) {
   constexpr int q = prec_of<T>();
   if (M.tuned[q] || M.nlist[q] == 0) return;
+  // A launch of fewer blocks than this lasts a few microseconds, where timing noise outweighs any
+  // tile's gain and fewer workgroups only lose parallelism: such a matrix keeps the plain tile.
+  constexpr int kTuneMinBlocks = 2048;
+  if (M.nlist[q] < kTuneMinBlocks) {
+    M.tile[q] = Tile{};
+    M.tuned[q] = true;
+    return;
+  }
   const long long rows = std::max(h->m, h->n);
   if (h->tune_out == nullptr) h->tune_out = h->alloc<double>(3 * std::max(rows, 1LL));
   TuneOp<T> op;
@@ -488,16 +496,9 @@ int time_adaptive_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms
   staging_sent(h);
   std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   refresh_stale_ax(h, prec_of<T>() == SMY_FP32 ? 2 : 1);
-  // One attempt is the primal, dual and decision kernels, in that order.
-  double ms[3];
-  time_in_sequence(h, reps, 3, [&](int k) {
-    if (k == 0) adapt_primal<T>(h);
-    else if (k == 1) adapt_dual<T>(h);
-    else adapt_decide(h);
-  }, ms);
-  *primal_ms = ms[0];
-  *dual_ms = ms[1];
-  *decide_ms = ms[2];
+  *primal_ms = time_repeated(h, reps, [&] { adapt_primal<T>(h); });
+  *dual_ms = time_repeated(h, reps, [&] { adapt_dual<T>(h); });
+  *decide_ms = time_repeated(h, reps, [&] { adapt_decide(h); });
   SMY_HIP(hipGetLastError());
   std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   h->timed = false;
@@ -593,9 +594,9 @@ int run_adaptive_impl(smy_pdhg* h, smy_pdhg_adaptive* st, int count) {
 }
 
 // Average duration of each of the two step kernels of the first precision in use (FP32 before
-// FP64) over `reps` steps, dual then primal as a solve issues them (HIP events between the
-// launches on the handle's stream). The launches repeat one step from the current state, so the
-// iterates and averages change: a measurement helper, not part of a solve.
+// FP64) over `reps` back-to-back launches (HIP events on the handle's stream). The launches repeat
+// one step from the current state, so the iterates and averages change: a measurement helper,
+// not part of a solve.
 template <typename T>
 int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   constexpr int q = prec_of<T>();
@@ -634,10 +635,8 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   const auto primal_go = [&] {
     launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
   };
-  double ms[2];
-  time_in_sequence(h, reps, 2, [&](int k) { k == 0 ? dual_go() : primal_go(); }, ms);
-  *dual_ms = ms[0];
-  *primal_ms = ms[1];
+  *dual_ms = time_repeated(h, reps, dual_go);
+  *primal_ms = time_repeated(h, reps, primal_go);
   SMY_HIP(hipGetLastError());
   h->timed = false;
   return SMY_OK;

@@ -278,37 +278,20 @@ inline void staging_sent(smy_pdhg* h) {
   h->staged = true;
 }
 
-// Average duration of each of `count` launch steps, issued in order `reps` times as a solve issues
-// them (each step reads what the previous one wrote, so none finds its inputs left in cache by a
-// repeat of itself), from events recorded between the steps on the handle's stream. One untimed
-// round first.
+// Average duration of one launch step over `reps` back-to-back repeats (after one untimed), from
+// one HIP event pair around them on the handle's stream. Repeats keep the dispatch pipelined as a
+// solve does; an event between every two launches would add its own serialization to each
+// (measured: 5-8 us on a 90 us kernel).
 template <typename Step>
-void time_in_sequence(smy_pdhg* h, int reps, int count, Step&& step, double* ms) {
-  for (int k = 0; k < count; ++k) step(k);
-  std::vector<hipEvent_t> ev(static_cast<size_t>(reps) * count + 1, nullptr);
-  struct Release {
-    std::vector<hipEvent_t>& ev;
-    ~Release() {
-      for (hipEvent_t e : ev)
-        if (e) (void)hipEventDestroy(e);
-    }
-  } release{ev};
-  for (hipEvent_t& e : ev) SMY_HIP(hipEventCreate(&e));
-  SMY_HIP(hipEventRecord(ev[0], h->stream));
-  for (int r = 0; r < reps; ++r)
-    for (int k = 0; k < count; ++k) {
-      step(k);
-      SMY_HIP(hipEventRecord(ev[static_cast<size_t>(r) * count + k + 1], h->stream));
-    }
-  SMY_HIP(hipEventSynchronize(ev.back()));
-  for (int k = 0; k < count; ++k) ms[k] = 0.0;
-  for (int r = 0; r < reps; ++r)
-    for (int k = 0; k < count; ++k) {
-      const size_t e = static_cast<size_t>(r) * count + k;
-      float t = 0.0f;
-      SMY_HIP(hipEventElapsedTime(&t, ev[e], ev[e + 1]));
-      ms[k] += static_cast<double>(t) / reps;
-    }
+double time_repeated(smy_pdhg* h, int reps, Step&& step) {
+  step();
+  SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+  for (int r = 0; r < reps; ++r) step();
+  SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+  SMY_HIP(hipEventSynchronize(h->ev_stop));
+  float t = 0.0f;
+  SMY_HIP(hipEventElapsedTime(&t, h->ev_start, h->ev_stop));
+  return static_cast<double>(t) / reps;
 }
 
 // The shared-matrix batch (pdhg_shared.hip): the batch entry points of a handle whose `shared` is

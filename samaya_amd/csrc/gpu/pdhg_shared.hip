@@ -32,6 +32,9 @@ struct SharedIter {
   T* row_upper = nullptr;
   T* col_lower = nullptr;  // n x kp
   T* col_upper = nullptr;
+  // The same bounds as codes (see SharedBatch::codes): n x 4 pairs of each column.
+  T* code_lower = nullptr;
+  T* code_upper = nullptr;
   T* tau = nullptr;        // kp
   T* sigma = nullptr;
   T* alpha = nullptr;      // kMaxRunSteps x kp
@@ -70,6 +73,12 @@ struct SharedBatch {
   int nba = 0, nbt = 0;
   SharedIter<double> d;
   SharedIter<float> f;
+  // Column bounds as codes, when every column has at most 4 distinct (lower, upper) pairs over
+  // the instances (node relaxations: the root's bounds or a fixing): codes[j * kp + k] is the
+  // index of instance k's pair in column j's table (SharedIter::code_lower / code_upper). The
+  // full-width primal step reads 1 byte per (column, instance) instead of two values. Null when
+  // some column has more pairs.
+  unsigned char* codes = nullptr;
   std::vector<int> mode_host;  // kp: SMY_FP64 / SMY_FP32 / kFrozen (padding: frozen)
   std::vector<int> state_prec;  // kp: precision of the instance's state (also when frozen)
   int* mode = nullptr;
@@ -208,6 +217,12 @@ void ensure_fp32_shared(smy_pdhg* h) {
   convert_array(h, s->d.row_upper, s->f.row_upper, s->m);
   convert_array(h, s->d.col_lower, s->f.col_lower, static_cast<long long>(s->n) * s->kp);
   convert_array(h, s->d.col_upper, s->f.col_upper, static_cast<long long>(s->n) * s->kp);
+  if (s->codes != nullptr) {
+    s->f.code_lower = h->alloc<float>(4LL * s->n);
+    s->f.code_upper = h->alloc<float>(4LL * s->n);
+    convert_array(h, s->d.code_lower, s->f.code_lower, 4LL * s->n);
+    convert_array(h, s->d.code_upper, s->f.code_upper, 4LL * s->n);
+  }
   SMY_HIP(hipGetLastError());
 }
 
@@ -240,6 +255,9 @@ SharedStep<T> step_args(smy_pdhg* h, int j, bool dual) {
   a.row_upper = S.row_upper;
   a.col_lower = S.col_lower;
   a.col_upper = S.col_upper;
+  a.codes = s->codes;
+  a.code_lower = S.code_lower;
+  a.code_upper = S.code_upper;
   return a;
 }
 
@@ -279,6 +297,7 @@ SharedStep<T> compact_args(smy_pdhg* h, int j, bool dual) {
   const int p = h->p;
   const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
   a.kp = compact_width(s, q);
+  a.codes = nullptr;  // the compact block carries its own bounds
   a.tab = StepTable<T>{C.tau, C.sigma, C.alpha, static_cast<T>(h->theta_vbar)};
   a.vbar = C.vbar[p];
   a.v = C.v[p];
@@ -656,7 +675,6 @@ void set_start_part(smy_pdhg* h, int k, const double* v, const double* y) {
 template <typename T>
 int time_part(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   SharedBatch* s = h->shared;
-  // The dual then the primal kernel, as a step issues them; a side with no rows or columns has none.
   const auto go = [&](bool dual) {
     if (dual)
       hipLaunchKernelGGL((shared_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
@@ -665,14 +683,8 @@ int time_part(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
       hipLaunchKernelGGL((shared_primal_kernel<T>), dim3(s->n, chunks(s)), dim3(kBlock), 0,
                          h->stream, step_args<T>(h, 0, false));
   };
-  bool sides[2];
-  int count = 0;
-  if (s->m > 0) sides[count++] = true;
-  if (s->n > 0) sides[count++] = false;
-  double ms[2] = {0.0, 0.0};
-  if (count > 0) time_in_sequence(h, reps, count, [&](int k) { go(sides[k]); }, ms);
-  *dual_ms = s->m > 0 ? ms[0] : 0.0;
-  *primal_ms = s->n > 0 ? ms[count - 1] : 0.0;
+  *dual_ms = s->m > 0 ? time_repeated(h, reps, [&] { go(true); }) : 0.0;
+  *primal_ms = s->n > 0 ? time_repeated(h, reps, [&] { go(false); }) : 0.0;
   SMY_HIP(hipGetLastError());
   h->timed = false;
   return SMY_OK;
@@ -945,6 +957,33 @@ int shared_create(const smy_csc* A, const smy_csc* At, int count, const double* 
         }
       SMY_HIP(hipMemcpy(s->d.col_lower, lo.data(), lo.size() * sizeof(double), hipMemcpyHostToDevice));
       SMY_HIP(hipMemcpy(s->d.col_upper, up.data(), up.size() * sizeof(double), hipMemcpyHostToDevice));
+      // The codes, if every column has at most 4 distinct pairs (padding instances included).
+      std::vector<unsigned char> codes(static_cast<size_t>(n) * s->kp, 0);
+      std::vector<double> tlo(4 * static_cast<size_t>(n), 0.0), tup(4 * static_cast<size_t>(n), 0.0);
+      bool coded = n > 0;
+      for (int j = 0; j < n && coded; ++j) {
+        int pairs = 0;
+        for (int k = 0; k < s->kp && coded; ++k) {
+          const size_t o = static_cast<size_t>(j) * s->kp + k;
+          int c = 0;
+          while (c < pairs && !(tlo[4 * j + c] == lo[o] && tup[4 * j + c] == up[o])) ++c;
+          if (c == pairs) {
+            if (pairs == 4) {
+              coded = false;
+              break;
+            }
+            tlo[4 * j + c] = lo[o];
+            tup[4 * j + c] = up[o];
+            ++pairs;
+          }
+          codes[o] = static_cast<unsigned char>(c);
+        }
+      }
+      if (coded) {
+        s->codes = h->upload(codes.data(), static_cast<long long>(codes.size()));
+        s->d.code_lower = h->upload(tlo.data(), static_cast<long long>(tlo.size()));
+        s->d.code_upper = h->upload(tup.data(), static_cast<long long>(tup.size()));
+      }
     }
     if (precision == SMY_FP32) ensure_fp32_shared(h.get());
     s->mode = h->alloc<int>(s->kp);

@@ -74,6 +74,54 @@ def test_shared_device_matches_stacked_device(gpu, K):
             assert rel_close(ds[key], dt[key], 1e-9), key
 
 
+def test_coded_bounds_give_the_same_bits(gpu):
+    # Node bounds (the root's or a fixing) have at most 4 (lower, upper) pairs per column, so the
+    # shared batch keeps them as one-byte codes. Four more instances with pairs of their own in
+    # column 0 make it keep the full bound arrays instead. The 72 common instances step to the same
+    # bits either way, in FP64 and in FP32.
+    md = G.knapsack(74, 60, 150, 4)
+    lp = identity_lp(M.from_product(md))
+    rng = np.random.default_rng(6)
+    rs, cs = rng.uniform(0.5, 2.0, lp.m), rng.uniform(0.5, 2.0, lp.n)
+    eq = equilibrated(lp, rs, cs)
+    K = 72
+    lows, ups = [], []
+    for k in range(K):
+        lo, up = G.fixings(820 + k, eq.lower[:lp.n], eq.upper[:lp.n], 0.05, 0.2)
+        lows.append(np.concatenate([lo, eq.lower[lp.n:]]))
+        ups.append(np.concatenate([up, eq.upper[lp.n:]]))
+    extra_lo, extra_up = [], []
+    for e in range(4):
+        lo, up = lows[0].copy(), ups[0].copy()
+        lo[0], up[0] = 0.01 * (e + 1), 0.5 + 0.01 * e
+        extra_lo.append(lo)
+        extra_up.append(up)
+    a = (eq.m, eq.n, eq.start, eq.index, eq.value)
+    at = transpose_csc(eq)
+    coded = D.PdhgBatch.create_shared(a, at, eq.cost, lows, ups)
+    full = D.PdhgBatch.create_shared(a, at, eq.cost, lows + extra_lo, ups + extra_up)
+    ol = [np.concatenate([x[:lp.n] * cs, lp.lower[lp.n:]]) for x in lows + extra_lo]
+    ou = [np.concatenate([x[:lp.n] * cs, lp.upper[lp.n:]]) for x in ups + extra_up]
+    for b, count in ((coded, K), (full, K + 4)):
+        for k in range(count):
+            b.set_original(k, cs, rs, lp.cost, ol[k], ou[k])
+    tau = 0.2 + 0.2 * (np.arange(K + 4) % 7) / 6
+    sigma = 0.4 - 0.2 * (np.arange(K + 4) % 5) / 4
+    coded.run(tau[:K], sigma[:K], 1.0, [0] * K, 64)
+    full.run(tau, sigma, 1.0, [0] * (K + 4), 64)
+    np.testing.assert_array_equal(coded.evaluate(), full.evaluate()[:K])
+    for b, count in ((coded, K), (full, K + 4)):
+        b.advance()
+        b.set_precision(["fp32"] * count)
+    coded.run(tau[:K], sigma[:K], 1.0, [64] * K, 64)
+    full.run(tau, sigma, 1.0, [64] * (K + 4), 64)
+    np.testing.assert_array_equal(coded.evaluate(), full.evaluate()[:K])
+    for k in (0, 17, 71):
+        dc, df = coded.download(k), full.download(k)
+        for key in dc:
+            np.testing.assert_array_equal(dc[key], df[key], err_msg=key)
+
+
 def test_identical_bound_vectors_give_identical_answers(gpu):
     md = G.knapsack(72, 300, 800, 6)
     lo, up = G.fixings(901, md.col_lower, md.col_upper)

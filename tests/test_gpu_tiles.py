@@ -1,7 +1,8 @@
 """Product tiles and their tuner: every tile (1, 2 or 4 row blocks per workgroup, blocks in
 launch order or one contiguous range per XCD) gives every sum bit for bit, so the tuner's timing
-choice never changes an answer; the tuner picks one of the tiles at the first run of a precision;
-and the solver answers equal with and without tuning."""
+choice never changes an answer; the tuner picks one of the tiles at the first run of a precision
+(timing them on a matrix of at least 2,048 row blocks, the plain tile below that); and the solver
+answers equal with and without tuning."""
 import itertools
 
 import numpy as np
@@ -74,6 +75,19 @@ def test_tuner_picks_a_tile_at_the_first_run(gpu):
     for matrix in (0, 1):
         assert b.tile(matrix, "fp64") in TILES
     assert b.tile(0, "fp32") is None  # no instance steps in FP32
+
+
+def test_tuner_times_the_tiles_of_a_large_matrix(gpu):
+    # 2.4M nonzeros: over 2,048 row blocks per matrix, so the tuner times every tile; a matrix of
+    # fewer blocks keeps the plain tile untimed (its launches last a few microseconds).
+    b = make_batch([_device_lp(G.sparse_uniform(150, 100000, 200000, 12), 151)], "fp32")
+    b.run([0.2], [0.3], 1.0, [0], 64)
+    for matrix in (0, 1):
+        assert b.tile(matrix, "fp32") in TILES
+    small = make_batch([_device_lp(G.sparse_uniform(152, 300, 600, 4), 153)], "fp32")
+    small.run([0.2], [0.3], 1.0, [0], 64)
+    for matrix in (0, 1):
+        assert small.tile(matrix, "fp32") == (1, False)
 
 
 def test_solver_answers_do_not_depend_on_the_tile(gpu):
