@@ -197,6 +197,20 @@ def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 128)) -> dict | None
                        f"{gpu_iterations} iterations")}
 
 
+def copy_calibration() -> int:
+    """One plain device-to-device copy of 1 GiB (HIP's own copy kernel). Under rocprofv3 --pmc its
+    FETCH_SIZE and WRITE_SIZE give the counters' scale: measured 524315 KiB and 1048576 KiB, so
+    fetches are multiplied by 2 and writes by 1 to get bytes."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {a.numel() * 8} bytes")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c5"])
@@ -218,7 +232,12 @@ def main() -> int:
                          "stall); default 1e-6 for the node relaxations, 1e-5 otherwise")
     ap.add_argument("--max-iterations", type=int, default=-1,
                     help="cap each solve (profiling runs); the line then reports no speedup")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device and exit: the reference transfer that "
+                         "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counts to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.fp32_switch_tol is None:
         args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
     args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload in ("c3", "c4"))
