@@ -95,14 +95,15 @@ def shared_bounds_coded(lows, ups) -> bool:
     return bool(distinct.max() <= 4)
 
 
-def profiled_traffic(op: str) -> int | None:
-    """Calibrated HBM bytes per launch of a step kernel, from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summaries of this workload (profiles/*_c2_mixed.md), if present."""
-    for path in sorted((ROOT / "profiles").glob("*_c2_mixed.md")):
+def profiled_traffic(workload: str, kernel: str) -> int | None:
+    """Calibrated HBM bytes per launch of a step kernel on a workload (c3, c4, c5), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries: lines `traffic_bytes <workload>
+    <kernel> <bytes>` in profiles/*.md. None if no summary has it."""
+    for path in sorted((ROOT / "profiles").glob("*.md")):
         for line in path.read_text().splitlines():
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == op:
-                return int(parts[2])
+            if len(parts) == 4 and parts[0] == "traffic_bytes" and parts[1] == workload and parts[2] == kernel:
+                return int(parts[3])
     return None
 
 
@@ -232,6 +233,9 @@ def main() -> int:
                          "stall); default 1e-6 for the node relaxations, 1e-5 otherwise")
     ap.add_argument("--max-iterations", type=int, default=-1,
                     help="cap each solve (profiling runs); the line then reports no speedup")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="--workload c4 / c5: time the batch's step kernels and exit, without the "
+                         "solve (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device and exit: the reference transfer that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counts to bytes")
@@ -335,7 +339,7 @@ def main() -> int:
         "objective": last.objective,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
-                     "traffic": profiled_traffic(f"{dom_op}<float>") if args.scale == 1.0 else None,
+                     "traffic": profiled_traffic("c3", f"{dom_op}<float>") if args.scale == 1.0 else None,
                      "kernel": f"csr_rows_kernel<float, {dom_op}<float>>",
                      "avg_us": round(dom_ms * 1e3, 2), "bytes_per_launch": rl["bytes"][dom]},
         "cpu_baseline": cpu,
@@ -375,10 +379,11 @@ def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict
 
 
 def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool = False,
-                   coded: bool = False) -> dict:
+                   coded: bool = False, traffic_of: str | None = None) -> dict:
     """Live timing of the FP32 step kernels of a batch handle; k: instances of a shared-matrix
     batch (None: stacked batch); adaptive: the adaptive attempt's kernels (stacked batch; same
-    byte model); coded: the shared batch keeps its column bounds as codes."""
+    byte model); coded: the shared batch keeps its column bounds as codes; traffic_of: the
+    workload whose profiled traffic applies (None: the handle is not that full workload)."""
     if adaptive:
         dual_ms, primal_ms, _ = dev.time_adaptive_kernels(20)
         ops = ("AdaptDualOp", "AdaptPrimalOp")
@@ -389,10 +394,11 @@ def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool 
     dom = "primal" if primal_ms >= dual_ms else "dual"
     ms = primal_ms if dom == "primal" else dual_ms
     achieved = b[dom] / (ms * 1e-3) / 1e9
-    kernel = (f"shared_{dom}_kernel<float>" if k else
-              f"csr_rows_kernel<float, {ops[1] if dom == 'primal' else ops[0]}<float>>")
+    key = f"shared_{dom}_kernel<float>" if k else f"{ops[1] if dom == 'primal' else ops[0]}<float>"
+    kernel = key if k else f"csr_rows_kernel<float, {key}>"
+    traffic = profiled_traffic(traffic_of, key) if traffic_of else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None, "kernel": kernel,
+            "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic, "kernel": kernel,
             "avg_us": round(ms * 1e3, 2), "bytes_per_launch": b[dom],
             "other_kernel_us": round((dual_ms if dom == "primal" else primal_ms) * 1e3, 2)}
 
@@ -430,6 +436,32 @@ def run_c4(args) -> int:
     nnz = sum(md.nnz for md in models)
     log(f"{K} blending LPs, rows {min(md.m for md in models)}..{max(md.m for md in models)}, "
         f"nnz {nnz}, generated in {time.perf_counter() - t0:.1f} s")
+    M = sum(md.m for md in models)
+    N = sum(md.n for md in models)
+
+    def step_kernels() -> dict:
+        """Live timing of the FP32 step kernels over the whole stacked batch."""
+        As, Ats, costs, los, ups = [], [], [], [], []
+        for md in models:
+            A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+            T = A.T.tocsc()
+            T.sort_indices()
+            As.append((md.m, md.n, md.start, md.index, md.value))
+            Ats.append((md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data))
+            costs.append(np.concatenate([md.obj, np.zeros(md.m)]))
+            los.append(np.concatenate([md.col_lower, md.row_lower]))
+            ups.append(np.concatenate([md.col_upper, md.row_upper]))
+        dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
+        out = batch_roofline(dev, nnz, M, N, None, args.adaptive,
+                             traffic_of="c4" if args.scale == 1.0 else None)
+        del dev
+        return out
+
+    if args.kernels_only:
+        # Profiling mode: the step kernels alone (no solve, no baselines); one JSON line.
+        print(json.dumps({"workload": f"{K} blending LPs, stacked FP32 step kernels",
+                          "roofline": step_kernels()}), flush=True)
+        return 0
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
                            adaptive_steps=args.adaptive)
@@ -464,22 +496,7 @@ def run_c4(args) -> int:
                 "obj_err": max(objs)}
         log(f"fp64 one at a time: {sum(secs):.2f} s for {len(sample)} LPs -> {fp64['seconds']:.1f} s")
 
-    # Roofline: the FP32 step kernels over the whole stacked batch.
-    As, Ats, costs, los, ups = [], [], [], [], []
-    for md in models:
-        A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
-        T = A.T.tocsc()
-        T.sort_indices()
-        As.append((md.m, md.n, md.start, md.index, md.value))
-        Ats.append((md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data))
-        costs.append(np.concatenate([md.obj, np.zeros(md.m)]))
-        los.append(np.concatenate([md.col_lower, md.row_lower]))
-        ups.append(np.concatenate([md.col_upper, md.row_upper]))
-    dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
-    M = sum(md.m for md in models)
-    N = sum(md.n for md in models)
-    rl = batch_roofline(dev, nnz, M, N, None, args.adaptive)
-    del dev
+    rl = step_kernels()
 
     cpu = None
     if not args.no_cpu_baseline and fp64 is not None:
@@ -522,6 +539,30 @@ def run_c5(args) -> int:
     md, lows, ups = smy.generators.c5(K)
     nm = md.native()
     log(f"knapsack m={md.m} n={md.n} nnz={md.nnz}, {K} bound vectors, {time.perf_counter() - t0:.1f} s")
+
+    def step_kernels() -> dict:
+        """Live timing of the shared-matrix FP32 step kernels over all K instances."""
+        A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+        T = A.T.tocsc()
+        T.sort_indices()
+        a = (md.m, md.n, md.start, md.index, md.value)
+        at = (md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data)
+        cost = np.concatenate([-md.obj, np.zeros(md.m)])
+        lo_full = [np.concatenate([lo, md.row_lower]) for lo in lows]
+        up_full = [np.concatenate([up, md.row_upper]) for up in ups]
+        coded = shared_bounds_coded(lows, ups)
+        dev = D.PdhgBatch.create_shared(a, at, cost, lo_full, up_full, "fp32")
+        out = batch_roofline(dev, md.nnz, md.m, md.n, K, coded=coded,
+                             traffic_of="c5" if args.scale == 1.0 else None)
+        out["bounds_coded"] = coded
+        del dev
+        return out
+
+    if args.kernels_only:
+        # Profiling mode: the step kernels alone (no solve, no baselines); one JSON line.
+        print(json.dumps({"workload": f"{K} node relaxations, shared-matrix FP32 step kernels",
+                          "roofline": step_kernels()}), flush=True)
+        return 0
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
                            adaptive_steps=args.adaptive)
@@ -574,17 +615,7 @@ def run_c5(args) -> int:
         fp64 = {"seconds": total, "sample": sample, "its": its64, "obj_err": max(objs) if objs else None}
         log(f"fp64 one at a time: {fp64['seconds']:.1f} s estimated for {K} nodes")
 
-    T = A.T.tocsc()
-    T.sort_indices()
-    a = (md.m, md.n, md.start, md.index, md.value)
-    at = (md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data)
-    cost = np.concatenate([-md.obj, np.zeros(md.m)])
-    lo_full = [np.concatenate([l, md.row_lower]) for l in lows]
-    up_full = [np.concatenate([u, md.row_upper]) for u in ups]
-    dev = D.PdhgBatch.create_shared(a, at, cost, lo_full, up_full, "fp32")
-    rl = batch_roofline(dev, md.nnz, md.m, md.n, K, coded=shared_bounds_coded(lows, ups))
-    rl["bounds_coded"] = shared_bounds_coded(lows, ups)
-    del dev
+    rl = step_kernels()
 
     cpu = None
     if not args.no_cpu_baseline and fp64 is not None:
