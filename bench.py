@@ -454,6 +454,13 @@ def run_c4(args) -> int:
         dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
         out = batch_roofline(dev, nnz, M, N, None, args.adaptive,
                              traffic_of="c4" if args.scale == 1.0 else None)
+        # The tuner's choice on this handle: a tile per size bin, and whether the bins go apart.
+        out["size_bins"] = {}
+        for matrix, name in ((0, "A"), (1, "At")):
+            bins = [dev.bin_tile(matrix, "fp32", b) for b in range(3)]
+            out["size_bins"][name] = {"apart": bool(bins[0][2]) if bins[0] else False,
+                                      "tiles": [list(t[:2]) if t else None for t in bins],
+                                      "one_tile": list(dev.tile(matrix, "fp32") or ())}
         del dev
         return out
 

@@ -29,6 +29,7 @@ enum {
 };
 
 enum { SMY_FP64 = 0, SMY_FP32 = 1 }; /* iterate and matrix-value precision of the step kernels */
+enum { SMY_TILE_BINS = 3 };           /* size bins of smy_pdhg_get_bin_tile / set_bin_tile */
 enum { SMY_CURRENT = 0, SMY_AVERAGE = 1 }; /* PdhgDevice::Iterate (pdhg_core.hpp:53) */
 
 /* A compressed sparse column matrix as samaya's SparseMatrix holds it
@@ -226,9 +227,22 @@ int smy_pdhg_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter
  * workgroup, and with xcd = 1 one contiguous range of blocks per XCD. The first run in a
  * precision times every tile on that matrix's product and keeps the fastest; since each row is
  * summed inside its own block the same way under every tile, the choice never changes a result.
- * get returns SMY_STATE before the tuner ran; set fixes a tile (the tuner then leaves it). */
+ * get returns SMY_STATE before the tuner ran; set fixes one tile for all blocks (the tuner then
+ * leaves it).
+ *
+ * Size bins (a tile per size bin of a batch; no reference counterpart). A stacked batch's
+ * instances fall into SMY_TILE_BINS bins by their row blocks in the matrix (about 1,024
+ * nonzeros per block): bin 0 under 64 blocks, bin 1 under 512, bin 2 the rest. The tuner also
+ * times each tile on each bin's blocks alone, and launches the bins apart, each with its fastest
+ * tile, when that beats one launch of all blocks by 3%. get_bin_tile reports bin `bin`'s tile
+ * and whether the bins are launched apart (*split); set_bin_tile fixes one bin's tile and turns
+ * the split on (the other bins keep theirs, the plain tile unless set). No tile or split changes
+ * a result. */
 int smy_pdhg_get_tile(smy_pdhg* h, int matrix, int precision, int* per_wg, int* xcd);
 int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xcd);
+int smy_pdhg_get_bin_tile(smy_pdhg* h, int matrix, int precision, int bin, int* per_wg, int* xcd,
+                          int* split);
+int smy_pdhg_set_bin_tile(smy_pdhg* h, int matrix, int precision, int bin, int per_wg, int xcd);
 
 /* Replaces gpu::csr_spmv (spmv.hpp:16-21): y = A x for a CSR matrix in device memory (rows x ?,
  * ptr[rows + 1], idx[nnz], val[nnz]), x and y device arrays. threads_per_row is accepted for

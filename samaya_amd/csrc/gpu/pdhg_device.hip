@@ -99,12 +99,18 @@ void rebuild_lists(smy_pdhg* h) {
   for (DevMatrix* M : {&h->a, &h->at}) {
     const bool is_a = M == &h->a;
     for (int q = 0; q < 2; ++q) {
+      // The instances' blocks, size bin after size bin.
       std::vector<int> list;
-      for (const Layout& L : h->inst) {
-        if (L.prec != q) continue;
-        const int b0 = is_a ? L.ba0 : L.bt0, b1 = is_a ? L.ba1 : L.bt1;
-        for (int b = b0; b < b1; ++b) list.push_back(b);
+      for (int bin = 0; bin < kTileBins; ++bin) {
+        M->bin_start[q][bin] = static_cast<int>(list.size());
+        for (const Layout& L : h->inst) {
+          if (L.prec != q) continue;
+          const int b0 = is_a ? L.ba0 : L.bt0, b1 = is_a ? L.ba1 : L.bt1;
+          if (tile_bin(b1 - b0) != bin) continue;
+          for (int b = b0; b < b1; ++b) list.push_back(b);
+        }
       }
+      M->bin_start[q][kTileBins] = static_cast<int>(list.size());
       M->nlist[q] = static_cast<int>(list.size());
       if (!list.empty())
         SMY_HIP(hipMemcpy(M->list[q], list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -248,24 +254,53 @@ void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5])
   for (int k = 0; k < 5; ++k) op.in[k] = in[k];
   for (int k = 0; k < 3; ++k) op.out[k] = reinterpret_cast<T*>(h->tune_out) + k * rows;
   constexpr int kReps = 3;
-  float base = 0.0f, best = 0.0f;
+  // The fastest tile on blocks [first, first + count) of the list: the plain one unless another
+  // is 3% faster. Returns its time (ms for kReps launches).
+  const auto best_tile = [&](int first, int count, Tile& pick) {
+    float base = 0.0f, best = 0.0f;
+    pick = Tile{};
+    for (const Tile& t : kTiles) {
+      launch_rows(M, avals<T>(M), M.list[q] + first, count, op, h->stream, &t);  // warm
+      SMY_HIP(hipEventRecord(h->ev_start, h->stream));
+      for (int r = 0; r < kReps; ++r) launch_rows(M, avals<T>(M), M.list[q] + first, count, op, h->stream, &t);
+      SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
+      SMY_HIP(hipEventSynchronize(h->ev_stop));
+      float ms = 0.0f;
+      SMY_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
+      if (&t == kTiles) {
+        base = best = ms;
+      } else if (ms < best && ms < 0.97f * base) {
+        best = ms;
+        pick = t;
+      }
+    }
+    return best;
+  };
   Tile pick;
-  for (const Tile& t : kTiles) {
-    launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream, &t);  // warm
+  const float whole = best_tile(0, M.nlist[q], pick);
+  M.tile[q] = pick;
+  M.split[q] = false;
+  // Size bins launched apart, each with its own fastest tile (ranked on the bin alone), when
+  // the bins launched one after the other beat one launch of all blocks by 3%. That comparison
+  // times the whole sequence: a bin timed alone keeps its smaller data in cache between repeats.
+  int filled = 0;
+  for (int b = 0; b < kTileBins; ++b) filled += M.bin_start[q][b + 1] > M.bin_start[q][b];
+  if (filled > 1) {
+    for (int b = 0; b < kTileBins; ++b) {
+      M.bin_tile[q][b] = Tile{};
+      const int first = M.bin_start[q][b], count = M.bin_start[q][b + 1] - first;
+      if (count > 0) best_tile(first, count, M.bin_tile[q][b]);
+    }
+    M.split[q] = true;
+    launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream);  // warm
     SMY_HIP(hipEventRecord(h->ev_start, h->stream));
-    for (int r = 0; r < kReps; ++r) launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream, &t);
+    for (int r = 0; r < kReps; ++r) launch_rows(M, avals<T>(M), M.list[q], M.nlist[q], op, h->stream);
     SMY_HIP(hipEventRecord(h->ev_stop, h->stream));
     SMY_HIP(hipEventSynchronize(h->ev_stop));
-    float ms = 0.0f;
-    SMY_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
-    if (&t == kTiles) {
-      base = best = ms;
-    } else if (ms < best && ms < 0.97f * base) {
-      best = ms;
-      pick = t;
-    }
+    float apart = 0.0f;
+    SMY_HIP(hipEventElapsedTime(&apart, h->ev_start, h->ev_stop));
+    M.split[q] = apart < 0.97f * whole;
   }
-  M.tile[q] = pick;
   M.tuned[q] = true;
 }
 
@@ -1404,6 +1439,35 @@ int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xc
   if (per_wg != 1 && per_wg != 2 && per_wg != 4) return fail(SMY_INVALID, "per_wg must be 1, 2 or 4");
   DevMatrix& M = matrix == 0 ? h->a : h->at;
   M.tile[precision] = Tile{per_wg, xcd != 0};
+  M.split[precision] = false;
+  M.tuned[precision] = true;
+  ++h->list_gen;
+  return SMY_OK;
+}
+
+int smy_pdhg_get_bin_tile(smy_pdhg* h, int matrix, int precision, int bin, int* per_wg, int* xcd,
+                          int* split) {
+  if (h == nullptr || per_wg == nullptr || xcd == nullptr || split == nullptr ||
+      (matrix != 0 && matrix != 1) || (precision != SMY_FP64 && precision != SMY_FP32) || bin < 0 ||
+      bin >= kTileBins)
+    return fail(SMY_INVALID, "bad argument");
+  if (h->shared) return fail(SMY_INVALID, "the shared-matrix batch has no product tiles");
+  const DevMatrix& M = matrix == 0 ? h->a : h->at;
+  *per_wg = M.bin_tile[precision][bin].per_wg;
+  *xcd = M.bin_tile[precision][bin].xcd ? 1 : 0;
+  *split = M.split[precision] ? 1 : 0;
+  return M.tuned[precision] ? SMY_OK : SMY_STATE;
+}
+
+int smy_pdhg_set_bin_tile(smy_pdhg* h, int matrix, int precision, int bin, int per_wg, int xcd) {
+  if (h == nullptr || (matrix != 0 && matrix != 1) || (precision != SMY_FP64 && precision != SMY_FP32) ||
+      bin < 0 || bin >= kTileBins)
+    return fail(SMY_INVALID, "bad argument");
+  if (h->shared) return fail(SMY_INVALID, "the shared-matrix batch has no product tiles");
+  if (per_wg != 1 && per_wg != 2 && per_wg != 4) return fail(SMY_INVALID, "per_wg must be 1, 2 or 4");
+  DevMatrix& M = matrix == 0 ? h->a : h->at;
+  M.bin_tile[precision][bin] = Tile{per_wg, xcd != 0};
+  M.split[precision] = true;
   M.tuned[precision] = true;
   ++h->list_gen;
   return SMY_OK;

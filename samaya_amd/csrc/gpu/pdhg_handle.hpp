@@ -59,6 +59,11 @@ struct Tile {
   bool xcd = false;
 };
 
+// Size bins of a stacked batch's instances, by their row blocks in a matrix (about 1,024
+// nonzeros each): < 64, < 512, the rest. A block list holds the bins one after the other.
+constexpr int kTileBins = SMY_TILE_BINS;
+inline int tile_bin(int blocks) { return blocks < 64 ? 0 : blocks < 512 ? 1 : 2; }
+
 struct DevMatrix {
   int rows = 0;
   long long nnz = 0;
@@ -73,6 +78,11 @@ struct DevMatrix {
   int nlist[2] = {0, 0};
   Tile tile[2];                 // by value precision (SMY_FP64, SMY_FP32): chosen by the tuner
   bool tuned[2] = {false, false};
+  // Per size bin: where each bin's blocks start in list[q] (bin_start[q][kTileBins] = nlist[q]),
+  // and, when `split`, the launches of list[q] go bin by bin, each with its own tile.
+  int bin_start[2][kTileBins + 1] = {};
+  Tile bin_tile[2][kTileBins];
+  bool split[2] = {false, false};
   // One instance: launches pass no block-instance table, so a workgroup knows its instance (0)
   // without a dependent load and reads the instance's step state in parallel with its schedule.
   bool single = false;
@@ -113,12 +123,22 @@ constexpr int prec_of() {
   return std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
 }
 
-// Returns the grid used (0: nothing launched). The tile is the matrix's for values of type V.
+// Returns the grid used (0: nothing launched). The tile is the matrix's for values of type V; the
+// blocks of list[q] with values of precision q go bin by bin when the tuner split them.
 template <typename V, typename Op>
 int launch_rows(const DevMatrix& M, const V* val, const int* list, int nlist, const Op& op,
                 hipStream_t s, const Tile* force = nullptr) {
   if (nlist == 0) return 0;
-  const Tile t = force != nullptr ? *force : M.tile[prec_of<V>()];
+  constexpr int q = prec_of<V>();
+  if (force == nullptr && M.split[q] && list == M.list[q] && nlist == M.nlist[q]) {
+    int grid = 0;
+    for (int b = 0; b < kTileBins; ++b) {
+      const int first = M.bin_start[q][b], count = M.bin_start[q][b + 1] - first;
+      if (count > 0) grid += launch_rows(M, val, list + first, count, op, s, &M.bin_tile[q][b]);
+    }
+    return grid;
+  }
+  const Tile t = force != nullptr ? *force : M.tile[q];
   CsrRows<V> A;
   A.rows = M.rows;
   A.nblocks = M.nblocks;

@@ -214,6 +214,23 @@ class PdhgBatch:
         q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
         N.check(self.lib.smy_pdhg_set_tile(self.h, matrix, q, per_wg, int(xcd)), "set_tile")
 
+    def bin_tile(self, matrix: int, precision: str, size_bin: int):
+        """(per_wg, xcd, split) of one size bin of A (0) or A' (1); None before tuning."""
+        pw, x, sp = C.c_int(), C.c_int(), C.c_int()
+        q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
+        rc = self.lib.smy_pdhg_get_bin_tile(self.h, matrix, q, size_bin, C.byref(pw), C.byref(x),
+                                            C.byref(sp))
+        if rc == N.SMY_STATE:
+            return None
+        N.check(rc, "get_bin_tile")
+        return pw.value, bool(x.value), bool(sp.value)
+
+    def set_bin_tile(self, matrix: int, precision: str, size_bin: int, per_wg: int, xcd: bool):
+        """Fixes one size bin's tile; the bins are then launched apart."""
+        q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
+        N.check(self.lib.smy_pdhg_set_bin_tile(self.h, matrix, q, size_bin, per_wg, int(xcd)),
+                "set_bin_tile")
+
     @staticmethod
     def adaptive_state(eta, omega, weight=0.0, attempts=0, rejected=0):
         """A state array for run_adaptive (per-instance sequences or scalars)."""

@@ -1,8 +1,9 @@
 """Product tiles and their tuner: every tile (1, 2 or 4 row blocks per workgroup, blocks in
 launch order or one contiguous range per XCD) gives every sum bit for bit, so the tuner's timing
 choice never changes an answer; the tuner picks one of the tiles at the first run of a precision
-(timing them on a matrix of at least 2,048 row blocks, the plain tile below that); and the solver
-answers equal with and without tuning."""
+(timing them on a matrix of at least 2,048 row blocks, the plain tile below that), and one per size
+bin of a batch's instances, launched apart when that is faster; and the solver answers equal with
+and without tuning."""
 import itertools
 
 import numpy as np
@@ -88,6 +89,54 @@ def test_tuner_times_the_tiles_of_a_large_matrix(gpu):
     small.run([0.2], [0.3], 1.0, [0], 64)
     for matrix in (0, 1):
         assert small.tile(matrix, "fp32") == (1, False)
+
+
+def binned_models():
+    # Instances in all three size bins of both matrices: < 64 row blocks, < 512 (120k nonzeros),
+    # and more (2.4M nonzeros, so that the batch has enough blocks for the tuner to time tiles).
+    return models()[:3] + [G.sparse_uniform(165, 20000, 40000, 3), G.sparse_uniform(166, 100000, 200000, 12)]
+
+
+def test_size_bins_launched_apart_give_the_same_bits(gpu):
+    inputs = [_device_lp(md, 170 + i) for i, md in enumerate(binned_models())]
+    K = len(inputs)
+    results = []
+    for apart in (False, True):
+        b = make_batch(inputs, "fp32")
+        for matrix in (0, 1):
+            b.set_tile(matrix, "fp32", 1, False)
+            if apart:
+                for size_bin, (pw, xcd) in enumerate([(4, True), (2, False), (1, True)]):
+                    b.set_bin_tile(matrix, "fp32", size_bin, pw, xcd)
+                assert b.bin_tile(matrix, "fp32", 0) == (4, True, True)
+                assert b.bin_tile(matrix, "fp32", 2) == (1, True, True)
+            else:
+                assert b.bin_tile(matrix, "fp32", 1)[2] is False
+        b.run([0.2] * K, [0.3] * K, 1.0, [0] * K, 64)  # a captured run
+        b.advance()
+        b.run([0.2] * K, [0.3] * K, 1.0, [64] * K, 3)  # single launches
+        st = D.PdhgBatch.adaptive_state([0.1] * K, 1.0)
+        b.advance()
+        b.run_adaptive(st, 7)
+        results.append((b.evaluate(), [b.download(i) for i in range(K)], [(s.eta, s.rejected) for s in st]))
+    (s0, g0, a0), (s1, g1, a1) = results
+    np.testing.assert_array_equal(s1, s0)
+    assert a1 == a0
+    for x, y in zip(g1, g0):
+        for key in x:
+            np.testing.assert_array_equal(x[key], y[key], err_msg=key)
+
+
+def test_tuner_picks_a_tile_per_size_bin(gpu):
+    inputs = [_device_lp(md, 180 + i) for i, md in enumerate(binned_models())]
+    K = len(inputs)
+    b = make_batch(inputs, "fp32")
+    b.run([0.2] * K, [0.3] * K, 1.0, [0] * K, 64)
+    for matrix in (0, 1):
+        assert b.tile(matrix, "fp32") in TILES
+        for size_bin in range(3):
+            pw, xcd, split = b.bin_tile(matrix, "fp32", size_bin)
+            assert (pw, xcd) in TILES
 
 
 def test_solver_answers_do_not_depend_on_the_tile(gpu):
