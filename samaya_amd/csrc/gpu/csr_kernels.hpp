@@ -611,36 +611,45 @@ struct AdaptDualOp {
   }
 };
 
-// One 1024-thread workgroup per instance: the attempt's totals from the block partials (the
-// instance's A blocks hold slots 0-2, its A' blocks slot 0), then the rule. It runs after every
-// attempt, so the partials (tens of thousands for a large LP) are read with several loads in
-// flight per thread; each thread still adds its entries in index order, so the totals are fixed.
-// Instances that do not step (frozen) keep their state.
+// The attempt's totals from the block partials (the instance's A blocks hold slots 0-2, its A'
+// blocks slot 0), then the rule. It runs after every attempt, so the partials (tens of thousands
+// for a large LP) are spread over `nslices` 1024-thread workgroups per instance (grid: nslices x
+// instances), each thread with many loads in flight. Thread t of slice r adds entries
+// r 1024 + t, then every nslices 1024 further, in order. With one slice the workgroup applies the
+// rule itself. With more, each writes its slice totals and the last to arrive adds the slices in
+// slice order, applies the rule and sets the arrival count back to 0 for the next attempt; the
+// arrival needs a device-scope fence, which writes back the XCD's L2, so the host asks for more
+// slices only for an instance of many blocks. The totals are fixed either way for a given
+// nslices. Instances that do not step (frozen) keep their state.
 constexpr int kDecideThreads = 1024;
 constexpr int kDecideUnroll = 12;
+constexpr int kDecideSlices = 4;            // at most
+constexpr int kDecideBlocksPerSlice = 4096;  // partials per slice the host aims for
 
 static __global__ __launch_bounds__(kDecideThreads) void adapt_decide_kernel(
     int stride, const double* __restrict__ partial, const int2* __restrict__ ranges,
-    const int* __restrict__ prec, AdaptState* __restrict__ st) {
+    const int* __restrict__ prec, AdaptState* __restrict__ st, int nslices,
+    double* __restrict__ slices, unsigned* __restrict__ arrivals) {
   constexpr int W = kDecideThreads / kWave;
+  const int step = nslices * kDecideThreads;
   __shared__ double wsum[3][W];
-  const int i = blockIdx.x;
-  if (prec[i] < 0) return;  // frozen
+  __shared__ int last;
+  const int slice = blockIdx.x, i = blockIdx.y;
+  if (prec[i] < 0) return;  // frozen (the same for every slice of the instance)
   const int tid = threadIdx.x;
   const int2 ra = ranges[2 * i];
   const int2 rt = ranges[2 * i + 1];
-  // Both ranges in one loop, so every round has the loads of both in flight; each thread still
-  // adds entries tid, tid + 1024, ... of each range in that order.
+  // Both ranges in one loop, so every round has the loads of both in flight.
   const int len_a = ra.y - ra.x, len_t = rt.y - rt.x;
   const int span = max(len_a, len_t);
   double acc[3] = {0.0, 0.0, 0.0};
   double at = 0.0;
-  for (int o = tid; o < span; o += kDecideUnroll * kDecideThreads) {
+  for (int o = slice * kDecideThreads + tid; o < span; o += kDecideUnroll * step) {
     double v[kDecideUnroll][3];
     double w[kDecideUnroll];
 #pragma unroll
     for (int u = 0; u < kDecideUnroll; ++u) {
-      const int b = o + u * kDecideThreads;
+      const int b = o + u * step;
 #pragma unroll
       for (int q = 0; q < 3; ++q)
         v[u][q] = b < len_a ? partial[static_cast<size_t>(q) * stride + ra.x + b] : 0.0;
@@ -660,12 +669,33 @@ static __global__ __launch_bounds__(kDecideThreads) void adapt_decide_kernel(
     if ((tid & (kWave - 1)) == 0) wsum[q][tid / kWave] = t;
   }
   __syncthreads();
-  if (tid != 0) return;
-  double tot[3];
+  double tot[3] = {0.0, 0.0, 0.0};
+  if (nslices == 1) {
+    if (tid != 0) return;
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    tot[q] = 0.0;
-    for (int w = 0; w < W; ++w) tot[q] += wsum[q][w];
+    for (int q = 0; q < 3; ++q)
+      for (int w = 0; w < W; ++w) tot[q] += wsum[q][w];
+  } else {
+    if (tid == 0) {
+      double* mine = slices + (static_cast<size_t>(i) * kDecideSlices + slice) * 3;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double t = 0.0;
+        for (int w = 0; w < W; ++w) t += wsum[q][w];
+        mine[q] = t;
+      }
+      __threadfence();  // this slice's totals are visible before its arrival is
+      last = atomicAdd(arrivals + i, 1u) == static_cast<unsigned>(nslices - 1);
+    }
+    __syncthreads();
+    if (tid != 0 || !last) return;
+    __threadfence();
+    for (int r = 0; r < nslices; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        tot[q] += __hip_atomic_load(slices + (static_cast<size_t>(i) * kDecideSlices + r) * 3 + q,
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    arrivals[i] = 0;
   }
   AdaptState s = st[i];
   const double movement = 0.5 * s.omega * tot[0] + 0.5 / s.omega * tot[1];

@@ -427,6 +427,13 @@ void ensure_adaptive(smy_pdhg* h) {
   if (h->adapt == nullptr) {
     h->adapt = h->alloc<AdaptState>(h->K);
     h->ax_flags = h->alloc<int>(h->K);
+    h->decide_slices = h->alloc<double>(3LL * kDecideSlices * h->K);
+    h->decide_arrivals = h->zeros<unsigned>(h->K);
+    // Slices of the decision: about kDecideBlocksPerSlice partials each for the largest instance.
+    int most = 0;
+    for (const Layout& L : h->inst) most = std::max({most, L.ba1 - L.ba0, L.bt1 - L.bt0});
+    h->decide_nslices =
+        std::clamp((most + kDecideBlocksPerSlice - 1) / kDecideBlocksPerSlice, 1, kDecideSlices);
     SMY_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->adapt_host), sizeof(AdaptState) * h->K,
                           hipHostMallocDefault));
   }
@@ -492,8 +499,9 @@ void adapt_dual(smy_pdhg* h) {
 }
 
 void adapt_decide(smy_pdhg* h) {
-  hipLaunchKernelGGL(adapt_decide_kernel, dim3(h->K), dim3(kDecideThreads), 0, h->stream, stride(h),
-                     h->partial, h->ranges, h->step_prec, h->adapt);
+  hipLaunchKernelGGL(adapt_decide_kernel, dim3(h->decide_nslices, h->K), dim3(kDecideThreads), 0,
+                     h->stream, stride(h), h->partial, h->ranges, h->step_prec, h->adapt,
+                     h->decide_nslices, h->decide_slices, h->decide_arrivals);
 }
 
 void adapt_attempt(smy_pdhg* h) {

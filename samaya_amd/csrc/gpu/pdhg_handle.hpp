@@ -214,6 +214,9 @@ struct smy_pdhg {
   AdaptState* adapt_host = nullptr;
   std::vector<int> ax_stale;  // per instance: A x must be formed again before its next attempt
   int* ax_flags = nullptr;    // device copy for the refresh launch
+  double* decide_slices = nullptr;     // per instance and slice: the decision's 3 partial totals
+  unsigned* decide_arrivals = nullptr;  // per instance: slices done (0 between attempts)
+  int decide_nslices = 1;               // slices per instance of the decision kernel
   // Restart rule on the device (allocated by smy_pdhg_rule_init).
   RuleState* rule = nullptr;
   RuleState* rule_host = nullptr;    // pinned

@@ -127,6 +127,35 @@ def test_attempts_match_the_rule_restated(gpu):
         assert close(r.v_erg, w @ vs / w.sum(), 1e-12)
 
 
+def test_decision_over_several_workgroups_matches_the_rule(gpu):
+    # An instance of over 4,096 row blocks (4.5M nonzeros) has its decision reduced by several
+    # workgroups, the last of which applies the rule; a small instance shares that launch. Both
+    # follow the rule restated with NumPy, attempt by attempt.
+    mds = [G.sparse_uniform(190, 150000, 300000, 15), G.sparse_uniform(191, 60, 150, 4)]
+    inputs = [_device_lp(md, 192 + i) for i, md in enumerate(mds)]
+    batch = make_batch(inputs)
+    eta0 = []
+    for _, eq, _, _ in inputs:
+        K = k_matrix(eq)
+        x = np.ones(K.shape[1])
+        for _ in range(30):
+            x = K.T @ (K @ x)
+            x /= np.linalg.norm(x)
+        eta0.append(3.0 / np.sqrt(np.linalg.norm(K.T @ (K @ x))))  # above 1/||K||
+    rules = [Rule(eq, eta0[i], 1.0) for i, (_, eq, _, _) in enumerate(inputs)]
+    st = D.PdhgBatch.adaptive_state(eta0, 1.0)
+    batch.run_adaptive(st, 8)
+    for _ in range(8):
+        for r in rules:
+            r.attempt()
+    for i, r in enumerate(rules):
+        got = batch.download(i)
+        assert st[i].attempts == r.total and st[i].rejected == r.rejected, (i, st[i].rejected, r.rejected)
+        assert abs(st[i].eta - r.eta) <= 1e-9 * r.eta
+        assert close(got["v_new"], r.v, 1e-9) and close(got["y_new"], r.y, 1e-9)
+        assert close(got["v_erg"], r.v_erg, 1e-9) and close(got["y_erg"], r.y_erg, 1e-9)
+
+
 def test_steps_within_the_norm_bound_are_never_rejected(gpu):
     # |dy' K dv| <= ||K|| ||dv|| ||dy|| <= ||K|| movement, so the limit is >= 1 / ||K||: with the
     # growth factor made negligible by a huge attempt count, eta stays below 1 / ||K|| and every
