@@ -234,7 +234,7 @@ int smy_pdhg_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter
  * instances fall into SMY_TILE_BINS bins by their row blocks in the matrix (about 1,024
  * nonzeros per block): bin 0 under 64 blocks, bin 1 under 512, bin 2 the rest. The tuner also
  * times each tile on each bin's blocks alone, and launches the bins apart, each with its fastest
- * tile, when that beats one launch of all blocks by 3%. get_bin_tile reports bin `bin`'s tile
+ * tile, when that beats one launch of all blocks by 5%. get_bin_tile reports bin `bin`'s tile
  * and whether the bins are launched apart (*split); set_bin_tile fixes one bin's tile and turns
  * the split on (the other bins keep theirs, the plain tile unless set). No tile or split changes
  * a result. */

@@ -230,7 +230,7 @@ void step_kernels(smy_pdhg* h, int j) {
 // The first run in a precision times the candidate tiles of both matrices with the values of that
 // precision (over the blocks of the instances stepping in it) on a stand-in for the step kernels
 // (TuneOp: the product plus an epilogue's reads and writes, into scratch), and keeps the fastest;
-// the plain one-block-per-workgroup tile unless another is at least 3% faster. Every tile sums
+// the plain one-block-per-workgroup tile unless another is at least 5% faster. Every tile sums
 // every row in the same order, so the choice changes speed only: answers do not depend on the
 // timing.
 constexpr Tile kTiles[] = {{1, false}, {1, true}, {2, false}, {2, true}, {4, false}, {4, true}};
@@ -253,9 +253,13 @@ void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5])
   op.x = x;
   for (int k = 0; k < 5; ++k) op.in[k] = in[k];
   for (int k = 0; k < 3; ++k) op.out[k] = reinterpret_cast<T*>(h->tune_out) + k * rows;
-  constexpr int kReps = 3;
+  // 5 launches per tile, and a tile must be 5% faster than the plain one (or the bins than one
+  // launch): closer calls are within the timing noise of this stand-in, and a pick by noise would
+  // let two handles of the same LP launch differently.
+  constexpr int kReps = 5;
+  constexpr float kMargin = 0.95f;
   // The fastest tile on blocks [first, first + count) of the list: the plain one unless another
-  // is 3% faster. Returns its time (ms for kReps launches).
+  // is faster by the margin. Returns its time (ms for kReps launches).
   const auto best_tile = [&](int first, int count, Tile& pick) {
     float base = 0.0f, best = 0.0f;
     pick = Tile{};
@@ -269,7 +273,7 @@ void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5])
       SMY_HIP(hipEventElapsedTime(&ms, h->ev_start, h->ev_stop));
       if (&t == kTiles) {
         base = best = ms;
-      } else if (ms < best && ms < 0.97f * base) {
+      } else if (ms < best && ms < kMargin * base) {
         best = ms;
         pick = t;
       }
@@ -281,7 +285,7 @@ void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5])
   M.tile[q] = pick;
   M.split[q] = false;
   // Size bins launched apart, each with its own fastest tile (ranked on the bin alone), when
-  // the bins launched one after the other beat one launch of all blocks by 3%. That comparison
+  // the bins launched one after the other beat one launch of all blocks by the margin. That comparison
   // times the whole sequence: a bin timed alone keeps its smaller data in cache between repeats.
   int filled = 0;
   for (int b = 0; b < kTileBins; ++b) filled += M.bin_start[q][b + 1] > M.bin_start[q][b];
@@ -299,7 +303,7 @@ void tune_matrix(smy_pdhg* h, DevMatrix& M, const T* x, const T* const (&in)[5])
     SMY_HIP(hipEventSynchronize(h->ev_stop));
     float apart = 0.0f;
     SMY_HIP(hipEventElapsedTime(&apart, h->ev_start, h->ev_stop));
-    M.split[q] = apart < 0.97f * whole;
+    M.split[q] = apart < kMargin * whole;
   }
   M.tuned[q] = true;
 }
