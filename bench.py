@@ -143,9 +143,10 @@ def spmv_sweep(md) -> dict:
     return out
 
 
-def kernel_roofline(md, precision: str, adaptive: bool = False) -> dict:
+def kernel_roofline(md, precision: str, adaptive: bool = False, acc64: bool = False) -> dict:
     """Live per-kernel timing of the two step kernels on the workload's matrix (the adaptive
-    attempt's kernels when `adaptive`: same byte model, plus the per-attempt decision kernel)."""
+    attempt's kernels when `adaptive`: same byte model, plus the per-attempt decision kernel;
+    `acc64`: the FP32 kernels that accumulate in FP64)."""
     import numpy as np
     import scipy.sparse as sp
 
@@ -161,6 +162,8 @@ def kernel_roofline(md, precision: str, adaptive: bool = False) -> dict:
     dev = PdhgDevice.create((md.m, md.n, md.start, md.index, md.value),
                             (md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data),
                             cost, lower, upper, "fp32" if precision == "fp32" else "fp64")
+    if acc64:
+        dev.set_fp64_accumulation(True)
     decide_ms = None
     if adaptive:
         dual_ms, primal_ms, decide_ms = dev.time_adaptive_kernels(50)
@@ -224,6 +227,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
+    ap.add_argument("--fp64-accumulation", action="store_true",
+                    help="mixed precision: FP32 steps sum their products in FP64 (the default line "
+                         "is specified with FP32 accumulation)")
     ap.add_argument("--adaptive-steps", default="auto", choices=["auto", "on", "off"],
                     help="PDLP's adaptive step sizes for the mixed-precision solves (auto: on for "
                          "the single LP and the stacked batch; the shared-matrix batch has none)")
@@ -267,7 +273,8 @@ def main() -> int:
     nm = md.native()
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, detect=False,
                            verify=True, polish_rounds=0, max_iterations=args.max_iterations,
-                           fp32_switch_tol=args.fp32_switch_tol, adaptive_steps=args.adaptive)
+                           fp32_switch_tol=args.fp32_switch_tol, adaptive_steps=args.adaptive,
+                           fp64_accumulation=args.fp64_accumulation)
 
     for w in range(args.warmup):
         r = smy.solve_lp(nm, opts)
@@ -298,7 +305,7 @@ def main() -> int:
         fp64 = f
         log(f"fp64: {f.status_name} {f.iterations} it, iterate {f.solve_seconds - f.setup_seconds:.3f} s")
 
-    rl = kernel_roofline(md, "fp32", args.adaptive)
+    rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation)
     dom = "primal" if rl["primal_ms"] >= rl["dual_ms"] else "dual"
     dom_ms = rl[f"{dom}_ms"]
     dom_op = rl["ops"][dom]
@@ -326,6 +333,7 @@ def main() -> int:
         "dtype": "fp32 steps + fp64 evaluation/polish",
         "algorithm": args.algorithm,
         "adaptive_steps": args.adaptive,
+        "fp64_accumulation": args.fp64_accumulation,
         "status": last.status_name,
         "data": "synthetic (seeded generator)",
         "config": {"workload": "sparse LP m=500000 n=1000000 nnz~12M power-law columns, "
@@ -371,6 +379,7 @@ def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict
         "dtype": "fp32 steps + fp64 evaluation/polish",
         "algorithm": args.algorithm,
         "adaptive_steps": args.adaptive,
+        "fp64_accumulation": args.fp64_accumulation,
         "data": "synthetic (seeded generator)",
         "config": {"workload": workload},
     }
@@ -452,6 +461,8 @@ def run_c4(args) -> int:
             los.append(np.concatenate([md.col_lower, md.row_lower]))
             ups.append(np.concatenate([md.col_upper, md.row_upper]))
         dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
+        if args.fp64_accumulation:
+            dev.set_fp64_accumulation(True)
         out = batch_roofline(dev, nnz, M, N, None, args.adaptive,
                              traffic_of="c4" if args.scale == 1.0 else None)
         # The tuner's choice on this handle: a tile per size bin, and whether the bins go apart.
@@ -471,7 +482,7 @@ def run_c4(args) -> int:
         return 0
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
-                           adaptive_steps=args.adaptive)
+                           adaptive_steps=args.adaptive, fp64_accumulation=args.fp64_accumulation)
     for w in range(args.warmup):
         smy.solve_lp_batch(nms[:8], opts)
     runs = []
@@ -572,7 +583,7 @@ def run_c5(args) -> int:
         return 0
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
-                           adaptive_steps=args.adaptive)
+                           adaptive_steps=args.adaptive, fp64_accumulation=args.fp64_accumulation)
     root = smy.solve_lp(nm, opts)
     log(f"root: {root.status_name} {root.iterations} it, omega {root.omega:.3e}")
     start = (root.col_value, root.row_dual)

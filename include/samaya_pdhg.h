@@ -239,6 +239,12 @@ int smy_pdhg_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter
  * the split on (the other bins keep theirs, the plain tile unless set). No tile or split changes
  * a result. */
 int smy_pdhg_get_tile(smy_pdhg* h, int matrix, int precision, int* per_wg, int* xcd);
+/* FP32 steps with FP64 accumulation (no reference counterpart; the reference has no FP32 path).
+ * With on = 1, the FP32 step kernels (constant and adaptive) sum each product in FP64 and form
+ * the update that uses it in FP64, rounding once to FP32; values, iterates and every stored
+ * vector stay FP32. Off (0) by default: FP32 accumulation. FP64 steps are unchanged. Stacked
+ * batches only (SMY_INVALID for the shared-matrix batch). */
+int smy_pdhg_set_fp64_accumulation(smy_pdhg* h, int on);
 int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xcd);
 int smy_pdhg_get_bin_tile(smy_pdhg* h, int matrix, int precision, int bin, int* per_wg, int* xcd,
                           int* split);

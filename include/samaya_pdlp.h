@@ -103,6 +103,8 @@ typedef struct smy_pdlp_params {
                                 switches to FP64 (1e-4; 0: never) */
   int device_restart;        /* restart decisions, distances and primal weights on the device (1)
                                 or on the host from downloaded sums (0) */
+  int fp64_accumulation;     /* mixed: FP32 steps sum their products in FP64 (1) or FP32 (0);
+                                storage stays FP32; batches then use the stacked layout */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);

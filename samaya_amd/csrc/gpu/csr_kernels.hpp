@@ -334,9 +334,11 @@ struct DualPre {
 //   y_new_i = y_i + sigma (A vbar_x - vbar_{n+i})_i,  y_erg_i += alpha (y_new_i - y_erg_i)
 //   v_new_{n+i} = clamp(v_{n+i} - tau (cost_{n+i} - y_new_i)),  its average and next vbar.
 // (n: the number of structural variables of the whole stack; the logical variables follow them.)
-template <typename T>
+// A: the accumulator. A wider one (FP32 steps, FP64 accumulation) also forms the update that uses
+// the product in A and rounds it once to T; A = T is the plain step.
+template <typename T, typename A = T>
 struct DualRowsOp {
-  using Acc = T;
+  using Acc = A;
   using Pre = DualPre<T>;
   static constexpr int kVec = 1;
   static constexpr int kSums = 0;
@@ -367,7 +369,11 @@ struct DualRowsOp {
     const T theta = static_cast<T>(s.theta);
     const T alpha = static_cast<T>(1.0 / static_cast<double>(s.k0 + j + 1));
     const int k = n + i;
-    const T yn = p.y + sigma * (acc[0] - p.vbar);
+    T yn;
+    if constexpr (std::is_same_v<A, T>)
+      yn = p.y + sigma * (acc[0] - p.vbar);
+    else
+      yn = static_cast<T>(static_cast<A>(p.y) + static_cast<A>(sigma) * (acc[0] - static_cast<A>(p.vbar)));
     y_new[i] = yn;
     y_erg[i] = (T(1) - alpha) * p.y_erg + alpha * yn;
     const T grad = p.cost + (-yn);
@@ -385,9 +391,10 @@ struct PrimalPre {
 
 // Rows of A' (structural variables j < n): v_new_j = clamp(v_j - tau (cost_j + (A' y_new)_j)),
 // its running average and the next extrapolation vbar_j = v_new_j + theta (v_new_j - v_j).
-template <typename T>
+// A: the accumulator, as for DualRowsOp.
+template <typename T, typename A = T>
 struct PrimalColsOp {
-  using Acc = T;
+  using Acc = A;
   using Pre = PrimalPre<T>;
   static constexpr int kVec = 1;
   static constexpr int kSums = 0;
@@ -411,8 +418,14 @@ struct PrimalColsOp {
     const T tau = static_cast<T>(s.tau);
     const T theta = static_cast<T>(s.theta);
     const T alpha = static_cast<T>(1.0 / static_cast<double>(s.k0 + j + 1));
-    const T grad = p.cost + acc[0];
-    const T vn = clamp_to(p.v - tau * grad, p.lower, p.upper);
+    T vn;
+    if constexpr (std::is_same_v<A, T>) {
+      const T grad = p.cost + acc[0];
+      vn = clamp_to(p.v - tau * grad, p.lower, p.upper);
+    } else {
+      vn = clamp_to(static_cast<T>(static_cast<A>(p.v) - static_cast<A>(tau) * (static_cast<A>(p.cost) + acc[0])),
+                    p.lower, p.upper);
+    }
     v_new[col] = vn;
     v_erg[col] = (T(1) - alpha) * p.v_erg + alpha * vn;
     vbar_next[col] = vn + theta * (vn - p.v);
@@ -483,9 +496,9 @@ struct AdaptPrimalPre {
 
 // Rows of A' (structural variables): v+_j = clamp(v_j - tau (c_j + (A'y)_j)), the point
 // 2 v+ - v the dual kernel multiplies, the weighted average, and ||dx||^2 (partial slot 0).
-template <typename T>
+template <typename T, typename A = T>
 struct AdaptPrimalOp {
-  using Acc = T;
+  using Acc = A;  // a wider accumulator as for DualRowsOp
   using Pre = AdaptPrimalPre<T>;
   using Ctx = AdaptPrimalCtx<T>;
   static constexpr int kVec = 1;
@@ -517,7 +530,12 @@ struct AdaptPrimalOp {
   }
   __device__ Acc gather(int, int c, const Ctx& x) const { return x.y[c]; }
   __device__ void row(int col, const Ctx& x, const Acc* acc, const Pre& p, double* s) const {
-    const T vn = clamp_to(p.v - x.tau * (p.cost + acc[0]), p.lower, p.upper);
+    T vn;
+    if constexpr (std::is_same_v<A, T>)
+      vn = clamp_to(p.v - x.tau * (p.cost + acc[0]), p.lower, p.upper);
+    else
+      vn = clamp_to(static_cast<T>(static_cast<A>(p.v) - static_cast<A>(x.tau) * (static_cast<A>(p.cost) + acc[0])),
+                    p.lower, p.upper);
     x.v_out[col] = vn;
     xbar[col] = vn + (vn - p.v);
     x.e_out[col] = (T(1) - x.alpha) * p.v_erg + x.alpha * vn;
@@ -550,9 +568,9 @@ struct AdaptDualPre {
 // then y+_i = y_i + sigma ((A xbar)_i - (2 r+ - r)), with xbar = 2 x+ - x from the primal kernel.
 // A x+ = ((A xbar)_i + h_i) / 2 replaces h, and (A dx)_i = ((A xbar)_i - h_i) / 2 gives row i's
 // share of dy' K dv. Partial slots: 0 ||dr||^2, 1 ||dy||^2, 2 dy' K dv.
-template <typename T>
+template <typename T, typename A = T>
 struct AdaptDualOp {
-  using Acc = T;
+  using Acc = A;  // a wider accumulator as for DualRowsOp
   using Pre = AdaptDualPre<T>;
   using Ctx = AdaptDualCtx<T>;
   static constexpr int kVec = 1;
@@ -596,11 +614,19 @@ struct AdaptDualOp {
     const int k = n + i;
     const T rn = clamp_to(p.r - x.tau * (p.cost - p.y), p.lower, p.upper);
     const T rbar = rn + (rn - p.r);
-    const T yn = p.y + x.sigma * (acc[0] - rbar);
-    const T adx = (acc[0] - p.h) * T(0.5);
+    T yn, adx, hn;
+    if constexpr (std::is_same_v<A, T>) {
+      yn = p.y + x.sigma * (acc[0] - rbar);
+      adx = (acc[0] - p.h) * T(0.5);
+      hn = (acc[0] + p.h) * T(0.5);
+    } else {
+      yn = static_cast<T>(static_cast<A>(p.y) + static_cast<A>(x.sigma) * (acc[0] - static_cast<A>(rbar)));
+      adx = static_cast<T>((acc[0] - static_cast<A>(p.h)) * A(0.5));
+      hn = static_cast<T>((acc[0] + static_cast<A>(p.h)) * A(0.5));
+    }
     x.y_out[i] = yn;
     x.v_out[k] = rn;
-    x.h_out[i] = (acc[0] + p.h) * T(0.5);
+    x.h_out[i] = hn;
     x.ye_out[i] = (T(1) - x.alpha) * p.y_erg + x.alpha * yn;
     x.ve_out[k] = (T(1) - x.alpha) * p.r_erg + x.alpha * rn;
     const double dr = static_cast<double>(rn) - static_cast<double>(p.r);

@@ -191,12 +191,13 @@ bool uses(const smy_pdhg* h, int q) {
   return false;
 }
 
-template <typename T>
+// A: the products' accumulator (T, or double for FP32 steps with FP64 accumulation).
+template <typename T, typename A = T>
 void step_kernels(smy_pdhg* h, int j) {
   constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
   const int p = h->p;
-  DualRowsOp<T> dual;
+  DualRowsOp<T, A> dual;
   dual.n = h->n;
   dual.j = j;
   dual.sc = h->scalars;
@@ -212,7 +213,7 @@ void step_kernels(smy_pdhg* h, int j) {
   dual.lower = S.lower;
   dual.upper = S.upper;
   launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], dual, h->stream);
-  PrimalColsOp<T> primal;
+  PrimalColsOp<T, A> primal;
   primal.j = j;
   primal.sc = h->scalars;
   primal.y_new = S.y[1 - p];
@@ -328,7 +329,12 @@ void tune(smy_pdhg* h) {
 }
 
 void step_all(smy_pdhg* h, int j) {
-  if (uses(h, SMY_FP32)) step_kernels<float>(h, j);
+  if (uses(h, SMY_FP32)) {
+    if (h->acc64)
+      step_kernels<float, double>(h, j);
+    else
+      step_kernels<float>(h, j);
+  }
   if (uses(h, SMY_FP64)) step_kernels<double>(h, j);
 }
 
@@ -455,11 +461,11 @@ void refresh_ax(smy_pdhg* h) {
   launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], op, h->stream);
 }
 
-template <typename T>
+template <typename T, typename A = T>
 void adapt_primal(smy_pdhg* h) {
   constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
-  AdaptPrimalOp<T> primal;
+  AdaptPrimalOp<T, A> primal;
   primal.st = h->adapt;
   primal.y0 = S.y[0];
   primal.y1 = S.y[1];
@@ -476,11 +482,11 @@ void adapt_primal(smy_pdhg* h) {
   launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
 }
 
-template <typename T>
+template <typename T, typename A = T>
 void adapt_dual(smy_pdhg* h) {
   constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
-  AdaptDualOp<T> dual;
+  AdaptDualOp<T, A> dual;
   dual.n = h->n;
   dual.st = h->adapt;
   dual.xbar = S.vbar[0];
@@ -510,8 +516,13 @@ void adapt_decide(smy_pdhg* h) {
 
 void adapt_attempt(smy_pdhg* h) {
   if (uses(h, SMY_FP32)) {
-    adapt_primal<float>(h);
-    adapt_dual<float>(h);
+    if (h->acc64) {
+      adapt_primal<float, double>(h);
+      adapt_dual<float, double>(h);
+    } else {
+      adapt_primal<float>(h);
+      adapt_dual<float>(h);
+    }
   }
   if (uses(h, SMY_FP64)) {
     adapt_primal<double>(h);
@@ -533,7 +544,7 @@ void refresh_stale_ax(smy_pdhg* h, int mask) {
 
 // Measurement helper: average milliseconds of the adaptive primal (rows of A'), dual (rows of A)
 // and decision kernels of the first precision in use, from a small step size. Changes the state.
-template <typename T>
+template <typename T, typename A = T>
 int time_adaptive_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms, double* decide_ms) {
   ensure_adaptive(h);
   staging_wait(h);
@@ -543,8 +554,8 @@ int time_adaptive_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms
   staging_sent(h);
   std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
   refresh_stale_ax(h, prec_of<T>() == SMY_FP32 ? 2 : 1);
-  *primal_ms = time_repeated(h, reps, [&] { adapt_primal<T>(h); });
-  *dual_ms = time_repeated(h, reps, [&] { adapt_dual<T>(h); });
+  *primal_ms = time_repeated(h, reps, [&] { adapt_primal<T, A>(h); });
+  *dual_ms = time_repeated(h, reps, [&] { adapt_dual<T, A>(h); });
   *decide_ms = time_repeated(h, reps, [&] { adapt_decide(h); });
   SMY_HIP(hipGetLastError());
   std::fill(h->ax_stale.begin(), h->ax_stale.end(), 1);
@@ -644,7 +655,7 @@ int run_adaptive_impl(smy_pdhg* h, smy_pdhg_adaptive* st, int count) {
 // FP64) over `reps` back-to-back launches (HIP events on the handle's stream). The launches repeat
 // one step from the current state, so the iterates and averages change: a measurement helper,
 // not part of a solve.
-template <typename T>
+template <typename T, typename A = T>
 int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
@@ -652,7 +663,7 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   std::vector<double> tau(h->K, 0.5), sigma(h->K, 0.5);
   std::vector<long long> k0(h->K, 0);
   set_scalars(h, tau.data(), sigma.data(), 1.0, k0.data());
-  DualRowsOp<T> dual;
+  DualRowsOp<T, A> dual;
   dual.n = h->n;
   dual.j = 0;
   dual.sc = h->scalars;
@@ -667,7 +678,7 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   dual.cost = S.cost;
   dual.lower = S.lower;
   dual.upper = S.upper;
-  PrimalColsOp<T> primal;
+  PrimalColsOp<T, A> primal;
   primal.j = 0;
   primal.sc = h->scalars;
   primal.y_new = S.y[1 - p];
@@ -1401,7 +1412,9 @@ int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal
   return guarded([&]() -> int {
     refresh_lists(h);
     tune(h);
-    if (uses(h, SMY_FP32)) return time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
+    if (uses(h, SMY_FP32))
+      return h->acc64 ? time_kernels_impl<float, double>(h, reps, dual_ms, primal_ms)
+                      : time_kernels_impl<float>(h, reps, dual_ms, primal_ms);
     if (uses(h, SMY_FP64)) return time_kernels_impl<double>(h, reps, dual_ms, primal_ms);
     return fail(SMY_STATE, "no instance steps");
   });
@@ -1415,7 +1428,9 @@ int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, doubl
   return guarded([&]() -> int {
     refresh_lists(h);
     tune(h);
-    if (uses(h, SMY_FP32)) return time_adaptive_impl<float>(h, reps, dual_ms, primal_ms, decide_ms);
+    if (uses(h, SMY_FP32))
+      return h->acc64 ? time_adaptive_impl<float, double>(h, reps, dual_ms, primal_ms, decide_ms)
+                      : time_adaptive_impl<float>(h, reps, dual_ms, primal_ms, decide_ms);
     if (uses(h, SMY_FP64)) return time_adaptive_impl<double>(h, reps, dual_ms, primal_ms, decide_ms);
     return fail(SMY_STATE, "no instance steps");
   });
@@ -1454,6 +1469,16 @@ int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xc
   M.split[precision] = false;
   M.tuned[precision] = true;
   ++h->list_gen;
+  return SMY_OK;
+}
+
+int smy_pdhg_set_fp64_accumulation(smy_pdhg* h, int on) {
+  if (h == nullptr) return fail(SMY_INVALID, "null handle");
+  if (h->shared && on) return fail(SMY_INVALID, "the shared-matrix batch accumulates in its step precision");
+  if (h->acc64 != (on != 0)) {
+    h->acc64 = on != 0;
+    ++h->list_gen;  // graphs captured before hold the other kernels
+  }
   return SMY_OK;
 }
 

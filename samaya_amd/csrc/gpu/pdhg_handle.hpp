@@ -223,6 +223,8 @@ struct smy_pdhg {
   RuleInput* rule_in = nullptr;
   RuleInput* rule_in_host = nullptr;  // pinned staging
   double* rule_dist = nullptr;        // per instance and segment: ||dv||^2, ||dy||^2
+  // FP32 steps accumulate their products in FP64 (smy_pdhg_set_fp64_accumulation; stacked only).
+  bool acc64 = false;
   // Tile tuner: on by default; its scratch output (max(m, n) doubles, at first use).
   bool autotune = true;
   double* tune_out = nullptr;

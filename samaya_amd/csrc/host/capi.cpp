@@ -52,6 +52,7 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.device_detect = p->device_detect != 0;
   o.start_omega = p->start_omega;
   o.adaptive_steps = p->adaptive_steps != 0;
+  o.fp64_accumulation = p->fp64_accumulation != 0;
   o.fp32_stall_level = p->fp32_stall_level;
   o.device_restart = p->device_restart != 0;
   return o;
@@ -135,6 +136,7 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->device_detect = 1;
   p->start_omega = 0.0;
   p->adaptive_steps = 0;
+  p->fp64_accumulation = 0;
   p->fp32_stall_level = 1e-4;
   p->device_restart = 1;
 }

@@ -108,6 +108,7 @@ class Device {
   Device& operator=(const Device&) = delete;
 
   int size() const { return k_; }
+  void set_fp64_accumulation(bool on) { check(smy_pdhg_set_fp64_accumulation(h_, on ? 1 : 0)); }
   void set_original(int i, const std::vector<double>& cs, const std::vector<double>& rs,
                     const std::vector<double>& cost, const std::vector<double>& lower,
                     const std::vector<double>& upper) {
@@ -701,7 +702,8 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
 
   // One problem for all lanes, with one set of row bounds: the shared-matrix batch (iterates
   // stored instance-minor, SpMM products); otherwise the stacked batch.
-  bool share = opt.share_matrix && !opt.adaptive_steps && K > 1 && owners.size() == 1;
+  bool share = opt.share_matrix && !opt.adaptive_steps && !opt.fp64_accumulation && K > 1 &&
+               owners.size() == 1;
   for (int i = 1; share && i < K; ++i) {
     const int n = lanes[0].lp_.n;
     share = std::equal(lanes[i].lower.begin() + n, lanes[i].lower.end(), lanes[0].lower.begin() + n) &&
@@ -725,6 +727,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
                    : Device::create(A, At, c, l, u, mixed ? SMY_FP32 : SMY_FP64);
     if (device) {
       if (share) lanes[0].log(2, ": %d LPs share one matrix: shared-matrix batch", K);
+      if (opt.fp64_accumulation) device->set_fp64_accumulation(true);
       for (int i = 0; i < K; ++i) {
         Lane& L = lanes[i];
         device->set_original(i, L.eq->cs, L.eq->rs, L.lp_.cost, L.lo(), L.up());

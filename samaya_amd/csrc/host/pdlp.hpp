@@ -55,6 +55,9 @@ struct PdlpOptions {
   // PDLP's adaptive step sizes (decided on the device) instead of the constant 0.99 / L; the
   // averages are then weighted by step size. Uses the stacked layout for every batch.
   bool adaptive_steps = false;
+  // Mixed precision: FP32 steps sum their products (and form the update that uses them) in FP64;
+  // storage stays FP32. Uses the stacked layout for every batch.
+  bool fp64_accumulation = false;
   bool use_graph = true;          // replay each 64-step run from a HIP graph
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
   bool device_detect = true;      // ... computed on the device (false: host, from downloads)

@@ -83,6 +83,10 @@ class PdhgDevice:
                 "time_adaptive_kernels")
         return d.value, p.value, q.value
 
+    def set_fp64_accumulation(self, on: bool):
+        """FP32 steps sum their products in FP64 (storage stays FP32)."""
+        N.check(self.lib.smy_pdhg_set_fp64_accumulation(self.h, int(on)), "set_fp64_accumulation")
+
     def restart(self):
         N.check(self.lib.smy_pdhg_restart(self.h), "restart")
 
@@ -213,6 +217,10 @@ class PdhgBatch:
     def set_tile(self, matrix: int, precision: str, per_wg: int, xcd: bool):
         q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
         N.check(self.lib.smy_pdhg_set_tile(self.h, matrix, q, per_wg, int(xcd)), "set_tile")
+
+    def set_fp64_accumulation(self, on: bool):
+        """FP32 steps sum their products in FP64 (storage stays FP32)."""
+        N.check(self.lib.smy_pdhg_set_fp64_accumulation(self.h, int(on)), "set_fp64_accumulation")
 
     def bin_tile(self, matrix: int, precision: str, size_bin: int):
         """(per_wg, xcd, split) of one size bin of A (0) or A' (1); None before tuning."""

@@ -189,6 +189,7 @@ class PdlpOptions:
     adaptive_steps: bool = False    # PDLP's adaptive step sizes (decided on the GPU)
     fp32_stall_level: float = 1e-4  # mixed: FP32 plateaus below this switch to FP64 (0: never)
     device_restart: bool = True     # restart decisions, distances and primal weights on the GPU
+    fp64_accumulation: bool = False  # mixed: FP32 steps sum their products in FP64
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -211,6 +212,7 @@ class PdlpOptions:
         p.adaptive_steps = int(self.adaptive_steps)
         p.fp32_stall_level = self.fp32_stall_level
         p.device_restart = int(self.device_restart)
+        p.fp64_accumulation = int(self.fp64_accumulation)
         return p
 
 
