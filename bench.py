@@ -278,7 +278,7 @@ def main() -> int:
     args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload in ("c3", "c4"))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the batched and "
-              "single-LP shardings are described in DESIGN.md); run with --gpus 1",
+              "single-LP shardings are designed but not built); run with --gpus 1",
               file=sys.stderr)
         return 2
 
