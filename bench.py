@@ -215,26 +215,6 @@ def copy_calibration() -> int:
     return 0
 
 
-def gather_ceiling() -> int:
-    """The ceiling of the products' memory pattern (smy_probe_gather) at the 1M-column LP's sizes:
-    11,910,022 entries gathering from the 1,000,000-entry x (A x) and the 500,000-entry y (A' y),
-    in FP64 and FP32. One JSON line of microseconds per launch."""
-    import ctypes as C
-    import torch  # noqa: F401  (torch's HIP runtime first)
-    from samaya_amd import _native as N
-    lib = N.load()
-    out = {}
-    for prec, q in (("fp64", N.SMY_FP64), ("fp32", N.SMY_FP32)):
-        for name, length in (("x_1M", 1_000_000), ("y_500k", 500_000)):
-            us = (C.c_double * 4)()
-            N.check(lib.smy_probe_gather(11_910_022, length, q, us), "probe_gather")
-            out[f"{prec}_{name}"] = dict(zip(("stream", "random", "window_64k", "sorted"),
-                                             [round(v, 1) for v in us]))
-            log(f"{prec} {name}: " + ", ".join(f"{k} {v} us" for k, v in out[f"{prec}_{name}"].items()))
-    print(json.dumps({"gather_ceiling_us": out, "entries": 11_910_022}), flush=True)
-    return 0
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c5"])
@@ -262,17 +242,12 @@ def main() -> int:
     ap.add_argument("--kernels-only", action="store_true",
                     help="--workload c4 / c5: time the batch's step kernels and exit, without the "
                          "solve (for counter runs)")
-    ap.add_argument("--gather-ceiling", action="store_true",
-                    help="only time the products' memory pattern without row structure (stream, "
-                         "random / windowed / sorted gathers) at the 1M-column LP's sizes and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device and exit: the reference transfer that "
                          "scales rocprofv3's FETCH_SIZE / WRITE_SIZE counts to bytes")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
-    if args.gather_ceiling:
-        return gather_ceiling()
     if args.fp32_switch_tol is None:
         args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
     args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload in ("c3", "c4"))

@@ -246,13 +246,6 @@ int smy_pdhg_get_tile(smy_pdhg* h, int matrix, int precision, int* per_wg, int* 
  * batches only (SMY_INVALID for the shared-matrix batch). */
 int smy_pdhg_set_fp64_accumulation(smy_pdhg* h, int on);
 
-/* Measurement helper (no reference counterpart): the ceiling of the products' memory pattern.
- * Over n synthetic entries (values of the precision, int32 indices) and a vector of len entries,
- * us[0..3] receive the average microseconds of: 0 streaming values and indices only; 1 the same
- * with a gather at a random position of the vector per entry; 2 random gathers within a
- * 64K-entry window (L2-resident); 3 gathers at sorted positions. Allocates and frees its own
- * device memory. */
-int smy_probe_gather(long long n, int len, int precision, double* us);
 int smy_pdhg_set_tile(smy_pdhg* h, int matrix, int precision, int per_wg, int xcd);
 int smy_pdhg_get_bin_tile(smy_pdhg* h, int matrix, int precision, int bin, int* per_wg, int* xcd,
                           int* split);

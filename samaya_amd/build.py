@@ -29,7 +29,7 @@ LIB = LIB_DIR / "libsamaya_amd.so"
 SOURCES = [
     SRC / "gpu" / "pdhg_device.hip",
     SRC / "gpu" / "pdhg_shared.hip",
-    SRC / "gpu" / "probe.hip",
+    SRC / "gpu" / "spmv.hip",
     SRC / "host" / "lp_data.cpp",
     SRC / "host" / "pdlp.cpp",
     SRC / "host" / "verify.cpp",

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -21,17 +22,21 @@
 
 namespace smy {
 
-inline thread_local std::string g_last_error;
+// The calling thread's last error message, in a fixed buffer: reporting a failure allocates
+// nothing (the allocation-free SpMV entry point reports through it too).
+inline thread_local char g_last_error[512] = {0};
 
-inline int fail(int code, const std::string& what) {
-  g_last_error = what;
+inline int fail(int code, const char* what) {
+  std::snprintf(g_last_error, sizeof g_last_error, "%s", what);
   return code;
 }
+
+inline int fail(int code, const std::string& what) { return fail(code, what.c_str()); }
 
 template <typename F>
 int guarded(F&& f) {
   try {
-    g_last_error.clear();
+    g_last_error[0] = '\0';
     return f();
   } catch (const HipError& e) {
     return fail(SMY_ERROR, e.what());

@@ -290,6 +290,22 @@ def oracle_pdhg_steps(lp: LP, tau, sigma, theta, steps):
     return vn, yn, ve, ye
 
 
+def equilibrate_lp(lp: LP, rs, cs) -> LP:
+    """A_eq = diag(rs) A diag(cs) and the scaled cost and bounds (tests/test_gpu.cpp:527-545;
+    src/lp/pdlp.cpp:190-209,284-299)."""
+    cols = np.repeat(np.arange(lp.n), np.diff(lp.start))
+    val = rs[lp.index] * lp.value * cs[cols]
+    n = lp.n
+    cost, lower, upper = lp.cost.copy(), lp.lower.copy(), lp.upper.copy()
+    cost[:n] *= cs
+    lower[:n] /= cs
+    upper[:n] /= cs
+    cost[n:] *= rs
+    lower[n:] *= rs
+    upper[n:] *= rs
+    return LP(lp.m, lp.n, lp.start.copy(), lp.index.copy(), val, cost, lower, upper)
+
+
 def oracle_verify_lp(md: M, x, y, ptol=1e-6, dtol=1e-6):
     out = np.zeros(3)
     msg = C.create_string_buffer(256)

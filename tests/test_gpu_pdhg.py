@@ -6,7 +6,7 @@ import pytest
 import scipy.sparse as sp
 
 from conftest import load_golden
-from oracle_lib import LP, M, oracle_pdhg_steps, oracle_sums
+from oracle_lib import LP, M, equilibrate_lp, oracle_pdhg_steps, oracle_sums
 
 pytestmark = pytest.mark.gpu
 CURRENT, AVERAGE = 0, 1
@@ -23,18 +23,7 @@ def identity_lp(md: M) -> LP:
 
 
 def equilibrated(lp: LP, rs, cs):
-    """A_eq = diag(rs) A diag(cs) and the scaled cost and bounds (tests/test_gpu.cpp:527-545)."""
-    cols = np.repeat(np.arange(lp.n), np.diff(lp.start))
-    val = rs[lp.index] * lp.value * cs[cols]
-    n, m = lp.n, lp.m
-    cost, lower, upper = lp.cost.copy(), lp.lower.copy(), lp.upper.copy()
-    cost[:n] *= cs
-    lower[:n] /= cs
-    upper[:n] /= cs
-    cost[n:] *= rs
-    lower[n:] *= rs
-    upper[n:] *= rs
-    return LP(m, n, lp.start.copy(), lp.index.copy(), val, cost, lower, upper)
+    return equilibrate_lp(lp, rs, cs)
 
 
 def transpose_csc(lp: LP):

@@ -224,6 +224,65 @@ def test_sums_hand_computed():
     assert s2[3] == 3.0 * 1.0  # only r's lambda = 3 > 0 meets a finite bound (lower 1)
 
 
+def test_pdhg_steps_hand_algebra_one_by_one():
+    # min x s.t. 2 x = r, r in [3, 3], 0 <= x <= 10; tau = sigma = 0.5, theta = 1, from zero.
+    # Step k: vbar = v + (v - v_prev); y+ = y + sigma (2 vbar_x - vbar_r);
+    # v+ = proj(v - tau (c + [A -I]' y+)); averages with weight 1 / k (pdlp.cpp:574-641).
+    # By hand: step 1 gives v+ = (0, 3), y+ = 0; step 2 vbar = (0, 6), y+ = -3, v+ = (2.5, 3);
+    # step 3 vbar = (5, 3), y+ = 0.5, v+ = (1.5, 3.25 -> 3).
+    lp = lp_from_dense([[2.0]], cost=[1.0, 0.0], lower=[0.0, 3.0], upper=[10.0, 3.0])
+    want = {1: ([0.0, 3.0], [0.0], [0.0, 3.0], [0.0]),
+            2: ([2.5, 3.0], [-3.0], [1.25, 3.0], [-1.5]),
+            3: ([1.5, 3.0], [0.5], [4.0 / 3.0, 3.0], [-5.0 / 6.0])}
+    for steps, (vn, yn, ve, ye) in want.items():
+        got = O.oracle_pdhg_steps(lp, 0.5, 0.5, 1.0, steps)
+        for g, w in zip(got, (vn, yn, ve, ye)):
+            assert np.allclose(g, w, rtol=0, atol=1e-15), (steps, g, w)
+
+
+def test_pdhg_steps_hand_algebra_one_by_two():
+    # min x1 - x2 s.t. x1 - x2 = r, r <= 2 (no lower bound), 0 <= x <= 4; tau 0.5, sigma 0.25.
+    # By hand: step 1 v+ = (0, 0.5, 0), y+ = 0; step 2 vbar = (0, 1, 0), A vbar_x - vbar_r = -1,
+    # y+ = -0.25, [A -I]' y+ = (-0.25, 0.25, 0.25), v+ = (0, 0.875, -0.125); step 3 vbar =
+    # (0, 1.25, -0.25), y+ = -0.5, v+ = (0, 1.125, -0.375); the one-sided r is never clipped.
+    lp = lp_from_dense([[1.0, -1.0]], cost=[1.0, -1.0, 0.0], lower=[0.0, 0.0, -np.inf],
+                       upper=[4.0, 4.0, 2.0])
+    want = {1: ([0.0, 0.5, 0.0], [0.0], [0.0, 0.5, 0.0], [0.0]),
+            2: ([0.0, 0.875, -0.125], [-0.25], [0.0, 0.6875, -0.0625], [-0.125]),
+            3: ([0.0, 1.125, -0.375], [-0.5], [0.0, 5.0 / 6.0, -1.0 / 6.0], [-0.25])}
+    for steps, (vn, yn, ve, ye) in want.items():
+        got = O.oracle_pdhg_steps(lp, 0.5, 0.25, 1.0, steps)
+        for g, w in zip(got, (vn, yn, ve, ye)):
+            assert np.allclose(g, w, rtol=0, atol=1e-15), (steps, g, w)
+
+
+def test_pdhg_steps_equal_the_pinned_solve_loop():
+    # The solve loop (bit-pinned against the reference above) stops at max_iterations = 63 with
+    # its first evaluation, where the artificial rule (63 steps >= 0.36 x 63) restarts to the
+    # candidate: so its answer is the current or average iterate of 63 plain steps with
+    # tau = sigma = 0.99 / L on the equilibrated LP, unscaled (x = v cs, r = v / rs, y = y rs).
+    cases = load_golden("gpu_vs_cpu")["cases"][:4] + load_golden("pdlp_random_feasible")["cases"][:4]
+    checked = 0
+    for case in cases:
+        md = O.M.from_json(case["model"])
+        if md.m == 0:
+            continue
+        lp, _, _ = O.make_problem(md)
+        run = O.oracle_solve_pdlp(lp, 1e-300, 63)
+        assert run.status == 3 and [r[0] for r in run.restarts] == [62]
+        rs, cs = run.row_scale, run.col_scale
+        eq = O.equilibrate_lp(lp, rs, cs)
+        eta = 0.99 / run.L
+        vn, yn, ve, ye = O.oracle_pdhg_steps(eq, eta, eta, 1.0, 63)
+        v, y = (ve, ye) if run.restarts[0][1] else (vn, yn)
+        n = lp.n
+        want_x = np.concatenate([v[:n] * cs, v[n:] / rs])
+        assert np.allclose(run.x, want_x, rtol=1e-12, atol=1e-13)
+        assert np.allclose(run.y, y * rs, rtol=1e-12, atol=1e-13)
+        checked += 1
+    assert checked >= 6
+
+
 def tiny_lp_model() -> O.M:
     # tests/instances/tiny_lp.mps: max 3x + 2y, x + y <= 4, x + 3y <= 6, 0 <= x <= 3, y >= 0;
     # optimum x = 3, y = 1, objective 11 (tests/CMakeLists.txt:48-51).
