@@ -99,6 +99,13 @@ int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y);
 /* Kernel time of the last smy_pdhg_run, in milliseconds (HIP events on the handle's stream). */
 int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
 
+/* Test hook for the reference's GPU-failure path (src/lp/pdlp.cpp:228-246,341-360): once
+ * the handle's step entry points (smy_pdhg_step, _run, _run_batch, _run_adaptive) would run past
+ * step `step_no` (counted over the handle's life), they return SMY_ERROR with the message
+ * "injected fault at step N (test hook)" and launch nothing; the handle stays valid and can be
+ * destroyed. A host-side failure, not a GPU fault. step_no < 0 turns it off. */
+int smy_pdhg_inject_fault(smy_pdhg* h, long long step_no);
+
 /* Measurement helper: average milliseconds of the dual (rows of A) and primal (rows of A') step
  * kernels over `reps` back-to-back launches on the handle's stream. Changes the iterates. */
 int smy_pdhg_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);

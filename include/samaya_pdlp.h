@@ -105,6 +105,9 @@ typedef struct smy_pdlp_params {
                                 or on the host from downloaded sums (0) */
   int fp64_accumulation;     /* mixed: FP32 steps sum their products in FP64 (1) or FP32 (0);
                                 storage stays FP32; batches then use the stacked layout */
+  long long inject_fault_step; /* test hook: the device handle fails its steps after this many
+                                  (smy_pdhg_inject_fault), so the solve takes the reference's
+                                  GPU-failure path (src/lp/pdlp.cpp:231-242); -1: off */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);

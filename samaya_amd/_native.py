@@ -30,7 +30,7 @@ EXPORTS = (
     "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared",
     "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_run_adaptive",
     "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_get_bin_tile", "smy_pdhg_set_bin_tile",
-    "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels",
+    "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels", "smy_pdhg_inject_fault",
     "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
@@ -55,7 +55,8 @@ class SmyPdlpParams(C.Structure):
                 ("verify", C.c_int), ("polish_rounds", C.c_int), ("algorithm", C.c_int),
                 ("share_matrix", C.c_int), ("device_detect", C.c_int), ("start_omega", C.c_double),
                 ("adaptive_steps", C.c_int), ("fp32_stall_level", C.c_double),
-                ("device_restart", C.c_int), ("fp64_accumulation", C.c_int)]
+                ("device_restart", C.c_int), ("fp64_accumulation", C.c_int),
+                ("inject_fault_step", C.c_longlong)]
 
 
 class SmyPdlpInfo(C.Structure):
@@ -152,6 +153,7 @@ def load() -> C.CDLL:
            C.POINTER(i32))
     _proto(lib, "smy_pdhg_set_bin_tile", i32, vp, i32, i32, i32, i32, i32)
     _proto(lib, "smy_pdhg_set_fp64_accumulation", i32, vp, i32)
+    _proto(lib, "smy_pdhg_inject_fault", i32, vp, C.c_longlong)
     _proto(lib, "smy_pdhg_rule_init", i32, vp, vp)
     _proto(lib, "smy_pdhg_rule_decide", i32, vp, vp, C.c_longlong, vp)
     _proto(lib, "smy_pdhg_time_adaptive_kernels", i32, vp, i32, C.POINTER(f64), C.POINTER(f64),

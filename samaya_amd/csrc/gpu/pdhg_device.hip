@@ -1381,13 +1381,33 @@ int smy_pdhg_set_original_at(smy_pdhg* h, int instance, const double* col_scale,
   return set_original_impl(h, instance, col_scale, row_scale, cost, lower, upper);
 }
 
+// The test hook's check before `count` more steps: SMY_ERROR (nothing launched) once they would
+// pass the injected step, else the count is taken.
+static int take_steps(smy_pdhg* h, int count) {
+  if (h->fault_at >= 0 && h->steps_run + count > h->fault_at) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "injected fault at step %lld (test hook)", h->fault_at);
+    return fail(SMY_ERROR, msg);
+  }
+  h->steps_run += count > 0 ? count : 0;
+  return SMY_OK;
+}
+
+int smy_pdhg_inject_fault(smy_pdhg* h, long long step_no) {
+  if (h == nullptr) return fail(SMY_INVALID, "null handle");
+  h->fault_at = step_no < 0 ? -1 : step_no;
+  return SMY_OK;
+}
+
 int smy_pdhg_step(smy_pdhg* h, double tau, double sigma, double theta, long long iter) {
   SMY_SINGLE(h);
+  if (take_steps(h, 1) != SMY_OK) return SMY_ERROR;
   return guarded([&]() -> int { return step_impl(h, &tau, &sigma, theta, &iter); });
 }
 
 int smy_pdhg_run(smy_pdhg* h, double tau, double sigma, double theta, long long iter0, int count) {
   SMY_SINGLE(h);
+  if (take_steps(h, count) != SMY_OK) return SMY_ERROR;
   return guarded([&]() -> int { return run_impl(h, &tau, &sigma, theta, &iter0, count); });
 }
 
@@ -1395,6 +1415,7 @@ int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, doub
                        const long long* iter0, int count) {
   if (h == nullptr || tau == nullptr || sigma == nullptr || iter0 == nullptr)
     return fail(SMY_INVALID, "null argument");
+  if (take_steps(h, count) != SMY_OK) return SMY_ERROR;
   if (h->shared) return guarded([&]() -> int { return shared_run(h, tau, sigma, theta, iter0, count); });
   return guarded([&]() -> int { return run_impl(h, tau, sigma, theta, iter0, count); });
 }
@@ -1402,6 +1423,7 @@ int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, doub
 int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count) {
   if (h == nullptr || state == nullptr) return fail(SMY_INVALID, "null argument");
   if (h->shared) return fail(SMY_INVALID, "adaptive steps need the stacked batch layout");
+  if (take_steps(h, count) != SMY_OK) return SMY_ERROR;
   return guarded([&]() -> int { return run_adaptive_impl(h, state, count); });
 }
 

@@ -173,6 +173,10 @@ using namespace smy;
 
 struct smy_pdhg {
   int K = 1;
+  // Test hook (smy_pdhg_inject_fault): steps run so far by the step entry points, and the step
+  // at which they start failing (-1: never).
+  long long steps_run = 0;
+  long long fault_at = -1;
   int n = 0, m = 0, nt = 0;
   std::vector<Layout> inst;
   DevMatrix a, at;  // A by rows, A' by rows

@@ -53,6 +53,7 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.start_omega = p->start_omega;
   o.adaptive_steps = p->adaptive_steps != 0;
   o.fp64_accumulation = p->fp64_accumulation != 0;
+  o.inject_fault_step = p->inject_fault_step;
   o.fp32_stall_level = p->fp32_stall_level;
   o.device_restart = p->device_restart != 0;
   return o;
@@ -137,6 +138,7 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->start_omega = 0.0;
   p->adaptive_steps = 0;
   p->fp64_accumulation = 0;
+  p->inject_fault_step = -1;
   p->fp32_stall_level = 1e-4;
   p->device_restart = 1;
 }

@@ -109,6 +109,7 @@ class Device {
 
   int size() const { return k_; }
   void set_fp64_accumulation(bool on) { check(smy_pdhg_set_fp64_accumulation(h_, on ? 1 : 0)); }
+  void inject_fault(long long step) { check(smy_pdhg_inject_fault(h_, step)); }
   void set_original(int i, const std::vector<double>& cs, const std::vector<double>& rs,
                     const std::vector<double>& cost, const std::vector<double>& lower,
                     const std::vector<double>& upper) {
@@ -728,6 +729,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     if (device) {
       if (share) lanes[0].log(2, ": %d LPs share one matrix: shared-matrix batch", K);
       if (opt.fp64_accumulation) device->set_fp64_accumulation(true);
+      if (opt.inject_fault_step >= 0) device->inject_fault(opt.inject_fault_step);
       for (int i = 0; i < K; ++i) {
         Lane& L = lanes[i];
         device->set_original(i, L.eq->cs, L.eq->rs, L.lp_.cost, L.lo(), L.up());

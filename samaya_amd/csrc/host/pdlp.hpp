@@ -58,6 +58,9 @@ struct PdlpOptions {
   // Mixed precision: FP32 steps sum their products (and form the update that uses them) in FP64;
   // storage stays FP32. Uses the stacked layout for every batch.
   bool fp64_accumulation = false;
+  // Test hook: the device handle fails its steps after this many (-1: off), so the solve takes
+  // the GPU-failure path.
+  long long inject_fault_step = -1;
   bool use_graph = true;          // replay each 64-step run from a HIP graph
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
   bool device_detect = true;      // ... computed on the device (false: host, from downloads)

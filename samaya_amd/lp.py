@@ -190,6 +190,7 @@ class PdlpOptions:
     fp32_stall_level: float = 1e-4  # mixed: FP32 plateaus below this switch to FP64 (0: never)
     device_restart: bool = True     # restart decisions, distances and primal weights on the GPU
     fp64_accumulation: bool = False  # mixed: FP32 steps sum their products in FP64
+    inject_fault_step: int = -1      # test hook: the device fails its steps after this many
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -213,6 +214,7 @@ class PdlpOptions:
         p.fp32_stall_level = self.fp32_stall_level
         p.device_restart = int(self.device_restart)
         p.fp64_accumulation = int(self.fp64_accumulation)
+        p.inject_fault_step = int(self.inject_fault_step)
         return p
 
 

@@ -10,6 +10,11 @@ objective of tests/reference_lp.hpp. The large CPU case (the 2,000 x 5,000 spars
 library's generator seed 1001) is solved by the reference at tolerance 1e-6 too.
 
 Run in the container that has the reference tree:  python tests/golden/make_golden.py
+The 1M-column LP's fixture (c2_reference.json) takes hours and has modes of its own:
+  python tests/golden/make_golden.py c2          the reference's CPU PDLP at 1e-6
+  python tests/golden/make_golden.py c2-highs    HiGHS's objective (dual simplex)
+The restatement-to-reference speed ratio on one core (cpu_calibration.json):
+  python tests/golden/make_golden.py cpu-calibration
 """
 from __future__ import annotations
 
@@ -141,5 +146,84 @@ def main():
         "model_sha": __import__("hashlib").sha256(np.asarray(md.value).tobytes()).hexdigest()})
 
 
+def c2_reference() -> None:
+    """The 1M-column LP (generators.c2(), seed 1002): the reference's own FP64 PDLP on one CPU
+    core at 1e-6 (about an hour), into c2_reference.json. HiGHS fields are kept when present."""
+    import samaya_amd as smy
+    md = O.M.from_product(smy.generators.c2())
+    t0 = time.perf_counter()
+    r = O.ref_solve_lp(md, 1e-6, -1)
+    seconds = time.perf_counter() - t0
+    path = HERE / "c2_reference.json"
+    old = json.loads(path.read_text()) if path.exists() else {}
+    out = {k: v for k, v in old.items() if k.startswith("highs")}
+    out.update({
+        "source": "reference solve_pdlp (CPU, one core) on generators.c2()", "tol": 1e-6,
+        "m": md.m, "n": md.n, "nnz": int(md.start[-1]),
+        "status": r["status"], "iterations": r["iterations"], "objective": r["objective"],
+        "verified": r["verified"], "max_primal_violation": r["max_primal_violation"],
+        "max_dual_violation": r["max_dual_violation"], "primal_residual": r["primal_residual"],
+        "dual_residual": r["dual_residual"], "gap": r["gap"],
+        "restarts": [[a, b] for a, b, _, _ in r["restarts"]], "seconds_this_container": seconds,
+        "col_value_head": f(r["col_value"][:50]), "row_dual_head": f(r["row_dual"][:50]),
+        "model_sha": __import__("hashlib").sha256(np.asarray(md.value).tobytes()).hexdigest()})
+    write("c2_reference", out)
+
+
+def c2_highs(method: str = "highs-ds") -> None:
+    """HiGHS (scipy) on the 1M-column LP: its objective merged into c2_reference.json."""
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+    import samaya_amd as smy
+    md = smy.generators.c2()
+    A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n)).tocsr()
+    eq = md.row_lower == md.row_upper
+    t0 = time.perf_counter()
+    res = linprog(md.obj, A_ub=A[~eq], b_ub=md.row_upper[~eq], A_eq=A[eq], b_eq=md.row_upper[eq],
+                  bounds=np.stack([md.col_lower, md.col_upper], 1), method=method,
+                  options={"time_limit": 6 * 3600, "disp": True})
+    seconds = time.perf_counter() - t0
+    path = HERE / "c2_reference.json"
+    out = json.loads(path.read_text()) if path.exists() else {}
+    out.update({"highs_method": method, "highs_status": int(res.status), "highs_message": res.message,
+                "highs_objective": float(res.fun) if res.status == 0 else None,
+                "highs_seconds_this_container": seconds})
+    write("c2_reference", out)
+
+
+def cpu_calibration(repeats: int = 5) -> None:
+    """The oracle restatement's speed against the reference's own CPU PDLP on one core: both solve
+    the 2,000 x 5,000 LP (generators.c1()) to 1e-6, best of `repeats`; their time ratio scales the
+    restatement's timings (the benchmark's CPU baseline, measured where the reference tree is
+    absent) to the reference's."""
+    import os
+    import platform
+    import samaya_amd as smy
+    md = O.M.from_product(smy.generators.c1())
+    os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[-1]})
+    ref_t, port_t = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        r = O.ref_solve_lp(md, 1e-6, -1, capture_log=False)
+        ref_t.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        p, _ = O.oracle_solve_lp(md, 1e-6, -1)
+        port_t.append(time.perf_counter() - t0)
+    cpu = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")), platform.processor())
+    write("cpu_calibration", {
+        "source": "reference solve_pdlp (oracle/_ref) and the oracle restatement, generators.c1(), tol 1e-6, one core",
+        "reference_seconds": min(ref_t), "port_seconds": min(port_t),
+        "reference_over_port": min(ref_t) / min(port_t),
+        "reference_iterations": r["iterations"], "port_iterations": p["iterations"],
+        "cpu": cpu, "flags": "-O3 -ffp-contract=off (both; oracle/Makefile)"})
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "c2":
+        c2_reference()
+    elif len(sys.argv) > 1 and sys.argv[1] == "cpu-calibration":
+        cpu_calibration()
+    elif len(sys.argv) > 1 and sys.argv[1] == "c2-highs":
+        c2_highs(*sys.argv[2:3])
+    else:
+        main()
