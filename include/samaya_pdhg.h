@@ -267,6 +267,16 @@ int smy_csr_spmv(int rows, const int* ptr, const int* idx, const double* val, co
 /* The same with FP32 values, x and y (FP32 accumulation). */
 int smy_csr_spmv_f32(int rows, const int* ptr, const int* idx, const float* val, const float* x,
                      float* y, void* stream);
+/* No counterpart in the reference: smy_csr_spmv / smy_csr_spmv_f32 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). y = A x with values, x and y of `precision` (SMY_FP64 / SMY_FP32), the same
+ * kernels and row-block schedule, so the same bits as those two. It allocates nothing: the
+ * schedule is built in a fixed per-GPU workspace of the library (about 8 MiB, held until the
+ * process ends), which limits rows to 1,048,576 (SMY_TOO_LARGE beyond). Asynchronous on `stream`
+ * (NULL: the default stream). Every call on one GPU shares that workspace: calls that overlap on
+ * one GPU (on different streams or threads) race; calls on one stream are ordered. */
+int smy_csr_spmv_noalloc(int rows, const int* ptr, const int* idx, const void* val, const void* x,
+                         void* y, int precision, void* stream);
 /* Replaces gpu::spmv_threads_per_row (spmv.hpp:23): smallest power of two in [2, 32] not below
  * the mean row length (the reference's CUSP rule, kept for its known answers). */
 int smy_spmv_threads_per_row(long long nnz, int rows);

@@ -31,7 +31,7 @@ EXPORTS = (
     "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_run_adaptive",
     "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_get_bin_tile", "smy_pdhg_set_bin_tile",
     "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels", "smy_pdhg_inject_fault",
-    "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32",
+    "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32", "smy_csr_spmv_noalloc",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
     "smy_model_create", "smy_model_generate", "smy_model_free", "smy_model_num_rows",
@@ -162,6 +162,7 @@ def load() -> C.CDLL:
            vp, i32, C.POINTER(vp))
     _proto(lib, "smy_csr_spmv", i32, i32, vp, vp, vp, vp, vp, vp, i32)
     _proto(lib, "smy_csr_spmv_f32", i32, i32, vp, vp, vp, vp, vp, vp)
+    _proto(lib, "smy_csr_spmv_noalloc", i32, i32, vp, vp, vp, vp, vp, i32, vp)
     _proto(lib, "smy_spmv_threads_per_row", i32, i64, i32)
     _proto(lib, "smy_spmv_plan_create", i32, i32, vp, vp, vp, vp, i32, C.POINTER(vp))
     _proto(lib, "smy_spmv_plan_run", i32, vp, vp, vp, vp)

@@ -30,6 +30,7 @@ SOURCES = [
     SRC / "gpu" / "pdhg_device.hip",
     SRC / "gpu" / "pdhg_shared.hip",
     SRC / "gpu" / "spmv.hip",
+    SRC / "gpu" / "spmv_noalloc.hip",
     SRC / "host" / "lp_data.cpp",
     SRC / "host" / "pdlp.cpp",
     SRC / "host" / "verify.cpp",
