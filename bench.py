@@ -176,29 +176,58 @@ def kernel_roofline(md, precision: str, adaptive: bool = False, acc64: bool = Fa
             "ops": {"dual": ops[0], "primal": ops[1]}}
 
 
-def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 128)) -> dict | None:
-    """The reference's CPU PDLP (oracle/_ref) on the same LP for two short runs; the difference
-    gives the time per iteration, extrapolated to the GPU FP64 run's iteration count."""
+def host_info() -> dict:
+    """The host cores the CPU baseline ran on: usable CPUs, the model name, the flags the oracle
+    is built with (oracle/Makefile, as the reference's own: -O3 -ffp-contract=off)."""
+    model = ""
+    try:
+        model = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo")
+                      if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "flags": "-std=c++20 -O3 -ffp-contract=off"}
+
+
+def calibration() -> dict | None:
+    """tests/golden/cpu_calibration.json: the reference's CPU PDLP over the oracle restatement,
+    time ratio on one core (the 2,000 x 5,000 LP, measured where the reference tree is)."""
+    path = ROOT / "tests" / "golden" / "cpu_calibration.json"
+    return json.loads(path.read_text()) if path.exists() else None
+
+
+def port_rate(md, its) -> tuple[list, float, float]:
+    """The oracle restatement of the reference's CPU PDLP (one core, FP64, parity mode) for two
+    short runs of `its` iterations: (times, seconds per iteration, setup seconds)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O
-    kind = "reference" if O.ref_available() else "port"
     m = O.M.from_product(md)
     times = []
-    for its in sample_iters:
+    for k in its:
         t0 = time.perf_counter()
-        if kind == "reference":
-            O.ref_solve_lp(m, 1e-6, its, capture_log=False)
-        else:
-            O.oracle_solve_lp(m, 1e-6, its)
+        O.oracle_solve_lp(m, 1e-6, k)
         times.append(time.perf_counter() - t0)
-    per_it = (times[1] - times[0]) / (sample_iters[1] - sample_iters[0])
-    setup = times[0] - per_it * sample_iters[0]
+    per_it = (times[1] - times[0]) / (its[1] - its[0])
+    return times, per_it, times[0] - per_it * its[0]
+
+
+def cpu_baseline(md, gpu_iterations: int, sample_iters=(64, 576)) -> dict:
+    """The CPU baseline of one LP: the oracle restatement timed for two short runs on one host
+    core; the difference gives the time per iteration, extrapolated to the GPU FP64 run's
+    iteration count. `reference_equivalent` scales it by the committed reference-over-restatement
+    ratio."""
+    times, per_it, setup = port_rate(md, sample_iters)
     value = setup + per_it * gpu_iterations
-    return {"value": round(value, 2), "unit": "s", "cores": 1, "kind": kind,
-            "sample": (f"{sample_iters[0]} and {sample_iters[1]} FP64 PDLP iterations of the same "
-                       f"LP on one core ({times[0]:.1f} s, {times[1]:.1f} s): "
-                       f"{per_it * 1e3:.1f} ms/iteration, setup {setup:.1f} s, extrapolated to "
-                       f"{gpu_iterations} iterations")}
+    cal = calibration()
+    out = {"value": round(value, 2), "unit": "s", "cores": 1, "kind": "port",
+           "sample": (f"{sample_iters[0]} and {sample_iters[1]} FP64 PDLP iterations of the same "
+                      f"LP on one core ({times[0]:.1f} s, {times[1]:.1f} s): "
+                      f"{per_it * 1e3:.1f} ms/iteration, setup {setup:.1f} s, extrapolated to "
+                      f"{gpu_iterations} iterations"), **host_info()}
+    if cal is not None:
+        out["reference_over_port"] = round(cal["reference_over_port"], 4)
+        out["reference_equivalent"] = round(value * cal["reference_over_port"], 2)
+    return out
 
 
 def copy_calibration() -> int:
@@ -215,7 +244,16 @@ def copy_calibration() -> int:
     return 0
 
 
+BUDGET_S = 480.0  # the default run ends well inside 10 minutes: sub-results past this are skipped
+SUB_COST_S = {"c4": 110.0, "c5": 60.0}  # measured cost of each sub-result, with margin
+
+
+def relerr(a, b) -> float | None:
+    return None if a is None or b is None else abs(a - b) / (1 + abs(b))
+
+
 def main() -> int:
+    t_begin = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c5"])
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +264,11 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmv", action="store_true")
+    ap.add_argument("--no-batches", action="store_true",
+                    help="the default (c3) line without its c4 / c5 sub-results")
+    ap.add_argument("--polish-rounds", type=int, default=0,
+                    help="c3: also time one solve with this many FP64 polish rounds (the "
+                         "continuation at tol / 10 after a verifier rejection)")
     ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
     ap.add_argument("--fp64-accumulation", action="store_true",
                     help="mixed precision: FP32 steps sum their products in FP64 (the default line "
@@ -248,14 +291,19 @@ def main() -> int:
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
-    if args.fp32_switch_tol is None:
-        args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
-    args.adaptive = args.adaptive_steps == "on" or (args.adaptive_steps == "auto" and args.workload in ("c3", "c4"))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the batched and "
               "single-LP shardings are designed but not built); run with --gpus 1",
               file=sys.stderr)
         return 2
+    on = args.adaptive_steps == "on"
+    auto = args.adaptive_steps == "auto"
+    args.adaptive = on or (auto and args.workload in ("c3", "c4"))
+    args.adaptive_c4 = on or auto
+    args.fp32_switch_tol_c5 = args.fp32_switch_tol if args.fp32_switch_tol is not None else 1e-6
+    if args.fp32_switch_tol is None:
+        args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
+    args.sub_steps = 1
 
     import torch
 
@@ -263,10 +311,8 @@ def main() -> int:
 
     assert torch.cuda.is_available() and smy.gpu_available(), "no usable MI355X"
     torch.cuda.set_device(0)
-    if args.workload == "c4":
-        return run_c4(args)
-    if args.workload == "c5":
-        return run_c5(args)
+    if args.workload in ("c4", "c5"):
+        return batch_line(args, args.workload)
     t0 = time.perf_counter()
     md = smy.generators.c2(args.scale)
     log(f"LP m={md.m} n={md.n} nnz={md.nnz} generated in {time.perf_counter() - t0:.1f} s")
@@ -282,9 +328,11 @@ def main() -> int:
 
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    runs = []
+    runs, walls = [], []
     for k in range(args.steps):
+        t1 = time.perf_counter()
         r = smy.solve_lp(nm, opts)
+        walls.append(time.perf_counter() - t1)
         runs.append(r)
         log(f"step {k}: {r.status_name} {r.iterations} it ({r.fp32_iterations} FP32), "
             f"setup {r.setup_seconds:.2f} s, iterate {r.solve_seconds - r.setup_seconds:.3f} s, "
@@ -299,11 +347,24 @@ def main() -> int:
 
     fp64 = None
     if not args.no_fp64 and not capped:
-        # Plain FP64 PDLP: the reference's algorithm and precision on the same GPU.
-        f = smy.solve_lp(nm, smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference",
-                                             detect=False))
-        fp64 = f
-        log(f"fp64: {f.status_name} {f.iterations} it, iterate {f.solve_seconds - f.setup_seconds:.3f} s")
+        # Plain FP64 PDLP: the reference's algorithm and precision on the same GPU, verified too.
+        fp64 = smy.solve_lp(nm, smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference",
+                                                detect=False, verify=True))
+        log(f"fp64: {fp64.status_name} {fp64.iterations} it, iterate "
+            f"{fp64.solve_seconds - fp64.setup_seconds:.3f} s, verified {fp64.verified} "
+            f"({fp64.max_primal_violation:.1e}, {fp64.max_dual_violation:.1e})")
+
+    polish = None
+    if args.polish_rounds > 0 and not capped:
+        po = smy.PdlpOptions(**{**opts.__dict__, "polish_rounds": args.polish_rounds})
+        t1 = time.perf_counter()
+        p = smy.solve_lp(nm, po)
+        polish = {"rounds": p.rounds, "iterations": p.iterations, "wall_s": round(time.perf_counter() - t1, 3),
+                  "verified": p.verified, "max_primal_violation": p.max_primal_violation,
+                  "max_dual_violation": p.max_dual_violation, "objective": p.objective,
+                  "time_to_verified_s": round(time.perf_counter() - t1, 3) if p.verified else None}
+        log(f"polish {args.polish_rounds}: {p.iterations} it, verified {p.verified} "
+            f"({p.max_primal_violation:.1e}, {p.max_dual_violation:.1e})")
 
     rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation)
     dom = "primal" if rl["primal_ms"] >= rl["dual_ms"] else "dual"
@@ -319,6 +380,7 @@ def main() -> int:
         gpu_its = fp64.iterations if fp64 is not None else last.iterations
         cpu = cpu_baseline(md, gpu_its)
 
+    ref = golden("c2_reference") if args.scale == 1.0 else None
     line = {
         "metric": METRIC,
         "value": round(iterate_s, 4),
@@ -334,6 +396,7 @@ def main() -> int:
         "algorithm": args.algorithm,
         "adaptive_steps": args.adaptive,
         "fp64_accumulation": args.fp64_accumulation,
+        "detect": False,
         "status": last.status_name,
         "data": "synthetic (seeded generator)",
         "config": {"workload": "sparse LP m=500000 n=1000000 nnz~12M power-law columns, "
@@ -343,7 +406,15 @@ def main() -> int:
         "iterations": last.iterations,
         "fp32_iterations": last.fp32_iterations,
         "setup_s": round(setup_s, 3),
+        "value_with_setup_s": round(iterate_s + setup_s, 4),
+        "solve_lp_wall_s": round(sum(walls) / len(walls), 4),
+        "rel_primal_residual": last.primal_residual,
+        "rel_dual_residual": last.dual_residual,
+        "rel_gap": last.gap,
         "verified": last.verified,
+        "max_primal_violation": last.max_primal_violation,
+        "max_dual_violation": last.max_dual_violation,
+        "time_to_verified_s": None,
         "objective": last.objective,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
@@ -352,14 +423,43 @@ def main() -> int:
                      "avg_us": round(dom_ms * 1e3, 2), "bytes_per_launch": rl["bytes"][dom]},
         "cpu_baseline": cpu,
     }
+    if not last.verified:
+        line["verifier_note"] = ("the verifier (1e-6 relative per row and per reduced cost) rejects "
+                                 "PDLP answers at rel KKT 1e-6 on this LP, as it rejects the "
+                                 "reference's own; FP64 polish rounds do not reach it either")
+    if ref is not None:
+        line["reference_fp64_objective"] = ref["objective"]
+        line["rel_objective_error_vs_reference_fp64"] = relerr(last.objective, ref["objective"])
+        if ref.get("highs_objective") is not None:
+            line["highs_objective"] = ref["highs_objective"]
+            line["rel_objective_error_vs_highs"] = relerr(last.objective, ref["highs_objective"])
     if fp64 is not None:
         line["fp64_time_to_tol_s"] = round(fp64.solve_seconds - fp64.setup_seconds, 4)
         line["fp64_iterations"] = fp64.iterations
         line["fp64_objective"] = fp64.objective
+        line["fp64_verified"] = fp64.verified
+        line["fp64_max_primal_violation"] = fp64.max_primal_violation
+        line["fp64_max_dual_violation"] = fp64.max_dual_violation
         line["speedup_vs_fp64"] = round(line["fp64_time_to_tol_s"] / iterate_s, 3)
-        line["rel_objective_error_vs_fp64"] = abs(last.objective - fp64.objective) / (1 + abs(fp64.objective))
+        line["rel_objective_error_vs_fp64"] = relerr(last.objective, fp64.objective)
+        if ref is not None:
+            line["fp64_rel_objective_error_vs_reference_fp64"] = relerr(fp64.objective, ref["objective"])
+    if polish is not None:
+        line["polish"] = polish
+        line["time_to_verified_s"] = polish["time_to_verified_s"]
     if spmv is not None:
         line["spmv"] = spmv
+    if not args.no_batches and args.scale == 1.0 and not capped:
+        for name, fn in (("c4", c4_result), ("c5", c5_result)):
+            spent = time.perf_counter() - t_begin
+            if spent + SUB_COST_S[name] > BUDGET_S:
+                line[name] = {"skipped": "budget", "elapsed_s": round(spent, 1)}
+                log(f"{name}: skipped ({spent:.0f} s spent of a {BUDGET_S:.0f} s budget)")
+                continue
+            t1 = time.perf_counter()
+            kw = {"fp64_every": 32, "cpu_probe_every": 4} if name == "c4" else {"feasible_sample": 16, "infeasible_sample": 8}
+            line[name] = fn(args, **kw)
+            line[name]["elapsed_s"] = round(time.perf_counter() - t1, 1)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -412,29 +512,35 @@ def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool 
             "other_kernel_us": round((dual_ms if dom == "primal" else primal_ms) * 1e3, 2)}
 
 
-def cpu_rate(models, its=(64, 128)) -> tuple[float, str]:
-    """Seconds per nonzero-iteration of the reference's CPU PDLP (oracle/_ref, one core), from
-    two short runs on each model."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    import oracle_lib as O
-    kind = "reference" if O.ref_available() else "port"
+def cpu_rate(models, its=(64, 192)) -> tuple[float, str]:
+    """Seconds per nonzero-iteration of the oracle restatement of the reference's CPU PDLP (one
+    core), from two short runs on each model."""
     rates = []
     for md in models:
-        mm = O.M.from_product(md)
-        times = []
-        for k in its:
-            t0 = time.perf_counter()
-            if kind == "reference":
-                O.ref_solve_lp(mm, 1e-6, k, capture_log=False)
-            else:
-                O.oracle_solve_lp(mm, 1e-6, k)
-            times.append(time.perf_counter() - t0)
-        rates.append((times[1] - times[0]) / (its[1] - its[0]) / md.nnz)
-    return float(np.mean(rates)), kind
+        _, per_it, _ = port_rate(md, its)
+        rates.append(per_it / md.nnz)
+    return float(np.mean(rates)), "port"
 
 
-def run_c4(args) -> int:
-    import numpy as np
+def cpu_block(value: float, sample: str) -> dict:
+    cal = calibration()
+    out = {"value": round(value, 1), "unit": "s", "cores": 1, "kind": "port", "sample": sample,
+           **host_info()}
+    if cal is not None:
+        out["reference_over_port"] = round(cal["reference_over_port"], 4)
+        out["reference_equivalent"] = round(value * cal["reference_over_port"], 1)
+    return out
+
+
+def golden(name: str) -> dict | None:
+    path = ROOT / "tests" / "golden" / f"{name}.json"
+    return json.loads(path.read_text()) if path.exists() else None
+
+
+def c4_result(args, fp64_every: int = 16, cpu_probe_every: int = 4) -> dict:
+    """The 256 blending LPs as one stacked batch: the fields of its line (value = seconds from the
+    batch resident on the GPU until the last LP stopped). FP64 one LP at a time on every
+    `fp64_every`-th LP, scaled; their objectives also against HiGHS where c4_sample.json has it."""
     import samaya_amd as smy
     import samaya_amd.device as D
     import scipy.sparse as sp
@@ -463,7 +569,7 @@ def run_c4(args) -> int:
         dev = D.PdhgBatch.create(As, Ats, costs, los, ups, "fp32")
         if args.fp64_accumulation:
             dev.set_fp64_accumulation(True)
-        out = batch_roofline(dev, nnz, M, N, None, args.adaptive,
+        out = batch_roofline(dev, nnz, M, N, None, args.adaptive_c4,
                              traffic_of="c4" if args.scale == 1.0 else None)
         # The tuner's choice on this handle: a tile per size bin, and whether the bins go apart.
         out["size_bins"] = {}
@@ -476,24 +582,21 @@ def run_c4(args) -> int:
         return out
 
     if args.kernels_only:
-        # Profiling mode: the step kernels alone (no solve, no baselines); one JSON line.
-        print(json.dumps({"workload": f"{K} blending LPs, stacked FP32 step kernels",
-                          "roofline": step_kernels()}), flush=True)
-        return 0
+        return {"kernels_only": True, "roofline": step_kernels(),
+                "config": {"workload": f"{K} blending LPs, stacked FP32 step kernels"}}
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
-                           adaptive_steps=args.adaptive, fp64_accumulation=args.fp64_accumulation)
-    for w in range(args.warmup):
-        smy.solve_lp_batch(nms[:8], opts)
+                           adaptive_steps=args.adaptive_c4, fp64_accumulation=args.fp64_accumulation)
+    smy.solve_lp_batch(nms[:8], opts)  # warm up the tuner and the graph path on a small batch
     runs = []
     t_start = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(args.sub_steps):
         res, times = smy.solve_lp_batch(nms, opts)
         runs.append((res, times))
         its = np.array([r.iterations for r in res])
-        log(f"step {k}: batch {times.solve_seconds - times.setup_seconds:.2f} s after setup "
+        log(f"c4 step {k}: batch {times.solve_seconds - times.setup_seconds:.2f} s after setup "
             f"{times.setup_seconds:.2f} s; iterations median {int(np.median(its))} max {its.max()}; "
-            f"statuses {sorted(set(r.status_name for r in res))}")
+            f"statuses {sorted(set(r.status_name for r in res))}; verified {sum(r.verified for r in res)}")
     wall = time.perf_counter() - t_start
     res, times = runs[-1]
     value = float(np.mean([t.solve_seconds - t.setup_seconds for _, t in runs]))
@@ -501,9 +604,9 @@ def run_c4(args) -> int:
 
     fp64 = None
     if not args.no_fp64:
-        # Plain FP64 PDLP (the reference's algorithm), one LP at a time: every 16th LP, scaled.
-        sample = list(range(0, K, max(1, K // 16)))
-        o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=False)
+        # Plain FP64 PDLP (the reference's algorithm), one LP at a time: every fp64_every-th LP.
+        sample = list(range(0, K, max(1, fp64_every)))
+        o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=True)
         secs, its64, objs = [], [], []
         for i in sample:
             r = smy.solve_lp(nms[i], o64)
@@ -512,43 +615,53 @@ def run_c4(args) -> int:
             objs.append(abs(res[i].objective - r.objective) / (1 + abs(r.objective)))
         fp64 = {"seconds": sum(secs) * K / len(sample), "sample": sample, "its": its64,
                 "obj_err": max(objs)}
-        log(f"fp64 one at a time: {sum(secs):.2f} s for {len(sample)} LPs -> {fp64['seconds']:.1f} s")
+        log(f"c4 fp64 one at a time: {sum(secs):.2f} s for {len(sample)} LPs -> {fp64['seconds']:.1f} s")
 
     rl = step_kernels()
+    highs = golden("c4_sample")
+    highs_err = None
+    if highs is not None and K == 256:
+        errs = [abs(res[e["index"]].objective - e["objective"]) / (1 + abs(e["objective"]))
+                for e in highs["lps"] if e["status"] == 0]
+        highs_err = max(errs) if errs else None
 
     cpu = None
     if not args.no_cpu_baseline and fp64 is not None:
-        probe = [models[i] for i in fp64["sample"][::4]]
-        rate, kind = cpu_rate(probe)
+        probe = [models[i] for i in fp64["sample"][::cpu_probe_every]]
+        rate, _ = cpu_rate(probe)
         est = sum(rate * models[i].nnz * it for i, it in zip(fp64["sample"], fp64["its"])) * K / len(fp64["sample"])
-        cpu = {"value": round(est, 1), "unit": "s", "cores": 1, "kind": kind,
-               "sample": (f"reference CPU PDLP, 64 and 128 FP64 iterations on {len(probe)} of the LPs "
-                          f"({rate * 1e9:.2f} ns per nonzero-iteration), times the FP64 iteration "
-                          f"counts of the 16 sampled LPs, scaled to {K}; extrapolated")}
+        cpu = cpu_block(est, f"oracle restatement of the reference's CPU PDLP, 64 and 192 FP64 "
+                             f"iterations on {len(probe)} of the LPs ({rate * 1e9:.2f} ns per "
+                             f"nonzero-iteration), times the FP64 iteration counts of the "
+                             f"{len(fp64['sample'])} sampled LPs, scaled to {K}; extrapolated")
     its = np.array([r.iterations for r in res])
-    extra = {
+    out = {
+        "value": round(value, 4), "unit": "s", "higher_is_better": False,
+        "ms_per_step": round(wall / len(runs) * 1e3, 1), "steps": len(runs),
         "status": "optimal", "iterations_median": int(np.median(its)), "iterations_max": int(its.max()),
         "fp32_iterations_sum": int(sum(r.fp32_iterations for r in res)),
-        "fp32_switch_tol": args.fp32_switch_tol,
+        "fp32_switch_tol": args.fp32_switch_tol, "adaptive_steps": args.adaptive_c4,
         "setup_s": round(times.setup_seconds, 3), "verified": int(sum(r.verified for r in res)),
-        "roofline": rl, "cpu_baseline": cpu,
+        "lps": K, "roofline": rl, "cpu_baseline": cpu,
+        "config": {"workload": f"{K} multi-period blending LPs (periods U{{4..48}}, 20 units x "
+                               f"30 streams), {M} rows, {N} columns, {nnz} nonzeros; stacked "
+                               f"batch, mixed precision, rel KKT {args.tol:g} each",
+                   "lps": K, "rows": M, "cols": N, "nnz": nnz, "tol": args.tol},
     }
-    extra["config"] = {"workload": f"{K} multi-period blending LPs (periods U{{4..48}}, 20 units x "
-                                   f"30 streams), {M} rows, {N} columns, {nnz} nonzeros; stacked "
-                                   f"batch, mixed precision, rel KKT {args.tol:g} each",
-                       "lps": K, "rows": M, "cols": N, "nnz": nnz, "tol": args.tol}
+    if highs_err is not None:
+        out["rel_objective_error_vs_highs_max"] = highs_err
+        out["highs_sample"] = f"{sum(e['status'] == 0 for e in highs['lps'])} LPs (every 32nd)"
     if fp64 is not None:
-        extra["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
-        extra["fp64_sample"] = f"{len(fp64['sample'])} of {K} LPs solved one at a time, scaled"
-        extra["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
-        extra["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
-    line = base_line(args, value, wall / args.steps * 1e3, extra["config"]["workload"], extra)
-    print(json.dumps(line), flush=True)
-    return 0
+        out["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
+        out["fp64_sample"] = f"{len(fp64['sample'])} of {K} LPs solved one at a time, scaled"
+        out["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
+        out["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
+    return out
 
 
-def run_c5(args) -> int:
-    import numpy as np
+def c5_result(args, feasible_sample: int = 32, infeasible_sample: int = 16) -> dict:
+    """The 1,024 knapsack node relaxations as one shared-matrix batch, warm-started from the root
+    LP: the fields of its line (value = seconds until the last node stopped)."""
     import samaya_amd as smy
     import samaya_amd.device as D
     import scipy.sparse as sp
@@ -577,28 +690,25 @@ def run_c5(args) -> int:
         return out
 
     if args.kernels_only:
-        # Profiling mode: the step kernels alone (no solve, no baselines); one JSON line.
-        print(json.dumps({"workload": f"{K} node relaxations, shared-matrix FP32 step kernels",
-                          "roofline": step_kernels()}), flush=True)
-        return 0
+        return {"kernels_only": True, "roofline": step_kernels(),
+                "config": {"workload": f"{K} node relaxations, shared-matrix FP32 step kernels"}}
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
-                           fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
-                           adaptive_steps=args.adaptive, fp64_accumulation=args.fp64_accumulation)
+                           fp32_switch_tol=args.fp32_switch_tol_c5, log_level=1 if args.verbose else 0,
+                           adaptive_steps=False, fp64_accumulation=args.fp64_accumulation)
     root = smy.solve_lp(nm, opts)
-    log(f"root: {root.status_name} {root.iterations} it, omega {root.omega:.3e}")
+    log(f"c5 root: {root.status_name} {root.iterations} it, omega {root.omega:.3e}")
     start = (root.col_value, root.row_dual)
     opts.start_omega = root.omega
-    for w in range(args.warmup):
-        smy.solve_lp_bounds_batch(nm, lows[:16], ups[:16], opts, start=start)
+    smy.solve_lp_bounds_batch(nm, lows[:16], ups[:16], opts, start=start)  # warm up
     runs = []
     t_start = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(args.sub_steps):
         res, times = smy.solve_lp_bounds_batch(nm, lows, ups, opts, start=start)
         runs.append((res, times))
         its = np.array([r.iterations for r in res])
-        log(f"step {k}: batch {times.solve_seconds - times.setup_seconds:.2f} s after setup "
+        log(f"c5 step {k}: batch {times.solve_seconds - times.setup_seconds:.2f} s after setup "
             f"{times.setup_seconds:.2f} s; iterations median {int(np.median(its))} max {its.max()}; "
-            f"statuses {sorted(set(r.status_name for r in res))}")
+            f"statuses {sorted(set(r.status_name for r in res))}; verified {sum(r.verified for r in res)}")
     wall = time.perf_counter() - t_start
     res, times = runs[-1]
     value = float(np.mean([t.solve_seconds - t.setup_seconds for _, t in runs]))
@@ -613,8 +723,9 @@ def run_c5(args) -> int:
         # when the Farkas test succeeds), each class scaled by its count.
         feas = [i for i in range(K) if not infeasible[i]]
         infs = [i for i in range(K) if infeasible[i]]
-        strata = [(feas, feas[::max(1, len(feas) // 32)][:32]), (infs, infs[::max(1, len(infs) // 16)][:16])]
-        o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=False,
+        strata = [(feas, feas[::max(1, len(feas) // feasible_sample)][:feasible_sample]),
+                  (infs, infs[::max(1, len(infs) // infeasible_sample)][:infeasible_sample])]
+        o64 = smy.PdlpOptions(tol=args.tol, precision="fp64", algorithm="reference", verify=True,
                               start_omega=root.omega)
         total, its64, objs, sample = 0.0, [], [], []
         for members, chosen in strata:
@@ -629,9 +740,9 @@ def run_c5(args) -> int:
                 if r.status == smy.OPTIMAL and res[i].status == smy.OPTIMAL:
                     objs.append(abs(res[i].objective - r.objective) / (1 + abs(r.objective)))
             total += float(np.mean(secs)) * len(members)
-            log(f"fp64 one at a time: {len(chosen)} of {len(members)} nodes, mean {np.mean(secs):.3f} s")
+            log(f"c5 fp64 one at a time: {len(chosen)} of {len(members)} nodes, mean {np.mean(secs):.3f} s")
         fp64 = {"seconds": total, "sample": sample, "its": its64, "obj_err": max(objs) if objs else None}
-        log(f"fp64 one at a time: {fp64['seconds']:.1f} s estimated for {K} nodes")
+        log(f"c5 fp64 one at a time: {fp64['seconds']:.1f} s estimated for {K} nodes")
 
     rl = step_kernels()
 
@@ -639,35 +750,52 @@ def run_c5(args) -> int:
     if not args.no_cpu_baseline and fp64 is not None:
         node = smy.Model(md.m, md.n, md.start, md.index, md.value, md.obj, lows[0], ups[0],
                          md.row_lower, md.row_upper, md.maximize, md.offset)
-        rate, kind = cpu_rate([node])
+        rate, _ = cpu_rate([node])
         est = rate * md.nnz * float(np.mean(fp64["its"])) * K
-        cpu = {"value": round(est, 1), "unit": "s", "cores": 1, "kind": kind,
-               "sample": (f"reference CPU PDLP, 64 and 128 FP64 iterations on one node "
-                          f"({rate * 1e9:.2f} ns per nonzero-iteration), times the mean FP64 "
-                          f"iteration count of the sampled nodes, times {K}; extrapolated")}
+        cpu = cpu_block(est, f"oracle restatement of the reference's CPU PDLP, 64 and 192 FP64 "
+                             f"iterations on one node ({rate * 1e9:.2f} ns per nonzero-iteration), "
+                             f"times the mean FP64 iteration count of the sampled nodes, times {K}; "
+                             f"extrapolated")
     its = np.array([r.iterations for r in res])
-    extra = {
+    out = {
+        "value": round(value, 4), "unit": "s", "higher_is_better": False,
+        "ms_per_step": round(wall / len(runs) * 1e3, 1), "steps": len(runs),
         "status": "expected statuses" if statuses_ok else "unexpected statuses",
         "optimal": int(sum(r.status == smy.OPTIMAL for r in res)),
         "infeasible": int(sum(r.status == smy.INFEASIBLE for r in res)),
         "infeasible_by_construction": int(sum(infeasible)),
         "iterations_median": int(np.median(its)), "iterations_max": int(its.max()),
         "root_iterations": root.iterations, "setup_s": round(times.setup_seconds, 3),
-        "fp32_switch_tol": args.fp32_switch_tol,
+        "fp32_switch_tol": args.fp32_switch_tol_c5, "adaptive_steps": False,
         "fp32_iterations_sum": int(sum(r.fp32_iterations for r in res)),
         "verified": int(sum(r.verified for r in res)), "roofline": rl, "cpu_baseline": cpu,
+        "config": {"workload": f"{K} node relaxations (1-5% of columns fixed) of a knapsack LP "
+                               f"m={md.m} n={md.n} nnz={md.nnz}; shared-matrix batch (SpMM), "
+                               f"mixed precision, warm-started from the root, rel KKT {args.tol:g}",
+                   "nodes": K, "m": md.m, "n": md.n, "nnz": md.nnz, "tol": args.tol},
     }
-    extra["config"] = {"workload": f"{K} node relaxations (1-5% of columns fixed) of a knapsack LP "
-                                   f"m={md.m} n={md.n} nnz={md.nnz}; shared-matrix batch (SpMM), "
-                                   f"mixed precision, warm-started from the root, rel KKT {args.tol:g}",
-                       "nodes": K, "m": md.m, "n": md.n, "nnz": md.nnz, "tol": args.tol}
     if fp64 is not None:
-        extra["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
-        extra["fp64_sample"] = (f"{len(fp64['sample'])} of {K} nodes solved one at a time, warm-started "
-                                f"alike: 32 feasible and 16 infeasible, each mean scaled by its class count")
-        extra["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
-        extra["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
-    line = base_line(args, value, wall / args.steps * 1e3, extra["config"]["workload"], extra)
+        out["fp64_one_at_a_time_s"] = round(fp64["seconds"], 2)
+        out["fp64_sample"] = (f"{len(fp64['sample'])} of {K} nodes solved one at a time, warm-started "
+                              f"alike: {feasible_sample} feasible and {infeasible_sample} infeasible, "
+                              f"each mean scaled by its class count")
+        out["speedup_vs_fp64"] = round(fp64["seconds"] / value, 3)
+        out["rel_objective_error_vs_fp64_max"] = fp64["obj_err"]
+    return out
+
+
+def batch_line(args, workload: str) -> int:
+    """--workload c4 / c5: the batch's own line (or its kernel timings with --kernels-only)."""
+    args.sub_steps = args.steps
+    res = c4_result(args) if workload == "c4" else c5_result(args)
+    if res.get("kernels_only"):
+        print(json.dumps({"workload": res["config"]["workload"], "roofline": res["roofline"]}), flush=True)
+        return 0
+    value, ms = res.pop("value"), res.pop("ms_per_step")
+    for k in ("unit", "higher_is_better", "steps"):
+        res.pop(k, None)
+    line = base_line(args, value, ms, res["config"]["workload"], res)
+    line["warmup"] = 1
     print(json.dumps(line), flush=True)
     return 0
 

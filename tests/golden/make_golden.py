@@ -13,6 +13,8 @@ Run in the container that has the reference tree:  python tests/golden/make_gold
 The 1M-column LP's fixture (c2_reference.json) takes hours and has modes of its own:
   python tests/golden/make_golden.py c2          the reference's CPU PDLP at 1e-6
   python tests/golden/make_golden.py c2-highs    HiGHS's objective (dual simplex)
+HiGHS objectives of the larger GPU test models (c1, knapsack nodes, a blending sample):
+  python tests/golden/make_golden.py highs-fixtures
 The restatement-to-reference speed ratio on one core (cpu_calibration.json):
   python tests/golden/make_golden.py cpu-calibration
 """
@@ -218,9 +220,66 @@ def cpu_calibration(repeats: int = 5) -> None:
         "cpu": cpu, "flags": "-O3 -ffp-contract=off (both; oracle/Makefile)"})
 
 
+def highs_solve(md, col_lower=None, col_upper=None, method="highs"):
+    """(status, objective, seconds) of HiGHS (scipy) on a Model, optionally with other column
+    bounds; the objective in the model's sense, offset included."""
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+    A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n)).tocsr()
+    sense = -1.0 if md.maximize else 1.0
+    eq = md.row_lower == md.row_upper
+    fin_u = np.isfinite(md.row_upper) & ~eq
+    fin_l = np.isfinite(md.row_lower) & ~eq
+    A_ub = sp.vstack([A[fin_u], -A[fin_l]]).tocsr()
+    b_ub = np.concatenate([md.row_upper[fin_u], -md.row_lower[fin_l]])
+    lo = md.col_lower if col_lower is None else col_lower
+    up = md.col_upper if col_upper is None else col_upper
+    t0 = time.perf_counter()
+    res = linprog(sense * md.obj, A_ub=A_ub if A_ub.shape[0] else None,
+                  b_ub=b_ub if A_ub.shape[0] else None,
+                  A_eq=A[eq] if eq.any() else None, b_eq=md.row_lower[eq] if eq.any() else None,
+                  bounds=np.stack([lo, up], 1), method=method)
+    obj = sense * res.fun + md.offset if res.status == 0 else None
+    return int(res.status), obj, time.perf_counter() - t0
+
+
+def highs_fixtures() -> None:
+    """HiGHS objectives for the GPU tests' larger models: the 2,000 x 5,000 LP (merged into
+    c1_reference.json), the knapsack node relaxations of the shared-matrix tests
+    (node_relaxations.json), and every 32nd blending LP of the 256-LP batch (c4_sample.json)."""
+    import samaya_amd as smy
+    G = smy.generators
+    path = HERE / "c1_reference.json"
+    c1 = json.loads(path.read_text())
+    st, obj, sec = highs_solve(G.c1(), method="highs-ipm")
+    c1.update({"highs_status": st, "highs_objective": obj, "highs_method": "highs-ipm"})
+    write("c1_reference", c1)
+    nodes = []
+    for seed, m, n, per_col, count in ((73, 400, 1000, 6, 9), (74, 300, 700, 5, 5), (75, 400, 1000, 6, 6)):
+        md = G.knapsack(seed, m, n, per_col)
+        for k in range(count):
+            lo, up = G.fixings(900 + k, md.col_lower, md.col_upper, 0.01, 0.05)
+            st, obj, _ = highs_solve(md, lo, up)
+            nodes.append({"knapsack": [seed, m, n, per_col], "fixings_seed": 900 + k,
+                          "status": st, "objective": obj})
+    write("node_relaxations", {"source": "HiGHS (scipy linprog, method highs) on knapsack node "
+                                         "relaxations: generators.knapsack, generators.fixings "
+                                         "(fractions 0.01..0.05)", "nodes": nodes})
+    lps = []
+    for i in range(0, 256, 32):
+        md = G.blending(4000 + i)
+        st, obj, sec = highs_solve(md, method="highs-ipm")
+        lps.append({"index": i, "seed": 4000 + i, "m": md.m, "n": md.n, "status": st,
+                    "objective": obj, "seconds_this_container": sec})
+    write("c4_sample", {"source": "HiGHS (scipy linprog, highs-ipm) on every 32nd blending LP of "
+                                  "generators.c4()", "lps": lps})
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "c2":
         c2_reference()
+    elif len(sys.argv) > 1 and sys.argv[1] == "highs-fixtures":
+        highs_fixtures()
     elif len(sys.argv) > 1 and sys.argv[1] == "cpu-calibration":
         cpu_calibration()
     elif len(sys.argv) > 1 and sys.argv[1] == "c2-highs":
