@@ -143,23 +143,40 @@ def spmv_sweep(md) -> dict:
     return out
 
 
-def kernel_roofline(md, precision: str, adaptive: bool = False, acc64: bool = False) -> dict:
-    """Live per-kernel timing of the two step kernels on the workload's matrix (the adaptive
-    attempt's kernels when `adaptive`: same byte model, plus the per-attempt decision kernel;
-    `acc64`: the FP32 kernels that accumulate in FP64)."""
+def count_order(md, algorithm: str):
+    """The column order of the solver's model-level solve (PdlpOptions.column_order "auto"): by
+    decreasing count for the enhanced algorithm when the longest column is at least 16x the mean,
+    else None (the model's order)."""
+    counts = np.diff(md.start)
+    if algorithm != "enhanced" or md.n == 0 or counts.max() < 16 * max(md.nnz / md.n, 1.0):
+        return None
+    return np.argsort(-counts, kind="stable")
+
+
+def kernel_roofline(md, precision: str, adaptive: bool = False, acc64: bool = False,
+                    order=None) -> dict:
+    """Live per-kernel timing of the two step kernels on the workload's matrix, with its columns
+    in `order` when the solve uses one (the adaptive attempt's kernels when `adaptive`: same byte
+    model, plus the per-attempt decision kernel; `acc64`: the FP32 kernels that accumulate in
+    FP64)."""
     import numpy as np
     import scipy.sparse as sp
 
     from samaya_amd.device import PdhgDevice
 
     A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+    obj, col_lower, col_upper = md.obj, md.col_lower, md.col_upper
+    if order is not None:
+        A = A[:, order]
+        A.sort_indices()
+        obj, col_lower, col_upper = obj[order], col_lower[order], col_upper[order]
     T = A.T.tocsc()
     T.sort_indices()
     nt = md.n + md.m
-    lower = np.concatenate([md.col_lower, md.row_lower])
-    upper = np.concatenate([md.col_upper, md.row_upper])
-    cost = np.concatenate([md.obj, np.zeros(md.m)])
-    dev = PdhgDevice.create((md.m, md.n, md.start, md.index, md.value),
+    lower = np.concatenate([col_lower, md.row_lower])
+    upper = np.concatenate([col_upper, md.row_upper])
+    cost = np.concatenate([obj, np.zeros(md.m)])
+    dev = PdhgDevice.create((md.m, md.n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data),
                             (md.n, md.m, T.indptr.astype(np.int64), T.indices.astype(np.int32), T.data),
                             cost, lower, upper, "fp32" if precision == "fp32" else "fp64")
     if acc64:
@@ -366,7 +383,8 @@ def main() -> int:
         log(f"polish {args.polish_rounds}: {p.iterations} it, verified {p.verified} "
             f"({p.max_primal_violation:.1e}, {p.max_dual_violation:.1e})")
 
-    rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation)
+    order = count_order(md, args.algorithm)
+    rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation, order)
     dom = "primal" if rl["primal_ms"] >= rl["dual_ms"] else "dual"
     dom_ms = rl[f"{dom}_ms"]
     dom_op = rl["ops"][dom]
@@ -397,6 +415,7 @@ def main() -> int:
         "adaptive_steps": args.adaptive,
         "fp64_accumulation": args.fp64_accumulation,
         "detect": False,
+        "column_order": "by count" if order is not None else "model",
         "status": last.status_name,
         "data": "synthetic (seeded generator)",
         "config": {"workload": "sparse LP m=500000 n=1000000 nnz~12M power-law columns, "

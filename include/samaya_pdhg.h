@@ -99,6 +99,19 @@ int smy_pdhg_set_start(smy_pdhg* h, const double* v, const double* y);
 /* Kernel time of the last smy_pdhg_run, in milliseconds (HIP events on the handle's stream). */
 int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
 
+/* The enhanced algorithm's preconditioner computed and applied on the device (no reference
+ * counterpart: the reference equilibrates on the host, src/lp/pdlp.cpp:109-209,284-318). For a
+ * single-LP handle created from the unscaled LpProblem, before its first step: `passes`
+ * infinity-norm Ruiz rounds (every row and column divided by the square root of its largest
+ * magnitude, from the same current scaling) and, with pock_chambolle, one Pock-Chambolle step
+ * (alpha = 1: square roots of the 1-norms); then the handle's matrices, cost and bounds are
+ * scaled in place (FP32 copies included) and 20 power iterations from the reference's seeded start
+ * bound ||[A_eq -I]||. Writes col_scale (n), row_scale (m), out = {that bound, max column 2-norm
+ * of the unscaled A, max row 2-norm}. The factors and the norms equal smy_pdlp_equilibrate's bit
+ * for bit (the same per-row sums in the same order); the bound agrees to rounding. */
+int smy_pdhg_equilibrate(smy_pdhg* h, int passes, int pock_chambolle, double* col_scale,
+                         double* row_scale, double out[3]);
+
 /* Test hook for the reference's GPU-failure path (src/lp/pdlp.cpp:228-246,341-360): once
  * the handle's step entry points (smy_pdhg_step, _run, _run_batch, _run_adaptive) would run past
  * step `step_no` (counted over the handle's life), they return SMY_ERROR with the message

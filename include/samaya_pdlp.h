@@ -19,6 +19,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "samaya_pdhg.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -79,6 +81,15 @@ int smy_compute_scaling(const smy_model* model, double* col_scale, double* row_s
 int smy_make_problem(const smy_model* model, double* cost, double* lower, double* upper,
                      double* a_value);
 
+/* solve_pdlp's preconditioner on the host (ruiz_equilibration, src/lp/pdlp.cpp:109-187, for
+ * SMY_ALGORITHM_REFERENCE; for SMY_ALGORITHM_ENHANCED `passes` infinity-norm Ruiz rounds and,
+ * with pock_chambolle, one Pock-Chambolle step with alpha = 1) of an LpProblem's A (m x n CSC)
+ * and At: col_scale (n), row_scale (m), out = {the bound on ||[A_eq -I]|| of
+ * estimate_operator_norm (:36-103), max column 2-norm of A, max row 2-norm of A (:377-394)}.
+ * The device computes the same (smy_pdhg_equilibrate). */
+int smy_pdlp_equilibrate(const smy_csc* A, const smy_csc* At, int algorithm, int passes,
+                         int pock_chambolle, double* col_scale, double* row_scale, double out[3]);
+
 /* PdlpOptions (src/lp/pdlp.hpp:11-16) plus this library's additions. */
 typedef struct smy_pdlp_params {
   double tol;                /* 1e-8 (the solver's kPdlpTolerance, src/core/solver.cpp:31) */
@@ -105,6 +116,13 @@ typedef struct smy_pdlp_params {
                                 or on the host from downloaded sums (0) */
   int fp64_accumulation;     /* mixed: FP32 steps sum their products in FP64 (1) or FP32 (0);
                                 storage stays FP32; batches then use the stacked layout */
+  int device_scaling;        /* enhanced algorithm, one LP: the preconditioner, the scaled matrices,
+                                cost and bounds and the operator-norm bound computed on the device
+                                (smy_pdhg_equilibrate; 1) or on the host (0); the same factors */
+  int column_order;          /* model-level solves: 0 auto (enhanced algorithm with skewed column
+                                counts: columns by decreasing nonzero count, so the products'
+                                gathers of the hottest columns share cache lines), 1 the model's
+                                order, 2 by count always; answers come back in the model's order */
   long long inject_fault_step; /* test hook: the device handle fails its steps after this many
                                   (smy_pdhg_inject_fault), so the solve takes the reference's
                                   GPU-failure path (src/lp/pdlp.cpp:231-242); -1: off */
@@ -156,7 +174,19 @@ typedef struct smy_lp_result {
   int rounds;           /* PDLP solves run: 1 + polish rounds used */
   smy_pdlp_info pdlp;   /* the last solve */
   char message[256];
+  double dual_bound;    /* optimal answers: a bound on the LP's value from the answer's row duals
+                           by weak duality (smy_lp_dual_bound), valid whether or not the primal
+                           verifies; lower for minimization, upper for maximization; NaN else */
 } smy_lp_result;
+
+/* A bound on the model's optimal value from row duals y (m, d = c - A'y convention), by weak
+ * duality (no reference counterpart: the reference's branch and bound takes the dual simplex's
+ * objective as a node's bound, src/mip/branch_and_bound.hpp:133-145): y made sign-feasible, then
+ * sum_j min over the column's box of d_j x_j plus sum_i min over the row's range of y_i r_i, in
+ * double-double less a margin that covers its rounding. Lower bound for minimization, upper for
+ * maximization (offset included); -inf / +inf when an unbounded variable has a nonzero reduced
+ * cost. */
+int smy_lp_dual_bound(const smy_model* model, const double* row_dual, double* bound);
 
 /* LP by PDLP on the GPU (solve_pdlp_lp): power-of-two scaling, computational form, PDLP, mapping
  * back and the independent verifier. Vectors (any may be NULL): col_value (n), row_dual (m,

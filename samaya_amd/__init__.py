@@ -7,7 +7,7 @@ include/samaya_pdlp.h); it has no compute path of its own.
 from ._native import LIB_PATH, load, last_error, NativeError
 from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, PdlpOptions,
                  PdlpResult, LpResult, BatchTimes, solve_pdlp, solve_lp, solve_lp_batch,
-                 solve_lp_bounds_batch, solve, verify_lp_optimality, verify_infeasibility,
+                 solve_lp_bounds_batch, solve, verify_lp_optimality, verify_infeasibility, dual_bound,
                  verify_unbounded_ray, gpu_available, gpu_info, STATUS_NAMES, OPTIMAL,
                  INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR)
 from . import generators
@@ -17,7 +17,7 @@ __all__ = [
     "compute_scaling", "make_problem",
     "PdlpOptions", "PdlpResult", "LpResult", "BatchTimes", "solve_pdlp", "solve_lp",
     "solve_lp_batch", "solve_lp_bounds_batch", "solve",
-    "verify_lp_optimality", "verify_infeasibility", "verify_unbounded_ray", "gpu_available",
+    "verify_lp_optimality", "verify_infeasibility", "dual_bound", "verify_unbounded_ray", "gpu_available",
     "gpu_info", "STATUS_NAMES", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "ITERATION_LIMIT",
     "TIME_LIMIT", "NUMERICAL_ERROR", "generators",
 ]

@@ -31,6 +31,7 @@ EXPORTS = (
     "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_run_adaptive",
     "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_get_bin_tile", "smy_pdhg_set_bin_tile",
     "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels", "smy_pdhg_inject_fault",
+    "smy_pdhg_equilibrate", "smy_pdlp_equilibrate", "smy_lp_dual_bound",
     "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32", "smy_csr_spmv_noalloc",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
@@ -56,6 +57,7 @@ class SmyPdlpParams(C.Structure):
                 ("share_matrix", C.c_int), ("device_detect", C.c_int), ("start_omega", C.c_double),
                 ("adaptive_steps", C.c_int), ("fp32_stall_level", C.c_double),
                 ("device_restart", C.c_int), ("fp64_accumulation", C.c_int),
+                ("device_scaling", C.c_int), ("column_order", C.c_int),
                 ("inject_fault_step", C.c_longlong)]
 
 
@@ -78,7 +80,8 @@ class SmyLpResult(C.Structure):
     _fields_ = [("status", C.c_int), ("verified", C.c_int), ("objective", C.c_double),
                 ("max_primal_violation", C.c_double), ("max_dual_violation", C.c_double),
                 ("iterations", C.c_longlong), ("fp32_iterations", C.c_longlong),
-                ("rounds", C.c_int), ("pdlp", SmyPdlpInfo), ("message", C.c_char * 256)]
+                ("rounds", C.c_int), ("pdlp", SmyPdlpInfo), ("message", C.c_char * 256),
+                ("dual_bound", C.c_double)]
 
 
 _lib = None
@@ -154,6 +157,9 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_set_bin_tile", i32, vp, i32, i32, i32, i32, i32)
     _proto(lib, "smy_pdhg_set_fp64_accumulation", i32, vp, i32)
     _proto(lib, "smy_pdhg_inject_fault", i32, vp, C.c_longlong)
+    _proto(lib, "smy_pdhg_equilibrate", i32, vp, i32, i32, vp, vp, vp)
+    _proto(lib, "smy_lp_dual_bound", i32, vp, vp, C.POINTER(C.c_double))
+    _proto(lib, "smy_pdlp_equilibrate", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), i32, i32, i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_rule_init", i32, vp, vp)
     _proto(lib, "smy_pdhg_rule_decide", i32, vp, vp, C.c_longlong, vp)
     _proto(lib, "smy_pdhg_time_adaptive_kernels", i32, vp, i32, C.POINTER(f64), C.POINTER(f64),

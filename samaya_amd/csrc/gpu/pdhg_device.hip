@@ -25,7 +25,10 @@
 // aty vectors disappear. An instance's arithmetic does not depend on the rest of its batch: a
 // batch of K gives each LP the bits it gets alone.
 
+#include <random>
+
 #include "pdhg_handle.hpp"
+#include "scaling_kernels.hpp"
 
 namespace smy {
 namespace {
@@ -1138,6 +1141,153 @@ int create_impl(int count, const smy_csc* A, const smy_csc* At, const double* co
   });
 }
 
+// ---- Preconditioning on the device (single-LP handles created with the unscaled LP) ----
+
+// The enhanced algorithm's preconditioner of the handle's LP, computed and applied on the device
+// (the host's ruiz_pock_chambolle, csc_scaled of both copies, the cost and bound scaling of
+// Lane::prepare, operator_norm_bound and the termination norms, csrc/host/pdlp.cpp): `passes`
+// infinity-norm Ruiz rounds, then (pock_chambolle) one Pock-Chambolle step with alpha = 1.
+// out = {L, a_colmax, at_rowmax}.
+int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_scale,
+                     double* row_scale, double* out) {
+  const int n = h->n, m = h->m;
+  const hipStream_t s = h->stream;
+  DevMatrix& A = h->a;    // A by rows: row i, indices j
+  DevMatrix& At = h->at;  // A' by rows: row j, indices i
+  double* cs = h->cs;
+  double* rs = h->rs;
+  double* cn = h->alloc<double>(n);
+  double* rn = h->alloc<double>(m);
+  double* scalar = h->alloc<double>(2);
+  const auto fill_ones = [&](double* x, int count) {
+    if (count == 0) return;
+    std::vector<double> ones(static_cast<size_t>(count), 1.0);
+    SMY_HIP(hipMemcpyAsync(x, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    SMY_HIP(hipStreamSynchronize(s));
+  };
+  fill_ones(cs, n);
+  fill_ones(rs, m);
+  // Norms of the termination test from the unscaled matrix (src/lp/pdlp.cpp:377-394).
+  double a_colmax = 0.0, at_rowmax = 0.0;
+  if (m > 0 && n > 0) {
+    hipLaunchKernelGGL((segment_norm_kernel<kSegSumSq, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       n, At.ptr, At.idx, At.val64, rs, cs, cn);
+    hipLaunchKernelGGL(sqrt_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cn);
+    hipLaunchKernelGGL((reduce_finish_kernel<true>), dim3(1), dim3(kReduceThreads), 0, s,
+                       static_cast<long long>(n), cn, scalar);
+    hipLaunchKernelGGL((segment_norm_kernel<kSegSumSq, true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
+                       m, A.ptr, A.idx, A.val64, rs, cs, rn);
+    hipLaunchKernelGGL(sqrt_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rn);
+    hipLaunchKernelGGL((reduce_finish_kernel<true>), dim3(1), dim3(kReduceThreads), 0, s,
+                       static_cast<long long>(m), rn, scalar + 1);
+    SMY_HIP(hipGetLastError());
+    double two[2];
+    SMY_HIP(hipMemcpyAsync(two, scalar, sizeof two, hipMemcpyDeviceToHost, s));
+    SMY_HIP(hipStreamSynchronize(s));
+    a_colmax = two[0];
+    at_rowmax = two[1];
+  }
+  if (m > 0 && n > 0) {
+    const auto round = [&](bool one_norm) {
+      if (one_norm) {
+        hipLaunchKernelGGL((segment_norm_kernel<kSegSum, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                           n, At.ptr, At.idx, At.val64, rs, cs, cn);
+        hipLaunchKernelGGL((segment_norm_kernel<kSegSum, true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
+                           m, A.ptr, A.idx, A.val64, rs, cs, rn);
+      } else {
+        hipLaunchKernelGGL((segment_norm_kernel<kSegMax, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                           n, At.ptr, At.idx, At.val64, rs, cs, cn);
+        hipLaunchKernelGGL((segment_norm_kernel<kSegMax, true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
+                           m, A.ptr, A.idx, A.val64, rs, cs, rn);
+      }
+      hipLaunchKernelGGL(divide_by_sqrt_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cs, cn);
+      hipLaunchKernelGGL(divide_by_sqrt_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rs, rn);
+    };
+    for (int pass = 0; pass < passes; ++pass) round(false);
+    if (pock_chambolle) round(true);
+  }
+  // 20 power iterations on A_eq'A_eq from the host's seeded start (src/lp/pdlp.cpp:36-103), on the
+  // unscaled values and the factors in the host's order, norms on the host:
+  // L = sqrt((1.01 sigma)^2 + 1).
+  double L = 1.0;
+  if (m > 0 && n > 0) {
+    std::vector<double> hu(static_cast<size_t>(m)), hw(static_cast<size_t>(n));
+    std::mt19937_64 gen(12345);
+    std::uniform_real_distribution<double> uniform(-1.0, 1.0);
+    for (double& t : hu) t = uniform(gen);
+    const auto norm = [](const std::vector<double>& z) {
+      double acc = 0.0;
+      for (double t : z) acc += t * t;
+      return std::sqrt(acc);
+    };
+    double un = norm(hu);
+    for (double& t : hu) t /= un;
+    double* u = rn;  // reused: the row norms are no longer needed
+    double* w = cn;
+    SMY_HIP(hipMemcpyAsync(u, hu.data(), hu.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    double sigma = 0.0;
+    bool zero = false;
+    for (int it = 0; it < 20; ++it) {
+      hipLaunchKernelGGL(power_w_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, At.ptr, At.idx,
+                         At.val64, rs, cs, u, w);
+      SMY_HIP(hipMemcpyAsync(hw.data(), w, hw.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      SMY_HIP(hipStreamSynchronize(s));
+      const double wn = norm(hw);
+      if (wn == 0.0) {
+        zero = true;
+        break;
+      }
+      hipLaunchKernelGGL(power_u_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, A.ptr, A.idx,
+                         A.val64, rs, cs, w, u);
+      SMY_HIP(hipMemcpyAsync(hu.data(), u, hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      SMY_HIP(hipStreamSynchronize(s));
+      un = norm(hu);
+      if (un == 0.0) {
+        zero = true;
+        break;
+      }
+      sigma = un / wn;
+      for (double& t : hu) t /= un;
+      SMY_HIP(hipMemcpyAsync(u, hu.data(), hu.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    if (!zero) {
+      const double la = sigma * 1.01;
+      L = std::sqrt(la * la + 1.0);
+    }
+  }
+  if (m > 0 && n > 0) {
+    // The scaled copies: A by rows (a * cs_j) * rs_i, A' by rows (a * rs_i) * cs_j.
+    hipLaunchKernelGGL((scale_values_kernel<true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
+                       m, A.ptr, A.idx, A.val64, rs, cs);
+    hipLaunchKernelGGL((scale_values_kernel<false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       n, At.ptr, At.idx, At.val64, rs, cs);
+    SMY_HIP(hipGetLastError());
+  }
+  if (h->nt > 0)
+    hipLaunchKernelGGL(scale_vectors_kernel, dim3(blocks_for(h->nt)), dim3(kBlock), 0, s, n, m, cs, rs,
+                       h->d.cost, h->d.lower, h->d.upper);
+  if (h->f.ready) {  // the FP32 copies follow the scaled FP64 ones
+    for (DevMatrix* M : {&h->a, &h->at})
+      if (M->nnz > 0)
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(M->nnz)), dim3(kBlock), 0, s,
+                           M->nnz, M->val64, M->val32);
+    for (auto [src, dst] : {std::pair{h->d.cost, h->f.cost}, std::pair{h->d.lower, h->f.lower},
+                            std::pair{h->d.upper, h->f.upper}})
+      if (h->nt > 0)
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(h->nt)), dim3(kBlock), 0, s,
+                           static_cast<long long>(h->nt), src, dst);
+  }
+  SMY_HIP(hipGetLastError());
+  if (n > 0) SMY_HIP(hipMemcpyAsync(col_scale, cs, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (m > 0) SMY_HIP(hipMemcpyAsync(row_scale, rs, m * sizeof(double), hipMemcpyDeviceToHost, s));
+  SMY_HIP(hipStreamSynchronize(s));
+  out[0] = L;
+  out[1] = a_colmax;
+  out[2] = at_rowmax;
+  h->vbar_any = true;
+  return SMY_OK;
+}
+
 int set_original_impl(smy_pdhg* h, int i, const double* col_scale, const double* row_scale,
                       const double* cost, const double* lower, const double* upper) {
   const Layout& L = h->inst[i];
@@ -1391,6 +1541,16 @@ static int take_steps(smy_pdhg* h, int count) {
   }
   h->steps_run += count > 0 ? count : 0;
   return SMY_OK;
+}
+
+int smy_pdhg_equilibrate(smy_pdhg* h, int passes, int pock_chambolle, double* col_scale,
+                         double* row_scale, double out[3]) {
+  SMY_SINGLE(h);
+  if (passes < 0 || out == nullptr || (h->n > 0 && col_scale == nullptr) ||
+      (h->m > 0 && row_scale == nullptr))
+    return fail(SMY_INVALID, "bad argument");
+  if (h->stepped || h->steps_run > 0) return fail(SMY_STATE, "equilibrate before the first step");
+  return guarded([&]() -> int { return equilibrate_impl(h, passes, pock_chambolle, col_scale, row_scale, out); });
 }
 
 int smy_pdhg_inject_fault(smy_pdhg* h, long long step_no) {

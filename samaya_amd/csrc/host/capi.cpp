@@ -54,6 +54,8 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.adaptive_steps = p->adaptive_steps != 0;
   o.fp64_accumulation = p->fp64_accumulation != 0;
   o.inject_fault_step = p->inject_fault_step;
+  o.device_scaling = p->device_scaling != 0;
+  o.column_order = static_cast<ColumnOrder>(p->column_order == 1 ? 1 : p->column_order == 2 ? 2 : 0);
   o.fp32_stall_level = p->fp32_stall_level;
   o.device_restart = p->device_restart != 0;
   return o;
@@ -113,6 +115,7 @@ void fill_result(const LpSolveResult& r, smy_lp_result* result) {
   result->rounds = r.rounds;
   fill_info(r.pdlp, &result->pdlp);
   copy_text(result->message, sizeof result->message, r.message);
+  result->dual_bound = r.dual_bound;
 }
 
 }  // namespace
@@ -139,6 +142,8 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->adaptive_steps = 0;
   p->fp64_accumulation = 0;
   p->inject_fault_step = -1;
+  p->device_scaling = 1;
+  p->column_order = 0;
   p->fp32_stall_level = 1e-4;
   p->device_restart = 1;
 }
@@ -242,6 +247,41 @@ int smy_compute_scaling(const smy_model* model, double* col_scale, double* row_s
     const Scaling s = compute_scaling(model->model.A);
     put(s.col, col_scale);
     put(s.row, row_scale);
+    return SMY_OK;
+  } catch (...) {
+    return SMY_ERROR;
+  }
+}
+
+int smy_lp_dual_bound(const smy_model* model, const double* row_dual, double* bound) {
+  if (model == nullptr || bound == nullptr || (model->model.m > 0 && row_dual == nullptr)) return SMY_INVALID;
+  try {
+    *bound = safe_dual_bound(model->model, row_dual);
+    return SMY_OK;
+  } catch (...) {
+    return SMY_ERROR;
+  }
+}
+
+int smy_pdlp_equilibrate(const smy_csc* A, const smy_csc* At, int algorithm, int passes,
+                         int pock_chambolle, double* col_scale, double* row_scale, double out[3]) {
+  if (A == nullptr || At == nullptr || out == nullptr || passes < 0) return SMY_INVALID;
+  try {
+    const auto own = [](const smy_csc* c) {
+      Csc m;
+      m.rows = c->rows;
+      m.cols = c->cols;
+      m.start.assign(c->start, c->start + c->cols + 1);
+      m.index.assign(c->index, c->index + c->start[c->cols]);
+      m.value.assign(c->value, c->value + c->start[c->cols]);
+      return m;
+    };
+    const Csc a = own(A), at = own(At);
+    std::vector<double> rs, cs;
+    equilibrate_host(a, at, algorithm == SMY_ALGORITHM_ENHANCED ? Algorithm::kEnhanced : Algorithm::kReference,
+                     passes, pock_chambolle != 0, rs, cs, out);
+    put(cs, col_scale);
+    put(rs, row_scale);
     return SMY_OK;
   } catch (...) {
     return SMY_ERROR;

@@ -110,6 +110,12 @@ class Device {
   int size() const { return k_; }
   void set_fp64_accumulation(bool on) { check(smy_pdhg_set_fp64_accumulation(h_, on ? 1 : 0)); }
   void inject_fault(long long step) { check(smy_pdhg_inject_fault(h_, step)); }
+  // The enhanced preconditioner on the device (a handle of one unscaled LP): the factors, L and
+  // the norms of the termination test.
+  // out = {L, a_colmax, at_rowmax}.
+  void equilibrate(int passes, std::vector<double>& cs, std::vector<double>& rs, double out[3]) {
+    check(smy_pdhg_equilibrate(h_, passes, 1, cs.data(), rs.data(), out));
+  }
   void set_original(int i, const std::vector<double>& cs, const std::vector<double>& rs,
                     const std::vector<double>& cost, const std::vector<double>& lower,
                     const std::vector<double>& upper) {
@@ -232,7 +238,7 @@ void equilibrate(const Csc& A, const Csc& At, std::vector<double>& rs, std::vect
 // alpha = 1 (rows by the square root of their 1-norm, columns likewise). Factors of empty rows and
 // columns stay 1.
 void ruiz_pock_chambolle(const Csc& A, const Csc& At, std::vector<double>& rs,
-                         std::vector<double>& cs, int passes) {
+                         std::vector<double>& cs, int passes, bool pock_chambolle = true) {
   const int m = A.rows, n = A.cols;
   rs.assign(static_cast<size_t>(m), 1.0);
   cs.assign(static_cast<size_t>(n), 1.0);
@@ -261,7 +267,7 @@ void ruiz_pock_chambolle(const Csc& A, const Csc& At, std::vector<double>& rs,
       if (rn[i] > 0.0) rs[i] /= std::sqrt(rn[i]);
   };
   for (int pass = 0; pass < passes; ++pass) norms(false);
-  norms(true);
+  if (pock_chambolle) norms(true);
 }
 
 // Upper bound on ||[A -I]||_2 by 20 power iterations on A'A from a seeded random start
@@ -377,6 +383,8 @@ class Lane {
   bool fp32 = false;
   double fp32_best = kInf;       // best distance to termination in FP32 (by 20% steps)
   long long fp32_best_iter = 0;  // ... and where it was reached
+  int fp32_below = 0;            // consecutive evaluations below fp32_stall_level
+  bool fp32_armed = false;       // the stall test counts (fp32_below reached kStallArm once)
   bool active = true;
   Measure best{kInf, kInf, kInf, kInf, 0.0, 0.0};
 };
@@ -416,6 +424,36 @@ std::shared_ptr<const Equilibrated> equilibrate_problem(const LpProblem& lp, Alg
   }
   return e;
 }
+
+}  // namespace
+
+void equilibrate_host(const Csc& A, const Csc& At, Algorithm algorithm, int passes,
+                      bool pock_chambolle, std::vector<double>& rs, std::vector<double>& cs,
+                      double out[3]) {
+  if (algorithm == Algorithm::kEnhanced)
+    ruiz_pock_chambolle(A, At, rs, cs, passes, pock_chambolle);
+  else
+    equilibrate(A, At, rs, cs, passes);
+  const Csc A_eq = csc_scaled(A, rs, cs), At_eq = csc_scaled(At, cs, rs);
+  out[0] = operator_norm_bound(A_eq, At_eq);
+  double a_colmax = 0.0, at_rowmax = 0.0;
+  if (A.rows > 0) {
+    for (int j = 0; j < A.cols; ++j) {
+      double t = 0.0;
+      for (int64_t p = A.start[j]; p < A.start[j + 1]; ++p) t += A.value[p] * A.value[p];
+      a_colmax = std::max(a_colmax, std::sqrt(t));
+    }
+    for (int i = 0; i < At.cols; ++i) {
+      double t = 0.0;
+      for (int64_t p = At.start[i]; p < At.start[i + 1]; ++p) t += At.value[p] * At.value[p];
+      at_rowmax = std::max(at_rowmax, std::sqrt(t));
+    }
+  }
+  out[1] = a_colmax;
+  out[2] = at_rowmax;
+}
+
+namespace {
 
 void Lane::prepare(std::shared_ptr<const Equilibrated> e) {
   const int m = lp_.m, n = lp_.n, nt = n + m;
@@ -626,10 +664,16 @@ bool Lane::ray_from(long long iter, const double* o) const {
 
 void Lane::map_back(const std::vector<double>& v, const std::vector<double>& y) {
   // src/lp/pdlp.cpp:1167-1189: x = cs v_x, r = v_r / rs, y = rs y'.
+  // The enhanced algorithm first puts the point back into its box: an average kept in FP32 drifts
+  // off a fixed bound by rounding over many steps (by up to ~3e-6 relative on the node
+  // relaxations), though every averaged iterate lies in the box. The reference's algorithm
+  // returns the point as it is.
   const int m = lp_.m, n = lp_.n;
+  const bool clip = opt_.algorithm == Algorithm::kEnhanced;
+  const auto at = [&](int k) { return clip ? std::min(std::max(v[k], lower[k]), upper[k]) : v[k]; };
   res.x.assign(static_cast<size_t>(n + m), 0.0);
-  for (int j = 0; j < n; ++j) res.x[j] = v[j] * eq->cs[j];
-  for (int i = 0; i < m; ++i) res.x[n + i] = v[n + i] / eq->rs[i];
+  for (int j = 0; j < n; ++j) res.x[j] = at(j) * eq->cs[j];
+  for (int i = 0; i < m; ++i) res.x[n + i] = at(n + i) / eq->rs[i];
   res.y.assign(static_cast<size_t>(m), 0.0);
   for (int i = 0; i < m; ++i) res.y[i] = y[i] * eq->rs[i];
 }
@@ -694,17 +738,22 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       if (&lanes[o].lp_ == &lanes[i].lp_) first[i] = o;
     if (first[i] == i) owners.push_back(i);
   }
+  // The enhanced algorithm's preconditioner of a single LP runs on the device: the handle is
+  // created from the unscaled LP and scales itself (the same factors as the host's).
+  const bool device_eq = opt.device_scaling && opt.algorithm == Algorithm::kEnhanced && K == 1;
   std::vector<std::shared_ptr<const Equilibrated>> eqs(static_cast<size_t>(K));
   const int threads = K > 1 ? host_threads() : 1;
-  parallel_for(static_cast<int>(owners.size()), threads, [&](int q) {
-    eqs[owners[q]] = equilibrate_problem(lanes[owners[q]].lp_, opt.algorithm);
-  });
-  parallel_for(K, threads, [&](int i) { lanes[i].prepare(eqs[first[i]]); });
+  if (!device_eq) {
+    parallel_for(static_cast<int>(owners.size()), threads, [&](int q) {
+      eqs[owners[q]] = equilibrate_problem(lanes[owners[q]].lp_, opt.algorithm);
+    });
+    parallel_for(K, threads, [&](int i) { lanes[i].prepare(eqs[first[i]]); });
+  }
 
   // One problem for all lanes, with one set of row bounds: the shared-matrix batch (iterates
   // stored instance-minor, SpMM products); otherwise the stacked batch.
   bool share = opt.share_matrix && !opt.adaptive_steps && !opt.fp64_accumulation && K > 1 &&
-               owners.size() == 1;
+               owners.size() == 1 && !device_eq;
   for (int i = 1; share && i < K; ++i) {
     const int n = lanes[0].lp_.n;
     share = std::equal(lanes[i].lower.begin() + n, lanes[i].lower.end(), lanes[0].lower.begin() + n) &&
@@ -715,17 +764,34 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   std::unique_ptr<Device> device;
   std::string why;
   try {
-    std::vector<const Csc*> A, At;
-    std::vector<const std::vector<double>*> c, l, u;
-    for (Lane& L : lanes) {
-      A.push_back(&L.eq->A_eq);
-      At.push_back(&L.eq->At_eq);
-      c.push_back(&L.cost);
-      l.push_back(&L.lower);
-      u.push_back(&L.upper);
+    if (device_eq) {
+      Lane& L0 = lanes[0];
+      device = Device::create({&L0.lp_.A}, {&L0.lp_.At}, {&L0.lp_.cost}, {&L0.lo()}, {&L0.up()},
+                              mixed ? SMY_FP32 : SMY_FP64);
+      if (device) {
+        auto e = std::make_shared<Equilibrated>();
+        e->cs.assign(static_cast<size_t>(L0.lp_.n), 1.0);
+        e->rs.assign(static_cast<size_t>(L0.lp_.m), 1.0);
+        double out[3];
+        device->equilibrate(10, e->cs, e->rs, out);
+        e->L = out[0];
+        e->a_colmax = out[1];
+        e->at_rowmax = out[2];
+        L0.prepare(e);
+      }
+    } else {
+      std::vector<const Csc*> A, At;
+      std::vector<const std::vector<double>*> c, l, u;
+      for (Lane& L : lanes) {
+        A.push_back(&L.eq->A_eq);
+        At.push_back(&L.eq->At_eq);
+        c.push_back(&L.cost);
+        l.push_back(&L.lower);
+        u.push_back(&L.upper);
+      }
+      device = share ? Device::create_shared(A[0], At[0], c[0], l, u, mixed ? SMY_FP32 : SMY_FP64)
+                     : Device::create(A, At, c, l, u, mixed ? SMY_FP32 : SMY_FP64);
     }
-    device = share ? Device::create_shared(A[0], At[0], c[0], l, u, mixed ? SMY_FP32 : SMY_FP64)
-                   : Device::create(A, At, c, l, u, mixed ? SMY_FP32 : SMY_FP64);
     if (device) {
       if (share) lanes[0].log(2, ": %d LPs share one matrix: shared-matrix batch", K);
       if (opt.fp64_accumulation) device->set_fp64_accumulation(true);
@@ -967,17 +1033,23 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       // Mixed precision: from FP32 to FP64 steps once FP32 got close to the tolerance, or once
       // FP32 stopped improving near its rounding floor (which lies above the switch point): no
       // 20% gain on its best distance to termination within max(4096, iter / 4) iterations,
-      // counted from the first evaluation below fp32_stall_level (a plateau above it belongs to
-      // the algorithm, which FP64 steps would only make slower).
+      // counted from the point where the distance has stayed below fp32_stall_level for
+      // kStallArm evaluations in a row (a plateau above it belongs to the algorithm, which FP64
+      // steps would only make slower; one evaluation below it can be the relative gap passing
+      // through zero, seen at iteration 447 of the 1M-column LP in count order).
       if (L.fp32) {
+        constexpr int kStallArm = 4;
         const double w = std::min(worst(current), worst(average));
         const double level = opt.fp32_stall_level;
-        if (w < 0.8 * L.fp32_best || (w < level && !(L.fp32_best < level))) {
+        L.fp32_below = w < level ? L.fp32_below + 1 : 0;
+        const bool arm = !L.fp32_armed && L.fp32_below >= kStallArm;
+        if (arm) L.fp32_armed = true;
+        if (w < 0.8 * L.fp32_best || arm) {
           L.fp32_best = w;
           L.fp32_best_iter = iter;
         }
         const bool reached = w < opt.fp32_switch_tol;
-        const bool stalled = L.fp32_best < level &&
+        const bool stalled = L.fp32_armed &&
                              iter - L.fp32_best_iter >= std::max<long long>(4096, iter / 4);
         if (reached || stalled) {
           L.log(2, reached ? ": FP32 steps reached %.2e at iter %lld; FP64 steps from here"

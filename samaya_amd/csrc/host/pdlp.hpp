@@ -15,6 +15,7 @@
 #pragma once
 
 #include <cstdint>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,10 @@ enum class Precision : int { kFp64 = 0, kMixed = 1 };
 // kEnhanced: infinity-norm Ruiz (10 passes) followed by Pock-Chambolle (alpha = 1) diagonal
 // preconditioning, the usual PDLP preconditioner; every other decision unchanged.
 enum class Algorithm : int { kReference = 0, kEnhanced = 1 };
+// Column order of a model-level solve: kAuto puts the columns of an enhanced-algorithm solve in
+// decreasing order of their nonzero counts when those are skewed (the longest column at least 16
+// times the mean); kDegree always, kNatural never. Answers come back in the model's order.
+enum class ColumnOrder : int { kAuto = 0, kNatural = 1, kDegree = 2 };
 
 struct PdlpOptions {
   double tol = 1e-8;              // max(rel primal, rel dual, |rel gap|) < tol
@@ -61,6 +66,10 @@ struct PdlpOptions {
   // Test hook: the device handle fails its steps after this many (-1: off), so the solve takes
   // the GPU-failure path.
   long long inject_fault_step = -1;
+  // Enhanced algorithm, one LP: the preconditioner and the operator-norm bound computed on the
+  // device from the unscaled LP (the same factors as the host's).
+  bool device_scaling = true;
+  ColumnOrder column_order = ColumnOrder::kAuto;  // model-level solves (solve_lp*)
   bool use_graph = true;          // replay each 64-step run from a HIP graph
   bool detect = true;             // infeasibility / unboundedness checks (pdlp.cpp:715-1043)
   bool device_detect = true;      // ... computed on the device (false: host, from downloads)
@@ -108,6 +117,13 @@ struct PdlpResult {
   std::vector<EvalEvent> evals;
 };
 
+// The preconditioner of solve_pdlp on the host (the reference's 2-norm variant, or for the
+// enhanced algorithm `passes` infinity-norm Ruiz rounds and, with pock_chambolle, one
+// Pock-Chambolle step), then out = {bound on ||[A_eq -I]||, a_colmax, at_rowmax}.
+void equilibrate_host(const Csc& A, const Csc& At, Algorithm algorithm, int passes,
+                      bool pock_chambolle, std::vector<double>& rs, std::vector<double>& cs,
+                      double out[3]);
+
 PdlpResult solve_pdlp(const LpProblem& lp, const PdlpOptions& options);
 
 // ---- Batches (no reference counterpart) ----
@@ -150,6 +166,14 @@ struct VerifyReport {
 VerifyReport verify_lp_optimality(const Model& model, const double* x, const double* y,
                                   double primal_tol = 1e-6, double dual_tol = 1e-6);
 VerifyReport verify_infeasibility(const Model& model, const double* y, double primal_tol = 1e-6);
+// A bound on the LP's optimal value from any row duals y (d = c - A'y convention), by weak
+// duality: y is first made sign-feasible (the dual of a row with only an upper bound <= 0 in the
+// minimization sense, with only a lower bound >= 0), then
+//   sum_j min over [l_j, u_j] of d_j x_j + sum_i min over [rl_i, ru_i] of y_i r_i
+// in double-double, less a margin for its rounding. A lower bound for minimization, an upper bound
+// for maximization (offset included); -inf (+inf) where a needed bound is infinite. A node
+// relaxation's bound for branch and bound, valid whether or not the primal point verifies.
+double safe_dual_bound(const Model& model, const double* y);
 VerifyReport verify_unbounded_ray(const Model& model, const double* ray, double primal_tol = 1e-6,
                                   double dual_tol = 1e-6);
 
@@ -173,6 +197,8 @@ struct LpSolveResult {
   std::vector<double> unbounded_ray;              // n
   double max_primal_violation = 0.0;
   double max_dual_violation = 0.0;
+  // safe_dual_bound of the answer's row duals (optimal answers; NaN otherwise).
+  double dual_bound = std::numeric_limits<double>::quiet_NaN();
   long long iterations = 0;       // reported by the reference as barrier_iterations
   long long fp32_iterations = 0;  // of which FP32 steps (mixed precision)
   int rounds = 0;                 // solves run (1 + polish rounds used)

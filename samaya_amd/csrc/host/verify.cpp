@@ -317,11 +317,131 @@ VerifyReport verify_unbounded_ray(const Model& md, const double* ray, double pri
   return r;
 }
 
+double safe_dual_bound(const Model& md, const double* y) {
+  // src/verify/verify.cpp:132-247 conventions: d_j = sense (c_j - a_j'y), a row's reduced cost is
+  // sense y_i, and weak duality holds for the minimization of sense c'x.
+  const double sense = md.maximize ? -1.0 : 1.0;
+  std::vector<double> ys(static_cast<size_t>(md.m));
+  for (int i = 0; i < md.m; ++i) {
+    double yi = sense * y[i];
+    if (!std::isfinite(yi)) yi = 0.0;
+    if (md.row_lower[i] == -kInf && yi > 0.0) yi = 0.0;  // only an upper bound: dual <= 0
+    if (md.row_upper[i] == kInf && yi < 0.0) yi = 0.0;   // only a lower bound: dual >= 0
+    ys[i] = yi;
+  }
+  const auto term = [](DD g, double lo, double up, DD& acc, double& mag) -> bool {
+    const double gv = g.value();
+    if (gv == 0.0) return true;
+    const double b = gv > 0.0 ? lo : up;
+    if (!std::isfinite(b)) return false;
+    g *= b;
+    acc += g;
+    mag += std::fabs(g.value());
+    return true;
+  };
+  DD bound;
+  double mag = 0.0;
+  bool finite = true;
+  for (int j = 0; j < md.n && finite; ++j) {
+    DD d(sense * md.obj[j]);
+    double dmag = std::fabs(md.obj[j]);
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+      const DD t = DD::product(md.A.value[p], ys[md.A.index[p]]);
+      d += t.negated();
+      dmag += std::fabs(t.value());
+    }
+    mag += dmag * std::max(std::fabs(md.col_lower[j]) < kInf ? std::fabs(md.col_lower[j]) : 0.0,
+                           std::fabs(md.col_upper[j]) < kInf ? std::fabs(md.col_upper[j]) : 0.0);
+    finite = term(d, md.col_lower[j], md.col_upper[j], bound, mag);
+  }
+  for (int i = 0; i < md.m && finite; ++i)
+    finite = term(DD(ys[i]), md.row_lower[i], md.row_upper[i], bound, mag);
+  if (!finite) return md.maximize ? kInf : -kInf;
+  // Every double-double operation errs by at most ~2^-104 of its magnitudes; one margin of
+  // (nnz + n + m + 8) 2^-100 times their total covers them all.
+  const double margin =
+      static_cast<double>(md.A.nnz() + md.n + md.m + 8) * 0x1p-100 * (mag + std::fabs(bound.value()));
+  const double lower = bound.value() - margin;  // minimization of sense c'x
+  return sense * lower + md.offset;
+}
+
 namespace {
+
+// Whether a model-level solve puts the model's columns in decreasing order of their counts
+// (ColumnOrder::kAuto: the enhanced algorithm, and the longest column at least 16 times the mean).
+bool wants_degree_order(const Model& md, const PdlpOptions& o) {
+  if (o.column_order == ColumnOrder::kNatural || md.n == 0) return false;
+  if (o.column_order == ColumnOrder::kDegree) return true;
+  if (o.algorithm != Algorithm::kEnhanced) return false;
+  int64_t longest = 0;
+  for (int j = 0; j < md.n; ++j) longest = std::max(longest, md.A.start[j + 1] - md.A.start[j]);
+  const double mean = static_cast<double>(md.A.nnz()) / md.n;
+  return static_cast<double>(longest) >= 16.0 * std::max(mean, 1.0);
+}
+
+// The columns by decreasing count, ties in the model's order: order[k] is the model column that
+// becomes column k.
+std::vector<int> degree_order(const Model& md) {
+  std::vector<int> order(static_cast<size_t>(md.n));
+  for (int j = 0; j < md.n; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return md.A.start[a + 1] - md.A.start[a] > md.A.start[b + 1] - md.A.start[b];
+  });
+  return order;
+}
+
+template <typename T>
+std::vector<T> gather(const std::vector<T>& v, const std::vector<int>& order) {
+  std::vector<T> out(order.size());
+  for (size_t k = 0; k < order.size(); ++k) out[k] = v[order[k]];
+  return out;
+}
+
+// The model with its columns in `order` (rows and their order unchanged).
+std::shared_ptr<const Model> permute_columns(const Model& md, const std::vector<int>& order) {
+  auto out = std::make_shared<Model>();
+  out->m = md.m;
+  out->n = md.n;
+  out->maximize = md.maximize;
+  out->offset = md.offset;
+  out->A.rows = md.A.rows;
+  out->A.cols = md.A.cols;
+  out->A.start.assign(static_cast<size_t>(md.n) + 1, 0);
+  out->A.index.resize(md.A.index.size());
+  out->A.value.resize(md.A.value.size());
+  int64_t z = 0;
+  for (int k = 0; k < md.n; ++k) {
+    const int j = order[k];
+    const int64_t b = md.A.start[j], e = md.A.start[j + 1];
+    std::copy(md.A.index.begin() + b, md.A.index.begin() + e, out->A.index.begin() + z);
+    std::copy(md.A.value.begin() + b, md.A.value.begin() + e, out->A.value.begin() + z);
+    z += e - b;
+    out->A.start[k + 1] = z;
+  }
+  out->obj = gather(md.obj, order);
+  out->col_lower = gather(md.col_lower, order);
+  out->col_upper = gather(md.col_upper, order);
+  out->row_lower = md.row_lower;
+  out->row_upper = md.row_upper;
+  return out;
+}
+
+// Column vectors of a permuted solve back in the model's order.
+void unpermute(std::vector<double>& v, const std::vector<int>& order) {
+  if (v.size() != order.size()) return;
+  std::vector<double> out(v.size());
+  for (size_t k = 0; k < order.size(); ++k) out[order[k]] = v[k];
+  v = std::move(out);
+}
 
 // One model of solve_lp / solve_lp_batch between its PDLP rounds.
 struct LpJob {
   const Model* model = nullptr;
+  // A column-permuted copy of the model the job solves instead, and the permutation (order[k]:
+  // the model column at position k); empty order: the model itself.
+  std::shared_ptr<const Model> permuted;
+  std::vector<int> order;
+  std::vector<double> own_lower, own_upper;  // node bounds in the permuted order
   std::shared_ptr<const Scaling> scaled;
   std::shared_ptr<const LpProblem> lp;
   // Node relaxations: this job's column bounds (model space) and their computational-form
@@ -383,8 +503,10 @@ void absorb(LpJob& job, PdlpResult&& result, int round, const LpSolveOptions& op
   }
   VerifyReport rep;
   bool answered = true;
+  out.dual_bound = std::numeric_limits<double>::quiet_NaN();
   if (r.status == Status::kOptimal) {
     if (options.verify) rep = verify_lp_optimality(model, out.col_value.data(), out.row_dual.data());
+    out.dual_bound = safe_dual_bound(model, out.row_dual.data());
   } else if (r.status == Status::kInfeasible && !r.dual_ray.empty()) {
     out.infeasibility_certificate = r.dual_ray;
     for (int i = 0; i < m; ++i) out.infeasibility_certificate[i] *= scaled.row[i];
@@ -467,7 +589,14 @@ void run_jobs(std::vector<LpJob>& jobs, const LpSolveOptions& options, PdlpBatch
 std::vector<LpSolveResult> collect(std::vector<LpJob>& jobs) {
   std::vector<LpSolveResult> out;
   out.reserve(jobs.size());
-  for (LpJob& j : jobs) out.push_back(std::move(j.out));
+  for (LpJob& j : jobs) {
+    if (!j.order.empty()) {
+      unpermute(j.out.col_value, j.order);
+      unpermute(j.out.col_dual, j.order);
+      unpermute(j.out.unbounded_ray, j.order);
+    }
+    out.push_back(std::move(j.out));
+  }
   return out;
 }
 
@@ -479,8 +608,13 @@ std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& model
   std::vector<LpJob> jobs(static_cast<size_t>(K));
   for_each_parallel(K, [&](int i) {
     jobs[i].model = models[i];
-    auto sc = std::make_shared<Scaling>(compute_scaling(models[i]->A));
-    jobs[i].lp = std::make_shared<LpProblem>(make_problem(*models[i], *sc));
+    if (wants_degree_order(*models[i], options.pdlp)) {
+      jobs[i].order = degree_order(*models[i]);
+      jobs[i].permuted = permute_columns(*models[i], jobs[i].order);
+      jobs[i].model = jobs[i].permuted.get();
+    }
+    auto sc = std::make_shared<Scaling>(compute_scaling(jobs[i].model->A));
+    jobs[i].lp = std::make_shared<LpProblem>(make_problem(*jobs[i].model, *sc));
     jobs[i].scaled = std::move(sc);
   });
   run_jobs(jobs, options, info);
@@ -496,8 +630,23 @@ std::vector<LpSolveResult> solve_lp_bounds_batch(const Model& model,
                                                  PdlpBatchInfo* info) {
   const int K = static_cast<int>(col_lower.size());
   const int n = model.n, m = model.m;
-  auto scaled = std::make_shared<const Scaling>(compute_scaling(model.A));
-  auto lp = std::make_shared<const LpProblem>(make_problem(model, *scaled));
+  // A column order for the whole batch (the model's, or by decreasing count): the model, every
+  // node's bounds and the warm start follow it.
+  std::vector<int> order;
+  std::shared_ptr<const Model> permuted;
+  std::vector<double> start_perm;
+  if (wants_degree_order(model, options.pdlp)) {
+    order = degree_order(model);
+    permuted = permute_columns(model, order);
+    if (start_col_value != nullptr) {
+      start_perm.resize(static_cast<size_t>(n));
+      for (int k = 0; k < n; ++k) start_perm[k] = start_col_value[order[k]];
+      start_col_value = start_perm.data();
+    }
+  }
+  const Model& md = permuted ? *permuted : model;
+  auto scaled = std::make_shared<const Scaling>(compute_scaling(md.A));
+  auto lp = std::make_shared<const LpProblem>(make_problem(md, *scaled));
   // Warm start in the computational form (make_problem's space): x / col scale, the row
   // activities A x times the row scale, and y = -sense row dual / row scale.
   std::vector<double> sx, sy;
@@ -507,8 +656,8 @@ std::vector<LpSolveResult> solve_lp_bounds_batch(const Model& model,
     sy.assign(static_cast<size_t>(m), 0.0);
     std::vector<double> act(static_cast<size_t>(m), 0.0);
     for (int j = 0; j < n; ++j)
-      for (int64_t p = model.A.start[j]; p < model.A.start[j + 1]; ++p)
-        act[model.A.index[p]] += model.A.value[p] * start_col_value[j];
+      for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p)
+        act[md.A.index[p]] += md.A.value[p] * start_col_value[j];
     for (int j = 0; j < n; ++j) sx[j] = start_col_value[j] / scaled->col[j];
     for (int i = 0; i < m; ++i) sx[n + i] = act[i] * scaled->row[i];
     if (start_row_dual != nullptr)
@@ -517,16 +666,27 @@ std::vector<LpSolveResult> solve_lp_bounds_batch(const Model& model,
   std::vector<LpJob> jobs(static_cast<size_t>(K));
   for_each_parallel(K, [&](int k) {
     LpJob& job = jobs[k];
-    job.model = &model;
+    job.model = &md;
     job.scaled = scaled;
     job.lp = lp;
     job.col_lower = col_lower[k];
     job.col_upper = col_upper[k];
+    if (!order.empty()) {
+      job.order = order;
+      job.own_lower.resize(static_cast<size_t>(n));
+      job.own_upper.resize(static_cast<size_t>(n));
+      for (int q = 0; q < n; ++q) {
+        job.own_lower[q] = col_lower[k][order[q]];
+        job.own_upper[q] = col_upper[k][order[q]];
+      }
+      job.col_lower = job.own_lower.data();
+      job.col_upper = job.own_upper.data();
+    }
     job.lower = lp->lower;
     job.upper = lp->upper;
     for (int j = 0; j < n; ++j) {
-      job.lower[j] = col_lower[k][j] / scaled->col[j];
-      job.upper[j] = col_upper[k][j] / scaled->col[j];
+      job.lower[j] = job.col_lower[j] / scaled->col[j];
+      job.upper[j] = job.col_upper[j] / scaled->col[j];
     }
     if (!sx.empty()) {
       job.start_x = sx;

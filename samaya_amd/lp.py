@@ -190,6 +190,8 @@ class PdlpOptions:
     fp32_stall_level: float = 1e-4  # mixed: FP32 plateaus below this switch to FP64 (0: never)
     device_restart: bool = True     # restart decisions, distances and primal weights on the GPU
     fp64_accumulation: bool = False  # mixed: FP32 steps sum their products in FP64
+    device_scaling: bool = True      # enhanced, one LP: preconditioner and norm bound on the GPU
+    column_order: str = "auto"       # model-level solves: "auto", "natural" or "degree"
     inject_fault_step: int = -1      # test hook: the device fails its steps after this many
 
     def native(self) -> N.SmyPdlpParams:
@@ -215,6 +217,8 @@ class PdlpOptions:
         p.device_restart = int(self.device_restart)
         p.fp64_accumulation = int(self.fp64_accumulation)
         p.inject_fault_step = int(self.inject_fault_step)
+        p.device_scaling = int(self.device_scaling)
+        p.column_order = {"auto": 0, "natural": 1, "degree": 2}[self.column_order]
         return p
 
 
@@ -310,6 +314,7 @@ class LpResult:
     message: str
     omega: float = 1.0  # the last solve's final primal weight (start_omega of a warm start)
     rejected_steps: int = 0  # adaptive steps: rejected attempts of the last solve
+    dual_bound: float = float("nan")  # optimal answers: weak-duality bound from the row duals
 
     @property
     def status_name(self) -> str:
@@ -344,7 +349,7 @@ def _lp_result(res: "N.SmyLpResult", x, yd, d, act, cert) -> LpResult:
         gap=res.pdlp.gap, setup_seconds=res.pdlp.setup_seconds,
         solve_seconds=res.pdlp.solve_seconds, kernel_ms=res.pdlp.kernel_ms,
         message=res.message.decode(errors="replace"), omega=res.pdlp.omega,
-        rejected_steps=res.pdlp.rejected_steps)
+        rejected_steps=res.pdlp.rejected_steps, dual_bound=res.dual_bound)
 
 
 @dataclass
@@ -414,6 +419,15 @@ def verify_lp_optimality(model: Model | NativeModel, x, y, primal_tol=1e-6, dual
     msg = C.create_string_buffer(256)
     ok = N.load().smy_verify_lp(nm.h, _p(_f64(x)), _p(_f64(y)), primal_tol, dual_tol, _p(out), msg, 256)
     return bool(ok == 1), float(out[0]), float(out[1]), float(out[2]), msg.value.decode()
+
+
+def dual_bound(model: Model | NativeModel, y) -> float:
+    """Weak-duality bound on the model's optimal value from row duals y (d = c - A'y convention):
+    lower for minimization, upper for maximization (smy_lp_dual_bound)."""
+    nm = model if isinstance(model, NativeModel) else model.native()
+    out = C.c_double()
+    N.check(N.load().smy_lp_dual_bound(nm.h, _p(_f64(y)), C.byref(out)), "lp_dual_bound")
+    return out.value
 
 
 def verify_infeasibility(model: Model | NativeModel, y):
