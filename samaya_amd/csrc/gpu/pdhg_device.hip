@@ -1167,16 +1167,26 @@ int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_sc
   };
   fill_ones(cs, n);
   fill_ones(rs, m);
+  const auto cols_reduce = [&](auto term, auto max_tag, double* dst) {
+    constexpr bool kMax = decltype(max_tag)::value;
+    hipLaunchKernelGGL((segment_wave_kernel<decltype(term), kMax>), dim3(wave_blocks(n)), dim3(kBlock), 0,
+                       s, n, At.ptr, term, dst);
+  };
+  const auto rows_reduce = [&](auto term, auto max_tag, double* dst) {
+    constexpr bool kMax = decltype(max_tag)::value;
+    hipLaunchKernelGGL((segment_wave_kernel<decltype(term), kMax>), dim3(wave_blocks(m)), dim3(kBlock), 0,
+                       s, m, A.ptr, term, dst);
+  };
+  using Sum = std::false_type;
+  using Max = std::true_type;
   // Norms of the termination test from the unscaled matrix (src/lp/pdlp.cpp:377-394).
   double a_colmax = 0.0, at_rowmax = 0.0;
   if (m > 0 && n > 0) {
-    hipLaunchKernelGGL((segment_norm_kernel<kSegSumSq, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                       n, At.ptr, At.idx, At.val64, rs, cs, cn);
+    cols_reduce(Square{At.val64}, Sum{}, cn);
     hipLaunchKernelGGL(sqrt_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cn);
     hipLaunchKernelGGL((reduce_finish_kernel<true>), dim3(1), dim3(kReduceThreads), 0, s,
                        static_cast<long long>(n), cn, scalar);
-    hipLaunchKernelGGL((segment_norm_kernel<kSegSumSq, true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
-                       m, A.ptr, A.idx, A.val64, rs, cs, rn);
+    rows_reduce(Square{A.val64}, Sum{}, rn);
     hipLaunchKernelGGL(sqrt_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rn);
     hipLaunchKernelGGL((reduce_finish_kernel<true>), dim3(1), dim3(kReduceThreads), 0, s,
                        static_cast<long long>(m), rn, scalar + 1);
@@ -1188,17 +1198,15 @@ int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_sc
     at_rowmax = two[1];
   }
   if (m > 0 && n > 0) {
+    const ScaledAbs col_term{At.idx, At.val64, rs, cs, false};
+    const ScaledAbs row_term{A.idx, A.val64, rs, cs, true};
     const auto round = [&](bool one_norm) {
       if (one_norm) {
-        hipLaunchKernelGGL((segment_norm_kernel<kSegSum, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                           n, At.ptr, At.idx, At.val64, rs, cs, cn);
-        hipLaunchKernelGGL((segment_norm_kernel<kSegSum, true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
-                           m, A.ptr, A.idx, A.val64, rs, cs, rn);
+        cols_reduce(col_term, Sum{}, cn);
+        rows_reduce(row_term, Sum{}, rn);
       } else {
-        hipLaunchKernelGGL((segment_norm_kernel<kSegMax, false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                           n, At.ptr, At.idx, At.val64, rs, cs, cn);
-        hipLaunchKernelGGL((segment_norm_kernel<kSegMax, true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
-                           m, A.ptr, A.idx, A.val64, rs, cs, rn);
+        cols_reduce(col_term, Max{}, cn);
+        rows_reduce(row_term, Max{}, rn);
       }
       hipLaunchKernelGGL(divide_by_sqrt_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cs, cn);
       hipLaunchKernelGGL(divide_by_sqrt_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rs, rn);
@@ -1228,8 +1236,7 @@ int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_sc
     double sigma = 0.0;
     bool zero = false;
     for (int it = 0; it < 20; ++it) {
-      hipLaunchKernelGGL(power_w_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, At.ptr, At.idx,
-                         At.val64, rs, cs, u, w);
+      cols_reduce(PowerW{At.idx, At.val64, rs, cs, u}, Sum{}, w);
       SMY_HIP(hipMemcpyAsync(hw.data(), w, hw.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       SMY_HIP(hipStreamSynchronize(s));
       const double wn = norm(hw);
@@ -1237,8 +1244,7 @@ int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_sc
         zero = true;
         break;
       }
-      hipLaunchKernelGGL(power_u_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, A.ptr, A.idx,
-                         A.val64, rs, cs, w, u);
+      rows_reduce(PowerU{A.idx, A.val64, rs, cs, w}, Sum{}, u);
       SMY_HIP(hipMemcpyAsync(hu.data(), u, hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       SMY_HIP(hipStreamSynchronize(s));
       un = norm(hu);
@@ -1257,9 +1263,9 @@ int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_sc
   }
   if (m > 0 && n > 0) {
     // The scaled copies: A by rows (a * cs_j) * rs_i, A' by rows (a * rs_i) * cs_j.
-    hipLaunchKernelGGL((scale_values_kernel<true>), dim3(blocks_for(m)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL((scale_values_wave_kernel<true>), dim3(wave_blocks(m)), dim3(kBlock), 0, s,
                        m, A.ptr, A.idx, A.val64, rs, cs);
-    hipLaunchKernelGGL((scale_values_kernel<false>), dim3(blocks_for(n)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL((scale_values_wave_kernel<false>), dim3(wave_blocks(n)), dim3(kBlock), 0, s,
                        n, At.ptr, At.idx, At.val64, rs, cs);
     SMY_HIP(hipGetLastError());
   }

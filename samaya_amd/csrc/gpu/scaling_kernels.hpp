@@ -14,34 +14,108 @@
 
 namespace smy {
 
-enum SegmentNorm { kSegMax = 0, kSegSum = 1, kSegSumSq = 2 };
+// Per-row reductions of a CSR copy take one wavefront per row, for rows of any length (the
+// 1M-column LP has columns of 5,000 entries, which one thread would walk for milliseconds): the
+// 64 lanes form the terms of 64 consecutive entries at once, then every lane adds them in entry
+// order from the lanes' registers (v_readlane), so the total is the sequential one, bit for bit.
+// The power iteration's products are formed with the host's scaled values: A_eq' entries
+// (a * cs_j) * rs_i, A_eq entries (a * rs_i) * cs_j.
+__device__ inline double lane_value(double v, int lane) {
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  return __hiloint2double(hi, lo);
+}
 
-// out[s] over the entries of row s of a CSR copy: max or sum of |(a * rs_i) * cs_j|, or the sum
-// of a^2 (the unscaled 2-norm's square). SegIsRow: row s of the copy is row i of A (indices are
-// columns j); otherwise it is column j of A (indices are rows i).
-template <int Kind, bool SegIsRow>
-__global__ __launch_bounds__(kBlock) void segment_norm_kernel(int segs, const int* __restrict__ ptr,
-                                                              const int* __restrict__ idx,
-                                                              const double* __restrict__ val,
-                                                              const double* __restrict__ rs,
-                                                              const double* __restrict__ cs,
-                                                              double* __restrict__ out) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= segs) return;
+struct ScaledAbs {  // |(a * rs_i) * cs_j|
+  const int* __restrict__ idx;
+  const double* __restrict__ val;
+  const double* __restrict__ rs;
+  const double* __restrict__ cs;
+  bool seg_is_row;
+  __device__ double operator()(int s, int p) const {
+    const int i = seg_is_row ? s : idx[p];
+    const int j = seg_is_row ? idx[p] : s;
+    return fabs(val[p] * rs[i] * cs[j]);
+  }
+};
+
+struct Square {  // a^2
+  const double* __restrict__ val;
+  __device__ double operator()(int, int p) const { return val[p] * val[p]; }
+};
+
+struct PowerW {  // ((a * cs_j) * rs_i) * u_i over column j
+  const int* __restrict__ idx;
+  const double* __restrict__ val;
+  const double* __restrict__ rs;
+  const double* __restrict__ cs;
+  const double* __restrict__ u;
+  __device__ double operator()(int j, int p) const {
+    const int i = idx[p];
+    return val[p] * cs[j] * rs[i] * u[i];
+  }
+};
+
+struct PowerU {  // ((a * rs_i) * cs_j) * w_j over row i; w_j = 0 adds nothing (the host skips it)
+  const int* __restrict__ idx;
+  const double* __restrict__ val;
+  const double* __restrict__ rs;
+  const double* __restrict__ cs;
+  const double* __restrict__ w;
+  __device__ double operator()(int i, int p) const {
+    const int j = idx[p];
+    const double wj = w[j];
+    return wj == 0.0 ? 0.0 : val[p] * rs[i] * cs[j] * wj;
+  }
+};
+
+// out[s] = sum (Max: max) of term(s, p) over row s's entries, in entry order.
+template <typename Term, bool Max>
+__global__ __launch_bounds__(kBlock) void segment_wave_kernel(int segs, const int* __restrict__ ptr,
+                                                              Term term, double* __restrict__ out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  if (s >= segs) return;  // uniform over the wavefront
+  const int b = ptr[s], e = ptr[s + 1];
   double acc = 0.0;
-  const int end = ptr[s + 1];
-  for (int p = ptr[s]; p < end; ++p) {
-    if constexpr (Kind == kSegSumSq) {
-      const double a = val[p];
-      acc += a * a;
-    } else {
-      const int i = SegIsRow ? s : idx[p];
-      const int j = SegIsRow ? idx[p] : s;
-      const double t = fabs(val[p] * rs[i] * cs[j]);
-      acc = Kind == kSegMax ? fmax(acc, t) : acc + t;
+  for (int c = b; c < e; c += kWave) {
+    const int p = c + lane;
+    const double t = p < e ? term(s, p) : 0.0;
+    const int k = min(kWave, e - c);
+    for (int q = 0; q < k; ++q) {
+      const double v = lane_value(t, q);
+      if constexpr (Max)
+        acc = fmax(acc, v);
+      else
+        acc += v;
     }
   }
-  out[s] = acc;
+  if (lane == 0) out[s] = acc;
+}
+
+// In place, one wavefront per row of the copy: A by rows (a * cs_j) * rs_i, A' by rows
+// (a * rs_i) * cs_j.
+template <bool SegIsRow>
+__global__ __launch_bounds__(kBlock) void scale_values_wave_kernel(int segs, const int* __restrict__ ptr,
+                                                                   const int* __restrict__ idx,
+                                                                   double* __restrict__ val,
+                                                                   const double* __restrict__ rs,
+                                                                   const double* __restrict__ cs) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  if (s >= segs) return;
+  const int e = ptr[s + 1];
+  for (int p = ptr[s] + lane; p < e; p += kWave) {
+    if constexpr (SegIsRow)
+      val[p] = val[p] * cs[idx[p]] * rs[s];
+    else
+      val[p] = val[p] * rs[idx[p]] * cs[s];
+  }
+}
+
+inline int wave_blocks(long long segs) {
+  constexpr int per = kBlock / kWave;
+  return static_cast<int>((segs + per - 1) / per);
 }
 
 // f[k] /= sqrt(norm[k]) where norm[k] > 0 (the factors of empty rows and columns stay).
@@ -49,25 +123,6 @@ __global__ __launch_bounds__(kBlock) void divide_by_sqrt_kernel(int count, doubl
                                                                 const double* __restrict__ norm) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k < count && norm[k] > 0.0) f[k] /= sqrt(norm[k]);
-}
-
-// In place: A by rows gets (a * cs_j) * rs_i (the host scales A' that way and uploads it as A by
-// rows), A' by rows gets (a * rs_i) * cs_j.
-template <bool SegIsRow>
-__global__ __launch_bounds__(kBlock) void scale_values_kernel(int segs, const int* __restrict__ ptr,
-                                                              const int* __restrict__ idx,
-                                                              double* __restrict__ val,
-                                                              const double* __restrict__ rs,
-                                                              const double* __restrict__ cs) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= segs) return;
-  const int end = ptr[s + 1];
-  for (int p = ptr[s]; p < end; ++p) {
-    if constexpr (SegIsRow)
-      val[p] = val[p] * cs[idx[p]] * rs[s];
-    else
-      val[p] = val[p] * rs[idx[p]] * cs[s];
-  }
 }
 
 // The scaled cost and bounds: structural j: cost * cs, bounds / cs; logical n + i: all * rs.
@@ -87,48 +142,6 @@ __global__ __launch_bounds__(kBlock) void scale_vectors_kernel(int n, int m, con
     lower[k] *= r;
     upper[k] *= r;
   }
-}
-
-// The power iteration's products on the unscaled values and the factors, one thread per row
-// summing in the order the host's scatter adds (csrc/host/pdlp.cpp, operator_norm_bound), with the
-// host's scaled values: w = A_eq' u with A_eq' entries (a * cs_j) * rs_i, and u = A_eq w with
-// A_eq entries (a * rs_i) * cs_j, skipping w_j = 0 as the host does.
-__global__ __launch_bounds__(kBlock) void power_w_kernel(int n, const int* __restrict__ ptr,
-                                                         const int* __restrict__ idx,
-                                                         const double* __restrict__ val,
-                                                         const double* __restrict__ rs,
-                                                         const double* __restrict__ cs,
-                                                         const double* __restrict__ u,
-                                                         double* __restrict__ w) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  double acc = 0.0;
-  const double c = cs[j];
-  for (int p = ptr[j]; p < ptr[j + 1]; ++p) {
-    const int i = idx[p];
-    acc += val[p] * c * rs[i] * u[i];
-  }
-  w[j] = acc;
-}
-
-__global__ __launch_bounds__(kBlock) void power_u_kernel(int m, const int* __restrict__ ptr,
-                                                         const int* __restrict__ idx,
-                                                         const double* __restrict__ val,
-                                                         const double* __restrict__ rs,
-                                                         const double* __restrict__ cs,
-                                                         const double* __restrict__ w,
-                                                         double* __restrict__ u) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= m) return;
-  double acc = 0.0;
-  const double r = rs[i];
-  for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
-    const int j = idx[p];
-    const double wj = w[j];
-    if (wj == 0.0) continue;
-    acc += val[p] * r * cs[j] * wj;
-  }
-  u[i] = acc;
 }
 
 constexpr int kReduceThreads = 1024;

@@ -155,6 +155,9 @@ def test_node_relaxations_match_single_solves(gpu, precision, algorithm):
         # HiGHS's objective of the same node (golden node_relaxations): within 1e-6 once polished.
         want = highs[(73, 900 + k)]
         assert batch[k].verified and abs(batch[k].objective - want) <= 1e-6 * (1 + abs(want))
+        # The node's bound for branch and bound (a maximization: an upper bound) from its duals
+        # never falls below HiGHS's optimum and is tight to 1e-6.
+        assert want <= batch[k].dual_bound <= want + 1e-6 * (1 + abs(want))
         # The answer respects the node's fixings.
         fixed = lows[k] == ups[k]
         assert np.max(np.abs(batch[k].col_value[fixed] - lows[k][fixed])) <= 1e-6
