@@ -5,11 +5,13 @@
 // The functions below only marshal plain arrays into the reference's types and call its code:
 // samaya::compute_scaling (src/linalg/scaling.cpp:130), samaya::make_problem
 // (src/lp/lp_solver.cpp:28), samaya::solve_pdlp (src/lp/pdlp.cpp:228), the verifier
-// (src/verify/verify.cpp:251,269,345), and the test generators random_lp
+// (src/verify/verify.cpp:251,269,345), the MPS reader read_mps_from_string
+// (src/io/mps_reader.cpp:516-527), and the test generators random_lp
 // (tests/lp_generators.hpp:33) and ReferenceLp (tests/reference_lp.hpp:20). The PDLP result
 // mapping mirrors solve_pdlp_lp (src/core/solver.cpp:405-437), which is not compiled here
 // because solver.cpp pulls in every other method of the library.
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +25,7 @@
 #include "lp/pdlp.hpp"
 #include "lp_generators.hpp"
 #include "reference_lp.hpp"
+#include "samaya/io.hpp"
 #include "samaya/model.hpp"
 #include "samaya/verify.hpp"
 
@@ -237,5 +240,83 @@ int ref_make_problem(const ref_model* model, double* col_scale, double* row_scal
   for (int64_t q = 0; q < lp.A.nnz(); ++q) a_value[q] = lp.A.values()[q];
   return static_cast<int>(lp.A.nnz());
 }
+
+// read_mps_from_string on `len` bytes of text: a handle to the parsed model, or null with the
+// ParseError's message in err and its line in *line. *seconds: the parse's wall time.
+void* ref_mps_read(const char* text, long long len, char* err, int err_cap, long long* line,
+                   double* seconds) {
+  *line = 0;
+  if (err_cap > 0) err[0] = '\0';
+  const auto t0 = std::chrono::steady_clock::now();
+  try {
+    auto* md = new samaya::Model(samaya::read_mps_from_string(std::string_view(text, static_cast<size_t>(len))));
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return md;
+  } catch (const samaya::ParseError& e) {
+    std::snprintf(err, static_cast<size_t>(err_cap), "%s", e.what());
+    *line = e.line();
+  } catch (const std::exception& e) {
+    std::snprintf(err, static_cast<size_t>(err_cap), "%s", e.what());
+  }
+  *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return nullptr;
+}
+
+// out: m, n, nnz, maximize, then the bytes of the row names, the column names (each followed by
+// '\n') and the model name.
+void ref_mps_dims(const void* h, long long* out) {
+  const auto& md = *static_cast<const samaya::Model*>(h);
+  long long rb = 0, cb = 0;
+  for (const std::string& s : md.row_names) rb += static_cast<long long>(s.size()) + 1;
+  for (const std::string& s : md.col_names) cb += static_cast<long long>(s.size()) + 1;
+  out[0] = md.num_rows();
+  out[1] = md.num_cols();
+  out[2] = md.A.nnz();
+  out[3] = md.sense == samaya::ObjSense::kMaximize ? 1 : 0;
+  out[4] = rb;
+  out[5] = cb;
+  out[6] = static_cast<long long>(md.name.size());
+}
+
+// Copies a parsed model out: arrays into buf (sized by ref_mps_dims), col_type (n bytes, 1 =
+// integer), the names joined with '\n' terminators, the model name (not terminated). Returns the
+// number of quadratic entries (0 for an LP).
+long long ref_mps_copy(const void* h, ref_model_buf* buf, char* col_type, char* row_names,
+                       char* col_names, char* name) {
+  const auto& md = *static_cast<const samaya::Model*>(h);
+  const int m = md.num_rows(), n = md.num_cols();
+  buf->m = m;
+  buf->n = n;
+  buf->maximize = md.sense == samaya::ObjSense::kMaximize ? 1 : 0;
+  buf->offset = md.obj_offset;
+  buf->nnz = md.A.nnz();
+  const auto start = md.A.col_start();
+  const auto index = md.A.row_index();
+  for (int j = 0; j <= n; ++j) buf->start[j] = start[j];
+  for (long long q = 0; q < buf->nnz; ++q) {
+    buf->index[q] = index[q];
+    buf->value[q] = md.A.values()[q];
+  }
+  put(md.obj, buf->obj);
+  put(md.col_lower, buf->col_lower);
+  put(md.col_upper, buf->col_upper);
+  put(md.row_lower, buf->row_lower);
+  put(md.row_upper, buf->row_upper);
+  for (int j = 0; j < n; ++j) col_type[j] = md.col_type[j] == samaya::VarType::kInteger ? 1 : 0;
+  for (const std::string& s : md.row_names) {
+    std::memcpy(row_names, s.data(), s.size());
+    row_names += s.size();
+    *row_names++ = '\n';
+  }
+  for (const std::string& s : md.col_names) {
+    std::memcpy(col_names, s.data(), s.size());
+    col_names += s.size();
+    *col_names++ = '\n';
+  }
+  std::memcpy(name, md.name.data(), md.name.size());
+  return md.Q.nnz() + static_cast<long long>(md.row_quadratic.size());
+}
+
+void ref_mps_free(void* h) { delete static_cast<samaya::Model*>(h); }
 
 }  // extern "C"

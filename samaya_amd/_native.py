@@ -41,6 +41,9 @@ EXPORTS = (
     "smy_solve_pdlp", "smy_solve_lp", "smy_solve_lp_batch", "smy_solve_lp_bounds_batch",
     "smy_generate_fixings", "smy_solve", "smy_verify_lp", "smy_verify_infeasibility",
     "smy_verify_unbounded_ray", "smy_gpu_info",
+    "smy_read_mps", "smy_read_mps_text", "smy_model_name", "smy_model_names",
+    "smy_model_set_names", "smy_model_set_integrality", "smy_model_integrality", "smy_write_mps",
+    "smy_write_mps_text", "smy_write_solution", "smy_solution_text",
 )
 
 
@@ -200,6 +203,20 @@ def load() -> C.CDLL:
     _proto(lib, "smy_verify_infeasibility", i32, vp, vp, C.c_char_p, C.c_size_t)
     _proto(lib, "smy_verify_unbounded_ray", i32, vp, vp, C.c_char_p, C.c_size_t)
     _proto(lib, "smy_gpu_info", i32, C.POINTER(i32), C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_read_mps", i32, C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_read_mps_text", i32, C.c_char_p, C.c_size_t, i32, C.POINTER(vp), C.c_char_p,
+           C.c_size_t, C.POINTER(i64))
+    _proto(lib, "smy_model_name", i32, vp, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_model_names", i64, vp, i32, vp, C.c_size_t)
+    _proto(lib, "smy_model_set_names", i32, vp, i32, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_model_set_integrality", i32, vp, vp)
+    _proto(lib, "smy_model_integrality", i64, vp, vp)
+    _proto(lib, "smy_write_mps", i32, vp, C.c_char_p, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_write_mps_text", i64, vp, vp, C.c_size_t)
+    _proto(lib, "smy_write_solution", i32, vp, C.c_char_p, C.POINTER(SmyLpResult), vp, vp, vp, vp,
+           vp, C.c_char_p, C.c_size_t)
+    _proto(lib, "smy_solution_text", i64, vp, C.POINTER(SmyLpResult), vp, vp, vp, vp, vp, vp,
+           C.c_size_t)
     _lib = lib
     return lib
 

@@ -36,10 +36,12 @@ SOURCES = [
     SRC / "host" / "verify.cpp",
     SRC / "host" / "generators.cpp",
     SRC / "host" / "capi.cpp",
+    SRC / "host" / "mps.cpp",
+    SRC / "host" / "capi_io.cpp",
 ]
 
 COMMON = [
-    "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
+    "-O3", "-std=c++20", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
     f"-I{ROOT / 'include'}",
 ]
 HIP_FLAGS = [
@@ -75,7 +77,7 @@ def build(verbose: bool = False) -> Path:
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as pool:
         objs = list(pool.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), "-shared", "--offload-arch=gfx950", "-fPIC", "-Wl,--no-undefined", *map(str, objs),
+    cmd = [hipcc(), "-shared", "--offload-arch=gfx950", "-fPIC", "-pthread", "-Wl,--no-undefined", *map(str, objs),
            "-o", str(tmp)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:

@@ -7,17 +7,15 @@
 #include <exception>
 #include <limits>
 #include <memory>
+#include <stdexcept>
 
 #include "../../../include/samaya_pdhg.h"
 #include "../../../include/samaya_pdlp.h"
+#include "capi_model.hpp"
 #include "generators.hpp"
 #include "pdlp.hpp"
 
 using namespace smy::host;
-
-struct smy_model {
-  Model model;
-};
 
 namespace {
 

@@ -55,6 +55,12 @@ class Model:
     row_upper: np.ndarray
     maximize: bool = False
     offset: float = 0.0
+    # What an MPS file gives a model besides the LP (include/samaya/model.hpp name, row_names,
+    # col_names, col_type): empty / None for models built from arrays.
+    name: str = ""
+    row_names: list | None = None
+    col_names: list | None = None
+    col_integer: np.ndarray | None = None  # uint8[n], 1 = integer (the LP path solves the relaxation)
 
     def __post_init__(self):
         self.start = np.ascontiguousarray(self.start, dtype=np.int64)
@@ -122,6 +128,20 @@ class NativeModel:
                                           _p(md.col_upper), _p(md.row_lower), _p(md.row_upper),
                                           C.byref(self.h)), "model_create")
         self.m, self.n = md.m, md.n
+        for which, names in ((0, md.row_names), (1, md.col_names)):
+            if names:
+                blob = "".join(f"{x}\n" for x in names).encode()
+                N.check(self.lib.smy_model_set_names(self.h, which, blob, len(blob)), "set_names")
+        if md.col_integer is not None:
+            marks = np.ascontiguousarray(md.col_integer, dtype=np.uint8)
+            N.check(self.lib.smy_model_set_integrality(self.h, _p(marks)), "set_integrality")
+
+    @classmethod
+    def _adopt(cls, h, m: int, n: int) -> "NativeModel":
+        """Wraps a handle the library made (it is freed with the object)."""
+        self = cls.__new__(cls)
+        self.lib, self.h, self.m, self.n = N.load(), h, m, n
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -17,6 +17,9 @@ HiGHS objectives of the larger GPU test models (c1, knapsack nodes, a blending s
   python tests/golden/make_golden.py highs-fixtures
 The restatement-to-reference speed ratio on one core (cpu_calibration.json):
   python tests/golden/make_golden.py cpu-calibration
+The reference's MPS reader on its instance files, reader test texts and generated models
+(mps_reference.json):
+  python tests/golden/make_golden.py mps
 """
 from __future__ import annotations
 
@@ -172,6 +175,183 @@ def c2_reference() -> None:
     write("c2_reference", out)
 
 
+# MPS reader cases: the reference's instance files (tests/instances, copied under golden/mps) and
+# texts covering what tests/test_mps_reader.cpp checks (row types with ranges and an objective
+# offset, every bound type, integer markers, OBJSENSE with CRLF line ends, fixed MPS with blanks
+# in names, malformed input) plus further edge cases: duplicate entries, a column listed again
+# after others, OBJNAME, named RHS / RANGES sets, +-1e30, empty columns.
+MPS_FILES = ["tiny_lp", "haverly1", "haverly2", "haverly3", "mrpl_crude_small", "neos-2657525-crna"]
+
+
+def _fixed(*fields):
+    """A fixed-MPS data line with fields from columns 2, 5, 15, 25, 40 and 50."""
+    line = [" "] * 61
+    for start, text in zip((1, 4, 14, 24, 39, 49), fields):
+        line[start:start + len(text)] = list(text)
+    return "".join(line).rstrip() + "\n"
+
+
+def mps_texts() -> list[tuple[str, str]]:
+    head = "NAME T\nROWS\n N cost\n L lim\nCOLUMNS\n"
+    cases = [
+        ("row_types_ranges_offset",
+         "NAME RNG\nROWS\n N cost\n E eq_up\n E eq_down\n L le\n G ge\n N spare\nCOLUMNS\n"
+         " x cost 1 eq_up 1\n x eq_down 1 le 1\n x ge 1 spare 5\nRHS\n cost -2.5\n"
+         " eq_up 10 eq_down 10\n le 10 ge 10\nRANGES\n r eq_up 4 eq_down -4\n r le 3 ge -3\nENDATA\n"),
+        ("bound_types",
+         "NAME BND\nROWS\n N cost\n L lim\nCOLUMNS\n" +
+         "".join(f" {c} lim 1\n" for c in "abcdefghijk") +
+         "RHS\n lim 1\nBOUNDS\n UP B a 5\n LO B b -2\n FX B c 3.5\n FR B d\n MI B e\n"
+         " UP B f -1\n BV B g\n LI B h 2\n UI B h 9\n LO B i 1\n UP B i 1e30\n"
+         " LO B j -1\n UP B j -0.5\n PL B k\n MI k\nENDATA\n"),
+        ("integer_markers",
+         "NAME MIP\nROWS\n N cost\n G c\nCOLUMNS\n x cost 1 c 1\n M 'MARKER' 'INTORG'\n"
+         " y cost 2 c 1\n z cost 3 c 1\n M 'MARKER' 'INTEND'\n w cost 4 c 1\nRHS\n RHS c 1\nENDATA\n"),
+        ("objsense_inline_crlf",
+         "NAME X\r\nOBJSENSE MAXIMIZE\r\nROWS\r\n N cost\r\nCOLUMNS\r\n x cost 1\r\nENDATA\r\n"),
+        ("objsense_section",
+         "NAME X\nOBJSENSE\n    MAX\nROWS\n N cost\n L r\nCOLUMNS\n x cost 1 r 2\nRHS\n R r 3\nENDATA\n"),
+        ("fixed_format_blank_names",
+         "NAME          FIXED\nROWS\n" + _fixed("N", "COST") + _fixed("L", "LIM 1") + _fixed("G", "MY ROW")
+         + "COLUMNS\n" + _fixed("", "MARKER", "'MARKER'", "", "'INTORG'")
+         + _fixed("", "X ONE", "COST", "1.0", "LIM 1", "1.0") + _fixed("", "X ONE", "MY ROW", "1.0")
+         + _fixed("", "MARKER", "'MARKER'", "", "'INTEND'")
+         + _fixed("", "X TWO", "COST", "2.", "LIM 1", "1.") + _fixed("", "X TWO", "MY ROW", "-1.")
+         + "RHS\n" + _fixed("", "RHS 1", "LIM 1", "4.", "MY ROW", "-2.") + "RANGES\n"
+         + _fixed("", "RNG 1", "LIM 1", "3.") + "BOUNDS\n" + _fixed("UP", "BND 1", "X TWO", "3.")
+         + "ENDATA\n"),
+        ("duplicates_and_repeats",
+         head + " x lim 1 cost 1\n y lim 2\n x lim 0.5\n y cost 3 lim -2\n x cost 0.25\n"
+         " z lim 1e-300 cost 0\n x lim 0.125\nRHS\n RHS lim 4\nENDATA\n"),
+        ("objname_and_sets",
+         "NAME O\nOBJNAME\n second\nROWS\n N first\n N second\n G g\n E e\nCOLUMNS\n"
+         " a first 9 second 1\n a g 1 e 1\n b second -1 g 2\nRHS\n RHS1 g 1 e 2\n RHS1 second 7\n"
+         "RANGES\n RNG e 0\nBOUNDS\n UP BND1 a 4\nENDATA\n"),
+        ("infinite_values",
+         head + " x cost 1 lim 1\n y cost 1 lim 1\nRHS\n RHS lim 1e30\nBOUNDS\n LO B x -1e30\n"
+         " UP B y 1e31\n UP B x -1e30\nENDATA\n"),
+        ("empty_column_and_row",
+         "NAME E\nROWS\n N cost\n L used\n L unused\nCOLUMNS\n x cost 0\n y used 1\nRHS\n"
+         " RHS used 1 unused -1\nENDATA\n"),
+        ("comments_and_blank_lines",
+         "* comment\nNAME C\n\nROWS\n N cost\n* another\n L r\nCOLUMNS\n   \n x cost 1 r 1\n"
+         "RHS\n RHS r 2\nENDATA\n"),
+        ("numbers", head + " x cost +1.5e+2 lim -.5\n y cost 1E-3 lim 3.\nRHS\n RHS lim +7\nENDATA\n"),
+        ("no_enddata", head + " x cost 1 lim 1\nRHS\n RHS lim 1\n"),
+        ("crossed_bounds", "NAME X\nROWS\n N o\nCOLUMNS\n x o 1\nBOUNDS\n LO B x 2\n UP B x 1\nENDATA\n"),
+        # malformed (expected errors)
+        ("err_unknown_row", "NAME X\nROWS\n N obj\nCOLUMNS\n x nosuchrow 1\nENDATA\n"),
+        ("err_row_type", "NAME X\nROWS\n Q r\nENDATA\n"),
+        ("err_duplicate_row", "NAME X\nROWS\n N obj\n L r\n L r\nENDATA\n"),
+        ("err_bad_number", "NAME X\nROWS\n N o\nCOLUMNS\n x o abc\nENDATA\n"),
+        ("err_unknown_section", "NAME X\nBOGUS\nENDATA\n"),
+        ("err_semicontinuous", "NAME X\nROWS\n N o\nCOLUMNS\n x o 1\nBOUNDS\n SC B x 1\nENDATA\n"),
+        ("err_infinite_lower", "NAME X\nROWS\n N o\nCOLUMNS\n x o 1\nBOUNDS\n LO B x 1e30\nENDATA\n"),
+        ("err_unknown_column", head + " x lim 1\nBOUNDS\n UP B nosuchcol 1\nENDATA\n"),
+        ("err_unknown_bound", head + " x lim 1\nBOUNDS\n XX B x 1\nENDATA\n"),
+        ("err_field_count", head + " x lim 1 cost\nENDATA\n"),
+        ("err_marker", head + " M 'MARKER' 'INTWHAT'\nENDATA\n"),
+        ("err_data_outside", "NAME X\n x y 1\nENDATA\n"),
+        ("err_empty", "NAME X\nENDATA\n"),
+        ("err_empty_text", ""),
+        ("err_name_only", "NAME X"),
+        ("err_blank_lines_only", "NAME X\n\n\n"),
+        ("err_comment_only", "* only\n"),
+        ("err_range_on_objective", head + " x lim 1\nRANGES\n R cost 1\nENDATA\n"),
+        ("err_objname_late", "NAME X\nROWS\n N o\nOBJNAME o\nENDATA\n"),
+        ("err_sense", "NAME X\nOBJSENSE\n SIDEWAYS\nENDATA\n"),
+        ("err_late_line", head + "".join(f" c{k} lim 1\n" for k in range(200)) + " c7 nosuch 1\nENDATA\n"),
+        ("quadratic_objective",
+         "NAME QP\nROWS\n N obj\n E c\nCOLUMNS\n x obj 1 c 1\n y obj 1 c 1\nRHS\n c 1\nQUADOBJ\n"
+         " x x 2\n x y 1\n y y 4\nENDATA\n"),
+    ]
+    return cases
+
+
+def mps_entry(name: str, data: bytes, **extra) -> dict:
+    r = O.ref_read_mps_text(data)
+    e = {"name": name, **extra, "ok": r["ok"]}
+    if r["ok"]:
+        e.update({"model": r["model"].to_json(), "model_name": r["name"], "row_names": r["row_names"],
+                  "col_names": r["col_names"], "col_integer": r["col_integer"].tolist(),
+                  "quadratic": r["quadratic"]})
+    else:
+        e.update({"message": r["message"], "line": r["line"]})
+    return e
+
+
+def mps_fixtures() -> None:
+    """The reference's MPS reader on the cases above (mps_reference.json), and its time on the
+    1M-column LP's MPS text as this library writes it."""
+    import hashlib
+    import samaya_amd as smy
+    out = {"source": "reference read_mps_from_string (src/io/mps_reader.cpp) built by oracle/Makefile",
+           "cases": []}
+    for name in MPS_FILES:
+        out["cases"].append(mps_entry(name, (HERE / "mps" / f"{name}.mps").read_bytes(),
+                                      file=f"mps/{name}.mps"))
+    for name, text in mps_texts():
+        out["cases"].append(mps_entry(name, text.encode(), text=text))
+    # A generated model through this library's writer: the reference's parse of that text as
+    # array hashes (the text itself is regenerated by the test).
+    big = {}
+    for label, md in (("powerlaw_20k", smy.Model.generate(2, 77, 10_000, 20_000, 400, 8.0)),
+                      ("c2", smy.generators.c2())):
+        text = smy.mps_text(md).encode()
+        r = O.ref_read_mps_text(text)
+        assert r["ok"], r
+        rm = r["model"]
+        sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+        big[label] = {"text_bytes": len(text), "text_sha": hashlib.sha256(text).hexdigest(),
+                      "m": rm.m, "n": rm.n, "nnz": int(rm.start[-1]),
+                      "sha": {k: sha(getattr(rm, k)) for k in ("start", "index", "value", "obj",
+                                                              "col_lower", "col_upper", "row_lower",
+                                                              "row_upper")},
+                      "reference_seconds_this_container": r["seconds"]}
+        print(label, big[label]["text_bytes"], "bytes, reference parse", r["seconds"], "s")
+        if label == "powerlaw_20k":
+            # Malformed lines deep in the COLUMNS section (where the parse runs on several
+            # threads): line index -> replacement, with the reference's message and line.
+            lines = text.split(b"\n")
+            inj = []
+            for k, repl in ((90_001, b" C17 nosuchrow 1"), (150_000, b" C1 R3 1.5x"),
+                            (120_000, b" C5 R1"), (60_000, b" M 'MARKER' 'INTEND'")):
+                bad = b"\n".join(lines[:k] + [repl] + lines[k + 1:])
+                e = O.ref_read_mps_text(bad)
+                ent = {"index": k, "line": repl.decode(), "ok": e["ok"]}
+                if e["ok"]:
+                    ent["sha_value"] = sha(e["model"].value)
+                    ent["col_integer_sum"] = int(e["col_integer"].sum())
+                else:
+                    ent.update({"message": e["message"], "error_line": e["line"]})
+                inj.append(ent)
+            big[label]["injections"] = inj
+        del text, r, rm
+    out["generated"] = big
+    # HiGHS (scipy's dual simplex) on the LP relaxations of the LP / MIP instance files.
+    out["relaxations"] = {name: mps_relaxation(HERE / "mps" / f"{name}.mps")
+                          for name in ("tiny_lp", "mrpl_crude_small", "neos-2657525-crna")}
+    write("mps_reference", out)
+
+
+def mps_relaxation(path) -> dict:
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+    import samaya_amd as smy
+    md = smy.read_mps(path)
+    A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n)).tocsr()
+    eq = md.row_lower == md.row_upper
+    up = ~eq & np.isfinite(md.row_upper)
+    lo = ~eq & np.isfinite(md.row_lower)
+    res = linprog(-md.obj if md.maximize else md.obj, A_ub=sp.vstack([A[up], -A[lo]]),
+                  b_ub=np.concatenate([md.row_upper[up], -md.row_lower[lo]]), A_eq=A[eq],
+                  b_eq=md.row_upper[eq], bounds=np.stack([md.col_lower, md.col_upper], 1),
+                  method="highs-ds")
+    objective = ((-res.fun if md.maximize else res.fun) + md.offset) if res.status == 0 else None
+    return {"highs_status": int(res.status), "highs_objective": objective,
+            "integers": int(md.col_integer.sum())}
+
+
 def c2_highs(method: str = "highs-ds") -> None:
     """HiGHS (scipy) on the 1M-column LP: its objective merged into c2_reference.json."""
     import scipy.sparse as sp
@@ -282,6 +462,8 @@ if __name__ == "__main__":
         highs_fixtures()
     elif len(sys.argv) > 1 and sys.argv[1] == "cpu-calibration":
         cpu_calibration()
+    elif len(sys.argv) > 1 and sys.argv[1] == "mps":
+        mps_fixtures()
     elif len(sys.argv) > 1 and sys.argv[1] == "c2-highs":
         c2_highs(*sys.argv[2:3])
     else:

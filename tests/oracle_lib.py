@@ -163,6 +163,13 @@ def ref():
                                      C.POINTER(RLpOut)]
         lib.ref_verify_lp.argtypes = [C.POINTER(OModel), vp, vp, vp]
         lib.ref_make_problem.argtypes = [C.POINTER(OModel), vp, vp, vp, vp, vp, vp]
+        lib.ref_mps_read.argtypes = [C.c_char_p, C.c_longlong, C.c_char_p, C.c_int,
+                                     C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+        lib.ref_mps_read.restype = vp
+        lib.ref_mps_dims.argtypes = [vp, vp]
+        lib.ref_mps_copy.argtypes = [vp, C.POINTER(RModelBuf), vp, vp, vp, vp]
+        lib.ref_mps_copy.restype = C.c_longlong
+        lib.ref_mps_free.argtypes = [vp]
         _ref = lib
     return _ref
 
@@ -417,3 +424,40 @@ def ref_solve_lp(md: M, tol: float, max_iterations: int = -1, capture_log: bool 
                 max_dual_violation=out.max_dual_violation, primal_residual=out.primal_residual,
                 dual_residual=out.dual_residual, gap=out.gap, col_value=x, row_dual=y, col_dual=d,
                 ray=ray, pdlp_x=px, pdlp_y=py, restarts=restarts, message=out.message.decode())
+
+
+def ref_read_mps_text(text: bytes) -> dict:
+    """The reference's read_mps_from_string (src/io/mps_reader.cpp:516-527) on `text`: either
+    {"ok": True, "model": M, "name", "row_names", "col_names", "col_integer", "quadratic",
+    "seconds"} or {"ok": False, "message", "line", "seconds"}."""
+    lib = ref()
+    err, line, secs = C.create_string_buffer(1024), C.c_longlong(), C.c_double()
+    h = lib.ref_mps_read(text, len(text), err, len(err), C.byref(line), C.byref(secs))
+    if not h:
+        return {"ok": False, "message": err.value.decode(errors="replace"), "line": line.value,
+                "seconds": secs.value}
+    try:
+        d = np.zeros(7, np.int64)
+        lib.ref_mps_dims(h, _p(d))
+        m, n, nnz = int(d[0]), int(d[1]), int(d[2])
+        b = dict(start=np.zeros(n + 1, np.int64), index=np.zeros(max(nnz, 1), np.int32),
+                 value=np.zeros(max(nnz, 1)), obj=np.zeros(max(n, 1)), col_lower=np.zeros(max(n, 1)),
+                 col_upper=np.zeros(max(n, 1)), row_lower=np.zeros(max(m, 1)),
+                 row_upper=np.zeros(max(m, 1)))
+        buf = RModelBuf()
+        for k, a in b.items():
+            setattr(buf, k, _p(a))
+        ct = np.zeros(max(n, 1), np.uint8)
+        rn = C.create_string_buffer(int(d[4]) + 1)
+        cn = C.create_string_buffer(int(d[5]) + 1)
+        nm = C.create_string_buffer(int(d[6]) + 1)
+        quad = lib.ref_mps_copy(h, C.byref(buf), _p(ct), rn, cn, nm)
+        md = M(m, n, buf.maximize, buf.offset, b["start"], b["index"][:nnz], b["value"][:nnz],
+               b["obj"][:n], b["col_lower"][:n], b["col_upper"][:n], b["row_lower"][:m],
+               b["row_upper"][:m])
+        names = lambda raw, k: raw.raw[:k].decode(errors="surrogateescape").split("\n")[:-1]
+        return {"ok": True, "model": md, "name": nm.raw[:int(d[6])].decode(errors="surrogateescape"),
+                "row_names": names(rn, int(d[4])), "col_names": names(cn, int(d[5])),
+                "col_integer": ct[:n].copy(), "quadratic": int(quad), "seconds": secs.value}
+    finally:
+        lib.ref_mps_free(h)
