@@ -57,11 +57,18 @@ def spmv_bytes(nnz: int, rows: int, cols: int, s: int) -> int:
     return nnz * (s + 4) + 4 * (rows + 1) + s * cols + s * rows
 
 
-def step_kernel_bytes(nnz: int, m: int, n: int, s: int) -> dict:
+def step_kernel_bytes(nnz: int, m: int, n: int, s: int, halpern: bool = False) -> dict:
     """Algorithmic HBM bytes of the two fused step kernels (each value / index / offset once,
-    the gathered vector once, the per-row and per-column update vectors once)."""
-    dual = nnz * (s + 4) + 4 * (m + 1) + s * n + 13 * s * m
-    primal = nnz * (s + 4) + 4 * (n + 1) + s * m + 8 * s * n
+    the gathered vector once, the per-row and per-column update vectors once). PDHG steps with
+    averages: rows of A read 8 and write 5 vectors, rows of A' read 5 and write 3; Halpern steps:
+    rows of A read 7 (y, its anchor, the logical variable, its anchor, cost, bounds) and write 2,
+    rows of A' read 5 (v, its anchor, cost, bounds) and write 2 (v', 2 v+ - v)."""
+    if halpern:
+        dual = nnz * (s + 4) + 4 * (m + 1) + s * n + 9 * s * m
+        primal = nnz * (s + 4) + 4 * (n + 1) + s * m + 7 * s * n
+    else:
+        dual = nnz * (s + 4) + 4 * (m + 1) + s * n + 13 * s * m
+        primal = nnz * (s + 4) + 4 * (n + 1) + s * m + 8 * s * n
     return {"dual": dual, "primal": primal}
 
 
@@ -143,18 +150,20 @@ def spmv_sweep(md) -> dict:
     return out
 
 
-def count_order(md, algorithm: str):
-    """The column order of the solver's model-level solve (PdlpOptions.column_order "auto"): by
-    decreasing count for the enhanced algorithm when the longest column is at least 16x the mean,
-    else None (the model's order)."""
+def count_order(md, algorithm: str, column_order: str = "auto"):
+    """The column order of the solver's model-level solve (PdlpOptions.column_order): "auto" is
+    by decreasing count for the enhanced algorithm when the longest column is at least 16x the
+    mean, else None (the model's order); "degree" always by count; "natural" None."""
     counts = np.diff(md.start)
-    if algorithm != "enhanced" or md.n == 0 or counts.max() < 16 * max(md.nnz / md.n, 1.0):
+    if column_order == "natural" or md.n == 0:
+        return None
+    if column_order == "auto" and (algorithm != "enhanced" or counts.max() < 16 * max(md.nnz / md.n, 1.0)):
         return None
     return np.argsort(-counts, kind="stable")
 
 
 def kernel_roofline(md, precision: str, adaptive: bool = False, acc64: bool = False,
-                    order=None) -> dict:
+                    order=None, halpern: float | None = None) -> dict:
     """Live per-kernel timing of the two step kernels on the workload's matrix, with its columns
     in `order` when the solve uses one (the adaptive attempt's kernels when `adaptive`: same byte
     model, plus the per-attempt decision kernel; `acc64`: the FP32 kernels that accumulate in
@@ -181,14 +190,18 @@ def kernel_roofline(md, precision: str, adaptive: bool = False, acc64: bool = Fa
                             cost, lower, upper, "fp32" if precision == "fp32" else "fp64")
     if acc64:
         dev.set_fp64_accumulation(True)
+    if halpern is not None:
+        dev.set_halpern(True, halpern)
+        adaptive = False
     decide_ms = None
     if adaptive:
         dual_ms, primal_ms, decide_ms = dev.time_adaptive_kernels(50)
     else:
         dual_ms, primal_ms = dev.time_kernels(50)
     s = 4 if precision == "fp32" else 8
-    b = step_kernel_bytes(md.nnz, md.m, md.n, s)
-    ops = ("AdaptDualOp", "AdaptPrimalOp") if adaptive else ("DualRowsOp", "PrimalColsOp")
+    b = step_kernel_bytes(md.nnz, md.m, md.n, s, halpern is not None)
+    ops = (("HalpernRowsOp", "HalpernColsOp") if halpern is not None else
+           ("AdaptDualOp", "AdaptPrimalOp") if adaptive else ("DualRowsOp", "PrimalColsOp"))
     return {"dual_ms": dual_ms, "primal_ms": primal_ms, "decide_ms": decide_ms, "bytes": b,
             "ops": {"dual": ops[0], "primal": ops[1]}}
 
@@ -293,10 +306,22 @@ def main() -> int:
     ap.add_argument("--adaptive-steps", default="auto", choices=["auto", "on", "off"],
                     help="PDLP's adaptive step sizes for the mixed-precision solves (auto: on for "
                          "the single LP and the stacked batch; the shared-matrix batch has none)")
+    ap.add_argument("--halpern", action="store_true",
+                    help="restarted Halpern PDHG with reflection (constant steps, no averages) for "
+                         "the single LP instead of PDLP with adaptive steps")
+    ap.add_argument("--reflection", type=float, default=1.0, help="Halpern steps: the reflection")
+    ap.add_argument("--column-order", default="auto", choices=["auto", "natural", "degree"],
+                    help="the single LP's column order (auto: by count for the enhanced algorithm "
+                         "when the counts are skewed)")
+    ap.add_argument("--pid-weight", action="store_true",
+                    help="the primal weight by a PI controller on log(dy / dx) at restarts")
     ap.add_argument("--verbose", action="store_true", help="progress lines of long batches (stdout)")
     ap.add_argument("--fp32-switch-tol", type=float, default=None,
                     help="mixed precision: FP32 steps until this distance to termination (or a "
                          "stall); default 1e-6 for the node relaxations, 1e-5 otherwise")
+    ap.add_argument("--fp32-stall-level", type=float, default=1e-4,
+                    help="mixed precision: a plateau of the FP32 steps below this distance switches "
+                         "to FP64 (0: never)")
     ap.add_argument("--max-iterations", type=int, default=-1,
                     help="cap each solve (profiling runs); the line then reports no speedup")
     ap.add_argument("--kernels-only", action="store_true",
@@ -334,10 +359,14 @@ def main() -> int:
     md = smy.generators.c2(args.scale)
     log(f"LP m={md.m} n={md.n} nnz={md.nnz} generated in {time.perf_counter() - t0:.1f} s")
     nm = md.native()
+    if args.halpern:
+        args.adaptive = False
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, detect=False,
                            verify=True, polish_rounds=0, max_iterations=args.max_iterations,
                            fp32_switch_tol=args.fp32_switch_tol, adaptive_steps=args.adaptive,
-                           fp64_accumulation=args.fp64_accumulation)
+                           fp64_accumulation=args.fp64_accumulation, halpern=args.halpern,
+                           reflection=args.reflection, pid_primal_weight=args.pid_weight,
+                           column_order=args.column_order, fp32_stall_level=args.fp32_stall_level)
 
     for w in range(args.warmup):
         r = smy.solve_lp(nm, opts)
@@ -383,8 +412,9 @@ def main() -> int:
         log(f"polish {args.polish_rounds}: {p.iterations} it, verified {p.verified} "
             f"({p.max_primal_violation:.1e}, {p.max_dual_violation:.1e})")
 
-    order = count_order(md, args.algorithm)
-    rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation, order)
+    order = count_order(md, args.algorithm, args.column_order)
+    rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation, order,
+                         args.reflection if args.halpern else None)
     dom = "primal" if rl["primal_ms"] >= rl["dual_ms"] else "dual"
     dom_ms = rl[f"{dom}_ms"]
     dom_op = rl["ops"][dom]
@@ -413,6 +443,8 @@ def main() -> int:
         "dtype": "fp32 steps + fp64 evaluation/polish",
         "algorithm": args.algorithm,
         "adaptive_steps": args.adaptive,
+        "halpern": {"reflection": args.reflection} if args.halpern else False,
+        "pid_primal_weight": args.pid_weight,
         "fp64_accumulation": args.fp64_accumulation,
         "detect": False,
         "column_order": "by count" if order is not None else "model",

@@ -210,6 +210,22 @@ int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count);
 int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms,
                                    double* decide_ms);
 
+/* Restarted Halpern PDHG with reflection (no reference counterpart: the reference's PDLP is
+ * restarted PDHG with averages, src/lp/pdlp.cpp:574-641). With on != 0, smy_pdhg_run_batch and
+ * smy_pdhg_step take, from z = (v, y) and the restart point z0 as anchor, the steps
+ *   T(z): v+ = clamp(v - tau (c + [A -I]'y)), y+ = y + sigma [A -I] (2 v+ - v)
+ *   z'   = lambda ((1 + reflection) T(z) - reflection z) + (1 - lambda) z0,
+ * lambda = (k + 1) / (k + 2), k = iter0 + the step's index (steps since the restart); theta is
+ * ignored. After a run, v_new / y_new hold the last z', v / y the z it was taken from, and the
+ * averages (v_erg / y_erg, SMY_AVERAGE) hold T(z) of that last step: smy_pdhg_evaluate_batch
+ * measures z' (current) and T(z) (average), smy_pdhg_restart_distance_batch with SMY_AVERAGE
+ * gives ||T(z) - z0||^2, and smy_pdhg_restart_to_batch with SMY_AVERAGE restarts at T(z), which
+ * becomes the anchor. reflection in [0, 1]. Not available for the shared-matrix batch. */
+int smy_pdhg_set_halpern(smy_pdhg* h, int on, double reflection);
+/* After a run of Halpern steps, per instance: out[2i] = ||v+ - v||^2, out[2i + 1] = ||y+ - y||^2
+ * of the run's last step (the fixed-point residual z - T(z) of the restart test). Synchronizes. */
+int smy_pdhg_fixed_point_batch(smy_pdhg* h, double* out);
+
 /* The restart rule on the device (the reference decides on the host from downloaded sums:
  * src/lp/pdlp.cpp:1075-1132). smy_pdhg_rule_init sets each instance's normalizers (the measures
  * of src/lp/pdlp.cpp:493-519: rel primal = sqrt(||Ax-r||^2) / (1 + a_colmax ||x|| + ||r||), rel

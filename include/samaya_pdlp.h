@@ -126,6 +126,14 @@ typedef struct smy_pdlp_params {
   long long inject_fault_step; /* test hook: the device handle fails its steps after this many
                                   (smy_pdhg_inject_fault), so the solve takes the reference's
                                   GPU-failure path (src/lp/pdlp.cpp:231-242); -1: off */
+  int halpern;               /* restarted Halpern PDHG with reflection (smy_pdhg_set_halpern):
+                                constant steps, no averages, restarts on the fixed-point residual
+                                to T(z) (0: the reference's restarted PDHG); batches then use the
+                                stacked layout */
+  double reflection;         /* Halpern steps: the reflection in [0, 1] (1) */
+  int pid_primal_weight;     /* the primal weight at a restart follows log(dy / dx) through a PI
+                                controller (gains 0.99, 0.01) instead of the reference's
+                                omega <- sqrt(omega dy / dx) (0); decided on the host */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);

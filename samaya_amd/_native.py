@@ -32,6 +32,7 @@ EXPORTS = (
     "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_get_bin_tile", "smy_pdhg_set_bin_tile",
     "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels", "smy_pdhg_inject_fault",
     "smy_pdhg_equilibrate", "smy_pdlp_equilibrate", "smy_lp_dual_bound",
+    "smy_pdhg_set_halpern", "smy_pdhg_fixed_point_batch",
     "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32", "smy_csr_spmv_noalloc",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
@@ -61,7 +62,8 @@ class SmyPdlpParams(C.Structure):
                 ("adaptive_steps", C.c_int), ("fp32_stall_level", C.c_double),
                 ("device_restart", C.c_int), ("fp64_accumulation", C.c_int),
                 ("device_scaling", C.c_int), ("column_order", C.c_int),
-                ("inject_fault_step", C.c_longlong)]
+                ("inject_fault_step", C.c_longlong), ("halpern", C.c_int),
+                ("reflection", C.c_double), ("pid_primal_weight", C.c_int)]
 
 
 class SmyPdlpInfo(C.Structure):
@@ -153,6 +155,8 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_detect_batch", i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_download_ray", i32, vp, i32, vp)
     _proto(lib, "smy_pdhg_run_adaptive", i32, vp, vp, i32)
+    _proto(lib, "smy_pdhg_set_halpern", i32, vp, i32, f64)
+    _proto(lib, "smy_pdhg_fixed_point_batch", i32, vp, vp)
     _proto(lib, "smy_pdhg_get_tile", i32, vp, i32, i32, C.POINTER(i32), C.POINTER(i32))
     _proto(lib, "smy_pdhg_set_tile", i32, vp, i32, i32, i32, i32)
     _proto(lib, "smy_pdhg_get_bin_tile", i32, vp, i32, i32, i32, C.POINTER(i32), C.POINTER(i32),

@@ -433,6 +433,155 @@ struct PrimalColsOp {
 };
 
 // ------------------------------------------------------------------------------------------
+// Restarted Halpern PDHG with reflection (no reference counterpart; Lu and Yang, "Restarted
+// Halpern PDHG for linear programming", 2024, the iteration of cuPDLPx). With z = (v, y), the
+// PDHG operator in the primal-first order
+//   T(z):  v+ = clamp(v - tau (c + K'y)),  y+ = y + sigma K (2 v+ - v),   K = [A -I],
+// and the anchor z0 (the last restart point), a step is
+//   z' = lambda ((1 + rho) T(z) - rho z) + (1 - lambda) z0,   lambda = (k + 1) / (k + 2),
+// k the steps since the restart and rho in [0, 1] the reflection. No averages: a restart moves
+// to T(z) and makes it the anchor. Buffers of the stacked state: z in slot p of v, y; z' into
+// slot 1 - p; the anchor in v_rst / y_rst; w = 2 v+ - v of the structural variables in vbar[0]
+// (the dual kernel's gathered vector); T(z) itself into v_erg / y_erg at the last step of a run
+// only (`keep`), where the evaluation and the restart read it.
+// The primal kernel (rows of A': structural variables) runs first, then the dual kernel (rows of
+// A: the duals and, from y_i alone, the logical variables).
+// ------------------------------------------------------------------------------------------
+
+template <typename T>
+struct HalpernColsPre {
+  T v, anchor, cost, lower, upper;
+};
+
+template <typename T, typename A = T>
+struct HalpernColsOp {
+  using Acc = A;
+  using Pre = HalpernColsPre<T>;
+  static constexpr int kVec = 1;
+  static constexpr int kSums = 0;
+  int j;       // step index within the run
+  bool keep;   // write T(z) (last step of the run)
+  double rho;  // reflection
+  const StepScalars* __restrict__ sc;
+  const T* __restrict__ y;
+  const T* __restrict__ v;
+  const T* __restrict__ v_anchor;
+  T* __restrict__ v_next;
+  T* __restrict__ w;
+  T* __restrict__ v_plus;
+  const T* __restrict__ cost;
+  const T* __restrict__ lower;
+  const T* __restrict__ upper;
+  __device__ bool skip(int) const { return false; }
+  __device__ Pre pre(int col, int) const {
+    return Pre{v[col], v_anchor[col], cost[col], lower[col], upper[col]};
+  }
+  __device__ Acc gather(int, int c, int) const { return y[c]; }
+  __device__ void row(int col, int inst, const Acc* acc, const Pre& p, double*) const {
+    const StepScalars& s = sc[inst];
+    const T tau = static_cast<T>(s.tau);
+    const double k = static_cast<double>(s.k0 + j);
+    const T lam = static_cast<T>((k + 1.0) / (k + 2.0));
+    const T r = static_cast<T>(rho);
+    T vp;
+    if constexpr (std::is_same_v<A, T>)
+      vp = clamp_to(p.v - tau * (p.cost + acc[0]), p.lower, p.upper);
+    else
+      vp = clamp_to(static_cast<T>(static_cast<A>(p.v) - static_cast<A>(tau) * (static_cast<A>(p.cost) + acc[0])),
+                    p.lower, p.upper);
+    w[col] = vp + (vp - p.v);
+    v_next[col] = lam * ((T(1) + r) * vp - r * p.v) + (T(1) - lam) * p.anchor;
+    if (keep) v_plus[col] = vp;
+  }
+};
+
+template <typename T>
+struct HalpernRowsPre {
+  T y, y_anchor, v, v_anchor, cost, lower, upper;
+};
+
+template <typename T, typename A = T>
+struct HalpernRowsOp {
+  using Acc = A;
+  using Pre = HalpernRowsPre<T>;
+  static constexpr int kVec = 1;
+  static constexpr int kSums = 0;
+  int n;
+  int j;
+  bool keep;
+  double rho;
+  const StepScalars* __restrict__ sc;
+  const T* __restrict__ w;  // 2 v+ - v of the structural variables
+  const T* __restrict__ y;
+  const T* __restrict__ y_anchor;
+  T* __restrict__ y_next;
+  T* __restrict__ y_plus;
+  const T* __restrict__ v;
+  const T* __restrict__ v_anchor;
+  T* __restrict__ v_next;
+  T* __restrict__ v_plus;
+  const T* __restrict__ cost;
+  const T* __restrict__ lower;
+  const T* __restrict__ upper;
+  __device__ bool skip(int) const { return false; }
+  __device__ Pre pre(int i, int) const {
+    const int k = n + i;
+    return Pre{y[i], y_anchor[i], v[k], v_anchor[k], cost[k], lower[k], upper[k]};
+  }
+  __device__ Acc gather(int, int c, int) const { return w[c]; }
+  __device__ void row(int i, int inst, const Acc* acc, const Pre& p, double*) const {
+    const StepScalars& s = sc[inst];
+    const T tau = static_cast<T>(s.tau);
+    const T sigma = static_cast<T>(s.sigma);
+    const double kk = static_cast<double>(s.k0 + j);
+    const T lam = static_cast<T>((kk + 1.0) / (kk + 2.0));
+    const T r = static_cast<T>(rho);
+    const int k = n + i;
+    // The logical variable: its gradient is cost - y_i.
+    const T vp = clamp_to(p.v - tau * (p.cost + (-p.y)), p.lower, p.upper);
+    const T wr = vp + (vp - p.v);
+    T yp;
+    if constexpr (std::is_same_v<A, T>)
+      yp = p.y + sigma * (acc[0] - wr);
+    else
+      yp = static_cast<T>(static_cast<A>(p.y) + static_cast<A>(sigma) * (acc[0] - static_cast<A>(wr)));
+    y_next[i] = lam * ((T(1) + r) * yp - r * p.y) + (T(1) - lam) * p.y_anchor;
+    v_next[k] = lam * ((T(1) + r) * vp - r * p.v) + (T(1) - lam) * p.v_anchor;
+    if (keep) {
+      y_plus[i] = yp;
+      v_plus[k] = vp;
+    }
+  }
+};
+
+// Squared distances of T(z) (v_erg, y_erg) from z (slot p): the fixed-point residual of a
+// Halpern run's last step. 0: ||v+ - v||^2, 1: ||y+ - y||^2.
+template <typename T>
+struct ResidualOp {
+  static constexpr int kSums = 2;
+  static constexpr int kMaxFrom = 2;
+  const int* __restrict__ which;  // per instance: -1 none, else measured
+  const T* __restrict__ v;
+  const T* __restrict__ y;
+  const T* __restrict__ v_plus;
+  const T* __restrict__ y_plus;
+  double* __restrict__ partial;
+  size_t stride;
+  int inst_stride = 0;
+  __device__ int flag(int inst) const { return which[inst]; }
+  __device__ void row_a(int i, int k, int, double* s, int) const {
+    const double dv = static_cast<double>(v_plus[k]) - static_cast<double>(v[k]);
+    const double dy = static_cast<double>(y_plus[i]) - static_cast<double>(y[i]);
+    s[0] += dv * dv;
+    s[1] += dy * dy;
+  }
+  __device__ void col(int k, int, double* s, int) const {
+    const double dv = static_cast<double>(v_plus[k]) - static_cast<double>(v[k]);
+    s[0] += dv * dv;
+  }
+};
+
+// ------------------------------------------------------------------------------------------
 // Adaptive step sizes: PDLP's rule (Applegate et al., "Practical large-scale linear programming
 // using primal-dual hybrid gradient", NeurIPS 2021, sec. 3.1). The reference has no counterpart:
 // its adaptive step is disabled code (src/lp/pdlp.cpp:1051-1067). An attempt takes one PDHG step

@@ -230,6 +230,60 @@ void step_kernels(smy_pdhg* h, int j) {
   launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], primal, h->stream);
 }
 
+// The two kernels of a restarted Halpern step (csr_kernels.hpp): the primal one over the rows of
+// A' first, then the dual one over the rows of A. `keep`: also write T(z) (the run's last step).
+template <typename T, typename A = T>
+struct HalpernOps {
+  HalpernColsOp<T, A> primal;
+  HalpernRowsOp<T, A> dual;
+};
+
+template <typename T, typename A = T>
+HalpernOps<T, A> halpern_ops(smy_pdhg* h, int j, bool keep) {
+  Iter<T>& S = state<T>(h);
+  const int p = h->p;
+  HalpernOps<T, A> o;
+  o.primal.j = j;
+  o.primal.keep = keep;
+  o.primal.rho = h->rho;
+  o.primal.sc = h->scalars;
+  o.primal.y = S.y[p];
+  o.primal.v = S.v[p];
+  o.primal.v_anchor = S.v_rst;
+  o.primal.v_next = S.v[1 - p];
+  o.primal.w = S.vbar[0];
+  o.primal.v_plus = S.v_erg;
+  o.primal.cost = S.cost;
+  o.primal.lower = S.lower;
+  o.primal.upper = S.upper;
+  o.dual.n = h->n;
+  o.dual.j = j;
+  o.dual.keep = keep;
+  o.dual.rho = h->rho;
+  o.dual.sc = h->scalars;
+  o.dual.w = S.vbar[0];
+  o.dual.y = S.y[p];
+  o.dual.y_anchor = S.y_rst;
+  o.dual.y_next = S.y[1 - p];
+  o.dual.y_plus = S.y_erg;
+  o.dual.v = S.v[p];
+  o.dual.v_anchor = S.v_rst;
+  o.dual.v_next = S.v[1 - p];
+  o.dual.v_plus = S.v_erg;
+  o.dual.cost = S.cost;
+  o.dual.lower = S.lower;
+  o.dual.upper = S.upper;
+  return o;
+}
+
+template <typename T, typename A = T>
+void halpern_kernels(smy_pdhg* h, int j, bool keep) {
+  constexpr int q = prec_of<T>();
+  const HalpernOps<T, A> o = halpern_ops<T, A>(h, j, keep);
+  launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], o.primal, h->stream);
+  launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], o.dual, h->stream);
+}
+
 // ---- Tile tuner ----------------------------------------------------------------------------
 // The first run in a precision times the candidate tiles of both matrices with the values of that
 // precision (over the blocks of the instances stepping in it) on a stand-in for the step kernels
@@ -331,7 +385,19 @@ void tune(smy_pdhg* h) {
   h->timed = false;
 }
 
-void step_all(smy_pdhg* h, int j) {
+// Step j of a run of `count` (Halpern steps write T(z) at the run's last step).
+void step_all(smy_pdhg* h, int j, int count = 1) {
+  if (h->halpern) {
+    const bool keep = j + 1 == count;
+    if (uses(h, SMY_FP32)) {
+      if (h->acc64)
+        halpern_kernels<float, double>(h, j, keep);
+      else
+        halpern_kernels<float>(h, j, keep);
+    }
+    if (uses(h, SMY_FP64)) halpern_kernels<double>(h, j, keep);
+    return;
+  }
   if (uses(h, SMY_FP32)) {
     if (h->acc64)
       step_kernels<float, double>(h, j);
@@ -394,12 +460,12 @@ int run_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
   const int mask = (uses(h, SMY_FP64) ? 1 : 0) | (uses(h, SMY_FP32) ? 2 : 0);
   if (count == smy_pdhg::kGraphSteps && mask != 0) {
     const int p0 = h->p;
-    smy_pdhg::Captured& c = h->graphs[mask][p0];
+    smy_pdhg::Captured& c = (h->halpern ? h->hgraphs : h->graphs)[mask][p0];
     if (c.exec == nullptr || c.gen != h->list_gen) {
       h->drop_graph(c);
       SMY_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       for (int j = 0; j < count; ++j) {
-        step_all(h, j);
+        step_all(h, j, count);
         if (j + 1 < count) h->p ^= 1;
       }
       SMY_HIP(hipStreamEndCapture(h->stream, &c.graph));
@@ -411,7 +477,7 @@ int run_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
     if ((count - 1) & 1) h->p ^= 1;
   } else {
     for (int j = 0; j < count; ++j) {
-      step_all(h, j);
+      step_all(h, j, count);
       if (j + 1 < count) h->p ^= 1;
     }
   }
@@ -666,6 +732,18 @@ int time_kernels_impl(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms)
   std::vector<double> tau(h->K, 0.5), sigma(h->K, 0.5);
   std::vector<long long> k0(h->K, 0);
   set_scalars(h, tau.data(), sigma.data(), 1.0, k0.data());
+  if (h->halpern) {
+    const HalpernOps<T, A> o = halpern_ops<T, A>(h, 0, false);
+    *dual_ms = time_repeated(h, reps, [&] {
+      launch_rows(h->a, avals<T>(h->a), h->a.list[q], h->a.nlist[q], o.dual, h->stream);
+    });
+    *primal_ms = time_repeated(h, reps, [&] {
+      launch_rows(h->at, avals<T>(h->at), h->at.list[q], h->at.nlist[q], o.primal, h->stream);
+    });
+    SMY_HIP(hipGetLastError());
+    h->timed = false;
+    return SMY_OK;
+  }
   DualRowsOp<T, A> dual;
   dual.n = h->n;
   dual.j = 0;
@@ -811,6 +889,37 @@ void distance_launch(smy_pdhg* h) {
   op.stride = stride(h);
   hipLaunchKernelGGL((block_rows_kernel<DistanceOp<T>>), dim3(block_rows_grid(h)), dim3(kBlock), 0,
                      h->stream, block_rows(h), op);
+}
+
+template <typename T>
+void residual_launch(smy_pdhg* h) {
+  constexpr int q = prec_of<T>();
+  Iter<T>& S = state<T>(h);
+  ResidualOp<T> op;
+  op.which = h->flags + q * h->K;
+  op.v = S.v[h->p];
+  op.y = S.y[h->p];
+  op.v_plus = S.v_erg;
+  op.y_plus = S.y_erg;
+  op.partial = h->partial;
+  op.stride = stride(h);
+  hipLaunchKernelGGL((block_rows_kernel<ResidualOp<T>>), dim3(block_rows_grid(h)), dim3(kBlock), 0,
+                     h->stream, block_rows(h), op);
+}
+
+// After a run of Halpern steps: per instance ||v+ - v||^2, ||y+ - y||^2 of its last step.
+int fixed_point_batch_impl(smy_pdhg* h, double* out) {
+  if (block_rows_grid(h) == 0) {
+    std::fill(out, out + 2 * h->K, 0.0);
+    return SMY_OK;
+  }
+  std::vector<int> all(static_cast<size_t>(h->K), 0);
+  stage_flags(h, all.data());
+  if (uses(h, SMY_FP32)) residual_launch<float>(h);
+  if (uses(h, SMY_FP64)) residual_launch<double>(h);
+  SMY_HIP(hipGetLastError());
+  finish(h, 2, out);
+  return SMY_OK;
 }
 
 int distance_batch_impl(smy_pdhg* h, const int* which, double* out) {
@@ -1584,6 +1693,29 @@ int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, doub
   if (take_steps(h, count) != SMY_OK) return SMY_ERROR;
   if (h->shared) return guarded([&]() -> int { return shared_run(h, tau, sigma, theta, iter0, count); });
   return guarded([&]() -> int { return run_impl(h, tau, sigma, theta, iter0, count); });
+}
+
+int smy_pdhg_set_halpern(smy_pdhg* h, int on, double reflection) {
+  if (h == nullptr) return fail(SMY_INVALID, "null handle");
+  if (h->shared) return fail(SMY_INVALID, "Halpern steps need the stacked batch layout");
+  if (!(reflection >= 0.0 && reflection <= 1.0)) return fail(SMY_INVALID, "reflection outside [0, 1]");
+  if (h->stepped) return fail(SMY_STATE, "the step kind changed between two steps without advance()");
+  if (h->rho != reflection) {
+    for (auto& row : h->hgraphs)
+      for (auto& c : row) h->drop_graph(c);
+  }
+  h->halpern = on != 0;
+  h->rho = reflection;
+  return SMY_OK;
+}
+
+int smy_pdhg_fixed_point_batch(smy_pdhg* h, double* out) {
+  if (h == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  if (h->shared || !h->halpern) return fail(SMY_STATE, "no Halpern steps on this handle");
+  return guarded([&]() -> int {
+    refresh_lists(h);
+    return fixed_point_batch_impl(h, out);
+  });
 }
 
 int smy_pdhg_run_adaptive(smy_pdhg* h, smy_pdhg_adaptive* state, int count) {

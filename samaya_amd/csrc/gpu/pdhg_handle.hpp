@@ -214,6 +214,11 @@ struct smy_pdhg {
   };
   Captured graphs[4][2];
   Captured agraphs[4][2];    // the same for runs of adaptive attempts
+  Captured hgraphs[4][2];    // the same for runs of Halpern steps
+  // Restarted Halpern steps instead of PDHG steps with averages (smy_pdhg_set_halpern; stacked
+  // only), with reflection rho.
+  bool halpern = false;
+  double rho = 1.0;
   int list_gen = 0;
   bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
   int* step_prec = nullptr;  // per instance: precision of its steps, or kFrozen (device copy)
@@ -280,6 +285,8 @@ struct smy_pdhg {
     for (auto& row : graphs)
       for (auto& c : row) drop_graph(c);
     for (auto& row : agraphs)
+      for (auto& c : row) drop_graph(c);
+    for (auto& row : hgraphs)
       for (auto& c : row) drop_graph(c);
     for (void* ptr : allocations) (void)hipFree(ptr);
     if (sums_host) (void)hipHostFree(sums_host);

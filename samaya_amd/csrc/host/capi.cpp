@@ -56,6 +56,9 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.column_order = static_cast<ColumnOrder>(p->column_order == 1 ? 1 : p->column_order == 2 ? 2 : 0);
   o.fp32_stall_level = p->fp32_stall_level;
   o.device_restart = p->device_restart != 0;
+  o.halpern = p->halpern != 0;
+  o.reflection = p->reflection;
+  o.pid_primal_weight = p->pid_primal_weight != 0;
   return o;
 }
 
@@ -144,6 +147,9 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->column_order = 0;
   p->fp32_stall_level = 1e-4;
   p->device_restart = 1;
+  p->halpern = 0;
+  p->reflection = 1.0;
+  p->pid_primal_weight = 0;
 }
 
 int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,

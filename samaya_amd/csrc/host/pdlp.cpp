@@ -109,6 +109,12 @@ class Device {
 
   int size() const { return k_; }
   void set_fp64_accumulation(bool on) { check(smy_pdhg_set_fp64_accumulation(h_, on ? 1 : 0)); }
+  void set_halpern(bool on, double reflection) { check(smy_pdhg_set_halpern(h_, on ? 1 : 0, reflection)); }
+  // After a run of Halpern steps: ||v+ - v||^2, ||y+ - y||^2 of each instance's last step.
+  void fixed_point(std::vector<double>& out) {
+    out.resize(2 * static_cast<size_t>(k_));
+    check(smy_pdhg_fixed_point_batch(h_, out.data()));
+  }
   void inject_fault(long long step) { check(smy_pdhg_inject_fault(h_, step)); }
   // The enhanced preconditioner on the device (a handle of one unscaled LP): the factors, L and
   // the norms of the termination test.
@@ -380,6 +386,7 @@ class Lane {
   // Adaptive steps: the rule's state (eta above is then the next attempt's step size).
   smy_pdhg_adaptive adapt{0.0, 1.0, 0.0, 0, 0};
   double kkt_at_restart = kInf, kkt_previous = kInf;
+  double weight_integral = 0.0;  // pid_primal_weight: the controller's sum of errors
   bool fp32 = false;
   double fp32_best = kInf;       // best distance to termination in FP32 (by 20% steps)
   long long fp32_best_iter = 0;  // ... and where it was reached
@@ -473,7 +480,14 @@ void Lane::prepare(std::shared_ptr<const Equilibrated> e) {
     lower[n + i] *= r_s[i];
     upper[n + i] *= r_s[i];
   }
-  const double L = eq->L;
+  double L = eq->L;
+  // Halpern steps converge only while T is nonexpansive (eta ||[A_eq -I]|| <= 1), and 20 power
+  // iterations can fall short of the norm when the top singular values cluster, which the
+  // enhanced preconditioner makes them do (seen: L 1.391 for a norm of 1.410, and the iterates
+  // diverged). Its last step, Pock-Chambolle with alpha = 1, bounds ||A_eq|| by 1 (Schur's test
+  // with the row and column weights it divided by), so ||[A_eq -I]|| <= sqrt(2): that bound is
+  // used instead.
+  if (opt_.halpern && opt_.algorithm == Algorithm::kEnhanced && lp_.m > 0) L = std::sqrt(2.0);
   res.operator_norm = L;
   eta = 0.99 / L;
   omega = opt_.start_omega > 0.0 ? opt_.start_omega : 1.0;
@@ -752,7 +766,9 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
 
   // One problem for all lanes, with one set of row bounds: the shared-matrix batch (iterates
   // stored instance-minor, SpMM products); otherwise the stacked batch.
-  bool share = opt.share_matrix && !opt.adaptive_steps && !opt.fp64_accumulation && K > 1 &&
+  const bool halpern = opt.halpern;
+  const bool adaptive = opt.adaptive_steps && !halpern;
+  bool share = opt.share_matrix && !adaptive && !halpern && !opt.fp64_accumulation && K > 1 &&
                owners.size() == 1 && !device_eq;
   for (int i = 1; share && i < K; ++i) {
     const int n = lanes[0].lp_.n;
@@ -795,6 +811,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     if (device) {
       if (share) lanes[0].log(2, ": %d LPs share one matrix: shared-matrix batch", K);
       if (opt.fp64_accumulation) device->set_fp64_accumulation(true);
+      if (halpern) device->set_halpern(true, opt.reflection);
       if (opt.inject_fault_step >= 0) device->inject_fault(opt.inject_fault_step);
       for (int i = 0; i < K; ++i) {
         Lane& L = lanes[i];
@@ -832,13 +849,14 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   if (info != nullptr) info->setup_seconds = setup_seconds;
 
   const double theta = 1.0;
-  std::vector<double> tau(static_cast<size_t>(K)), sigma(static_cast<size_t>(K)), sums, dist, dout;
+  std::vector<double> tau(static_cast<size_t>(K)), sigma(static_cast<size_t>(K)), sums, dist, dout, fixed;
   std::vector<long long> k0(static_cast<size_t>(K));
   std::vector<int> which(static_cast<size_t>(K)), precision(static_cast<size_t>(K));
   std::vector<smy_pdhg_adaptive> adapt(static_cast<size_t>(K));
   // The restart rule on the device: decisions, distances, primal weights and restarts there; the
   // host keeps termination, detection and precision switches.
-  const bool device_rule = opt.device_restart;
+  // Halpern steps restart on their fixed-point residual, decided on the host.
+  const bool device_rule = opt.device_restart && !halpern && !opt.pid_primal_weight;
   std::vector<smy_pdhg_rule_in> rule_in(static_cast<size_t>(K));
   std::vector<smy_pdhg_rule_out> rule_out(static_cast<size_t>(K));
   if (device_rule) {
@@ -897,7 +915,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         eval_iter = std::min(eval_iter, std::min((iter / kCheckFrequency + 1) * kCheckFrequency - 1,
                                                  L.max_iter - 1));
     const int count = static_cast<int>(eval_iter - iter + 1);
-    if (opt.adaptive_steps) {
+    if (adaptive) {
       for (int i = 0; i < K; ++i) adapt[i] = lanes[i].adapt;
       device->run_adaptive(adapt, count);
       for (int i = 0; i < K; ++i) {
@@ -925,6 +943,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     iter = eval_iter;
 
     device->evaluate(sums);
+    fixed.clear();
     std::fill(which.begin(), which.end(), -1);
     std::fill(precision.begin(), precision.end(), -1);
     std::vector<char> force(static_cast<size_t>(K), 0), due(static_cast<size_t>(K), 0);
@@ -1061,6 +1080,24 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         }
       }
 
+      if (halpern) {
+        // Restarted Halpern steps: the same three tests on the fixed-point residual
+        // r = ||z - T(z)|| (primal part weighted by omega, dual part by 1 / omega) of the run's
+        // last step instead of the KKT error; a restart moves to T(z), the new anchor.
+        if (fixed.empty()) device->fixed_point(fixed);
+        const double r = std::sqrt(L.omega * fixed[2 * i] + fixed[2 * i + 1] / L.omega);
+        if (!(L.kkt_at_restart < kInf)) L.kkt_at_restart = r;
+        const bool restart =
+            force[i] || r <= 0.2 * L.kkt_at_restart ||
+            (r <= 0.8 * L.kkt_at_restart && r > L.kkt_previous) ||
+            static_cast<double>(L.k_since) >= 0.36 * static_cast<double>(iter + 1);
+        L.kkt_previous = r;
+        if (restart) {
+          which[i] = SMY_AVERAGE;
+          kkt_candidate[i] = r;
+        }
+        continue;
+      }
       // Adaptive restart (src/lp/pdlp.cpp:1075-1132): the decision; the primal weight follows
       // from the distance below. With the device rule, both happen on the device (below).
       if (device_rule) continue;
@@ -1108,7 +1145,14 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         const double dx = std::sqrt(dist[2 * i]);
         const double dy = std::sqrt(dist[2 * i + 1]);
         if (dx > kWeightDistanceFloor && dy > kWeightDistanceFloor) {
-          L.omega = std::exp(0.5 * std::log(dy / dx) + 0.5 * std::log(L.omega));
+          if (opt.pid_primal_weight) {
+            // log omega follows log(dy / dx) through a PI controller (gains 0.99 and 0.01).
+            const double err = std::log(dy / dx) - std::log(L.omega);
+            L.weight_integral += err;
+            L.omega = std::exp(std::log(L.omega) + 0.99 * err + 0.01 * L.weight_integral);
+          } else {
+            L.omega = std::exp(0.5 * std::log(dy / dx) + 0.5 * std::log(L.omega));
+          }
           L.tau = L.eta / L.omega;
           L.sigma = L.eta * L.omega;
         }

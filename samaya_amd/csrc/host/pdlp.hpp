@@ -75,6 +75,16 @@ struct PdlpOptions {
   bool device_detect = true;      // ... computed on the device (false: host, from downloads)
   bool share_matrix = true;       // batches of one problem: the shared-matrix (SpMM) batch
   bool device_restart = true;     // restart decisions, distances and primal weights on the device
+  // Restarted Halpern PDHG with reflection (Lu and Yang 2024, as in cuPDLPx) instead of restarted
+  // PDHG with averages: constant steps 0.99 / L, the anchored step
+  // z' = lambda ((1 + reflection) T(z) - reflection z) + (1 - lambda) z0, restarts decided on the
+  // fixed-point residual and made to T(z). Overrides adaptive_steps; batches use the stacked layout.
+  bool halpern = false;
+  double reflection = 1.0;
+  // The primal weight at a restart follows log(dy / dx) (the distances moved since the last
+  // restart) through a PI controller with gains 0.99 and 0.01 (as in cuPDLPx) instead of the
+  // reference's halfway step omega <- sqrt(omega dy / dx). Decided on the host.
+  bool pid_primal_weight = false;
   int log_level = 0;
   // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m),
   // and the primal weight to start with (<= 0: 1, the reference's start).

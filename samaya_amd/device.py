@@ -134,6 +134,16 @@ class PdhgDevice:
     def set_start(self, v, y):
         N.check(self.lib.smy_pdhg_set_start(self.h, _p(_f64(v)), _p(_f64(y))), "set_start")
 
+    def set_halpern(self, on: bool, reflection: float = 1.0):
+        """Restarted Halpern steps with this reflection (smy_pdhg_set_halpern)."""
+        N.check(self.lib.smy_pdhg_set_halpern(self.h, int(on), reflection), "set_halpern")
+
+    def fixed_point(self) -> tuple[float, float]:
+        """||v+ - v||^2, ||y+ - y||^2 of the last Halpern step of the last run."""
+        out = np.zeros(2)
+        N.check(self.lib.smy_pdhg_fixed_point_batch(self.h, _p(out)), "fixed_point")
+        return float(out[0]), float(out[1])
+
 
 class PdhgBatch:
     """Many LPs' PDHG state on one device handle (smy_pdhg_create_batch): instance i is the LP of

@@ -213,6 +213,9 @@ class PdlpOptions:
     device_scaling: bool = True      # enhanced, one LP: preconditioner and norm bound on the GPU
     column_order: str = "auto"       # model-level solves: "auto", "natural" or "degree"
     inject_fault_step: int = -1      # test hook: the device fails its steps after this many
+    halpern: bool = False            # restarted Halpern PDHG with reflection (no averages)
+    reflection: float = 1.0          # Halpern steps: the reflection in [0, 1]
+    pid_primal_weight: bool = False  # primal weight by a PI controller on log(dy / dx)
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -239,6 +242,9 @@ class PdlpOptions:
         p.inject_fault_step = int(self.inject_fault_step)
         p.device_scaling = int(self.device_scaling)
         p.column_order = {"auto": 0, "natural": 1, "degree": 2}[self.column_order]
+        p.halpern = int(self.halpern)
+        p.reflection = self.reflection
+        p.pid_primal_weight = int(self.pid_primal_weight)
         return p
 
 

@@ -1,0 +1,185 @@
+"""The benchmark configurations at their full sizes on the GPU (the model for these tests is the
+reference's tests/test_gpu.cpp:407-477, which runs its GPU PDLP on its largest generated LPs):
+
+  C2  the 1M-column LP (m 500,000, n 1,000,000, nnz ~12M), plain FP64 PDLP with the reference's
+      algorithm, against the reference's own CPU PDLP on the same LP (golden c2_reference);
+  C3  the same LP in mixed precision with the enhanced preconditioner and adaptive steps;
+  C4  256 blending LPs as one stacked batch, against HiGHS on every 32nd LP (golden c4_sample)
+      and against the same LPs solved alone;
+  C5  1,024 knapsack node relaxations as one shared-matrix batch, warm-started from the root:
+      statuses against the nodes' infeasibility worked out exactly on the host, objectives
+      against plain FP64 PDLP solving sampled nodes one at a time, and weak-duality bounds.
+
+Every answer's residuals are recomputed here from the returned point, in numpy, in the model's
+space, so a test does not rest on the solver's own report of them."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import samaya_amd as smy
+from conftest import GOLDEN, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return abs(a - b) / (1.0 + abs(b))
+
+
+def csc(md):
+    return sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+
+
+def model_residuals(md, r):
+    """Relative residuals of a model-space answer, recomputed on the host:
+    primal  ||A x - s||_2 / (1 + ||A||_max-col ||x|| + ||s||), s the row values the answer
+            reports (within the row bounds), x within its bounds;
+    dual    ||d - lambda||_2 / (1 + ||A'||_max-row ||y|| + ||c||), d = c - A'y (model sense) and
+            lambda the part of d that the column bounds absorb;
+    bounds  largest violation of a column or row value bound (0 when the answer is projected)."""
+    A = csc(md)
+    x, s, y = r.col_value, r.row_activity, r.row_dual
+    sense = -1.0 if md.maximize else 1.0
+    col_norm = np.sqrt(np.asarray(A.multiply(A).sum(axis=0)).ravel()).max()
+    row_norm = np.sqrt(np.asarray(A.multiply(A).sum(axis=1)).ravel()).max()
+    primal = np.linalg.norm(A @ x - s) / (1 + col_norm * np.linalg.norm(x) + np.linalg.norm(s))
+    # Columns (reduced costs d) and rows (a row is a variable whose reduced cost is its dual).
+    d = np.concatenate([sense * (md.obj - A.T @ y), sense * y])
+    lo = np.concatenate([md.col_lower, md.row_lower])
+    up = np.concatenate([md.col_upper, md.row_upper])
+    lam = np.where((d > 0) & np.isfinite(lo), d, np.where((d < 0) & np.isfinite(up), d, 0.0))
+    dual = np.linalg.norm(d - lam) / (1 + row_norm * np.linalg.norm(y) + np.linalg.norm(md.obj))
+    bounds = max(np.max(md.col_lower - x, initial=0), np.max(x - md.col_upper, initial=0),
+                 np.max(md.row_lower - s, initial=0), np.max(s - md.row_upper, initial=0))
+    return primal, dual, bounds
+
+
+@pytest.fixture(scope="module")
+def c2_model():
+    return smy.generators.c2()
+
+
+C2_FIXTURE = (GOLDEN / "c2_reference.json").exists()
+
+
+@pytest.mark.skipif(not C2_FIXTURE, reason="tests/golden/c2_reference.json not written yet "
+                                           "(make_golden.py c2: the reference's CPU PDLP, ~1 hour)")
+def test_c2_fp64_reference_algorithm(gpu, c2_model):
+    # Plain FP64 PDLP (the reference's preconditioner, constant steps, restart rule and
+    # detection) on the GPU, against the reference's own CPU PDLP on the same LP at 1e-6.
+    ref = load_golden("c2_reference")
+    md = c2_model
+    assert (md.m, md.n, md.nnz) == (ref["m"], ref["n"], ref["nnz"])
+    r = smy.solve_lp(md, smy.PdlpOptions(tol=1e-6, precision="fp64", algorithm="reference"))
+    primal, dual, bounds = model_residuals(md, r)
+    print(f"C2 fp64: {r.status_name} {r.iterations} it (reference {ref['iterations']}), objective "
+          f"{r.objective!r} (reference {ref['objective']!r}, rel {rel(r.objective, ref['objective']):.2e}), "
+          f"residuals {primal:.2e} {dual:.2e} bounds {bounds:.1e}, dual bound {r.dual_bound!r}")
+    assert r.status == smy.OPTIMAL == ref["status"]
+    assert max(r.primal_residual, r.dual_residual, abs(r.gap)) < 1e-6
+    assert primal < 2e-6 and dual < 2e-6 and bounds == 0.0
+    # The same algorithm from the same start: the GPU sums in another order, so the runs part at
+    # the rounding level, but they stop within a few evaluation periods of each other, at points
+    # whose objectives agree to the tolerance.
+    assert abs(r.iterations - ref["iterations"]) <= 0.1 * ref["iterations"]
+    assert rel(r.objective, ref["objective"]) <= 1e-6
+    # Weak duality: the bound from the answer's duals lies below the objective (minimization).
+    assert r.dual_bound <= r.objective + 1e-9 * abs(r.objective)
+    assert rel(r.dual_bound, r.objective) <= 1e-4
+
+
+def test_c3_mixed_precision(gpu, c2_model):
+    # The default benchmark line: FP32 steps (values, iterates, sums), FP64 evaluation, restarts
+    # and termination, the enhanced preconditioner and adaptive steps.
+    ref = load_golden("c2_reference") if C2_FIXTURE else None
+    md = c2_model
+    r = smy.solve_lp(md, smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced",
+                                         adaptive_steps=True, detect=False))
+    primal, dual, bounds = model_residuals(md, r)
+    against = f"rel {rel(r.objective, ref['objective']):.2e} to the reference's FP64" if ref else "no fixture"
+    print(f"C3 mixed: {r.status_name} {r.iterations} it ({r.fp32_iterations} FP32), objective "
+          f"{r.objective!r} ({against}), residuals {primal:.2e} {dual:.2e} bounds {bounds:.1e}, "
+          f"dual bound {r.dual_bound!r}, verified {r.verified} ({r.max_primal_violation:.1e}, "
+          f"{r.max_dual_violation:.1e})")
+    assert r.status == smy.OPTIMAL
+    assert r.fp32_iterations >= 0.9 * r.iterations
+    assert max(r.primal_residual, r.dual_residual, abs(r.gap)) < 1e-6
+    assert primal < 2e-6 and dual < 2e-6 and bounds == 0.0
+    # Two points within rel KKT 1e-6 of an LP whose right-hand side is large (||b|| ~ 2e5): the
+    # allowed primal residual moves c'x by up to ~|y|'|b - Ax|, which on this LP is a few 1e-5
+    # of the objective (both answers are within the tolerance, at different points).
+    if ref is not None:
+        assert rel(r.objective, ref["objective"]) <= 5e-5
+    assert r.dual_bound <= r.objective + 1e-9 * abs(r.objective)
+    assert rel(r.dual_bound, r.objective) <= 1e-4
+
+
+def test_c4_blending_batch(gpu):
+    # 256 blending LPs (2.6k-60k rows each) as one stacked batch, mixed precision with adaptive
+    # steps: every LP optimal within the tolerance, objectives against HiGHS on every 32nd LP, and
+    # sampled members equal to the same LP solved alone (same iterations, bit-equal objective).
+    highs = load_golden("c4_sample")
+    models = smy.generators.c4()
+    nms = [md.native() for md in models]
+    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", adaptive_steps=True)
+    res, times = smy.solve_lp_batch(nms, opts)
+    its = [r.iterations for r in res]
+    print(f"C4: {times.solve_seconds - times.setup_seconds:.2f} s, iterations median "
+          f"{int(np.median(its))} max {max(its)}, verified {sum(r.verified for r in res)}")
+    assert all(r.status == smy.OPTIMAL for r in res), [r.message for r in res if r.status != smy.OPTIMAL][:3]
+    assert all(max(r.primal_residual, r.dual_residual, abs(r.gap)) < 1e-6 for r in res)
+    errs = []
+    for e in highs["lps"]:
+        assert e["status"] == 0
+        errs.append(rel(res[e["index"]].objective, e["objective"]))
+    print(f"C4 vs HiGHS on {len(errs)} LPs: max rel objective error {max(errs):.2e}")
+    assert max(errs) <= 2e-6
+    for i in (0, 97, 255):
+        alone = smy.solve_lp(nms[i], opts)
+        assert alone.iterations == res[i].iterations, i
+        assert alone.objective == res[i].objective, i
+
+
+def test_c5_node_batch(gpu):
+    # 1,024 node relaxations of the 20,000 x 50,000 knapsack LP, warm-started from the root LP's
+    # answer and primal weight, as one shared-matrix batch.
+    md, lows, ups = smy.generators.c5(1024)
+    nm = md.native()
+    A = csc(md)
+    # A node is infeasible exactly when the columns it fixes at 1 already overfill a row: every
+    # weight is positive and every free column can sit at 0.
+    infeasible = [bool(np.any(A @ lo > md.row_upper)) for lo in lows]
+    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", fp32_switch_tol=1e-6)
+    root = smy.solve_lp(nm, opts)
+    assert root.status == smy.OPTIMAL
+    opts.start_omega = root.omega
+    start = (root.col_value, root.row_dual)
+    res, times = smy.solve_lp_bounds_batch(nm, lows, ups, opts, start=start)
+    n_inf = sum(infeasible)
+    print(f"C5: {times.solve_seconds - times.setup_seconds:.2f} s, {n_inf} infeasible by "
+          f"construction, statuses {sorted(set(r.status_name for r in res))}, verified "
+          f"{sum(r.verified for r in res)} (infeasible {sum(r.verified for r, f in zip(res, infeasible) if f)})")
+    assert 0 < n_inf < len(lows)
+    for k, (r, inf) in enumerate(zip(res, infeasible)):
+        assert r.status == (smy.INFEASIBLE if inf else smy.OPTIMAL), (k, r.status_name, r.message)
+    # Every infeasible node's Farkas certificate passes the verifier.
+    assert all(r.verified for r, inf in zip(res, infeasible) if inf)
+    # Optimal nodes: within the tolerance, and the weak-duality bound (an upper bound here, the
+    # model maximizes) from each node's duals lies above its objective and close to it.
+    for k, r in enumerate(res):
+        if infeasible[k]:
+            continue
+        assert max(r.primal_residual, r.dual_residual, abs(r.gap)) < 1e-6, k
+        assert r.dual_bound >= r.objective - 1e-9 * abs(r.objective), k
+        assert rel(r.dual_bound, r.objective) <= 1e-4, (k, r.dual_bound, r.objective)
+    # Against plain FP64 PDLP (the reference's algorithm) solving sampled feasible nodes alone,
+    # warm-started alike.
+    o64 = smy.PdlpOptions(tol=1e-6, precision="fp64", algorithm="reference", start_omega=root.omega)
+    feas = [k for k in range(len(lows)) if not infeasible[k]]
+    errs = []
+    for k in feas[:: len(feas) // 8][:8]:
+        r64 = smy.solve_lp_bounds_batch(nm, [lows[k]], [ups[k]], o64, start=start)[0][0]
+        assert r64.status == smy.OPTIMAL
+        errs.append(rel(res[k].objective, r64.objective))
+    print(f"C5 vs FP64 one at a time on {len(errs)} nodes: max rel objective error {max(errs):.2e}")
+    assert max(errs) <= 1e-6
