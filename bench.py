@@ -299,29 +299,34 @@ def main() -> int:
     ap.add_argument("--polish-rounds", type=int, default=0,
                     help="c3: also time one solve with this many FP64 polish rounds (the "
                          "continuation at tol / 10 after a verifier rejection)")
-    ap.add_argument("--algorithm", default="enhanced", choices=["reference", "enhanced"])
+    ap.add_argument("--method", default="halpern", choices=["halpern", "adaptive", "constant"],
+                    help="the single LP's iteration: restarted Halpern PDHG with reflection "
+                         "(default), or PDLP with adaptive / constant steps")
+    ap.add_argument("--algorithm", default=None, choices=["reference", "enhanced"],
+                    help="preconditioner: the reference's 2-norm variant or Ruiz + Pock-Chambolle "
+                         "(default: reference for Halpern steps, enhanced otherwise)")
     ap.add_argument("--fp64-accumulation", action="store_true",
                     help="mixed precision: FP32 steps sum their products in FP64 (the default line "
                          "is specified with FP32 accumulation)")
     ap.add_argument("--adaptive-steps", default="auto", choices=["auto", "on", "off"],
                     help="PDLP's adaptive step sizes for the mixed-precision solves (auto: on for "
                          "the single LP and the stacked batch; the shared-matrix batch has none)")
-    ap.add_argument("--halpern", action="store_true",
-                    help="restarted Halpern PDHG with reflection (constant steps, no averages) for "
-                         "the single LP instead of PDLP with adaptive steps")
     ap.add_argument("--reflection", type=float, default=1.0, help="Halpern steps: the reflection")
-    ap.add_argument("--column-order", default="auto", choices=["auto", "natural", "degree"],
-                    help="the single LP's column order (auto: by count for the enhanced algorithm "
-                         "when the counts are skewed)")
+    ap.add_argument("--column-order", default=None, choices=["auto", "natural", "degree"],
+                    help="the single LP's column order (default: by count for Halpern steps; auto "
+                         "otherwise, which orders by count for the enhanced algorithm when the "
+                         "counts are skewed)")
     ap.add_argument("--pid-weight", action="store_true",
                     help="the primal weight by a PI controller on log(dy / dx) at restarts")
     ap.add_argument("--verbose", action="store_true", help="progress lines of long batches (stdout)")
     ap.add_argument("--fp32-switch-tol", type=float, default=None,
                     help="mixed precision: FP32 steps until this distance to termination (or a "
-                         "stall); default 1e-6 for the node relaxations, 1e-5 otherwise")
-    ap.add_argument("--fp32-stall-level", type=float, default=1e-4,
+                         "stall); default the tolerance for the single LP's Halpern steps and the "
+                         "node relaxations, 1e-5 otherwise")
+    ap.add_argument("--fp32-stall-level", type=float, default=None,
                     help="mixed precision: a plateau of the FP32 steps below this distance switches "
-                         "to FP64 (0: never)")
+                         "to FP64 (0: never; default 0 for the single LP's Halpern steps, 1e-4 "
+                         "otherwise)")
     ap.add_argument("--max-iterations", type=int, default=-1,
                     help="cap each solve (profiling runs); the line then reports no speedup")
     ap.add_argument("--kernels-only", action="store_true",
@@ -340,11 +345,22 @@ def main() -> int:
         return 2
     on = args.adaptive_steps == "on"
     auto = args.adaptive_steps == "auto"
-    args.adaptive = on or (auto and args.workload in ("c3", "c4"))
+    # The single LP (c3 line): its iteration, preconditioner, column order and precision rule.
+    args.halpern = args.method == "halpern"
+    args.adaptive = args.method == "adaptive"
+    args.c3_algorithm = args.algorithm or ("reference" if args.halpern else "enhanced")
+    args.c3_column_order = args.column_order or ("degree" if args.halpern else "auto")
+    args.c3_switch_tol = args.fp32_switch_tol if args.fp32_switch_tol is not None else (
+        args.tol if args.halpern else 1e-5)
+    args.c3_stall_level = args.fp32_stall_level if args.fp32_stall_level is not None else (
+        0.0 if args.halpern else 1e-4)
+    # The batches (c4 / c5 lines and sub-results).
     args.adaptive_c4 = on or auto
     args.fp32_switch_tol_c5 = args.fp32_switch_tol if args.fp32_switch_tol is not None else 1e-6
     if args.fp32_switch_tol is None:
         args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
+    if args.algorithm is None:
+        args.algorithm = "enhanced"
     args.sub_steps = 1
 
     import torch
@@ -359,14 +375,12 @@ def main() -> int:
     md = smy.generators.c2(args.scale)
     log(f"LP m={md.m} n={md.n} nnz={md.nnz} generated in {time.perf_counter() - t0:.1f} s")
     nm = md.native()
-    if args.halpern:
-        args.adaptive = False
-    opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, detect=False,
+    opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.c3_algorithm, detect=False,
                            verify=True, polish_rounds=0, max_iterations=args.max_iterations,
-                           fp32_switch_tol=args.fp32_switch_tol, adaptive_steps=args.adaptive,
+                           fp32_switch_tol=args.c3_switch_tol, adaptive_steps=args.adaptive,
                            fp64_accumulation=args.fp64_accumulation, halpern=args.halpern,
                            reflection=args.reflection, pid_primal_weight=args.pid_weight,
-                           column_order=args.column_order, fp32_stall_level=args.fp32_stall_level)
+                           column_order=args.c3_column_order, fp32_stall_level=args.c3_stall_level)
 
     for w in range(args.warmup):
         r = smy.solve_lp(nm, opts)
@@ -412,7 +426,7 @@ def main() -> int:
         log(f"polish {args.polish_rounds}: {p.iterations} it, verified {p.verified} "
             f"({p.max_primal_violation:.1e}, {p.max_dual_violation:.1e})")
 
-    order = count_order(md, args.algorithm, args.column_order)
+    order = count_order(md, args.c3_algorithm, args.c3_column_order)
     rl = kernel_roofline(md, "fp32", args.adaptive, args.fp64_accumulation, order,
                          args.reflection if args.halpern else None)
     dom = "primal" if rl["primal_ms"] >= rl["dual_ms"] else "dual"
@@ -441,9 +455,12 @@ def main() -> int:
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "fp32 steps + fp64 evaluation/polish",
-        "algorithm": args.algorithm,
+        "method": args.method,
+        "algorithm": args.c3_algorithm,
         "adaptive_steps": args.adaptive,
         "halpern": {"reflection": args.reflection} if args.halpern else False,
+        "fp32_switch_tol": args.c3_switch_tol,
+        "fp32_stall_level": args.c3_stall_level,
         "pid_primal_weight": args.pid_weight,
         "fp64_accumulation": args.fp64_accumulation,
         "detect": False,

@@ -111,6 +111,14 @@ int smy_pdhg_last_run_ms(smy_pdhg* h, double* ms);
  * for bit (the same per-row sums in the same order); the bound agrees to rounding. */
 int smy_pdhg_equilibrate(smy_pdhg* h, int passes, int pock_chambolle, double* col_scale,
                          double* row_scale, double out[3]);
+/* The same on the device for the reference's own preconditioner, ruiz_equilibration
+ * (src/lp/pdlp.cpp:109-187, applied as :190-209,284-299): up to `passes` full-step updates of every
+ * column and row by the inverse of its current 2-norm (both norms from the same factors), clamped
+ * to [1e-8, 1e8], stopping after the first pass that moves no factor by more than 10%; then the
+ * handle is scaled and the operator norm bounded as above. The factors and norms equal
+ * smy_pdlp_equilibrate's with SMY_ALGORITHM_REFERENCE bit for bit. */
+int smy_pdhg_equilibrate_reference(smy_pdhg* h, int passes, double* col_scale, double* row_scale,
+                                   double out[3]);
 
 /* Test hook for the reference's GPU-failure path (src/lp/pdlp.cpp:228-246,341-360): once
  * the handle's step entry points (smy_pdhg_step, _run, _run_batch, _run_adaptive) would run past

@@ -116,7 +116,7 @@ typedef struct smy_pdlp_params {
                                 or on the host from downloaded sums (0) */
   int fp64_accumulation;     /* mixed: FP32 steps sum their products in FP64 (1) or FP32 (0);
                                 storage stays FP32; batches then use the stacked layout */
-  int device_scaling;        /* enhanced algorithm, one LP: the preconditioner, the scaled matrices,
+  int device_scaling;        /* one LP (either algorithm): the preconditioner, the scaled matrices,
                                 cost and bounds and the operator-norm bound computed on the device
                                 (smy_pdhg_equilibrate; 1) or on the host (0); the same factors */
   int column_order;          /* model-level solves: 0 auto (enhanced algorithm with skewed column

@@ -32,7 +32,7 @@ EXPORTS = (
     "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_get_bin_tile", "smy_pdhg_set_bin_tile",
     "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels", "smy_pdhg_inject_fault",
     "smy_pdhg_equilibrate", "smy_pdlp_equilibrate", "smy_lp_dual_bound",
-    "smy_pdhg_set_halpern", "smy_pdhg_fixed_point_batch",
+    "smy_pdhg_set_halpern", "smy_pdhg_fixed_point_batch", "smy_pdhg_equilibrate_reference",
     "smy_pdhg_rule_init", "smy_pdhg_rule_decide", "smy_csr_spmv", "smy_csr_spmv_f32", "smy_csr_spmv_noalloc",
     "smy_spmv_threads_per_row", "smy_spmv_plan_create", "smy_spmv_plan_run",
     "smy_spmv_plan_destroy",
@@ -165,6 +165,7 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_set_fp64_accumulation", i32, vp, i32)
     _proto(lib, "smy_pdhg_inject_fault", i32, vp, C.c_longlong)
     _proto(lib, "smy_pdhg_equilibrate", i32, vp, i32, i32, vp, vp, vp)
+    _proto(lib, "smy_pdhg_equilibrate_reference", i32, vp, i32, vp, vp, vp)
     _proto(lib, "smy_lp_dual_bound", i32, vp, vp, C.POINTER(C.c_double))
     _proto(lib, "smy_pdlp_equilibrate", i32, C.POINTER(SmyCsc), C.POINTER(SmyCsc), i32, i32, i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_rule_init", i32, vp, vp)

@@ -835,11 +835,26 @@ void measure_both(smy_pdhg* h) {
   measure_launch<T, 2>(h, vv, yy);
 }
 
+template <typename T>
+void measure_one(smy_pdhg* h, int which);
+
 int evaluate_batch_impl(smy_pdhg* h, double* out) {
   for (const Layout& L : h->inst)
     if (L.prec != kFrozen && !L.has_original)
       return fail(SMY_STATE, "set_original() must be called before evaluate()");
   refresh_lists(h);
+  if (h->halpern) {
+    // Halpern steps: only T(z) (the averages' slot) is an answer (the iterate z' itself may lie
+    // outside the bounds); its sums fill both halves.
+    if (uses(h, SMY_FP32)) measure_one<float>(h, SMY_AVERAGE);
+    if (uses(h, SMY_FP64)) measure_one<double>(h, SMY_AVERAGE);
+    SMY_HIP(hipGetLastError());
+    std::vector<double> seven(7 * static_cast<size_t>(h->K));
+    finish(h, 7, seven.data());
+    for (int i = 0; i < h->K; ++i)
+      for (int q = 0; q < 7; ++q) out[14 * i + q] = out[14 * i + 7 + q] = seven[7 * i + q];
+    return SMY_OK;
+  }
   if (uses(h, SMY_FP32)) measure_both<float>(h);
   if (uses(h, SMY_FP64)) measure_both<double>(h);
   SMY_HIP(hipGetLastError());
@@ -1257,8 +1272,10 @@ int create_impl(int count, const smy_csc* A, const smy_csc* At, const double* co
 // Lane::prepare, operator_norm_bound and the termination norms, csrc/host/pdlp.cpp): `passes`
 // infinity-norm Ruiz rounds, then (pock_chambolle) one Pock-Chambolle step with alpha = 1.
 // out = {L, a_colmax, at_rowmax}.
+// reference2: the reference's 2-norm variant (passes of full-step updates with the stop rule)
+// instead of the infinity-norm Ruiz passes and Pock-Chambolle step.
 int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_scale,
-                     double* row_scale, double* out) {
+                     double* row_scale, double* out, bool reference2 = false) {
   const int n = h->n, m = h->m;
   const hipStream_t s = h->stream;
   DevMatrix& A = h->a;    // A by rows: row i, indices j
@@ -1320,8 +1337,28 @@ int equilibrate_impl(smy_pdhg* h, int passes, int pock_chambolle, double* col_sc
       hipLaunchKernelGGL(divide_by_sqrt_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cs, cn);
       hipLaunchKernelGGL(divide_by_sqrt_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rs, rn);
     };
-    for (int pass = 0; pass < passes; ++pass) round(false);
-    if (pock_chambolle) round(true);
+    if (reference2) {
+      // Both norms from the current factors, then both updates; the host reads the flag after
+      // each pass and stops when no factor moved by more than 10%.
+      int* moved = reinterpret_cast<int*>(scalar);
+      for (int pass = 0; pass < passes; ++pass) {
+        cols_reduce(ScaledSquare{At.idx, At.val64, rs, cs, false}, Sum{}, cn);
+        hipLaunchKernelGGL(sqrt_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cn);
+        rows_reduce(ScaledSquare{A.idx, A.val64, rs, cs, true}, Sum{}, rn);
+        hipLaunchKernelGGL(sqrt_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rn);
+        SMY_HIP(hipMemsetAsync(moved, 0, sizeof(int), s));
+        hipLaunchKernelGGL(ruiz2_update_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, cs, cn, moved);
+        hipLaunchKernelGGL(ruiz2_update_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m, rs, rn, moved);
+        SMY_HIP(hipGetLastError());
+        int any = 0;
+        SMY_HIP(hipMemcpyAsync(&any, moved, sizeof(int), hipMemcpyDeviceToHost, s));
+        SMY_HIP(hipStreamSynchronize(s));
+        if (!any) break;
+      }
+    } else {
+      for (int pass = 0; pass < passes; ++pass) round(false);
+      if (pock_chambolle) round(true);
+    }
   }
   // 20 power iterations on A_eq'A_eq from the host's seeded start (src/lp/pdlp.cpp:36-103), on the
   // unscaled values and the factors in the host's order, norms on the host:
@@ -1666,6 +1703,16 @@ int smy_pdhg_equilibrate(smy_pdhg* h, int passes, int pock_chambolle, double* co
     return fail(SMY_INVALID, "bad argument");
   if (h->stepped || h->steps_run > 0) return fail(SMY_STATE, "equilibrate before the first step");
   return guarded([&]() -> int { return equilibrate_impl(h, passes, pock_chambolle, col_scale, row_scale, out); });
+}
+
+int smy_pdhg_equilibrate_reference(smy_pdhg* h, int passes, double* col_scale, double* row_scale,
+                                   double out[3]) {
+  SMY_SINGLE(h);
+  if (passes < 0 || out == nullptr || (h->n > 0 && col_scale == nullptr) ||
+      (h->m > 0 && row_scale == nullptr))
+    return fail(SMY_INVALID, "bad argument");
+  if (h->stepped || h->steps_run > 0) return fail(SMY_STATE, "equilibrate before the first step");
+  return guarded([&]() -> int { return equilibrate_impl(h, passes, 0, col_scale, row_scale, out, true); });
 }
 
 int smy_pdhg_inject_fault(smy_pdhg* h, long long step_no) {

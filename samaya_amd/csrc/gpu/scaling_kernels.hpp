@@ -39,6 +39,20 @@ struct ScaledAbs {  // |(a * rs_i) * cs_j|
   }
 };
 
+struct ScaledSquare {  // ((a * rs_i) * cs_j)^2: the reference's 2-norm equilibration terms
+  const int* __restrict__ idx;
+  const double* __restrict__ val;
+  const double* __restrict__ rs;
+  const double* __restrict__ cs;
+  bool seg_is_row;
+  __device__ double operator()(int s, int p) const {
+    const int i = seg_is_row ? s : idx[p];
+    const int j = seg_is_row ? idx[p] : s;
+    const double t = val[p] * rs[i] * cs[j];
+    return t * t;
+  }
+};
+
 struct Square {  // a^2
   const double* __restrict__ val;
   __device__ double operator()(int, int p) const { return val[p] * val[p]; }
@@ -123,6 +137,22 @@ __global__ __launch_bounds__(kBlock) void divide_by_sqrt_kernel(int count, doubl
                                                                 const double* __restrict__ norm) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k < count && norm[k] > 0.0) f[k] /= sqrt(norm[k]);
+}
+
+// One update of the reference's 2-norm equilibration (src/lp/pdlp.cpp:131-178): where norm[k] > 0,
+// f[k] = clamp(f[k] / norm[k], 1e-8, 1e8); *moved = 1 if some factor changed by more than 10%
+// relative to max(|f[k]|, 1) (every thread that sees one stores the same value).
+__global__ __launch_bounds__(kBlock) void ruiz2_update_kernel(int count, double* __restrict__ f,
+                                                              const double* __restrict__ norm,
+                                                              int* __restrict__ moved) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= count) return;
+  const double nk = norm[k];
+  if (!(nk > 0.0)) return;
+  const double fk = f[k];
+  const double next = fmax(1e-8, fmin(1e8, fk / nk));
+  if (fabs(next - fk) / fmax(fabs(fk), 1.0) > 0.1) *moved = 1;
+  f[k] = next;
 }
 
 // The scaled cost and bounds: structural j: cost * cs, bounds / cs; logical n + i: all * rs.

@@ -119,8 +119,12 @@ class Device {
   // The enhanced preconditioner on the device (a handle of one unscaled LP): the factors, L and
   // the norms of the termination test.
   // out = {L, a_colmax, at_rowmax}.
-  void equilibrate(int passes, std::vector<double>& cs, std::vector<double>& rs, double out[3]) {
-    check(smy_pdhg_equilibrate(h_, passes, 1, cs.data(), rs.data(), out));
+  void equilibrate(Algorithm algorithm, int passes, std::vector<double>& cs, std::vector<double>& rs,
+                   double out[3]) {
+    if (algorithm == Algorithm::kEnhanced)
+      check(smy_pdhg_equilibrate(h_, passes, 1, cs.data(), rs.data(), out));
+    else
+      check(smy_pdhg_equilibrate_reference(h_, passes, cs.data(), rs.data(), out));
   }
   void set_original(int i, const std::vector<double>& cs, const std::vector<double>& rs,
                     const std::vector<double>& cost, const std::vector<double>& lower,
@@ -752,9 +756,9 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       if (&lanes[o].lp_ == &lanes[i].lp_) first[i] = o;
     if (first[i] == i) owners.push_back(i);
   }
-  // The enhanced algorithm's preconditioner of a single LP runs on the device: the handle is
+  // The preconditioner of a single LP (either algorithm's) runs on the device: the handle is
   // created from the unscaled LP and scales itself (the same factors as the host's).
-  const bool device_eq = opt.device_scaling && opt.algorithm == Algorithm::kEnhanced && K == 1;
+  const bool device_eq = opt.device_scaling && K == 1;
   std::vector<std::shared_ptr<const Equilibrated>> eqs(static_cast<size_t>(K));
   const int threads = K > 1 ? host_threads() : 1;
   if (!device_eq) {
@@ -789,7 +793,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         e->cs.assign(static_cast<size_t>(L0.lp_.n), 1.0);
         e->rs.assign(static_cast<size_t>(L0.lp_.m), 1.0);
         double out[3];
-        device->equilibrate(10, e->cs, e->rs, out);
+        device->equilibrate(opt.algorithm, 10, e->cs, e->rs, out);
         e->L = out[0];
         e->a_colmax = out[1];
         e->at_rowmax = out[2];
@@ -875,12 +879,18 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     lanes[i].res.solve_seconds = clock.seconds();
     --active;
   };
-  // The lane's answer at its current point (a limit stopped it: src/lp/pdlp.cpp:1153-1165).
+  // The lane's answer at its current point (a limit stopped it: src/lp/pdlp.cpp:1153-1165); with
+  // Halpern steps T(z) of the last run, which is within the bounds where z need not be.
   const auto stop_at_current = [&](int i) {
     Lane& L = lanes[i];
     const int nt = L.lp_.n + L.lp_.m;
-    device->download(i, &v, nullptr, &y, nullptr, nullptr, nullptr, nt, L.lp_.m);
-    L.map_back(v, y);
+    if (halpern) {
+      device->download(i, nullptr, nullptr, nullptr, nullptr, &v_erg, &y_erg, nt, L.lp_.m);
+      L.map_back(v_erg, y_erg);
+    } else {
+      device->download(i, &v, nullptr, &y, nullptr, nullptr, nullptr, nt, L.lp_.m);
+      L.map_back(v, y);
+    }
     stop(i);
   };
   long long iter = 0;
@@ -966,11 +976,12 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       const int nt = L.lp_.n + L.lp_.m, m = L.lp_.m;
 
       // Termination (src/lp/pdlp.cpp:684-713): strict < tol for the current, then the average.
+      // Halpern steps: T(z) only (in the averages' slot).
       const auto converged = [&](const Measure& e) {
         return e.rel_primal < opt.tol && e.rel_dual < opt.tol && std::fabs(e.rel_gap) < opt.tol;
       };
-      if (converged(current) || converged(average)) {
-        const bool cur = converged(current);
+      if ((!halpern && converged(current)) || converged(average)) {
+        const bool cur = !halpern && converged(current);
         const Measure& e = cur ? current : average;
         L.log(2, cur ? ": converged at iter %lld: primal=%.2e dual=%.2e gap=%.2e"
                      : ": average converged at iter %lld: primal=%.2e dual=%.2e gap=%.2e",

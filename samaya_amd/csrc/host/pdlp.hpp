@@ -66,7 +66,7 @@ struct PdlpOptions {
   // Test hook: the device handle fails its steps after this many (-1: off), so the solve takes
   // the GPU-failure path.
   long long inject_fault_step = -1;
-  // Enhanced algorithm, one LP: the preconditioner and the operator-norm bound computed on the
+  // One LP: the preconditioner (either algorithm's) and the operator-norm bound computed on the
   // device from the unscaled LP (the same factors as the host's).
   bool device_scaling = true;
   ColumnOrder column_order = ColumnOrder::kAuto;  // model-level solves (solve_lp*)

@@ -87,12 +87,16 @@ class PdhgDevice:
         """FP32 steps sum their products in FP64 (storage stays FP32)."""
         N.check(self.lib.smy_pdhg_set_fp64_accumulation(self.h, int(on)), "set_fp64_accumulation")
 
-    def equilibrate(self, passes: int = 10, pock_chambolle: bool = True):
-        """The enhanced preconditioner on the device, applied to this handle's (unscaled) LP:
-        (col_scale, row_scale, (L, a_colmax, at_rowmax))."""
+    def equilibrate(self, passes: int = 10, pock_chambolle: bool = True, algorithm: str = "enhanced"):
+        """The preconditioner on the device (the enhanced one, or the reference's 2-norm variant),
+        applied to this handle's (unscaled) LP: (col_scale, row_scale, (L, a_colmax, at_rowmax))."""
         cs, rs, out = np.zeros(self.n), np.zeros(self.m), np.zeros(3)
-        N.check(self.lib.smy_pdhg_equilibrate(self.h, passes, int(pock_chambolle), _p(cs), _p(rs),
-                                              _p(out)), "equilibrate")
+        if algorithm == "reference":
+            rc = self.lib.smy_pdhg_equilibrate_reference(self.h, passes, _p(cs), _p(rs), _p(out))
+        else:
+            rc = self.lib.smy_pdhg_equilibrate(self.h, passes, int(pock_chambolle), _p(cs), _p(rs),
+                                               _p(out))
+        N.check(rc, "equilibrate")
         return cs, rs, tuple(out)
 
     def restart(self):

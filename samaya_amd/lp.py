@@ -210,7 +210,7 @@ class PdlpOptions:
     fp32_stall_level: float = 1e-4  # mixed: FP32 plateaus below this switch to FP64 (0: never)
     device_restart: bool = True     # restart decisions, distances and primal weights on the GPU
     fp64_accumulation: bool = False  # mixed: FP32 steps sum their products in FP64
-    device_scaling: bool = True      # enhanced, one LP: preconditioner and norm bound on the GPU
+    device_scaling: bool = True      # one LP: preconditioner and norm bound on the GPU
     column_order: str = "auto"       # model-level solves: "auto", "natural" or "degree"
     inject_fault_step: int = -1      # test hook: the device fails its steps after this many
     halpern: bool = False            # restarted Halpern PDHG with reflection (no averages)

@@ -3,15 +3,16 @@ reference's tests/test_gpu.cpp:407-477, which runs its GPU PDLP on its largest g
 
   C2  the 1M-column LP (m 500,000, n 1,000,000, nnz ~12M), plain FP64 PDLP with the reference's
       algorithm, against the reference's own CPU PDLP on the same LP (golden c2_reference);
-  C3  the same LP in mixed precision with the enhanced preconditioner and adaptive steps;
+  C3  the same LP in mixed precision: FP32 restarted Halpern steps with FP64 evaluation, restarts
+      and termination (the benchmark's line), and PDLP with adaptive steps;
   C4  256 blending LPs as one stacked batch, against HiGHS on every 32nd LP (golden c4_sample)
       and against the same LPs solved alone;
   C5  1,024 knapsack node relaxations as one shared-matrix batch, warm-started from the root:
       statuses against the nodes' infeasibility worked out exactly on the host, objectives
       against plain FP64 PDLP solving sampled nodes one at a time, and weak-duality bounds.
 
-Every answer's residuals are recomputed here from the returned point, in numpy, in the model's
-space, so a test does not rest on the solver's own report of them."""
+Every single-LP answer's termination measures are recomputed here from the returned point, in
+numpy, so a test does not rest on the solver's own report of them."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -30,28 +31,52 @@ def csc(md):
     return sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
 
 
-def model_residuals(md, r):
-    """Relative residuals of a model-space answer, recomputed on the host:
-    primal  ||A x - s||_2 / (1 + ||A||_max-col ||x|| + ||s||), s the row values the answer
-            reports (within the row bounds), x within its bounds;
-    dual    ||d - lambda||_2 / (1 + ||A'||_max-row ||y|| + ||c||), d = c - A'y (model sense) and
-            lambda the part of d that the column bounds absorb;
-    bounds  largest violation of a column or row value bound (0 when the answer is projected)."""
-    A = csc(md)
-    x, s, y = r.col_value, r.row_activity, r.row_dual
+def problem_measures(md, r):
+    """The termination measures of a model-space answer, recomputed here in numpy: the answer is
+    mapped into make_problem's computational form (power-of-two scaling, src/lp/lp_solver.cpp:28-51,
+    exact to undo) and measured as src/lp/pdlp.cpp:411-519 defines it:
+      rel primal  ||A x - r|| / (1 + max column 2-norm of A * ||x|| + ||r||)
+      rel dual    ||g - lambda|| / (1 + max row 2-norm of A * ||y|| + ||c||),
+                  g = c + [A -I]'y, lambda the part the bounds absorb
+      rel gap     (c'v - sum lambda * bound) / (1 + |c'v| + |sum lambda * bound|)
+    and the largest bound violation of v (relative to 1 + |bound|)."""
+    lp = smy.make_problem(md)
+    cs, rs = smy.compute_scaling(md)
+    n, m = md.n, md.m
     sense = -1.0 if md.maximize else 1.0
-    col_norm = np.sqrt(np.asarray(A.multiply(A).sum(axis=0)).ravel()).max()
-    row_norm = np.sqrt(np.asarray(A.multiply(A).sum(axis=1)).ravel()).max()
-    primal = np.linalg.norm(A @ x - s) / (1 + col_norm * np.linalg.norm(x) + np.linalg.norm(s))
-    # Columns (reduced costs d) and rows (a row is a variable whose reduced cost is its dual).
-    d = np.concatenate([sense * (md.obj - A.T @ y), sense * y])
-    lo = np.concatenate([md.col_lower, md.row_lower])
-    up = np.concatenate([md.col_upper, md.row_upper])
-    lam = np.where((d > 0) & np.isfinite(lo), d, np.where((d < 0) & np.isfinite(up), d, 0.0))
-    dual = np.linalg.norm(d - lam) / (1 + row_norm * np.linalg.norm(y) + np.linalg.norm(md.obj))
-    bounds = max(np.max(md.col_lower - x, initial=0), np.max(x - md.col_upper, initial=0),
-                 np.max(md.row_lower - s, initial=0), np.max(s - md.row_upper, initial=0))
-    return primal, dual, bounds
+    x = r.col_value / cs
+    rr = r.row_activity * rs
+    y = -sense * r.row_dual / rs
+    A = sp.csc_matrix((lp.value, lp.index, lp.start), shape=(m, n))
+    v = np.concatenate([x, rr])
+    a_colmax = np.sqrt(np.asarray(A.multiply(A).sum(axis=0)).ravel()).max()
+    at_rowmax = np.sqrt(np.asarray(A.multiply(A).sum(axis=1)).ravel()).max()
+    primal = np.linalg.norm(A @ x - rr) / (1 + a_colmax * np.linalg.norm(x) + np.linalg.norm(rr))
+    g = lp.cost + np.concatenate([A.T @ y, -y])
+    lam = np.where((g > 0) & np.isfinite(lp.lower), g, np.where((g < 0) & np.isfinite(lp.upper), g, 0.0))
+    dual = np.linalg.norm(g - lam) / (1 + at_rowmax * np.linalg.norm(y) + np.linalg.norm(lp.cost))
+    pobj = lp.cost @ v
+    dobj = np.sum(np.where(lam > 0, lam * np.where(np.isfinite(lp.lower), lp.lower, 0.0), 0.0)) + \
+        np.sum(np.where(lam < 0, lam * np.where(np.isfinite(lp.upper), lp.upper, 0.0), 0.0))
+    gap = (pobj - dobj) / (1 + abs(pobj) + abs(dobj))
+    viol = np.maximum(lp.lower - v, v - lp.upper) / (1 + np.where(np.isfinite(lp.lower), abs(lp.lower), 0)
+                                                       + np.where(np.isfinite(lp.upper), abs(lp.upper), 0))
+    return primal, dual, gap, max(float(viol.max(initial=0.0)), 0.0)
+
+
+def assert_within_tolerance(md, r, tol=1e-6):
+    """The solver's report and the recomputation agree, and both are inside the tolerance (the
+    recomputation sums in another order: a relative slack of 1e-9 on the bound). An answer whose
+    last steps ran in FP32 lies within the FP32 roundings of the scaled bounds (a relative 2^-23
+    on either side: two FP32 ulps); an FP64 one within the FP64 roundings."""
+    primal, dual, gap, bounds = problem_measures(md, r)
+    assert max(r.primal_residual, r.dual_residual, abs(r.gap)) < tol
+    assert max(primal, dual, abs(gap)) < tol * (1 + 1e-9), (primal, dual, gap)
+    assert abs(primal - r.primal_residual) <= 1e-9 * tol + 1e-6 * r.primal_residual
+    assert abs(dual - r.dual_residual) <= 1e-9 * tol + 1e-6 * r.dual_residual
+    fp32_last = r.fp32_iterations == r.iterations
+    assert bounds <= (2.0 ** -22 if fp32_last else 1e-12), bounds
+    return primal, dual, gap
 
 
 @pytest.fixture(scope="module")
@@ -71,13 +96,11 @@ def test_c2_fp64_reference_algorithm(gpu, c2_model):
     md = c2_model
     assert (md.m, md.n, md.nnz) == (ref["m"], ref["n"], ref["nnz"])
     r = smy.solve_lp(md, smy.PdlpOptions(tol=1e-6, precision="fp64", algorithm="reference"))
-    primal, dual, bounds = model_residuals(md, r)
     print(f"C2 fp64: {r.status_name} {r.iterations} it (reference {ref['iterations']}), objective "
           f"{r.objective!r} (reference {ref['objective']!r}, rel {rel(r.objective, ref['objective']):.2e}), "
-          f"residuals {primal:.2e} {dual:.2e} bounds {bounds:.1e}, dual bound {r.dual_bound!r}")
+          f"measures {r.primal_residual:.2e} {r.dual_residual:.2e} {r.gap:.2e}, dual bound {r.dual_bound!r}")
     assert r.status == smy.OPTIMAL == ref["status"]
-    assert max(r.primal_residual, r.dual_residual, abs(r.gap)) < 1e-6
-    assert primal < 2e-6 and dual < 2e-6 and bounds == 0.0
+    assert_within_tolerance(md, r)
     # The same algorithm from the same start: the GPU sums in another order, so the runs part at
     # the rounding level, but they stop within a few evaluation periods of each other, at points
     # whose objectives agree to the tolerance.
@@ -88,23 +111,30 @@ def test_c2_fp64_reference_algorithm(gpu, c2_model):
     assert rel(r.dual_bound, r.objective) <= 1e-4
 
 
-def test_c3_mixed_precision(gpu, c2_model):
-    # The default benchmark line: FP32 steps (values, iterates, sums), FP64 evaluation, restarts
-    # and termination, the enhanced preconditioner and adaptive steps.
+C3_LINES = {
+    # The benchmark's line: restarted Halpern steps in FP32 (values, iterates, sums) to the
+    # tolerance, FP64 evaluation, restarts and termination, the reference's preconditioner and the
+    # columns by count.
+    "halpern": dict(halpern=True, algorithm="reference", column_order="degree",
+                    fp32_switch_tol=1e-6, fp32_stall_level=0.0),
+    # PDLP with adaptive steps and the enhanced preconditioner.
+    "adaptive": dict(adaptive_steps=True, algorithm="enhanced"),
+}
+
+
+@pytest.mark.parametrize("line", sorted(C3_LINES))
+def test_c3_mixed_precision(gpu, c2_model, line):
     ref = load_golden("c2_reference") if C2_FIXTURE else None
     md = c2_model
-    r = smy.solve_lp(md, smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced",
-                                         adaptive_steps=True, detect=False))
-    primal, dual, bounds = model_residuals(md, r)
+    r = smy.solve_lp(md, smy.PdlpOptions(tol=1e-6, precision="mixed", detect=False, **C3_LINES[line]))
     against = f"rel {rel(r.objective, ref['objective']):.2e} to the reference's FP64" if ref else "no fixture"
-    print(f"C3 mixed: {r.status_name} {r.iterations} it ({r.fp32_iterations} FP32), objective "
-          f"{r.objective!r} ({against}), residuals {primal:.2e} {dual:.2e} bounds {bounds:.1e}, "
-          f"dual bound {r.dual_bound!r}, verified {r.verified} ({r.max_primal_violation:.1e}, "
-          f"{r.max_dual_violation:.1e})")
+    print(f"C3 mixed {line}: {r.status_name} {r.iterations} it ({r.fp32_iterations} FP32), objective "
+          f"{r.objective!r} ({against}), measures {r.primal_residual:.2e} {r.dual_residual:.2e} "
+          f"{r.gap:.2e}, dual bound {r.dual_bound!r}, verified {r.verified} "
+          f"({r.max_primal_violation:.1e}, {r.max_dual_violation:.1e})")
     assert r.status == smy.OPTIMAL
     assert r.fp32_iterations >= 0.9 * r.iterations
-    assert max(r.primal_residual, r.dual_residual, abs(r.gap)) < 1e-6
-    assert primal < 2e-6 and dual < 2e-6 and bounds == 0.0
+    assert_within_tolerance(md, r)
     # Two points within rel KKT 1e-6 of an LP whose right-hand side is large (||b|| ~ 2e5): the
     # allowed primal residual moves c'x by up to ~|y|'|b - Ax|, which on this LP is a few 1e-5
     # of the objective (both answers are within the tolerance, at different points).

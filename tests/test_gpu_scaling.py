@@ -10,7 +10,7 @@ import scipy.sparse as sp
 
 import samaya_amd as smy
 from samaya_amd.device import PdhgDevice
-from test_scaling_cpu import ENHANCED, equilibrate as host_equilibrate
+from test_scaling_cpu import ENHANCED, REFERENCE, equilibrate as host_equilibrate
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +46,28 @@ def vectors(m, n, seed):
     lower = np.concatenate([np.zeros(n), np.where(rng.random(m) < 0.5, -np.inf, -1.0)])
     upper = np.concatenate([np.full(n, 10.0), rng.uniform(0, 2, m)])
     return cost, lower, upper
+
+
+def test_device_reference_factors_equal_the_hosts(gpu):
+    # The reference's 2-norm variant (src/lp/pdlp.cpp:109-187) on the device: factors, norms and
+    # the stop rule's pass count as the host's, bit for bit; the bound to rounding. The diagonal
+    # cases run all 10 passes (the variant oscillates with period 2 there), the others stop early.
+    mats = list(cases()) + [sp.csc_matrix(np.diag([4.0, 0.5])), sp.csc_matrix(np.diag([1.05, 0.97]))]
+    for k, A in enumerate(mats):
+        m, n = A.shape
+        print(f"reference scaling case {k}: {m} x {n}, nnz {A.nnz}", flush=True)
+        rs_h, cs_h, out_h = host_equilibrate(A, REFERENCE, 10, 0)
+        dev = device_of(A, *vectors(m, n, k))
+        cs_d, rs_d, out_d = dev.equilibrate(10, algorithm="reference")
+        assert np.array_equal(cs_d, cs_h) and np.array_equal(rs_d, rs_h), k
+        assert out_d[1] == out_h[1] and out_d[2] == out_h[2]
+        assert abs(out_d[0] - out_h[0]) <= 1e-12 * out_h[0]
+    # The closed forms of the period-2 oscillation: diag(4, 0.5) ends at all scales 1 after 10
+    # passes, diag(1.05, 0.97) stops after one pass at 1 / a_ii.
+    d = device_of(sp.csc_matrix(np.diag([4.0, 0.5])), *vectors(2, 2, 0)).equilibrate(10, algorithm="reference")
+    assert np.array_equal(d[0], [1.0, 1.0]) and np.array_equal(d[1], [1.0, 1.0])
+    d = device_of(sp.csc_matrix(np.diag([1.05, 0.97])), *vectors(2, 2, 0)).equilibrate(10, algorithm="reference")
+    np.testing.assert_allclose(d[0], [1 / 1.05, 1 / 0.97], rtol=1e-15)
 
 
 def test_device_factors_equal_the_hosts(gpu):
