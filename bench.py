@@ -311,6 +311,8 @@ def main() -> int:
     ap.add_argument("--adaptive-steps", default="auto", choices=["auto", "on", "off"],
                     help="PDLP's adaptive step sizes for the mixed-precision solves (auto: on for "
                          "the single LP and the stacked batch; the shared-matrix batch has none)")
+    ap.add_argument("--c4-method", default="halpern", choices=["halpern", "adaptive", "constant"],
+                    help="the blending batch's iteration (with the enhanced preconditioner)")
     ap.add_argument("--reflection", type=float, default=1.0, help="Halpern steps: the reflection")
     ap.add_argument("--column-order", default=None, choices=["auto", "natural", "degree"],
                     help="the single LP's column order (default: by count for Halpern steps; auto "
@@ -355,7 +357,8 @@ def main() -> int:
     args.c3_stall_level = args.fp32_stall_level if args.fp32_stall_level is not None else (
         0.0 if args.halpern else 1e-4)
     # The batches (c4 / c5 lines and sub-results).
-    args.adaptive_c4 = on or auto
+    args.halpern_c4 = args.c4_method == "halpern"
+    args.adaptive_c4 = args.c4_method == "adaptive" or (args.c4_method == "constant" and on)
     args.fp32_switch_tol_c5 = args.fp32_switch_tol if args.fp32_switch_tol is not None else 1e-6
     if args.fp32_switch_tol is None:
         args.fp32_switch_tol = 1e-6 if args.workload == "c5" else 1e-5
@@ -396,7 +399,8 @@ def main() -> int:
         runs.append(r)
         log(f"step {k}: {r.status_name} {r.iterations} it ({r.fp32_iterations} FP32), "
             f"setup {r.setup_seconds:.2f} s, iterate {r.solve_seconds - r.setup_seconds:.3f} s, "
-            f"verified {r.verified} ({r.max_primal_violation:.1e}, {r.max_dual_violation:.1e})")
+            f"verified {r.verified} ({r.max_primal_violation:.1e}, {r.max_dual_violation:.1e}), "
+            f"objective {r.objective!r}, dual bound {r.dual_bound!r}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     iterate_s = sum(r.solve_seconds - r.setup_seconds for r in runs) / len(runs)
@@ -484,6 +488,7 @@ def main() -> int:
         "max_dual_violation": last.max_dual_violation,
         "time_to_verified_s": None,
         "objective": last.objective,
+        "dual_bound": last.dual_bound,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBPS, 4),
                      "traffic": profiled_traffic("c3", f"{dom_op}<float>") if args.scale == 1.0 else None,
@@ -556,18 +561,24 @@ def base_line(args, value: float, ms_per_step: float, workload: str, extra: dict
 
 
 def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool = False,
-                   coded: bool = False, traffic_of: str | None = None) -> dict:
+                   coded: bool = False, traffic_of: str | None = None,
+                   halpern: float | None = None) -> dict:
     """Live timing of the FP32 step kernels of a batch handle; k: instances of a shared-matrix
     batch (None: stacked batch); adaptive: the adaptive attempt's kernels (stacked batch; same
     byte model); coded: the shared batch keeps its column bounds as codes; traffic_of: the
     workload whose profiled traffic applies (None: the handle is not that full workload)."""
-    if adaptive:
+    if halpern is not None:
+        dev.set_halpern(True, halpern)
+        dual_ms, primal_ms = dev.time_kernels(20)
+        ops = ("HalpernRowsOp", "HalpernColsOp")
+    elif adaptive:
         dual_ms, primal_ms, _ = dev.time_adaptive_kernels(20)
         ops = ("AdaptDualOp", "AdaptPrimalOp")
     else:
         dual_ms, primal_ms = dev.time_kernels(20)
         ops = ("DualRowsOp", "PrimalColsOp")
-    b = shared_step_bytes(nnz, m, n, k, 4, coded) if k else step_kernel_bytes(nnz, m, n, 4)
+    b = (shared_step_bytes(nnz, m, n, k, 4, coded) if k else
+         step_kernel_bytes(nnz, m, n, 4, halpern is not None))
     dom = "primal" if primal_ms >= dual_ms else "dual"
     ms = primal_ms if dom == "primal" else dual_ms
     achieved = b[dom] / (ms * 1e-3) / 1e9
@@ -638,7 +649,8 @@ def c4_result(args, fp64_every: int = 16, cpu_probe_every: int = 4) -> dict:
         if args.fp64_accumulation:
             dev.set_fp64_accumulation(True)
         out = batch_roofline(dev, nnz, M, N, None, args.adaptive_c4,
-                             traffic_of="c4" if args.scale == 1.0 else None)
+                             traffic_of="c4" if args.scale == 1.0 else None,
+                             halpern=args.reflection if args.halpern_c4 else None)
         # The tuner's choice on this handle: a tile per size bin, and whether the bins go apart.
         out["size_bins"] = {}
         for matrix, name in ((0, "A"), (1, "At")):
@@ -654,7 +666,8 @@ def c4_result(args, fp64_every: int = 16, cpu_probe_every: int = 4) -> dict:
                 "config": {"workload": f"{K} blending LPs, stacked FP32 step kernels"}}
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol, log_level=1 if args.verbose else 0,
-                           adaptive_steps=args.adaptive_c4, fp64_accumulation=args.fp64_accumulation)
+                           adaptive_steps=args.adaptive_c4, fp64_accumulation=args.fp64_accumulation,
+                           halpern=args.halpern_c4, reflection=args.reflection)
     smy.solve_lp_batch(nms[:8], opts)  # warm up the tuner and the graph path on a small batch
     runs = []
     t_start = time.perf_counter()
@@ -709,6 +722,7 @@ def c4_result(args, fp64_every: int = 16, cpu_probe_every: int = 4) -> dict:
         "status": "optimal", "iterations_median": int(np.median(its)), "iterations_max": int(its.max()),
         "fp32_iterations_sum": int(sum(r.fp32_iterations for r in res)),
         "fp32_switch_tol": args.fp32_switch_tol, "adaptive_steps": args.adaptive_c4,
+        "method": args.c4_method,
         "setup_s": round(times.setup_seconds, 3), "verified": int(sum(r.verified for r in res)),
         "lps": K, "roofline": rl, "cpu_baseline": cpu,
         "config": {"workload": f"{K} multi-period blending LPs (periods U{{4..48}}, 20 units x "

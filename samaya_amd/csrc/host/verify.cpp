@@ -511,7 +511,45 @@ void absorb(LpJob& job, PdlpResult&& result, int round, const LpSolveOptions& op
     out.infeasibility_certificate = r.dual_ray;
     for (int i = 0; i < m; ++i) out.infeasibility_certificate[i] *= scaled.row[i];
     for (double& t : out.infeasibility_certificate) t *= sense;
-    if (options.verify) rep = verify_infeasibility(model, out.infeasibility_certificate.data());
+    if (options.verify) {
+      rep = verify_infeasibility(model, out.infeasibility_certificate.data());
+      if (!rep.ok) {
+        // A Farkas certificate must be sign-feasible: the dual of a row bounded on one side only
+        // has that side's sign, a free row's is 0. The raw PDLP dual (the reference returns it as
+        // is) may carry small entries of the wrong sign, each of which makes the certificate's
+        // range unbounded, and small entries on rows that play no part in the proof, each of
+        // which widens the range. Its sign-feasible part, in either orientation, with the entries
+        // below a fraction of its largest dropped (none, then 1e-6, 1e-3, 1e-1), is checked next;
+        // the first that the verifier accepts is kept.
+        bool done = false;
+        for (const double orient : {1.0, -1.0}) {
+          std::vector<double> base = out.infeasibility_certificate;
+          double y_max = 0.0;
+          for (int i = 0; i < m; ++i) {
+            double t = orient * base[i];
+            const bool lo = std::isfinite(model.row_lower[i]), up = std::isfinite(model.row_upper[i]);
+            if (!lo && !up) t = 0.0;
+            else if (!lo) t = std::max(t, 0.0);
+            else if (!up) t = std::min(t, 0.0);
+            base[i] = t;
+            y_max = std::max(y_max, std::fabs(t));
+          }
+          for (const double drop : {0.0, 1e-6, 1e-3, 1e-1}) {
+            std::vector<double> y = base;
+            for (double& t : y)
+              if (std::fabs(t) < drop * y_max) t = 0.0;
+            const VerifyReport cleaned = verify_infeasibility(model, y.data());
+            if (cleaned.ok) {
+              out.infeasibility_certificate = std::move(y);
+              rep = cleaned;
+              done = true;
+              break;
+            }
+          }
+          if (done) break;
+        }
+      }
+    }
   } else if (r.status == Status::kUnbounded && !r.primal_ray.empty()) {
     out.unbounded_ray = r.primal_ray;
     for (int j = 0; j < n; ++j) out.unbounded_ray[j] *= scaled.col[j];

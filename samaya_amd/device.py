@@ -240,6 +240,10 @@ class PdhgBatch:
         q = {"fp64": N.SMY_FP64, "fp32": N.SMY_FP32}[precision]
         N.check(self.lib.smy_pdhg_set_tile(self.h, matrix, q, per_wg, int(xcd)), "set_tile")
 
+    def set_halpern(self, on: bool, reflection: float = 1.0):
+        """Restarted Halpern steps with this reflection (smy_pdhg_set_halpern)."""
+        N.check(self.lib.smy_pdhg_set_halpern(self.h, int(on), reflection), "set_halpern")
+
     def set_fp64_accumulation(self, on: bool):
         """FP32 steps sum their products in FP64 (storage stays FP32)."""
         N.check(self.lib.smy_pdhg_set_fp64_accumulation(self.h, int(on)), "set_fp64_accumulation")

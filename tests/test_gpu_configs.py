@@ -135,23 +135,29 @@ def test_c3_mixed_precision(gpu, c2_model, line):
     assert r.status == smy.OPTIMAL
     assert r.fp32_iterations >= 0.9 * r.iterations
     assert_within_tolerance(md, r)
-    # Two points within rel KKT 1e-6 of an LP whose right-hand side is large (||b|| ~ 2e5): the
-    # allowed primal residual moves c'x by up to ~|y|'|b - Ax|, which on this LP is a few 1e-5
-    # of the objective (both answers are within the tolerance, at different points).
+    # What rel KKT 1e-6 allows on this LP. Its normalizers are large (||b|| ~ 2e5, the largest
+    # column 2-norm ~ 1e3): the allowed primal residual moves c'x by up to |y|'|Ax - r| and the
+    # allowed dual residual moves the weak-duality bound by up to sum_j |d_j - lambda_j| |x_j|,
+    # each ~1e-4 of the objective here. Measured: the Halpern line's answer at 1e-6 lies 1.7e-4
+    # above its own dual bound, the adaptive line's 1e-5; a Halpern run to 1e-7 brackets the
+    # optimum in [-1506808.32, -1506806.35] (its dual bound and objective), 4.7e-5 and 2.7e-5
+    # from the two 1e-6 answers. So the answers are checked at 1e-4 against the reference's own
+    # FP64 PDLP answer at 1e-6 and at 3e-4 against their rigorous dual bound.
     if ref is not None:
-        assert rel(r.objective, ref["objective"]) <= 5e-5
+        assert rel(r.objective, ref["objective"]) <= 1e-4
     assert r.dual_bound <= r.objective + 1e-9 * abs(r.objective)
-    assert rel(r.dual_bound, r.objective) <= 1e-4
+    assert rel(r.dual_bound, r.objective) <= 3e-4
 
 
 def test_c4_blending_batch(gpu):
-    # 256 blending LPs (2.6k-60k rows each) as one stacked batch, mixed precision with adaptive
-    # steps: every LP optimal within the tolerance, objectives against HiGHS on every 32nd LP, and
-    # sampled members equal to the same LP solved alone (same iterations, bit-equal objective).
+    # 256 blending LPs (2.6k-60k rows each) as one stacked batch, mixed precision with restarted
+    # Halpern steps and the enhanced preconditioner (the benchmark's batch line): every LP optimal
+    # within the tolerance, objectives against HiGHS on every 32nd LP, and sampled members equal
+    # to the same LP solved alone (same iterations, bit-equal objective).
     highs = load_golden("c4_sample")
     models = smy.generators.c4()
     nms = [md.native() for md in models]
-    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", adaptive_steps=True)
+    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", halpern=True)
     res, times = smy.solve_lp_batch(nms, opts)
     its = [r.iterations for r in res]
     print(f"C4: {times.solve_seconds - times.setup_seconds:.2f} s, iterations median "
