@@ -4,8 +4,9 @@
 Workload: the 1,000,000-column sparse LP (m 500,000, nnz ~12M, power-law column counts on
 [2, 5,000] with mean 12, values N(0,1) x 10^U[-2,2], feasible by construction; the library's
 generator, seed 1002), solved to max(rel primal, rel dual, |rel gap|) < 1e-6 with
-mixed-precision PDLP (FP32 values / iterates / accumulation in the steps, FP64 evaluation,
-restarts and termination, FP64 polish). One step = one such solve.
+mixed-precision restarted Halpern PDHG (FP32 values / iterates / accumulation in the steps, FP64
+evaluation, restarts and termination; the reference's preconditioner, on the device). One step =
+one such solve. --method adaptive gives PDLP with adaptive steps (the enhanced preconditioner).
 
   value        time to 1e-6 of the mixed-precision solve, in seconds, from the moment the
                problem is resident in HBM (host scaling and upload reported separately as
@@ -20,7 +21,8 @@ restarts and termination, FP64 polish). One step = one such solve.
 
 Batched workloads (--workload):
   c4  256 multi-period blending LPs (2.9k-59.5k rows each, 91M nonzeros in all) solved as one
-      stacked batch, mixed precision, to 1e-6 each; value = seconds until the last LP stopped;
+      stacked batch, mixed precision with Halpern steps and the enhanced preconditioner, to 1e-6
+      each; value = seconds until the last LP stopped;
       compared with plain FP64 PDLP solving them one at a time (a timed sample of 16, scaled)
   c5  1,024 node relaxations of one knapsack LP (m 20,000, n 50,000, nnz 400k; 1-5% of the
       columns fixed at 0 or 1 per node) solved as one shared-matrix batch (SpMM steps), warm-started
