@@ -619,8 +619,13 @@ void run_jobs(std::vector<LpJob>& jobs, const LpSolveOptions& options, PdlpBatch
     }
     for_each_parallel(static_cast<int>(open.size()),
                       [&](int q) { absorb(jobs[open[q]], std::move(results[q]), round, options); });
+    // Polish rounds: FP64 PDHG steps with averages and constant step sizes (the reference's
+    // iteration) from the rejected point, to a tenth of the tolerance. Halpern steps move slowly
+    // below ~1e-6 on some LPs (the 1M-column LP took 508k FP64 steps from 1e-6 to 1e-7).
     popt.tol /= 10.0;
     popt.precision = Precision::kFp64;
+    popt.halpern = false;
+    popt.adaptive_steps = false;
   }
 }
 

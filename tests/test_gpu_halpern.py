@@ -206,6 +206,17 @@ def test_halpern_c1_against_highs(gpu):
         assert abs(r.objective - highs) <= 1e-6 * (1 + abs(highs))
 
 
+def test_halpern_answer_polished_to_the_verifier(gpu):
+    # A Halpern answer rejected by the verifier is polished by FP64 PDHG steps with averages (the
+    # reference's iteration) at a tenth of the tolerance: the 2,000 x 5,000 LP then verifies, with
+    # HiGHS's objective to 1e-6.
+    highs = load_golden("c1_reference")["highs_objective"]
+    r = smy.solve_lp(smy.generators.c1(), smy.PdlpOptions(tol=1e-6, halpern=True, precision="mixed",
+                                                          polish_rounds=3))
+    assert r.status == smy.OPTIMAL and r.verified, r.message
+    assert abs(r.objective - highs) <= 1e-6 * (1 + abs(highs))
+
+
 def test_halpern_batch_equals_single_solves(gpu):
     G = smy.generators
     mods = [G.sparse_uniform(121, 60, 150, 4), G.sparse_powerlaw(122, 300, 700, 40, 5.0),
