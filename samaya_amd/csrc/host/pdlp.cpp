@@ -395,6 +395,8 @@ class Lane {
   double fp32_best = kInf;       // best distance to termination in FP32 (by 20% steps)
   long long fp32_best_iter = 0;  // ... and where it was reached
   int fp32_below = 0;            // consecutive evaluations below fp32_stall_level
+  double fp32_recent[4] = {kInf, kInf, kInf, kInf};  // Halpern steps: the last 4 distances
+  int fp32_seen = 0;
   bool fp32_armed = false;       // the stall test counts (fp32_below reached kStallArm once)
   bool active = true;
   Measure best{kInf, kInf, kInf, kInf, 0.0, 0.0};
@@ -1069,7 +1071,15 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
       // through zero, seen at iteration 447 of the 1M-column LP in count order).
       if (L.fp32) {
         constexpr int kStallArm = 4;
-        const double w = std::min(worst(current), worst(average));
+        double w = std::min(worst(current), worst(average));
+        if (halpern) {
+          // T(z)'s distance at FP32's rounding floor scatters over an order of magnitude from one
+          // evaluation to the next (measured on a 3-row LP: 5e-6 to 1e-4), which would keep the
+          // stall rule from arming: the rule sees the least of the last 4 evaluations.
+          L.fp32_recent[L.fp32_seen++ % 4] = w;
+          w = std::min(std::min(L.fp32_recent[0], L.fp32_recent[1]),
+                       std::min(L.fp32_recent[2], L.fp32_recent[3]));
+        }
         const double level = opt.fp32_stall_level;
         L.fp32_below = w < level ? L.fp32_below + 1 : 0;
         const bool arm = !L.fp32_armed && L.fp32_below >= kStallArm;
