@@ -435,6 +435,7 @@ void set_scalars(smy_pdhg* h, const double* tau, const double* sigma, double the
 
 int step_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
               const long long* k0) {
+  h->fixed_fresh = false;
   refresh_lists(h);
   const int rc = refresh_vbar(h, theta);
   if (rc != SMY_OK) return rc;
@@ -451,6 +452,7 @@ int step_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
 int run_impl(smy_pdhg* h, const double* tau, const double* sigma, double theta,
              const long long* k0, int count) {
   if (count <= 0) return SMY_OK;
+  h->fixed_fresh = false;
   refresh_lists(h);
   tune(h);
   const int rc = refresh_vbar(h, theta);
@@ -837,6 +839,9 @@ void measure_both(smy_pdhg* h) {
 
 template <typename T>
 void measure_one(smy_pdhg* h, int which);
+void stage_flags(smy_pdhg* h, const int* value);
+template <typename T>
+void residual_launch(smy_pdhg* h, int row0 = 0);
 
 int evaluate_batch_impl(smy_pdhg* h, double* out) {
   for (const Layout& L : h->inst)
@@ -845,14 +850,25 @@ int evaluate_batch_impl(smy_pdhg* h, double* out) {
   refresh_lists(h);
   if (h->halpern) {
     // Halpern steps: only T(z) (the averages' slot) is an answer (the iterate z' itself may lie
-    // outside the bounds); its sums fill both halves.
+    // outside the bounds); its sums fill both halves. The fixed-point residual of the run's last
+    // step is summed in the same pass (partial rows 7-8) and kept for smy_pdhg_fixed_point_batch,
+    // so the restart test costs no second synchronization.
     if (uses(h, SMY_FP32)) measure_one<float>(h, SMY_AVERAGE);
     if (uses(h, SMY_FP64)) measure_one<double>(h, SMY_AVERAGE);
+    std::vector<int> all(static_cast<size_t>(h->K), 0);
+    stage_flags(h, all.data());
+    if (uses(h, SMY_FP32)) residual_launch<float>(h, 7);
+    if (uses(h, SMY_FP64)) residual_launch<double>(h, 7);
     SMY_HIP(hipGetLastError());
-    std::vector<double> seven(7 * static_cast<size_t>(h->K));
-    finish(h, 7, seven.data());
-    for (int i = 0; i < h->K; ++i)
-      for (int q = 0; q < 7; ++q) out[14 * i + q] = out[14 * i + 7 + q] = seven[7 * i + q];
+    std::vector<double> nine(9 * static_cast<size_t>(h->K));
+    finish(h, 9, nine.data());
+    h->fixed_cache.resize(2 * static_cast<size_t>(h->K));
+    for (int i = 0; i < h->K; ++i) {
+      for (int q = 0; q < 7; ++q) out[14 * i + q] = out[14 * i + 7 + q] = nine[9 * i + q];
+      h->fixed_cache[2 * i] = nine[9 * i + 7];
+      h->fixed_cache[2 * i + 1] = nine[9 * i + 8];
+    }
+    h->fixed_fresh = true;
     return SMY_OK;
   }
   if (uses(h, SMY_FP32)) measure_both<float>(h);
@@ -906,8 +922,10 @@ void distance_launch(smy_pdhg* h) {
                      h->stream, block_rows(h), op);
 }
 
+// Partial rows `row0` and row0 + 1 of the block partials (the evaluation passes 7: its own sums
+// take rows 0-6).
 template <typename T>
-void residual_launch(smy_pdhg* h) {
+void residual_launch(smy_pdhg* h, int row0) {
   constexpr int q = prec_of<T>();
   Iter<T>& S = state<T>(h);
   ResidualOp<T> op;
@@ -916,7 +934,7 @@ void residual_launch(smy_pdhg* h) {
   op.y = S.y[h->p];
   op.v_plus = S.v_erg;
   op.y_plus = S.y_erg;
-  op.partial = h->partial;
+  op.partial = h->partial + static_cast<size_t>(row0) * stride(h);
   op.stride = stride(h);
   hipLaunchKernelGGL((block_rows_kernel<ResidualOp<T>>), dim3(block_rows_grid(h)), dim3(kBlock), 0,
                      h->stream, block_rows(h), op);
@@ -924,6 +942,10 @@ void residual_launch(smy_pdhg* h) {
 
 // After a run of Halpern steps: per instance ||v+ - v||^2, ||y+ - y||^2 of its last step.
 int fixed_point_batch_impl(smy_pdhg* h, double* out) {
+  if (h->fixed_fresh) {  // formed with the evaluation since the last run
+    std::copy(h->fixed_cache.begin(), h->fixed_cache.end(), out);
+    return SMY_OK;
+  }
   if (block_rows_grid(h) == 0) {
     std::fill(out, out + 2 * h->K, 0.0);
     return SMY_OK;
@@ -972,6 +994,7 @@ void restart_launch(smy_pdhg* h) {
 }
 
 int restart_to_batch_impl(smy_pdhg* h, const int* which) {
+  h->fixed_fresh = false;
   if (block_rows_grid(h) == 0) return SMY_OK;
   stage_flags(h, which);
   if (any_flag(h, which, SMY_FP32)) restart_launch<float>(h);
@@ -1078,6 +1101,7 @@ void convert_launch(smy_pdhg* h) {
 // New step precisions (SMY_FP64 / SMY_FP32; -1 keeps the instance as it is); the state of each
 // instance that changes is converted.
 int set_precision_batch_impl(smy_pdhg* h, const int* precision) {
+  h->fixed_fresh = false;
   std::vector<int> flag(h->K, -1);
   bool any = false, to32 = false;
   for (int i = 0; i < h->K; ++i) {
@@ -1145,6 +1169,7 @@ int download_impl(smy_pdhg* h, int i, double* v, double* v_new, double* y, doubl
 
 template <typename T>
 void set_start_impl(smy_pdhg* h, int i, const double* v, const double* y) {
+  h->fixed_fresh = false;
   Iter<T>& S = state<T>(h);
   const Layout& L = h->inst[i];
   const auto up = [&](const double* src, T* dst, long long count) {

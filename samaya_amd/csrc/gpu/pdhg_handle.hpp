@@ -219,6 +219,9 @@ struct smy_pdhg {
   // only), with reflection rho.
   bool halpern = false;
   double rho = 1.0;
+  // The fixed-point residual sums formed with the last evaluation (valid until the next step).
+  std::vector<double> fixed_cache;
+  bool fixed_fresh = false;
   int list_gen = 0;
   bool lists_dirty = false;  // an instance changed precision or froze since the lists were built
   int* step_prec = nullptr;  // per instance: precision of its steps, or kFrozen (device copy)
