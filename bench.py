@@ -25,8 +25,9 @@ Batched workloads (--workload):
       each; value = seconds until the last LP stopped;
       compared with plain FP64 PDLP solving them one at a time (a timed sample of 16, scaled)
   c5  1,024 node relaxations of one knapsack LP (m 20,000, n 50,000, nnz 400k; 1-5% of the
-      columns fixed at 0 or 1 per node) solved as one shared-matrix batch (SpMM steps), warm-started
-      from the root LP's answer and primal weight; infeasible nodes detected on the device;
+      columns fixed at 0 or 1 per node) solved as one shared-matrix batch (SpMM Halpern steps),
+      warm-started from the root LP's answer and primal weight; infeasible nodes detected on the
+      device (with the sign-projected Farkas test);
       compared with plain FP64 PDLP solving the nodes one at a time (a stratified sample of 32
       feasible and 16 infeasible nodes, scaled)
 
@@ -74,13 +75,21 @@ def step_kernel_bytes(nnz: int, m: int, n: int, s: int, halpern: bool = False) -
     return {"dual": dual, "primal": primal}
 
 
-def shared_step_bytes(nnz: int, m: int, n: int, k: int, s: int, coded: bool = False) -> dict:
+def shared_step_bytes(nnz: int, m: int, n: int, k: int, s: int, coded: bool = False,
+                      halpern: bool = False) -> dict:
     """Algorithmic HBM bytes of the shared-matrix step kernels for k instances: the matrix once,
     the gathered iterate block once (k values per row), and the per-row update streams of all
     instances (rows of A: 5 read, 5 written; rows of A': v, average and both bounds read, 3 written;
-    with coded bounds one byte per (column, instance) and 4 pairs per column replace the bounds)."""
-    dual = nnz * (s + 4) + 4 * (m + 1) + s * k * n + 10 * s * k * m
+    with coded bounds one byte per (column, instance) and 4 pairs per column replace the bounds).
+    Halpern steps: rows of A read y, its anchor, the logical v and its anchor and write y', v'
+    (4 + 2); rows of A' read v, its anchor and the bounds and write w, v' (2 + 2); T(z), written
+    at a run's last step only, is left out."""
     bounds = k * n + 8 * s * n if coded else 2 * s * k * n
+    if halpern:
+        dual = nnz * (s + 4) + 4 * (m + 1) + s * k * n + 6 * s * k * m
+        primal = nnz * (s + 4) + 4 * (n + 1) + s * k * m + 4 * s * k * n + bounds
+        return {"dual": dual, "primal": primal}
+    dual = nnz * (s + 4) + 4 * (m + 1) + s * k * n + 10 * s * k * m
     primal = nnz * (s + 4) + 4 * (n + 1) + s * k * m + 5 * s * k * n + bounds
     return {"dual": dual, "primal": primal}
 
@@ -315,6 +324,8 @@ def main() -> int:
                          "the single LP and the stacked batch; the shared-matrix batch has none)")
     ap.add_argument("--c4-method", default="halpern", choices=["halpern", "adaptive", "constant"],
                     help="the blending batch's iteration (with the enhanced preconditioner)")
+    ap.add_argument("--c5-method", default="halpern", choices=["halpern", "constant"],
+                    help="the node-relaxation batch's iteration (shared-matrix batch)")
     ap.add_argument("--reflection", type=float, default=1.0, help="Halpern steps: the reflection")
     ap.add_argument("--column-order", default=None, choices=["auto", "natural", "degree"],
                     help="the single LP's column order (default: by count for Halpern steps; auto "
@@ -579,12 +590,13 @@ def batch_roofline(dev, nnz: int, m: int, n: int, k: int | None, adaptive: bool 
     else:
         dual_ms, primal_ms = dev.time_kernels(20)
         ops = ("DualRowsOp", "PrimalColsOp")
-    b = (shared_step_bytes(nnz, m, n, k, 4, coded) if k else
+    b = (shared_step_bytes(nnz, m, n, k, 4, coded, halpern is not None) if k else
          step_kernel_bytes(nnz, m, n, 4, halpern is not None))
     dom = "primal" if primal_ms >= dual_ms else "dual"
     ms = primal_ms if dom == "primal" else dual_ms
     achieved = b[dom] / (ms * 1e-3) / 1e9
-    key = f"shared_{dom}_kernel<float>" if k else f"{ops[1] if dom == 'primal' else ops[0]}<float>"
+    shared_name = f"shared_halpern_{dom}_kernel" if halpern is not None else f"shared_{dom}_kernel"
+    key = f"{shared_name}<float>" if k else f"{ops[1] if dom == 'primal' else ops[0]}<float>"
     kernel = key if k else f"csr_rows_kernel<float, {key}>"
     traffic = profiled_traffic(traffic_of, key) if traffic_of else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
@@ -768,7 +780,8 @@ def c5_result(args, feasible_sample: int = 32, infeasible_sample: int = 16) -> d
         coded = shared_bounds_coded(lows, ups)
         dev = D.PdhgBatch.create_shared(a, at, cost, lo_full, up_full, "fp32")
         out = batch_roofline(dev, md.nnz, md.m, md.n, K, coded=coded,
-                             traffic_of="c5" if args.scale == 1.0 else None)
+                             traffic_of="c5" if args.scale == 1.0 else None,
+                             halpern=args.reflection if args.c5_method == "halpern" else None)
         out["bounds_coded"] = coded
         del dev
         return out
@@ -778,7 +791,8 @@ def c5_result(args, feasible_sample: int = 32, infeasible_sample: int = 16) -> d
                 "config": {"workload": f"{K} node relaxations, shared-matrix FP32 step kernels"}}
     opts = smy.PdlpOptions(tol=args.tol, precision="mixed", algorithm=args.algorithm, verify=True,
                            fp32_switch_tol=args.fp32_switch_tol_c5, log_level=1 if args.verbose else 0,
-                           adaptive_steps=False, fp64_accumulation=args.fp64_accumulation)
+                           adaptive_steps=False, fp64_accumulation=args.fp64_accumulation,
+                           halpern=args.c5_method == "halpern", reflection=args.reflection)
     root = smy.solve_lp(nm, opts)
     log(f"c5 root: {root.status_name} {root.iterations} it, omega {root.omega:.3e}")
     start = (root.col_value, root.row_dual)
@@ -851,6 +865,7 @@ def c5_result(args, feasible_sample: int = 32, infeasible_sample: int = 16) -> d
         "iterations_median": int(np.median(its)), "iterations_max": int(its.max()),
         "root_iterations": root.iterations, "setup_s": round(times.setup_seconds, 3),
         "fp32_switch_tol": args.fp32_switch_tol_c5, "adaptive_steps": False,
+        "method": args.c5_method,
         "fp32_iterations_sum": int(sum(r.fp32_iterations for r in res)),
         "verified": int(sum(r.verified for r in res)), "roofline": rl, "cpu_baseline": cpu,
         "config": {"workload": f"{K} node relaxations (1-5% of columns fixed) of a knapsack LP "

@@ -186,7 +186,11 @@ int smy_pdhg_freeze(smy_pdhg* h, int instance);
  * 4 and 5 the counts of their infinite terms; 6 scale (max |t bound|, |t| > 1e-9, t = +-w/y_max);
  * 7 d_max = max |v_new - v|; 8 ||A x - r||^2 of the normalized difference; 9 its cost slope; 10 the
  * number of bound failures of the projected ray and of A times it. The host decides as the
- * reference does from these. Synchronizes. */
+ * reference does from these. flags[i] = 2 or 3 (no reference counterpart): the Farkas sums of the
+ * dual's sign projection instead, its entries that make a row's term of f_min (2) or f_max (3)
+ * infinite set to 0 (a y that passes is a Farkas certificate all the same). The ray test is
+ * skipped, its slots 7-10 left 0, when no instance with flag 1 has a structural variable with an
+ * infinite bound (a bounded LP has no improving ray). Flags above 3 are invalid. Synchronizes. */
 int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out);
 /* The projected ray (n_i structural entries) of instance i's last detection. */
 int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray);
@@ -226,9 +230,10 @@ int smy_pdhg_time_adaptive_kernels(smy_pdhg* h, int reps, double* dual_ms, doubl
  * lambda = (k + 1) / (k + 2), k = iter0 + the step's index (steps since the restart); theta is
  * ignored. After a run, v_new / y_new hold the last z', v / y the z it was taken from, and the
  * averages (v_erg / y_erg, SMY_AVERAGE) hold T(z) of that last step: smy_pdhg_evaluate_batch
- * measures z' (current) and T(z) (average), smy_pdhg_restart_distance_batch with SMY_AVERAGE
- * gives ||T(z) - z0||^2, and smy_pdhg_restart_to_batch with SMY_AVERAGE restarts at T(z), which
- * becomes the anchor. reflection in [0, 1]. Not available for the shared-matrix batch. */
+ * measures T(z) into both halves (z' may lie outside the bounds and is no answer),
+ * smy_pdhg_restart_distance_batch with SMY_AVERAGE gives ||T(z) - z0||^2, and
+ * smy_pdhg_restart_to_batch with SMY_AVERAGE restarts at T(z), which becomes the anchor.
+ * reflection in [0, 1]. Both batch layouts (stacked and shared-matrix). */
 int smy_pdhg_set_halpern(smy_pdhg* h, int on, double reflection);
 /* After a run of Halpern steps, per instance: out[2i] = ||v+ - v||^2, out[2i + 1] = ||y+ - y||^2
  * of the run's last step (the fixed-point residual z - T(z) of the restart test). Synchronizes. */

@@ -128,12 +128,16 @@ typedef struct smy_pdlp_params {
                                   GPU-failure path (src/lp/pdlp.cpp:231-242); -1: off */
   int halpern;               /* restarted Halpern PDHG with reflection (smy_pdhg_set_halpern):
                                 constant steps, no averages, restarts on the fixed-point residual
-                                to T(z) (0: the reference's restarted PDHG); batches then use the
-                                stacked layout */
+                                to T(z) (0: the reference's restarted PDHG) */
   double reflection;         /* Halpern steps: the reflection in [0, 1] (1) */
   int pid_primal_weight;     /* the primal weight at a restart follows log(dy / dx) through a PI
                                 controller (gains 0.99, 0.01) instead of the reference's
                                 omega <- sqrt(omega dy / dx) (0); decided on the host */
+  int farkas_projection;     /* when the reference's Farkas test fails, test the dual with its
+                                entries of the wrong sign for a row's infinite bound set to 0,
+                                for f_min and for f_max (smy_pdhg_detect_batch flags 2, 3):
+                                1 on, 0 off (the reference's test only), -1 (default) on unless
+                                algorithm is the reference's */
 } smy_pdlp_params;
 
 void smy_pdlp_params_default(smy_pdlp_params* params);

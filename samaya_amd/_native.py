@@ -63,7 +63,8 @@ class SmyPdlpParams(C.Structure):
                 ("device_restart", C.c_int), ("fp64_accumulation", C.c_int),
                 ("device_scaling", C.c_int), ("column_order", C.c_int),
                 ("inject_fault_step", C.c_longlong), ("halpern", C.c_int),
-                ("reflection", C.c_double), ("pid_primal_weight", C.c_int)]
+                ("reflection", C.c_double), ("pid_primal_weight", C.c_int),
+                ("farkas_projection", C.c_int)]
 
 
 class SmyPdlpInfo(C.Structure):

@@ -67,6 +67,31 @@ struct DetectView {
   __device__ bool skip(int inst) const { return flags[inst] <= 0; }
 };
 
+// Before phase 1 when some flag is 2 or 3: the dual the Farkas test takes, into a copy (ydst, the
+// layout of y_erg) that the phases then read for every flagged instance: y_erg itself (flag 1), or
+// its sign projection for the f_min (2) or f_max (3) test: the entry of a row whose term in that
+// sum would be infinite (f_min: -yf_i > 0 against an infinite lower bound of the row's logical
+// variable, or -yf_i < 0 against an infinite upper bound; f_max: the opposite signs) set to 0. The
+// reference tests y_erg only (pdlp.cpp:724-842); every y that passes the test is a certificate.
+template <typename T, bool Shared>
+struct FarkasProjectOp : DetectView<T, Shared> {
+  static constexpr int kSums = 0;
+  static constexpr int kMaxFrom = 0;
+  T* __restrict__ ydst;
+  __device__ int flag(int inst) const { return this->flags[inst] > 0 ? this->flags[inst] : -1; }
+  __device__ void row_a(int i, int k, int f, double*, int inst) const {
+    const size_t o = this->at(i, inst);
+    const T y = this->y_erg[o];
+    bool cut = false;
+    if (f >= 2) {
+      const double s = f == 2 ? static_cast<double>(y) : -static_cast<double>(y);  // as for f_min
+      cut = (s < 0.0 && !(this->lo(k, inst) > -INFINITY)) || (s > 0.0 && !(this->up(k, inst) < INFINITY));
+    }
+    ydst[o] = cut ? T(0) : y;
+  }
+  __device__ void col(int, int, double*, int) const {}
+};
+
 // Phase 1, rows of A': w_j = (A' yf)_j = (A_eq' y_erg)_j / cs_j, kept in scratch; slot 0 += |w_j|.
 template <typename T, bool Shared>
 struct FarkasColsOp : DetectView<T, Shared> {

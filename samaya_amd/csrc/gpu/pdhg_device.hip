@@ -1480,6 +1480,7 @@ int set_original_impl(smy_pdhg* h, int i, const double* col_scale, const double*
     put_instance_vec(h, h->lower_o, i, lower);
     put_instance_vec(h, h->upper_o, i, upper);
     h->inst[i].has_original = true;
+    h->inst[i].boxed = all_boxed(L.n, lower, upper);
     return SMY_OK;
   });
 }
@@ -1499,6 +1500,13 @@ void ensure_detect(smy_pdhg* h) {
                         hipHostMallocDefault));
 }
 
+template <typename T>
+T*& det_y(smy_pdhg* h);
+template <>
+double*& det_y<double>(smy_pdhg* h) { return h->det_y64; }
+template <>
+float*& det_y<float>(smy_pdhg* h) { return h->det_y32; }
+
 template <typename T, template <typename, bool> class Op>
 Op<T, false> stacked_detect_op(smy_pdhg* h, int slot) {
   Iter<T>& S = state<T>(h);
@@ -1507,7 +1515,7 @@ Op<T, false> stacked_detect_op(smy_pdhg* h, int slot) {
   op.kp = 1;
   op.v = S.v[h->p];
   op.v_new = S.v[1 - h->p];
-  op.y_erg = S.y_erg;
+  op.y_erg = h->det_projected ? det_y<T>(h) : S.y_erg;
   op.rs = h->rs;
   op.cs = h->cs;
   op.cost = h->cost_o;
@@ -1551,18 +1559,29 @@ void detect_rows(smy_pdhg* h, const Op& op, int first, int count) {
 template <typename T>
 void stacked_detect_phase(smy_pdhg* h, int phase) {
   const int nba = h->a.nblocks, nbt = h->at.nblocks;
+  if (phase == 1 && h->det_projected) {
+    // The tested duals (y_erg or its sign projection) into det_y, which the Farkas sums read.
+    if (det_y<T>(h) == nullptr) det_y<T>(h) = h->alloc<T>(std::max(1, h->m));
+    h->det_projected = false;  // the copy reads y_erg
+    auto project = stacked_detect_op<T, FarkasProjectOp>(h, 0);
+    h->det_projected = true;
+    project.ydst = det_y<T>(h);
+    detect_rows(h, project, 0, nba);
+  }
   if (phase == 1) {
     // Block partials of A' rows follow those of A (the instance ranges' layout).
     auto cols = stacked_detect_op<T, FarkasColsOp>(h, 0);
     cols.partial += nba;
     detect_csr(h, h->at, cols);
     detect_rows(h, stacked_detect_op<T, FarkasRowsOp>(h, 0), 0, nba);
-    detect_rows(h, stacked_detect_op<T, RayMaxOp>(h, 7), 0, nba + nbt);
+    if (h->det_ray_test) detect_rows(h, stacked_detect_op<T, RayMaxOp>(h, 7), 0, nba + nbt);
   } else {
     detect_rows(h, stacked_detect_op<T, FarkasBoundsOp>(h, 2), 0, nba + nbt);
-    detect_csr(h, h->a, stacked_detect_op<T, RayRowsOp>(h, 8));
-    detect_rows(h, stacked_detect_op<T, RayColsOp>(h, 9), nba, nbt);
-    detect_csr(h, h->a, stacked_detect_op<T, RayCheckOp>(h, 10));
+    if (h->det_ray_test) {
+      detect_csr(h, h->a, stacked_detect_op<T, RayRowsOp>(h, 8));
+      detect_rows(h, stacked_detect_op<T, RayColsOp>(h, 9), nba, nbt);
+      detect_csr(h, h->a, stacked_detect_op<T, RayCheckOp>(h, 10));
+    }
   }
 }
 
@@ -1576,6 +1595,9 @@ int stacked_detect(smy_pdhg* h, const int* flags, double* out) {
   refresh_lists(h);
   ensure_detect(h);
   stage_flags(h, flags);
+  h->det_projected = std::any_of(flags, flags + h->K, [](int f) { return f >= 2; });
+  h->det_ray_test = false;
+  for (int i = 0; i < h->K; ++i) h->det_ray_test = h->det_ray_test || (flags[i] == 1 && !h->inst[i].boxed);
   const bool f32 = any_flag(h, flags, SMY_FP32), f64 = any_flag(h, flags, SMY_FP64);
   const size_t st = static_cast<size_t>(stride(h));
   const auto finish_det = [&]() {
@@ -1611,6 +1633,7 @@ int stacked_detect(smy_pdhg* h, const int* flags, double* out) {
     if (f64) stacked_detect_phase<double>(h, 2);
   }
   finish_det();
+  h->det_projected = false;
   for (int i = 0; i < h->K; ++i)
     for (int q : {2, 3, 4, 5, 6, 8, 9, 10})
       out[static_cast<size_t>(i) * kDetectSlots + q] = h->det_out_host[static_cast<size_t>(i) * kDetectSlots + q];
@@ -1769,7 +1792,6 @@ int smy_pdhg_run_batch(smy_pdhg* h, const double* tau, const double* sigma, doub
 
 int smy_pdhg_set_halpern(smy_pdhg* h, int on, double reflection) {
   if (h == nullptr) return fail(SMY_INVALID, "null handle");
-  if (h->shared) return fail(SMY_INVALID, "Halpern steps need the stacked batch layout");
   if (!(reflection >= 0.0 && reflection <= 1.0)) return fail(SMY_INVALID, "reflection outside [0, 1]");
   if (h->stepped) return fail(SMY_STATE, "the step kind changed between two steps without advance()");
   if (h->rho != reflection) {
@@ -1783,7 +1805,8 @@ int smy_pdhg_set_halpern(smy_pdhg* h, int on, double reflection) {
 
 int smy_pdhg_fixed_point_batch(smy_pdhg* h, double* out) {
   if (h == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
-  if (h->shared || !h->halpern) return fail(SMY_STATE, "no Halpern steps on this handle");
+  if (!h->halpern) return fail(SMY_STATE, "no Halpern steps on this handle");
+  if (h->shared) return guarded([&]() -> int { return shared_fixed_point(h, out); });
   return guarded([&]() -> int {
     refresh_lists(h);
     return fixed_point_batch_impl(h, out);
@@ -2056,6 +2079,8 @@ int smy_pdhg_set_start_at(smy_pdhg* h, int instance, const double* v, const doub
 
 int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out) {
   if (h == nullptr || flags == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
+  for (int i = 0; i < h->K; ++i)
+    if (flags[i] > 3) return fail(SMY_INVALID, "detection flags are 0, 1, 2 or 3");
   for (int i = 0; i < h->K; ++i)
     if (flags[i] > 0 && (h->inst[i].prec == kFrozen || !h->inst[i].has_original))
       return fail(SMY_STATE, "detection needs a running instance with set_original()");

@@ -121,7 +121,17 @@ struct Layout {
   int prec = SMY_FP64;   // precision of its steps, or kFrozen
   int last_prec = SMY_FP64;  // precision of its state (also when frozen)
   bool has_original = false;
+  // Every structural variable has two finite model-space bounds: the LP is bounded, so it has no
+  // improving ray, and its detection skips the ray test (whose failure is then certain).
+  bool boxed = false;
 };
+
+// Whether the first n entries of both bound vectors are finite.
+inline bool all_boxed(int n, const double* lower, const double* upper) {
+  for (int j = 0; j < n; ++j)
+    if (!(lower[j] > -INFINITY && upper[j] < INFINITY)) return false;
+  return true;
+}
 
 template <typename T>
 constexpr int prec_of() {
@@ -255,6 +265,14 @@ struct smy_pdhg {
   double* det_out = nullptr;       // per instance: the detection slots
   double* det_out_host = nullptr;  // pinned
   int2* det_ranges = nullptr;      // per instance: its blocks in the detection partials
+  // The Farkas test's dual when some instance tests a sign projection (flags 2, 3): a copy in
+  // the layout of y_erg per precision (at first use), and whether the running detection reads it.
+  double* det_y64 = nullptr;
+  float* det_y32 = nullptr;
+  bool det_projected = false;
+  // Whether the running detection takes the ray test: some instance with flag 1 is not boxed (the
+  // ray slots of the others stay 0, so their test fails as it would).
+  bool det_ray_test = true;
   std::vector<void*> allocations;
 
   template <typename T>
@@ -348,6 +366,7 @@ int shared_set_original(smy_pdhg* h, int k, const double* cs, const double* rs, 
 int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta,
                const long long* k0, int count);
 int shared_evaluate(smy_pdhg* h, double* out);
+int shared_fixed_point(smy_pdhg* h, double* out);
 int shared_distance(smy_pdhg* h, const int* which, double* out);
 int shared_restart_to(smy_pdhg* h, const int* which);
 int shared_set_precision(smy_pdhg* h, const int* precision);

@@ -47,8 +47,10 @@ struct CompactState {
   T* v[2] = {nullptr, nullptr};
   T* vbar[2] = {nullptr, nullptr};
   T* v_erg = nullptr;
+  T* v_rst = nullptr;      // Halpern steps' anchor
   T* y[2] = {nullptr, nullptr};
   T* y_erg = nullptr;
+  T* y_rst = nullptr;
   T* col_lower = nullptr;  // n x kc
   T* col_upper = nullptr;
   T* tau = nullptr;        // kc
@@ -258,6 +260,10 @@ SharedStep<T> step_args(smy_pdhg* h, int j, bool dual) {
   a.codes = s->codes;
   a.code_lower = S.code_lower;
   a.code_upper = S.code_upper;
+  a.v_rst = S.v_rst;
+  a.y_rst = S.y_rst;
+  a.rho = static_cast<T>(h->rho);
+  a.keep = false;
   return a;
 }
 
@@ -281,6 +287,8 @@ void ensure_compact(smy_pdhg* h) {
   }
   C.v_erg = h->alloc<T>(s->nt * w);
   C.y_erg = h->alloc<T>(s->m * w);
+  C.v_rst = h->alloc<T>(s->nt * w);
+  C.y_rst = h->alloc<T>(s->m * w);
   C.col_lower = h->alloc<T>(s->n * w);
   C.col_upper = h->alloc<T>(s->n * w);
   C.tau = h->alloc<T>(w);
@@ -309,11 +317,14 @@ SharedStep<T> compact_args(smy_pdhg* h, int j, bool dual) {
   a.y_erg = C.y_erg;
   a.col_lower = C.col_lower;
   a.col_upper = C.col_upper;
+  a.v_rst = C.v_rst;
+  a.y_rst = C.y_rst;
   return a;
 }
 
 // Copies the stepping columns of precision T between the big layout and the compact block
-// (in: before a run, with the column bounds; out: after it, the iterates only).
+// (in: before a run, with the column bounds and, for Halpern steps, the anchor; out: after it,
+// the iterates only).
 template <typename T>
 void compact_copy(smy_pdhg* h, bool in) {
   SharedBatch* s = h->shared;
@@ -343,6 +354,10 @@ void compact_copy(smy_pdhg* h, bool in) {
   if (in) {
     add(S.col_lower, C.col_lower, s->n);
     add(S.col_upper, C.col_upper, s->n);
+    if (h->halpern) {
+      add(S.v_rst, C.v_rst, s->nt);
+      add(S.y_rst, C.y_rst, s->m);
+    }
   }
   c.count = a;
   const long long threads = static_cast<long long>(s->nt) * c.kc;
@@ -351,11 +366,21 @@ void compact_copy(smy_pdhg* h, bool in) {
                        s->nt);
   if (in)
     hipLaunchKernelGGL((compact_table_kernel<T>), dim3(1), dim3(kCompactInstances), 0, h->stream, c.kc,
-                       c.na, c.list, s->scalars, C.tau, C.sigma, C.alpha);
+                       c.na, c.list, s->scalars, C.tau, C.sigma, C.alpha, h->halpern);
+}
+
+// Halpern steps' arguments: the primal kernel writes w = 2 v+ - v into vbar[0], the dual kernel
+// gathers it; T(z) into the averages' slot at the run's last step.
+template <typename T>
+SharedStep<T> halpern_args(SharedStep<T> a, bool keep, T* w) {
+  a.vbar = w;
+  a.vbar_next = w;
+  a.keep = keep;
+  return a;
 }
 
 template <typename T>
-void shared_step_kernels(smy_pdhg* h, int j) {
+void shared_step_kernels(smy_pdhg* h, int j, bool keep) {
   SharedBatch* s = h->shared;
   const int q = std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32;
   if (compact_steps(s, q)) {
@@ -365,12 +390,32 @@ void shared_step_kernels(smy_pdhg* h, int j) {
     const int lanes = compact_width(s, q) / kInstPerThread;
     const int threads = std::min(kBlock, std::max(64, 16 * lanes));
     const int rows_per_wg = threads / lanes;
+    if (h->halpern) {
+      T* w = cstate<T>(s).vbar[0];
+      if (s->n > 0)
+        hipLaunchKernelGGL((compact_halpern_primal_kernel<T>), dim3((s->n + rows_per_wg - 1) / rows_per_wg),
+                           dim3(threads), 0, h->stream, halpern_args(compact_args<T>(h, j, false), keep, w), na);
+      if (s->m > 0)
+        hipLaunchKernelGGL((compact_halpern_dual_kernel<T>), dim3((s->m + rows_per_wg - 1) / rows_per_wg),
+                           dim3(threads), 0, h->stream, halpern_args(compact_args<T>(h, j, true), keep, w), na);
+      return;
+    }
     if (s->m > 0)
       hipLaunchKernelGGL((compact_dual_kernel<T>), dim3((s->m + rows_per_wg - 1) / rows_per_wg), dim3(threads),
                          0, h->stream, compact_args<T>(h, j, true), na);
     if (s->n > 0)
       hipLaunchKernelGGL((compact_primal_kernel<T>), dim3((s->n + rows_per_wg - 1) / rows_per_wg), dim3(threads),
                          0, h->stream, compact_args<T>(h, j, false), na);
+    return;
+  }
+  if (h->halpern) {
+    T* w = sstate<T>(s).vbar[0];
+    if (s->n > 0)
+      hipLaunchKernelGGL((shared_halpern_primal_kernel<T>), dim3(s->n, chunks(s)), dim3(kBlock), 0, h->stream,
+                         halpern_args(step_args<T>(h, j, false), keep, w));
+    if (s->m > 0)
+      hipLaunchKernelGGL((shared_halpern_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
+                         halpern_args(step_args<T>(h, j, true), keep, w));
     return;
   }
   if (s->m > 0)
@@ -389,9 +434,10 @@ int step_signature(const SharedBatch* s) {
   return a * 1024 + b;
 }
 
-void shared_step_all(smy_pdhg* h, int j) {
-  if (s_uses(h->shared, SMY_FP32)) shared_step_kernels<float>(h, j);
-  if (s_uses(h->shared, SMY_FP64)) shared_step_kernels<double>(h, j);
+// Step j of a run (Halpern steps write T(z) at the run's last step: keep).
+void shared_step_all(smy_pdhg* h, int j, bool keep) {
+  if (s_uses(h->shared, SMY_FP32)) shared_step_kernels<float>(h, j, keep);
+  if (s_uses(h->shared, SMY_FP64)) shared_step_kernels<double>(h, j, keep);
 }
 
 template <typename T>
@@ -399,7 +445,7 @@ void step_table(smy_pdhg* h) {
   SharedBatch* s = h->shared;
   SharedIter<T>& S = sstate<T>(s);
   hipLaunchKernelGGL((step_table_kernel<T>), dim3(blocks_for(s->kp)), dim3(kBlock), 0, h->stream,
-                     s->kp, s->scalars, S.tau, S.sigma, S.alpha);
+                     s->kp, s->scalars, S.tau, S.sigma, S.alpha, h->halpern);
 }
 
 // Per-instance flags by precision: which[q * kp + k] = value[k] for instances whose state is in
@@ -675,8 +721,15 @@ void set_start_part(smy_pdhg* h, int k, const double* v, const double* y) {
 template <typename T>
 int time_part(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms) {
   SharedBatch* s = h->shared;
+  T* w = sstate<T>(s).vbar[0];
   const auto go = [&](bool dual) {
-    if (dual)
+    if (h->halpern && dual)
+      hipLaunchKernelGGL((shared_halpern_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
+                         halpern_args(step_args<T>(h, 0, true), false, w));
+    else if (h->halpern)
+      hipLaunchKernelGGL((shared_halpern_primal_kernel<T>), dim3(s->n, chunks(s)), dim3(kBlock), 0,
+                         h->stream, halpern_args(step_args<T>(h, 0, false), false, w));
+    else if (dual)
       hipLaunchKernelGGL((shared_dual_kernel<T>), dim3(s->m, chunks(s)), dim3(kBlock), 0, h->stream,
                          step_args<T>(h, 0, true));
     else
@@ -716,6 +769,13 @@ void ensure_detect_shared(smy_pdhg* h) {
                         hipHostMallocDefault));
 }
 
+template <typename T>
+T*& det_y_shared(smy_pdhg* h);
+template <>
+double*& det_y_shared<double>(smy_pdhg* h) { return h->det_y64; }
+template <>
+float*& det_y_shared<float>(smy_pdhg* h) { return h->det_y32; }
+
 template <typename T, template <typename, bool> class Op>
 Op<T, true> shared_detect_op(smy_pdhg* h, int slot) {
   SharedBatch* s = h->shared;
@@ -725,7 +785,7 @@ Op<T, true> shared_detect_op(smy_pdhg* h, int slot) {
   op.kp = s->kp;
   op.v = S.v[h->p];
   op.v_new = S.v[1 - h->p];
-  op.y_erg = S.y_erg;
+  op.y_erg = h->det_projected ? det_y_shared<T>(h) : S.y_erg;
   op.rs = s->rs;
   op.cs = s->cs;
   op.cost = s->cost_o;
@@ -774,18 +834,30 @@ template <typename T>
 void shared_detect_phase(smy_pdhg* h, int phase) {
   SharedBatch* s = h->shared;
   const int nba = s->nba, nbt = s->nbt;
+  if (phase == 1 && h->det_projected) {
+    // The tested duals (y_erg or its sign projection) into det_y, which the Farkas sums read.
+    if (det_y_shared<T>(h) == nullptr)
+      det_y_shared<T>(h) = h->alloc<T>(std::max<long long>(1, static_cast<long long>(s->m) * s->kp));
+    h->det_projected = false;  // the copy reads y_erg
+    auto project = shared_detect_op<T, FarkasProjectOp>(h, 0);
+    h->det_projected = true;
+    project.ydst = det_y_shared<T>(h);
+    shared_detect_rows(h, project, 0, nba);
+  }
   if (phase == 1) {
     // Block partials of A' rows follow those of A within each instance's range.
     auto cols = shared_detect_op<T, FarkasColsOp>(h, 0);
     cols.partial += nba;
     shared_detect_csr(h, false, cols);
     shared_detect_rows(h, shared_detect_op<T, FarkasRowsOp>(h, 0), 0, nba);
-    shared_detect_rows(h, shared_detect_op<T, RayMaxOp>(h, 7), 0, nba + nbt);
+    if (h->det_ray_test) shared_detect_rows(h, shared_detect_op<T, RayMaxOp>(h, 7), 0, nba + nbt);
   } else {
     shared_detect_rows(h, shared_detect_op<T, FarkasBoundsOp>(h, 2), 0, nba + nbt);
-    shared_detect_csr(h, true, shared_detect_op<T, RayRowsOp>(h, 8));
-    shared_detect_rows(h, shared_detect_op<T, RayColsOp>(h, 9), nba, nbt);
-    shared_detect_csr(h, true, shared_detect_op<T, RayCheckOp>(h, 10));
+    if (h->det_ray_test) {
+      shared_detect_csr(h, true, shared_detect_op<T, RayRowsOp>(h, 8));
+      shared_detect_rows(h, shared_detect_op<T, RayColsOp>(h, 9), nba, nbt);
+      shared_detect_csr(h, true, shared_detect_op<T, RayCheckOp>(h, 10));
+    }
   }
 }
 
@@ -795,6 +867,9 @@ int shared_detect(smy_pdhg* h, const int* flags, double* out) {
   SharedBatch* s = h->shared;
   ensure_detect_shared(h);
   stage_which(h, flags);
+  h->det_projected = std::any_of(flags, flags + s->K, [](int f) { return f >= 2; });
+  h->det_ray_test = false;
+  for (int k = 0; k < s->K; ++k) h->det_ray_test = h->det_ray_test || (flags[k] == 1 && !h->inst[k].boxed);
   const bool f32 = any_which(s, flags, SMY_FP32), f64 = any_which(s, flags, SMY_FP64);
   const size_t st = det_stride(s);
   const auto zero = [&](int first, int count) {
@@ -834,6 +909,7 @@ int shared_detect(smy_pdhg* h, const int* flags, double* out) {
     if (f64) shared_detect_phase<double>(h, 2);
   }
   finish_det();
+  h->det_projected = false;
   for (int k = 0; k < s->K; ++k)
     for (int q : {2, 3, 4, 5, 6, 8, 9, 10})
       out[static_cast<size_t>(k) * kDetectSlots + q] = h->det_out_host[static_cast<size_t>(k) * kDetectSlots + q];
@@ -1047,6 +1123,7 @@ int shared_set_original(smy_pdhg* h, int k, const double* cs, const double* rs, 
   SMY_HIP(hipGetLastError());
   s->has_original[k] = 1;
   h->inst[k].has_original = true;
+  h->inst[k].boxed = all_boxed(s->n, lower, upper);
   return SMY_OK;
 }
 
@@ -1081,13 +1158,13 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
   const int mask = (u64 ? 1 : 0) | (u32 ? 2 : 0);
   if (count == smy_pdhg::kGraphSteps && mask != 0) {
     const int p0 = h->p;
-    smy_pdhg::Captured& c = h->graphs[mask][p0];
+    smy_pdhg::Captured& c = (h->halpern ? h->hgraphs : h->graphs)[mask][p0];
     const int sig = step_signature(s);
     if (c.exec == nullptr || c.gen != sig) {
       h->drop_graph(c);
       SMY_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       for (int j = 0; j < count; ++j) {
-        shared_step_all(h, j);
+        shared_step_all(h, j, j + 1 == count);
         if (j + 1 < count) h->p ^= 1;
       }
       SMY_HIP(hipStreamEndCapture(h->stream, &c.graph));
@@ -1099,7 +1176,7 @@ int shared_run(smy_pdhg* h, const double* tau, const double* sigma, double theta
     if ((count - 1) & 1) h->p ^= 1;
   } else {
     for (int j = 0; j < count; ++j) {
-      shared_step_all(h, j);
+      shared_step_all(h, j, j + 1 == count);
       if (j + 1 < count) h->p ^= 1;
     }
   }
@@ -1136,6 +1213,50 @@ int shared_evaluate(smy_pdhg* h, double* out) {
   if (c64) compact_measure<double>(h);
   SMY_HIP(hipGetLastError());
   shared_download_sums(h, 14, true, out);
+  if (h->halpern) {
+    // Halpern steps: only T(z) (the averages' slot) is an answer (the iterate z' itself may lie
+    // outside the bounds); its sums fill both halves.
+    for (int k = 0; k < s->K; ++k)
+      for (int q = 0; q < 7; ++q) out[14 * static_cast<size_t>(k) + q] = out[14 * static_cast<size_t>(k) + 7 + q];
+  }
+  return SMY_OK;
+}
+
+// After a run of Halpern steps: per instance ||v+ - v||^2, ||y+ - y||^2 of its last step, T(z)
+// (v_erg, y_erg) against z (slot p), with the restart distance's kernel.
+template <typename T>
+void residual_part(smy_pdhg* h) {
+  SharedBatch* s = h->shared;
+  SharedIter<T>& S = sstate<T>(s);
+  SharedDistance<T> a;
+  a.nt = s->nt;
+  a.m = s->m;
+  a.kp = s->kp;
+  a.which = s->which + (std::is_same_v<T, double> ? SMY_FP64 : SMY_FP32) * s->kp;
+  a.v_new = S.v[h->p];
+  a.v_erg = S.v_erg;
+  a.v_rst = S.v[h->p];
+  a.y_new = S.y[h->p];
+  a.y_erg = S.y_erg;
+  a.y_rst = S.y[h->p];
+  a.partial = s->partial;
+  hipLaunchKernelGGL((shared_distance_kernel<T>), dim3(groups_of(s->nt), chunks(s)), dim3(kBlock), 0,
+                     h->stream, a);
+}
+
+int shared_fixed_point(smy_pdhg* h, double* out) {
+  SharedBatch* s = h->shared;
+  if (s->nt == 0) {
+    std::fill(out, out + 2 * s->K, 0.0);
+    return SMY_OK;
+  }
+  upload_modes(h);
+  const std::vector<int> average(static_cast<size_t>(s->K), 1);  // the distance of v_erg / y_erg
+  stage_which(h, average.data());
+  if (s_uses(s, SMY_FP32)) residual_part<float>(h);
+  if (s_uses(s, SMY_FP64)) residual_part<double>(h);
+  SMY_HIP(hipGetLastError());
+  shared_finish(h, 2, groups_of(s->nt), -1, out);
   return SMY_OK;
 }
 

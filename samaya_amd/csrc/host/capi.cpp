@@ -59,6 +59,8 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.halpern = p->halpern != 0;
   o.reflection = p->reflection;
   o.pid_primal_weight = p->pid_primal_weight != 0;
+  o.farkas_projection = p->farkas_projection < 0 ? o.algorithm != Algorithm::kReference
+                                                 : p->farkas_projection != 0;
   return o;
 }
 
@@ -150,6 +152,7 @@ void smy_pdlp_params_default(smy_pdlp_params* p) {
   p->halpern = 0;
   p->reflection = 1.0;
   p->pid_primal_weight = 0;
+  p->farkas_projection = -1;
 }
 
 int smy_model_create(int m, int n, int maximize, double offset, const int64_t* start,

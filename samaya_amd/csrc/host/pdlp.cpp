@@ -358,10 +358,16 @@ class Lane {
                      e.rel_dual * e.rel_dual / (omega * omega) + e.rel_gap * e.rel_gap);
   }
   bool detect_farkas(long long iter, const std::vector<double>& y_erg);
+  bool farkas_test(long long iter, const std::vector<double>& y_erg, const char* what);
   bool detect_ray(long long iter, const std::vector<double>& v, const std::vector<double>& v_new);
   // The same decisions from the device-side totals (11 numbers, smy_pdhg_detect_batch).
-  bool farkas_from(long long iter, const double* o) const;
+  bool farkas_from(long long iter, const double* o, const char* what = "") const;
   bool ray_from(long long iter, const double* o) const;
+  // The Farkas test's decision on the totals, without logging.
+  static bool farkas_passes(const double* o);
+  // The sign projection of a scaled dual for the f_min (orient 2) or f_max (3) test: entries that
+  // make a row's term infinite set to 0 (smy_pdhg_detect_batch's flags 2 and 3).
+  std::vector<double> sign_projected(const std::vector<double>& y, int orient) const;
   void map_back(const std::vector<double>& v, const std::vector<double>& y);
 
   // The bounds of this LP: the task's own when given, else the problem's.
@@ -399,6 +405,9 @@ class Lane {
   int fp32_seen = 0;
   bool fp32_armed = false;       // the stall test counts (fp32_below reached kStallArm once)
   bool active = true;
+  // Every structural variable has two finite bounds: the LP is bounded and has no improving ray,
+  // so the ray test is not taken (the device skips it when every tested lane is boxed).
+  bool boxed = false;
   Measure best{kInf, kInf, kInf, kInf, 0.0, 0.0};
 };
 
@@ -476,6 +485,8 @@ void Lane::prepare(std::shared_ptr<const Equilibrated> e) {
   cost = lp_.cost;
   lower = lo();
   upper = up();
+  boxed = true;
+  for (int j = 0; j < n && boxed; ++j) boxed = lower[j] > -kInf && upper[j] < kInf;
   for (int j = 0; j < n; ++j) {
     cost[j] *= c_s[j];
     lower[j] /= c_s[j];
@@ -535,7 +546,26 @@ Measure Lane::finish(const Sums& s) const {
 }
 
 // Farkas certificate test on the average dual (src/lp/pdlp.cpp:724-842).
+std::vector<double> Lane::sign_projected(const std::vector<double>& y, int orient) const {
+  const int n = lp_.n;
+  std::vector<double> out(y);
+  for (int i = 0; i < lp_.m; ++i) {
+    const double s = orient == 2 ? y[i] : -y[i];  // oriented as for f_min
+    if ((s < 0.0 && !(lo()[n + i] > -kInf)) || (s > 0.0 && !(up()[n + i] < kInf))) out[i] = 0.0;
+  }
+  return out;
+}
+
 bool Lane::detect_farkas(long long iter, const std::vector<double>& y_erg) {
+  // The reference's test on y_erg, then (farkas_projection) on its sign projections.
+  if (farkas_test(iter, y_erg, "")) return true;
+  if (!opt_.farkas_projection) return false;
+  for (int orient : {2, 3})
+    if (farkas_test(iter, sign_projected(y_erg, orient), ", sign-projected dual")) return true;
+  return false;
+}
+
+bool Lane::farkas_test(long long iter, const std::vector<double>& y_erg, const char* what) {
   const int m = lp_.m, n = lp_.n;
   std::vector<double> yf(static_cast<size_t>(m));
   double y_max = 0.0;
@@ -579,7 +609,7 @@ bool Lane::detect_farkas(long long iter, const std::vector<double>& y_erg) {
   log(2, ": infeas check iter=%lld y_max=%.2e min_f=%.2e max_f=%.2e margin=%.2e scale=%.2e",
       iter, y_max, f_min, f_max, margin, scale);
   if ((std::isfinite(f_min) && f_min > margin) || (std::isfinite(f_max) && f_max < -margin)) {
-    log(1, ": detected primal infeasibility at iter %lld (Farkas certificate)", iter);
+    log(1, ": detected primal infeasibility at iter %lld (Farkas certificate%s)", iter, what);
     res.dual_ray = y_erg;
     return true;
   }
@@ -654,19 +684,23 @@ bool Lane::detect_ray(long long iter, const std::vector<double>& v, const std::v
   return false;
 }
 
-bool Lane::farkas_from(long long iter, const double* o) const {
+bool Lane::farkas_passes(const double* o) {
   // src/lp/pdlp.cpp:724-842 from the device totals.
-  const double y_max = o[1];
-  if (!(y_max > 1e-6)) return false;
+  if (!(o[1] > 1e-6)) return false;
   const double f_min = o[4] > 0.0 ? -INFINITY : o[2];
   const double f_max = o[5] > 0.0 ? INFINITY : o[3];
-  double scale = o[6];
-  if (!std::isfinite(scale)) scale = 0.0;
+  const double scale = std::isfinite(o[6]) ? o[6] : 0.0;
   const double margin = 1e-8 * (1.0 + scale);
-  log(2, ": infeas check iter=%lld y_max=%.2e min_f=%.2e max_f=%.2e margin=%.2e scale=%.2e", iter,
-      y_max, f_min, f_max, margin, scale);
-  if ((std::isfinite(f_min) && f_min > margin) || (std::isfinite(f_max) && f_max < -margin)) {
-    log(1, ": detected primal infeasibility at iter %lld (Farkas certificate)", iter);
+  return (std::isfinite(f_min) && f_min > margin) || (std::isfinite(f_max) && f_max < -margin);
+}
+
+bool Lane::farkas_from(long long iter, const double* o, const char* what) const {
+  if (!(o[1] > 1e-6)) return false;
+  const double scale = std::isfinite(o[6]) ? o[6] : 0.0;
+  log(2, ": infeas check iter=%lld y_max=%.2e min_f=%.2e max_f=%.2e margin=%.2e scale=%.2e%s", iter,
+      o[1], o[4] > 0.0 ? -INFINITY : o[2], o[5] > 0.0 ? INFINITY : o[3], 1e-8 * (1.0 + scale), o[6], what);
+  if (farkas_passes(o)) {
+    log(1, ": detected primal infeasibility at iter %lld (Farkas certificate%s)", iter, what);
     return true;
   }
   return false;
@@ -774,7 +808,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
   // stored instance-minor, SpMM products); otherwise the stacked batch.
   const bool halpern = opt.halpern;
   const bool adaptive = opt.adaptive_steps && !halpern;
-  bool share = opt.share_matrix && !adaptive && !halpern && !opt.fp64_accumulation && K > 1 &&
+  bool share = opt.share_matrix && !adaptive && !opt.fp64_accumulation && K > 1 &&
                owners.size() == 1 && !device_eq;
   for (int i = 1; share && i < K; ++i) {
     const int n = lanes[0].lp_.n;
@@ -1018,6 +1052,29 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         any_detect = true;
       }
     if (any_detect && opt.device_detect) device->detect(dflag, dout);
+    // farkas_projection: the lanes whose Farkas test failed on infinite terms test the sign
+    // projection of their dual for f_min (flag 2), then for f_max (3), in one pass each.
+    std::vector<int> projected(static_cast<size_t>(K), 0);
+    if (any_detect && opt.device_detect && opt.farkas_projection) {
+      for (int orient : {2, 3}) {
+        std::vector<int> pflag(static_cast<size_t>(K), 0);
+        bool any = false;
+        for (int i = 0; i < K; ++i) {
+          const double* o = &dout[static_cast<size_t>(i) * kDetectSlotsHost];
+          if (dflag[i] && projected[i] == 0 && !Lane::farkas_passes(o) && o[1] > 1e-6 &&
+              o[orient == 2 ? 4 : 5] > 0.0) {
+            pflag[i] = orient;
+            any = true;
+          }
+        }
+        if (!any) continue;
+        std::vector<double> pout;
+        device->detect(pflag, pout);
+        for (int i = 0; i < K; ++i)
+          if (pflag[i] && Lane::farkas_passes(&pout[static_cast<size_t>(i) * kDetectSlotsHost]))
+            projected[i] = orient;
+      }
+    }
     for (int i = 0; any_detect && i < K; ++i) {
       if (!dflag[i]) continue;
       Lane& L = lanes[i];
@@ -1030,7 +1087,13 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
           L.res.dual_ray = y_erg;
           L.res.status = Status::kInfeasible;
           done = true;
-        } else if (L.ray_from(iter, o)) {
+        } else if (projected[i] != 0) {
+          L.log(1, ": detected primal infeasibility at iter %lld (Farkas certificate, sign-projected dual)", iter);
+          device->download(i, nullptr, nullptr, nullptr, nullptr, nullptr, &y_erg, nt, m);
+          L.res.dual_ray = L.sign_projected(y_erg, projected[i]);
+          L.res.status = Status::kInfeasible;
+          done = true;
+        } else if (!L.boxed && L.ray_from(iter, o)) {
           device->download_ray(i, L.res.primal_ray, L.lp_.n);
           L.res.status = Status::kUnbounded;
           done = true;
@@ -1041,7 +1104,7 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         if (L.detect_farkas(iter, y_erg)) {
           L.res.status = Status::kInfeasible;
           done = true;
-        } else if (L.detect_ray(iter, v, v_new)) {
+        } else if (!L.boxed && L.detect_ray(iter, v, v_new)) {
           L.res.status = Status::kUnbounded;
           done = true;
         }

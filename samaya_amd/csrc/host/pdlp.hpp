@@ -85,6 +85,11 @@ struct PdlpOptions {
   // restart) through a PI controller with gains 0.99 and 0.01 (as in cuPDLPx) instead of the
   // reference's halfway step omega <- sqrt(omega dy / dx). Decided on the host.
   bool pid_primal_weight = false;
+  // When the reference's Farkas test fails on a dual y, test its sign projection: the entries
+  // whose sign makes a row's term of f_min (then of f_max) infinite set to 0. A certificate it
+  // accepts is a Farkas certificate as the reference's test defines one; the reference tests y
+  // only.
+  bool farkas_projection = false;
   int log_level = 0;
   // Warm start (scaled computational-form space, before equilibration): v (n + m) and y (m),
   // and the primal weight to start with (<= 0: 1, the reference's start).

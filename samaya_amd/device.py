@@ -244,6 +244,12 @@ class PdhgBatch:
         """Restarted Halpern steps with this reflection (smy_pdhg_set_halpern)."""
         N.check(self.lib.smy_pdhg_set_halpern(self.h, int(on), reflection), "set_halpern")
 
+    def fixed_point(self) -> np.ndarray:
+        """(K, 2): ||v+ - v||^2, ||y+ - y||^2 of each instance's last Halpern step."""
+        out = np.zeros((self.size, 2))
+        N.check(self.lib.smy_pdhg_fixed_point_batch(self.h, _p(out)), "fixed_point_batch")
+        return out
+
     def set_fp64_accumulation(self, on: bool):
         """FP32 steps sum their products in FP64 (storage stays FP32)."""
         N.check(self.lib.smy_pdhg_set_fp64_accumulation(self.h, int(on)), "set_fp64_accumulation")

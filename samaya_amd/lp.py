@@ -216,6 +216,7 @@ class PdlpOptions:
     halpern: bool = False            # restarted Halpern PDHG with reflection (no averages)
     reflection: float = 1.0          # Halpern steps: the reflection in [0, 1]
     pid_primal_weight: bool = False  # primal weight by a PI controller on log(dy / dx)
+    farkas_projection: int = -1      # sign-projected Farkas test too (-1: unless algorithm reference)
 
     def native(self) -> N.SmyPdlpParams:
         p = N.SmyPdlpParams()
@@ -245,6 +246,7 @@ class PdlpOptions:
         p.halpern = int(self.halpern)
         p.reflection = self.reflection
         p.pid_primal_weight = int(self.pid_primal_weight)
+        p.farkas_projection = int(self.farkas_projection)
         return p
 
 

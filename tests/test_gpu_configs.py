@@ -176,16 +176,18 @@ def test_c4_blending_batch(gpu):
         assert alone.objective == res[i].objective, i
 
 
-def test_c5_node_batch(gpu):
+@pytest.mark.parametrize("halpern", [False, True], ids=["pdlp", "halpern"])
+def test_c5_node_batch(gpu, halpern):
     # 1,024 node relaxations of the 20,000 x 50,000 knapsack LP, warm-started from the root LP's
-    # answer and primal weight, as one shared-matrix batch.
+    # answer and primal weight, as one shared-matrix batch (PDLP's steps or Halpern steps).
     md, lows, ups = smy.generators.c5(1024)
     nm = md.native()
     A = csc(md)
     # A node is infeasible exactly when the columns it fixes at 1 already overfill a row: every
     # weight is positive and every free column can sit at 0.
     infeasible = [bool(np.any(A @ lo > md.row_upper)) for lo in lows]
-    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", fp32_switch_tol=1e-6)
+    opts = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", fp32_switch_tol=1e-6,
+                           halpern=halpern)
     root = smy.solve_lp(nm, opts)
     assert root.status == smy.OPTIMAL
     opts.start_omega = root.omega

@@ -127,6 +127,44 @@ def test_detection_numbers_match_the_reference_formulas(gpu, layout):
         want, ray = reference_numbers(lp, lows[k], ups[k], rs, cs, st["v"], st["v_new"], st["y_erg"])
         assert close(got[k], want), (k, got[k], want)
         assert close(b.download_ray(k), ray)
+    # Flags 2 and 3: the Farkas numbers of the dual's sign projection (rows whose term of f_min,
+    # resp. f_max, would be infinite set to 0); no ray test.
+    for f in (2, 3):
+        got = b.detect([f, 0, f])
+        for k in (0, 2):
+            st = b.download(k)
+            y = st["y_erg"].copy()
+            s = y if f == 2 else -y
+            cut = ((s < 0) & ~np.isfinite(lows[k][n:])) | ((s > 0) & ~np.isfinite(ups[k][n:]))
+            y[cut] = 0.0
+            want, _ = reference_numbers(lp, lows[k], ups[k], rs, cs, st["v"], st["v_new"], y)
+            assert close(got[k][:7], want[:7]), (f, k, got[k], want)
+            assert np.all(got[k][7:] == 0.0)
+    with pytest.raises(RuntimeError):
+        b.detect([4, 0, 0])
+
+
+def test_bounded_instances_skip_the_ray_test(gpu):
+    # Every structural variable boxed: the LP has no improving ray; the device leaves the ray
+    # slots 0 and computes the Farkas numbers as always.
+    md = G.knapsack(82, 50, 120, 4)
+    lp = identity_lp(M.from_product(md))
+    n, m = lp.n, lp.m
+    assert np.all(np.isfinite(lp.lower[:n])) and np.all(np.isfinite(lp.upper[:n]))
+    rng = np.random.default_rng(6)
+    rs, cs = rng.uniform(0.5, 2.0, m), rng.uniform(0.5, 2.0, n)
+    eq = equilibrated(lp, rs, cs)
+    a = (m, n, eq.start, eq.index, eq.value)
+    b = D.PdhgBatch.create([a], [transpose_csc(eq)], [eq.cost], [eq.lower], [eq.upper])
+    b.set_original(0, cs, rs, lp.cost, lp.lower, lp.upper)
+    b.run([0.3], [0.3], 1.0, [0], 64)
+    b.advance()
+    b.run([0.3], [0.3], 1.0, [64], 3)
+    got = b.detect([1])[0]
+    st = b.download(0)
+    want, _ = reference_numbers(lp, lp.lower, lp.upper, rs, cs, st["v"], st["v_new"], st["y_erg"])
+    assert close(got[:7], want[:7])
+    assert np.all(got[7:] == 0.0) and want[7] > 0.0
 
 
 def test_device_and_host_detection_agree(gpu):

@@ -195,3 +195,82 @@ def test_warm_start_is_used(gpu):
     for c, w in zip(cold, warm):
         assert c.status == w.status == smy.OPTIMAL
         assert abs(c.objective - w.objective) <= 1e-4 * (1 + abs(c.objective))
+
+
+@pytest.mark.parametrize("K,rho", [(7, 1.0), (72, 0.5)])  # compact block / full-width SpMM rows
+def test_shared_halpern_steps_match_stacked(gpu, K, rho):
+    # Restarted Halpern steps (T(z) in the averages' slot, z' in the iterates) on both layouts:
+    # evaluation sums, fixed-point residuals, restart distances and the iterates agree to 1e-9 in
+    # FP64, through a restart at T(z), a frozen instance and an FP32 instance.
+    md = G.knapsack(71, 60, 150, 4)
+    lp = identity_lp(M.from_product(md))
+    rng = np.random.default_rng(5)
+    rs, cs = rng.uniform(0.5, 2.0, lp.m), rng.uniform(0.5, 2.0, lp.n)
+    eq = equilibrated(lp, rs, cs)
+    lows, ups = [], []
+    for k in range(K):
+        lo, up = G.fixings(850 + k, eq.lower[:lp.n], eq.upper[:lp.n], 0.05, 0.2)
+        lows.append(np.concatenate([lo, eq.lower[lp.n:]]))
+        ups.append(np.concatenate([up, eq.upper[lp.n:]]))
+    a = (eq.m, eq.n, eq.start, eq.index, eq.value)
+    at = transpose_csc(eq)
+    shared = D.PdhgBatch.create_shared(a, at, eq.cost, lows, ups)
+    stacked = D.PdhgBatch.create([a] * K, [at] * K, [eq.cost] * K, lows, ups)
+    ol = [np.concatenate([lows[k][:lp.n] * cs, lp.lower[lp.n:]]) for k in range(K)]
+    ou = [np.concatenate([ups[k][:lp.n] * cs, lp.upper[lp.n:]]) for k in range(K)]
+    for b in (shared, stacked):
+        b.set_halpern(True, rho)
+        for k in range(K):
+            b.set_original(k, cs, rs, lp.cost, ol[k], ou[k])
+    tau = 0.2 + 0.2 * (np.arange(K) % 7) / 6
+    sigma = 0.4 - 0.2 * (np.arange(K) % 5) / 4
+    for b in (shared, stacked):
+        b.run(tau, sigma, 1.0, [0] * K, 64)
+    es, et = shared.evaluate(), stacked.evaluate()
+    assert rel_close(es, et, 1e-9)
+    np.testing.assert_array_equal(es[:, :7], es[:, 7:])  # T(z) in both halves
+    assert rel_close(shared.fixed_point(), stacked.fixed_point(), 1e-9)
+    which = [(1, -1, 1, -1, 1, 1, -1)[k % 7] for k in range(K)]
+    flagged = [k for k in range(K) if which[k] >= 0]
+    assert rel_close(shared.restart_distance(which)[flagged], stacked.restart_distance(which)[flagged], 1e-9)
+    k0 = [(0 if which[k] >= 0 else 64) for k in range(K)]
+    for b in (shared, stacked):
+        b.restart_to(which)
+        b.advance()
+        b.freeze(3)
+        b.set_precision(["fp32" if k == 1 else -1 for k in range(K)])
+        b.run(tau, sigma, 1.0, k0, 5)
+    es, et = shared.evaluate(), stacked.evaluate()
+    fs, ft = shared.fixed_point(), stacked.fixed_point()
+    for k in range(K):
+        if k not in (1, 3):
+            assert rel_close(es[k], et[k], 1e-9), k
+            assert rel_close(fs[k], ft[k], 1e-9), k
+    assert rel_close(es[1], et[1], 1e-4)  # FP32 steps: rounding of FP32 sums in another order
+    for k in (0, 2, 5):
+        ds, dt = shared.download(k), stacked.download(k)
+        for key in ds:
+            assert rel_close(ds[key], dt[key], 1e-9), key
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_shared_halpern_node_relaxations(gpu, precision):
+    # Halpern steps on the shared-matrix batch: every node as the stacked batch solves it, and
+    # HiGHS's objective of the node (golden node_relaxations) once polished.
+    md = G.knapsack(73, 400, 1000, 6)
+    lows, ups = node_bounds(md, 9)
+    highs = {(e["knapsack"][0], e["fixings_seed"]): e["objective"]
+             for e in load_golden("node_relaxations")["nodes"] if e["status"] == 0}
+    opts = smy.PdlpOptions(tol=1e-7, precision=precision, algorithm="enhanced", halpern=True,
+                           polish_rounds=2)
+    shared, times = smy.solve_lp_bounds_batch(md, lows, ups, opts)
+    assert times.kernel_ms > 0
+    opts.share_matrix = False
+    stacked, _ = smy.solve_lp_bounds_batch(md, lows, ups, opts)
+    for k in range(len(lows)):
+        a, b = shared[k], stacked[k]
+        assert a.status == b.status == smy.OPTIMAL
+        assert abs(a.objective - b.objective) <= 1e-6 * (1 + abs(b.objective))
+        want = highs[(73, 900 + k)]
+        assert a.verified and abs(a.objective - want) <= 1e-6 * (1 + abs(want))
+        assert want <= a.dual_bound <= want + 1e-6 * (1 + abs(want))
