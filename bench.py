@@ -162,14 +162,24 @@ def spmv_sweep(md) -> dict:
 
 
 def count_order(md, algorithm: str, column_order: str = "auto"):
-    """The column order of the solver's model-level solve (PdlpOptions.column_order): "auto" is
-    by decreasing count for the enhanced algorithm when the longest column is at least 16x the
-    mean, else None (the model's order); "degree" always by count; "natural" None."""
+    """The column order of the solver's model-level solve (PdlpOptions.column_order): "degree" by
+    decreasing count; "natural" None (the model's order); "touch" the hot columns (count >= 2x the
+    mean) by count, then the others by the first row holding them; "auto" the touch order for the
+    enhanced algorithm when the longest column is at least 16x the mean, else None."""
     counts = np.diff(md.start)
     if column_order == "natural" or md.n == 0:
         return None
     if column_order == "auto" and (algorithm != "enhanced" or counts.max() < 16 * max(md.nnz / md.n, 1.0)):
         return None
+    if column_order in ("touch", "auto"):
+        idx = np.asarray(md.index)
+        first = np.full(md.n, md.m, dtype=np.int64)
+        nonempty = counts > 0
+        first[nonempty] = np.minimum.reduceat(idx, np.asarray(md.start)[:-1][nonempty])
+        hot = counts >= 2.0 * (md.nnz / md.n)
+        hot_idx, cold_idx = np.flatnonzero(hot), np.flatnonzero(~hot)
+        return np.concatenate([hot_idx[np.argsort(-counts[hot_idx], kind="stable")],
+                               cold_idx[np.argsort(first[cold_idx], kind="stable")]])
     return np.argsort(-counts, kind="stable")
 
 
@@ -327,10 +337,10 @@ def main() -> int:
     ap.add_argument("--c5-method", default="halpern", choices=["halpern", "constant"],
                     help="the node-relaxation batch's iteration (shared-matrix batch)")
     ap.add_argument("--reflection", type=float, default=1.0, help="Halpern steps: the reflection")
-    ap.add_argument("--column-order", default=None, choices=["auto", "natural", "degree"],
-                    help="the single LP's column order (default: by count for Halpern steps; auto "
-                         "otherwise, which orders by count for the enhanced algorithm when the "
-                         "counts are skewed)")
+    ap.add_argument("--column-order", default=None, choices=["auto", "natural", "degree", "touch"],
+                    help="the single LP's column order (default: touch for Halpern steps: hot "
+                         "columns by count, the rest by first row; auto otherwise, which takes the "
+                         "touch order for the enhanced algorithm when the counts are skewed)")
     ap.add_argument("--pid-weight", action="store_true",
                     help="the primal weight by a PI controller on log(dy / dx) at restarts")
     ap.add_argument("--verbose", action="store_true", help="progress lines of long batches (stdout)")
@@ -364,7 +374,7 @@ def main() -> int:
     args.halpern = args.method == "halpern"
     args.adaptive = args.method == "adaptive"
     args.c3_algorithm = args.algorithm or ("reference" if args.halpern else "enhanced")
-    args.c3_column_order = args.column_order or ("degree" if args.halpern else "auto")
+    args.c3_column_order = args.column_order or ("touch" if args.halpern else "auto")
     args.c3_switch_tol = args.fp32_switch_tol if args.fp32_switch_tol is not None else (
         args.tol if args.halpern else 1e-5)
     args.c3_stall_level = args.fp32_stall_level if args.fp32_stall_level is not None else (
@@ -481,7 +491,8 @@ def main() -> int:
         "pid_primal_weight": args.pid_weight,
         "fp64_accumulation": args.fp64_accumulation,
         "detect": False,
-        "column_order": "by count" if order is not None else "model",
+        "column_order": ("model" if order is None else
+                         "hot by count, then by first row" if args.c3_column_order == "touch" else "by count"),
         "status": last.status_name,
         "data": "synthetic (seeded generator)",
         "config": {"workload": "sparse LP m=500000 n=1000000 nnz~12M power-law columns, "

@@ -119,10 +119,13 @@ typedef struct smy_pdlp_params {
   int device_scaling;        /* one LP (either algorithm): the preconditioner, the scaled matrices,
                                 cost and bounds and the operator-norm bound computed on the device
                                 (smy_pdhg_equilibrate; 1) or on the host (0); the same factors */
-  int column_order;          /* model-level solves: 0 auto (enhanced algorithm with skewed column
-                                counts: columns by decreasing nonzero count, so the products'
-                                gathers of the hottest columns share cache lines), 1 the model's
-                                order, 2 by count always; answers come back in the model's order */
+  int column_order;          /* model-level solves: 1 the model's order; 2 by decreasing nonzero
+                                count (the products' gathers of the hottest columns share cache
+                                lines); 3 the hot columns (count >= 2x the mean) by count, the
+                                rest by the first row holding them (a row's new columns side by
+                                side); 0 auto: 3 for the enhanced algorithm with skewed column
+                                counts (the longest >= 16x the mean), else 1. Answers come back in
+                                the model's order */
   long long inject_fault_step; /* test hook: the device handle fails its steps after this many
                                   (smy_pdhg_inject_fault), so the solve takes the reference's
                                   GPU-failure path (src/lp/pdlp.cpp:231-242); -1: off */

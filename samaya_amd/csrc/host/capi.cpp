@@ -53,7 +53,7 @@ PdlpOptions to_options(const smy_pdlp_params* p) {
   o.fp64_accumulation = p->fp64_accumulation != 0;
   o.inject_fault_step = p->inject_fault_step;
   o.device_scaling = p->device_scaling != 0;
-  o.column_order = static_cast<ColumnOrder>(p->column_order == 1 ? 1 : p->column_order == 2 ? 2 : 0);
+  o.column_order = static_cast<ColumnOrder>(p->column_order >= 1 && p->column_order <= 3 ? p->column_order : 0);
   o.fp32_stall_level = p->fp32_stall_level;
   o.device_restart = p->device_restart != 0;
   o.halpern = p->halpern != 0;

@@ -43,7 +43,7 @@ enum class Algorithm : int { kReference = 0, kEnhanced = 1 };
 // Column order of a model-level solve: kAuto puts the columns of an enhanced-algorithm solve in
 // decreasing order of their nonzero counts when those are skewed (the longest column at least 16
 // times the mean); kDegree always, kNatural never. Answers come back in the model's order.
-enum class ColumnOrder : int { kAuto = 0, kNatural = 1, kDegree = 2 };
+enum class ColumnOrder : int { kAuto = 0, kNatural = 1, kDegree = 2, kTouch = 3 };
 
 struct PdlpOptions {
   double tol = 1e-8;              // max(rel primal, rel dual, |rel gap|) < tol

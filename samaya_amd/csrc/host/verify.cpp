@@ -367,12 +367,10 @@ double safe_dual_bound(const Model& md, const double* y) {
 
 namespace {
 
-// Whether a model-level solve puts the model's columns in decreasing order of their counts
-// (ColumnOrder::kAuto: the enhanced algorithm, and the longest column at least 16 times the mean).
-bool wants_degree_order(const Model& md, const PdlpOptions& o) {
-  if (o.column_order == ColumnOrder::kNatural || md.n == 0) return false;
-  if (o.column_order == ColumnOrder::kDegree) return true;
-  if (o.algorithm != Algorithm::kEnhanced) return false;
+// Whether ColumnOrder::kAuto reorders the model's columns: the enhanced algorithm, and the longest
+// column at least 16 times the mean.
+bool skewed_for_auto(const Model& md, const PdlpOptions& o) {
+  if (o.algorithm != Algorithm::kEnhanced || md.n == 0) return false;
   int64_t longest = 0;
   for (int j = 0; j < md.n; ++j) longest = std::max(longest, md.A.start[j + 1] - md.A.start[j]);
   const double mean = static_cast<double>(md.A.nnz()) / md.n;
@@ -388,6 +386,34 @@ std::vector<int> degree_order(const Model& md) {
     return md.A.start[a + 1] - md.A.start[a] > md.A.start[b + 1] - md.A.start[b];
   });
   return order;
+}
+
+// ColumnOrder::kTouch: the hot columns (count at least twice the mean) by decreasing count, then
+// the others by the first row that holds them (the order in which a pass over the rows first
+// reaches them), ties in the model's order. A row's columns seen for the first time then sit side
+// by side, so its gathers of them share cache lines.
+std::vector<int> touch_order(const Model& md) {
+  const double mean = md.n > 0 ? static_cast<double>(md.A.nnz()) / md.n : 0.0;
+  const auto count = [&](int j) { return md.A.start[j + 1] - md.A.start[j]; };
+  std::vector<int> hot, cold;
+  std::vector<int> first(static_cast<size_t>(md.n), md.m);
+  for (int j = 0; j < md.n; ++j) {
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) first[j] = std::min(first[j], md.A.index[p]);
+    (static_cast<double>(count(j)) >= 2.0 * mean ? hot : cold).push_back(j);
+  }
+  std::stable_sort(hot.begin(), hot.end(), [&](int a, int b) { return count(a) > count(b); });
+  std::stable_sort(cold.begin(), cold.end(), [&](int a, int b) { return first[a] < first[b]; });
+  hot.insert(hot.end(), cold.begin(), cold.end());
+  return hot;
+}
+
+// The column order of a model-level solve (empty: the model's own). kAuto takes the touch order
+// on skewed column counts.
+std::vector<int> solve_order(const Model& md, const PdlpOptions& o) {
+  if (md.n == 0 || o.column_order == ColumnOrder::kNatural) return {};
+  if (o.column_order == ColumnOrder::kDegree) return degree_order(md);
+  if (o.column_order == ColumnOrder::kTouch || skewed_for_auto(md, o)) return touch_order(md);
+  return {};
 }
 
 template <typename T>
@@ -651,8 +677,8 @@ std::vector<LpSolveResult> solve_lp_batch(const std::vector<const Model*>& model
   std::vector<LpJob> jobs(static_cast<size_t>(K));
   for_each_parallel(K, [&](int i) {
     jobs[i].model = models[i];
-    if (wants_degree_order(*models[i], options.pdlp)) {
-      jobs[i].order = degree_order(*models[i]);
+    jobs[i].order = solve_order(*models[i], options.pdlp);
+    if (!jobs[i].order.empty()) {
       jobs[i].permuted = permute_columns(*models[i], jobs[i].order);
       jobs[i].model = jobs[i].permuted.get();
     }
@@ -678,8 +704,8 @@ std::vector<LpSolveResult> solve_lp_bounds_batch(const Model& model,
   std::vector<int> order;
   std::shared_ptr<const Model> permuted;
   std::vector<double> start_perm;
-  if (wants_degree_order(model, options.pdlp)) {
-    order = degree_order(model);
+  order = solve_order(model, options.pdlp);
+  if (!order.empty()) {
     permuted = permute_columns(model, order);
     if (start_col_value != nullptr) {
       start_perm.resize(static_cast<size_t>(n));

@@ -211,7 +211,7 @@ class PdlpOptions:
     device_restart: bool = True     # restart decisions, distances and primal weights on the GPU
     fp64_accumulation: bool = False  # mixed: FP32 steps sum their products in FP64
     device_scaling: bool = True      # one LP: preconditioner and norm bound on the GPU
-    column_order: str = "auto"       # model-level solves: "auto", "natural" or "degree"
+    column_order: str = "auto"       # model-level solves: "auto", "natural", "degree" or "touch"
     inject_fault_step: int = -1      # test hook: the device fails its steps after this many
     halpern: bool = False            # restarted Halpern PDHG with reflection (no averages)
     reflection: float = 1.0          # Halpern steps: the reflection in [0, 1]
@@ -242,7 +242,7 @@ class PdlpOptions:
         p.fp64_accumulation = int(self.fp64_accumulation)
         p.inject_fault_step = int(self.inject_fault_step)
         p.device_scaling = int(self.device_scaling)
-        p.column_order = {"auto": 0, "natural": 1, "degree": 2}[self.column_order]
+        p.column_order = {"auto": 0, "natural": 1, "degree": 2, "touch": 3}[self.column_order]
         p.halpern = int(self.halpern)
         p.reflection = self.reflection
         p.pid_primal_weight = int(self.pid_primal_weight)

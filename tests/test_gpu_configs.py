@@ -115,7 +115,7 @@ C3_LINES = {
     # The benchmark's line: restarted Halpern steps in FP32 (values, iterates, sums) to the
     # tolerance, FP64 evaluation, restarts and termination, the reference's preconditioner and the
     # columns by count.
-    "halpern": dict(halpern=True, algorithm="reference", column_order="degree",
+    "halpern": dict(halpern=True, algorithm="reference", column_order="touch",
                     fp32_switch_tol=1e-6, fp32_stall_level=0.0),
     # PDLP with adaptive steps and the enhanced preconditioner.
     "adaptive": dict(adaptive_steps=True, algorithm="enhanced"),
