@@ -2,7 +2,8 @@
 
 A C++ host solver keeps PDLP's decisions; hand-written HIP kernels run the iterations. Python
 here only marshals arrays into the native library's C ABI (include/samaya_pdhg.h,
-include/samaya_pdlp.h, include/samaya_io.h); it has no compute path of its own.
+include/samaya_pdlp.h, include/samaya_io.h, include/samaya_mip.h); it has no compute path of its
+own.
 """
 from ._native import LIB_PATH, load, last_error, NativeError
 from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, PdlpOptions,
@@ -12,6 +13,7 @@ from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, P
                  INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR)
 from .mps import (MpsParseError, read_mps, read_mps_text, read_mps_native, read_mps_text_native,
                   mps_text, write_mps, solution_text, write_solution)
+from .mip import MipOptions, MipResult, solve_mip
 from . import generators
 
 __all__ = [
@@ -23,5 +25,5 @@ __all__ = [
     "gpu_info", "STATUS_NAMES", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "ITERATION_LIMIT",
     "TIME_LIMIT", "NUMERICAL_ERROR", "generators", "MpsParseError", "read_mps", "read_mps_text",
     "read_mps_native", "read_mps_text_native", "mps_text", "write_mps", "solution_text",
-    "write_solution",
+    "write_solution", "MipOptions", "MipResult", "solve_mip",
 ]

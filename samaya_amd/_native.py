@@ -45,6 +45,7 @@ EXPORTS = (
     "smy_read_mps", "smy_read_mps_text", "smy_model_name", "smy_model_names",
     "smy_model_set_names", "smy_model_set_integrality", "smy_model_integrality", "smy_write_mps",
     "smy_write_mps_text", "smy_write_solution", "smy_solution_text",
+    "smy_mip_params_default", "smy_mip_solve",
 )
 
 
@@ -88,6 +89,22 @@ class SmyLpResult(C.Structure):
                 ("iterations", C.c_longlong), ("fp32_iterations", C.c_longlong),
                 ("rounds", C.c_int), ("pdlp", SmyPdlpInfo), ("message", C.c_char * 256),
                 ("dual_bound", C.c_double)]
+
+
+class SmyMipParams(C.Structure):
+    _fields_ = [("lp", SmyPdlpParams), ("gap_tol", C.c_double), ("wave", C.c_int),
+                ("max_nodes", C.c_longlong), ("time_limit", C.c_double),
+                ("integrality_tol", C.c_double), ("feasibility_tol", C.c_double),
+                ("log_level", C.c_int)]
+
+
+class SmyMipResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("objective", C.c_double), ("bound", C.c_double),
+                ("gap", C.c_double), ("nodes", C.c_longlong), ("waves", C.c_longlong),
+                ("pruned_infeasible", C.c_longlong), ("pruned_bound", C.c_longlong),
+                ("integral", C.c_longlong), ("unresolved", C.c_longlong),
+                ("lp_iterations", C.c_longlong), ("seconds", C.c_double),
+                ("message", C.c_char * 256)]
 
 
 _lib = None
@@ -204,6 +221,8 @@ def load() -> C.CDLL:
     _proto(lib, "smy_solve_lp_bounds_batch", i32, vp, i32, vp, vp, C.POINTER(SmyPdlpParams), vp,
            vp, C.POINTER(SmyLpResult), vp, vp, vp)
     _proto(lib, "smy_generate_fixings", i32, C.c_uint64, i32, f64, f64, vp, vp, vp, vp)
+    _proto(lib, "smy_mip_params_default", None, C.POINTER(SmyMipParams))
+    _proto(lib, "smy_mip_solve", i32, vp, vp, C.POINTER(SmyMipParams), C.POINTER(SmyMipResult), vp)
     _proto(lib, "smy_solve", i32, vp, C.POINTER(f64))
     _proto(lib, "smy_verify_lp", i32, vp, vp, vp, f64, f64, vp, C.c_char_p, C.c_size_t)
     _proto(lib, "smy_verify_infeasibility", i32, vp, vp, C.c_char_p, C.c_size_t)

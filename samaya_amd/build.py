@@ -1,8 +1,8 @@
 """Builds the native library samaya_amd/_lib/libsamaya_amd.so with hipcc for gfx950.
 
 Every source is compiled in-tree (objects under samaya_amd/_build/), in parallel, then linked
-into one shared library that exports the C ABI of include/samaya_pdhg.h and
-include/samaya_pdlp.h. Flags:
+into one shared library that exports the C ABI of include/samaya_pdhg.h, include/samaya_pdlp.h,
+include/samaya_io.h and include/samaya_mip.h. Flags:
   --offload-arch=gfx950              CDNA4 only
   -ffp-contract=off                  no FMA contraction: the double-double verifier needs a*b+c
                                      rounded twice, and the FP64 step arithmetic matches the
@@ -34,6 +34,7 @@ SOURCES = [
     SRC / "host" / "lp_data.cpp",
     SRC / "host" / "pdlp.cpp",
     SRC / "host" / "verify.cpp",
+    SRC / "host" / "mip.cpp",
     SRC / "host" / "generators.cpp",
     SRC / "host" / "capi.cpp",
     SRC / "host" / "mps.cpp",

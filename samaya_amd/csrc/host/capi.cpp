@@ -10,9 +10,11 @@
 #include <stdexcept>
 
 #include "../../../include/samaya_pdhg.h"
+#include "../../../include/samaya_mip.h"
 #include "../../../include/samaya_pdlp.h"
 #include "capi_model.hpp"
 #include "generators.hpp"
+#include "mip.hpp"
 #include "pdlp.hpp"
 
 using namespace smy::host;
@@ -453,6 +455,70 @@ int smy_solve_lp_bounds_batch(const smy_model* model, int count, const double* c
       results[i].status = static_cast<int>(Status::kNumericalError);
       copy_text(results[i].message, sizeof results[i].message, e.what());
     }
+    return SMY_ERROR;
+  }
+}
+
+void smy_mip_params_default(smy_mip_params* params) {
+  if (params == nullptr) return;
+  smy_pdlp_params_default(&params->lp);
+  const MipOptions d;
+  params->gap_tol = d.gap_tol;
+  params->wave = d.wave;
+  params->max_nodes = d.max_nodes;
+  params->time_limit = d.time_limit;
+  params->integrality_tol = d.integrality_tol;
+  params->feasibility_tol = d.feasibility_tol;
+  params->log_level = d.log_level;
+}
+
+int smy_mip_solve(const smy_model* model, const unsigned char* integer, const smy_mip_params* params,
+                  smy_mip_result* result, double* col_value) {
+  if (model == nullptr || result == nullptr) return SMY_INVALID;
+  std::memset(result, 0, sizeof *result);
+  smy_mip_params d;
+  smy_mip_params_default(&d);
+  if (params == nullptr) params = &d;
+  const int n = model->model.n;
+  std::vector<char> flags(static_cast<size_t>(n), 0);
+  if (integer != nullptr) {
+    for (int j = 0; j < n; ++j) flags[j] = integer[j] != 0;
+  } else if (static_cast<int>(model->col_integer.size()) == n) {
+    flags.assign(model->col_integer.begin(), model->col_integer.end());
+  }
+  if (std::none_of(flags.begin(), flags.end(), [](char f) { return f != 0; })) {
+    copy_text(result->message, sizeof result->message, "no integer columns");
+    return SMY_INVALID;
+  }
+  if (!(params->gap_tol >= 0.0) || params->wave <= 0 || params->max_nodes <= 0) return SMY_INVALID;
+  try {
+    MipOptions o;
+    o.lp = to_lp_options(&params->lp);
+    o.gap_tol = params->gap_tol;
+    o.wave = params->wave;
+    o.max_nodes = params->max_nodes;
+    o.time_limit = params->time_limit;
+    o.integrality_tol = params->integrality_tol;
+    o.feasibility_tol = params->feasibility_tol;
+    o.log_level = params->log_level;
+    const MipResult r = solve_mip(model->model, flags, o);
+    result->status = static_cast<int>(r.status);
+    result->objective = r.objective;
+    result->bound = r.bound;
+    result->gap = r.gap;
+    result->nodes = r.nodes;
+    result->waves = r.waves;
+    result->pruned_infeasible = r.pruned_infeasible;
+    result->pruned_bound = r.pruned_bound;
+    result->integral = r.integral;
+    result->unresolved = r.unresolved;
+    result->lp_iterations = r.lp_iterations;
+    result->seconds = r.seconds;
+    if (col_value != nullptr && !r.col_value.empty()) put(r.col_value, col_value);
+    return SMY_OK;
+  } catch (const std::exception& e) {
+    result->status = static_cast<int>(Status::kNumericalError);
+    copy_text(result->message, sizeof result->message, e.what());
     return SMY_ERROR;
   }
 }

@@ -1,0 +1,244 @@
+// Best-bound branch and bound over waves of batched PDLP node relaxations (mip.hpp).
+//
+// Each wave takes up to `wave` open nodes with the smallest bounds (minimization sense) that can
+// still beat the incumbent, solves their relaxations as one batch (solve_lp_bounds_batch: the
+// model's matrix once, one column-bound vector per node), warm-started from the root relaxation's
+// answer and primal weight, and for each node:
+//  - prunes it when the relaxation is infeasible (a Farkas certificate);
+//  - takes its bound from the answer's safe dual bound (weak duality over the node's box with
+//    rounding margins: a valid bound whatever the duals' accuracy), never below its parent's, and
+//    prunes it when that cannot beat the incumbent by more than the gap tolerance;
+//  - rounds the answer's integer columns to the nearest integers within the node's bounds and
+//    accepts the point as an incumbent when it satisfies every row and bound;
+//  - branches on the integer column of largest fractionality: x_j <= floor(v), x_j >= ceil(v).
+// The reference's search (src/mip/branch_and_bound.cpp) adds propagation, cuts, strong branching,
+// pseudocosts, diving heuristics and conflict analysis, none of which is here: this is the
+// bounding of nodes through the batched GPU path and the loop that uses it.
+#include "mip.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+
+namespace smy::host {
+
+namespace {
+
+struct Node {
+  double bound;  // a lower bound of the node's optimum, minimization sense
+  int depth;
+  std::vector<double> lower, upper;  // the node's column bounds (n)
+};
+
+using NodePtr = std::unique_ptr<Node>;
+
+// Min-heap by bound (ties: the deeper node first, which finds incumbents sooner).
+bool heap_less(const NodePtr& a, const NodePtr& b) {
+  if (a->bound != b->bound) return a->bound > b->bound;
+  return a->depth < b->depth;
+}
+
+double objective_of(const Model& md, const std::vector<double>& x) {
+  double f = md.offset;
+  for (int j = 0; j < md.n; ++j) f += md.obj[j] * x[j];
+  return f;
+}
+
+bool within(double v, double lo, double up, double tol) {
+  return v >= lo - tol * (1.0 + std::fabs(lo)) && v <= up + tol * (1.0 + std::fabs(up));
+}
+
+// Whether x satisfies the node's column bounds and the model's rows within tol.
+bool feasible(const Model& md, const std::vector<double>& x, const Node& node, double tol) {
+  for (int j = 0; j < md.n; ++j)
+    if (!within(x[j], node.lower[j], node.upper[j], tol)) return false;
+  std::vector<double> act(static_cast<size_t>(md.m), 0.0);
+  for (int j = 0; j < md.n; ++j)
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) act[md.A.index[p]] += md.A.value[p] * x[j];
+  for (int i = 0; i < md.m; ++i)
+    if (!within(act[i], md.row_lower[i], md.row_upper[i], tol)) return false;
+  return true;
+}
+
+}  // namespace
+
+MipResult solve_mip(const Model& md, const std::vector<char>& integer, const MipOptions& opt) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const auto seconds = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  const int n = md.n;
+  const double sense = md.maximize ? -1.0 : 1.0;  // minimization sense: sense * objective
+  MipResult out;
+  double incumbent = kInf;  // minimization sense
+  // The smallest bound of the nodes closed without being beaten by the incumbent: pruned within
+  // the gap tolerance, or left unresolved (their parent's bound).
+  double closed_min = kInf;
+  const auto gap_abs = [&](double f) { return std::isfinite(f) ? opt.gap_tol * (1.0 + std::fabs(f)) : 0.0; };
+  const auto log = [&](const char* what) {
+    if (opt.log_level >= 1) {
+      std::printf("mip: %s: nodes %lld, waves %lld, incumbent %.9g, %.1f s\n", what, out.nodes, out.waves,
+                  std::isfinite(incumbent) ? sense * incumbent : NAN, seconds());
+      std::fflush(stdout);
+    }
+  };
+
+  // The root relaxation (its answer and primal weight warm-start every node).
+  LpSolveOptions lpo = opt.lp;
+  const LpSolveResult root = solve_lp(md, lpo);
+  out.nodes = 1;
+  out.lp_iterations = root.iterations;
+  if (root.status != Status::kOptimal) {
+    // An infeasible relaxation makes the MIP infeasible; an unbounded one leaves it unbounded or
+    // infeasible (reported as the relaxation's status); any other stop is the relaxation's.
+    out.status = root.status;
+    if (root.status == Status::kInfeasible) ++out.pruned_infeasible;
+    out.seconds = seconds();
+    return out;
+  }
+  lpo.pdlp.start_omega = root.pdlp.omega;
+  const std::vector<double> start_x = root.col_value, start_y = root.row_dual;
+
+  std::vector<NodePtr> heap;
+  // A solved relaxation of `node`: prune, update the incumbent, or branch.
+  const auto handle = [&](const Node& node, const LpSolveResult& r) {
+    if (&r != &root) out.lp_iterations += r.iterations;
+    if (r.status == Status::kInfeasible) {
+      ++out.pruned_infeasible;
+      return;
+    }
+    if (r.status != Status::kOptimal || r.col_value.size() != static_cast<size_t>(n)) {
+      ++out.unresolved;
+      closed_min = std::min(closed_min, node.bound);
+      return;
+    }
+    const double lb = std::max(node.bound, std::isfinite(r.dual_bound) ? sense * r.dual_bound : -kInf);
+    if (lb >= incumbent - gap_abs(incumbent)) {
+      ++out.pruned_bound;
+      if (lb < incumbent) closed_min = std::min(closed_min, lb);
+      return;
+    }
+    std::vector<double> x = r.col_value;
+    int branch = -1;
+    double widest = opt.integrality_tol;
+    for (int j = 0; j < n; ++j) {
+      if (!integer[j]) continue;
+      const double f = x[j] - std::floor(x[j]);
+      const double frac = std::min(f, 1.0 - f);
+      if (frac > widest) {
+        widest = frac;
+        branch = j;
+      }
+      x[j] = std::min(std::max(std::nearbyint(x[j]), node.lower[j]), node.upper[j]);
+    }
+    const bool ok = feasible(md, x, node, opt.feasibility_tol);
+    if (ok) {
+      const double f = sense * objective_of(md, x);
+      if (f < incumbent) {
+        incumbent = f;
+        out.col_value = x;
+        log("new incumbent");
+      }
+    }
+    if (branch < 0) {
+      // The relaxation's answer is integral: its rounding is the node's best point. If that point
+      // missed a row by more than the tolerance (an inexact relaxation), the node is counted as
+      // unresolved and its bound kept, rather than claimed.
+      ++out.integral;
+      if (!ok) {
+        ++out.unresolved;
+        closed_min = std::min(closed_min, lb);
+      }
+      return;
+    }
+    if (lb >= incumbent - gap_abs(incumbent)) {
+      ++out.pruned_bound;
+      if (lb < incumbent) closed_min = std::min(closed_min, lb);
+      return;
+    }
+    const double v = r.col_value[branch];
+    for (int side = 0; side < 2; ++side) {
+      auto child = std::make_unique<Node>(Node{lb, node.depth + 1, node.lower, node.upper});
+      if (side == 0)
+        child->upper[branch] = std::floor(v);
+      else
+        child->lower[branch] = std::ceil(v);
+      if (child->lower[branch] > child->upper[branch]) continue;
+      heap.push_back(std::move(child));
+      std::push_heap(heap.begin(), heap.end(), heap_less);
+    }
+  };
+  {
+    const Node root_node{-kInf, 0, md.col_lower, md.col_upper};
+    handle(root_node, root);
+  }
+
+  out.status = Status::kOptimal;
+  while (!heap.empty()) {
+    if (seconds() > opt.time_limit) {
+      out.status = Status::kTimeLimit;
+      break;
+    }
+    if (out.nodes >= opt.max_nodes) {
+      out.status = Status::kIterationLimit;
+      break;
+    }
+    // The wave: the open nodes with the smallest bounds that can still beat the incumbent.
+    std::vector<NodePtr> wave;
+    while (!heap.empty() && static_cast<int>(wave.size()) < opt.wave &&
+           out.nodes + static_cast<long long>(wave.size()) < opt.max_nodes) {
+      std::pop_heap(heap.begin(), heap.end(), heap_less);
+      NodePtr node = std::move(heap.back());
+      heap.pop_back();
+      if (node->bound >= incumbent - gap_abs(incumbent)) {
+        // Every open node's bound is at least this one's: all are pruned.
+        out.pruned_bound += 1 + static_cast<long long>(heap.size());
+        if (node->bound < incumbent) closed_min = std::min(closed_min, node->bound);
+        heap.clear();
+        break;
+      }
+      wave.push_back(std::move(node));
+    }
+    if (wave.empty()) break;
+    std::vector<const double*> lows, ups;
+    for (const NodePtr& node : wave) {
+      lows.push_back(node->lower.data());
+      ups.push_back(node->upper.data());
+    }
+    const std::vector<LpSolveResult> rs =
+        solve_lp_bounds_batch(md, lows, ups, lpo, start_x.data(), start_y.data());
+    ++out.waves;
+    out.nodes += static_cast<long long>(wave.size());
+    for (size_t k = 0; k < wave.size(); ++k) handle(*wave[k], rs[k]);
+    if (opt.log_level >= 1) {
+      double open_min = kInf;
+      for (const NodePtr& node : heap) open_min = std::min(open_min, node->bound);
+      std::printf("mip: wave %lld: %zu nodes, %zu open, incumbent %.9g, open bound %.9g, %.1f s\n", out.waves,
+                  wave.size(), heap.size(), std::isfinite(incumbent) ? sense * incumbent : NAN,
+                  std::isfinite(open_min) ? sense * open_min : NAN, seconds());
+      std::fflush(stdout);
+    }
+  }
+
+  // The bound: the incumbent, the open nodes and the nodes closed below the incumbent.
+  double bound = std::min(incumbent, closed_min);
+  for (const NodePtr& node : heap) bound = std::min(bound, node->bound);
+  if (std::isfinite(incumbent)) {
+    out.objective = sense * incumbent;
+    out.gap = (incumbent - bound) / (1.0 + std::fabs(incumbent));
+  }
+  out.bound = std::isfinite(bound) ? sense * bound : sense * kInf;
+  if (out.status == Status::kOptimal) {
+    if (!std::isfinite(incumbent))
+      out.status = out.unresolved > 0 ? Status::kNumericalError : Status::kInfeasible;
+    else if (out.gap > opt.gap_tol)
+      out.status = Status::kNumericalError;  // unresolved nodes hold the bound below the gap
+  }
+  out.seconds = seconds();
+  log("done");
+  return out;
+}
+
+}  // namespace smy::host

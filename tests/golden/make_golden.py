@@ -20,6 +20,8 @@ The restatement-to-reference speed ratio on one core (cpu_calibration.json):
 The reference's MPS reader on its instance files, reader test texts and generated models
 (mps_reference.json):
   python tests/golden/make_golden.py mps
+HiGHS's MILP optima of small knapsack MIPs for the branch and bound (mip_knapsack.json):
+  python tests/golden/make_golden.py mip
 """
 from __future__ import annotations
 
@@ -455,6 +457,36 @@ def highs_fixtures() -> None:
                                   "generators.c4()", "lps": lps})
 
 
+def mip_fixtures() -> None:
+    """Knapsack MIPs (generators.knapsack, integer columns with upper bound 1 or a general upper
+    bound) and a two-column MIP whose LP is feasible but has no integer point, solved by HiGHS's
+    MILP (scipy.optimize.milp): mip_knapsack.json, the branch and bound's parity fixture."""
+    import scipy.sparse as sp
+    from scipy.optimize import Bounds, LinearConstraint, milp
+    import samaya_amd as smy
+    G = smy.generators
+    cases = []
+    for seed, m, n, per_col, ub in [(7001, 3, 30, 3, 1), (7002, 5, 40, 3, 1), (7003, 4, 40, 3, 3),
+                                    (7004, 8, 50, 4, 1)]:
+        md = G.knapsack(seed, m, n, per_col)
+        A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+        up = np.full(md.n, float(ub))
+        c = -md.obj if md.maximize else md.obj
+        r = milp(c, constraints=LinearConstraint(A, md.row_lower, md.row_upper),
+                 bounds=Bounds(md.col_lower, up), integrality=np.ones(md.n))
+        assert r.status == 0
+        cases.append({"knapsack": [seed, m, n, per_col], "col_upper": ub, "status": 0,
+                      "objective": float(-r.fun if md.maximize else r.fun),
+                      "col_value": [float(round(t)) for t in r.x]})
+    # min x1 + x2 s.t. 2 x1 + 2 x2 = 1, x integer in [0, 1]: LP optimum 0.5, no integer point.
+    r = milp(np.array([1.0, 1.0]), constraints=LinearConstraint(np.array([[2.0, 2.0]]), [1.0], [1.0]),
+             bounds=Bounds([0.0, 0.0], [1.0, 1.0]), integrality=np.ones(2))
+    cases.append({"handmade": "2 x1 + 2 x2 = 1", "status": 1 if r.status == 2 else r.status})
+    write("mip_knapsack", {"source": "HiGHS MILP (scipy.optimize.milp) on generators.knapsack with "
+                                     "all columns integer, and a handmade integer-infeasible MIP",
+                           "cases": cases})
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "c2":
         c2_reference()
@@ -466,5 +498,7 @@ if __name__ == "__main__":
         mps_fixtures()
     elif len(sys.argv) > 1 and sys.argv[1] == "c2-highs":
         c2_highs(*sys.argv[2:3])
+    elif len(sys.argv) > 1 and sys.argv[1] == "mip":
+        mip_fixtures()
     else:
         main()
