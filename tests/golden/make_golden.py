@@ -482,9 +482,17 @@ def mip_fixtures() -> None:
     r = milp(np.array([1.0, 1.0]), constraints=LinearConstraint(np.array([[2.0, 2.0]]), [1.0], [1.0]),
              bounds=Bounds([0.0, 0.0], [1.0, 1.0]), integrality=np.ones(2))
     cases.append({"handmade": "2 x1 + 2 x2 = 1", "status": 1 if r.status == 2 else r.status})
+    # The bench's MIP workload (bench.py --workload mip).
+    md = G.knapsack(7101, 10, 60, 4)
+    A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))
+    t0 = time.perf_counter()
+    r = milp(-md.obj, constraints=LinearConstraint(A, md.row_lower, md.row_upper),
+             bounds=Bounds(md.col_lower, md.col_upper), integrality=np.ones(md.n))
+    bench = {"knapsack": [7101, 10, 60, 4], "status": int(r.status), "objective": float(-r.fun),
+             "seconds_this_container": time.perf_counter() - t0}
     write("mip_knapsack", {"source": "HiGHS MILP (scipy.optimize.milp) on generators.knapsack with "
                                      "all columns integer, and a handmade integer-infeasible MIP",
-                           "cases": cases})
+                           "cases": cases, "bench": bench})
 
 
 if __name__ == "__main__":
