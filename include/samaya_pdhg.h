@@ -186,14 +186,19 @@ int smy_pdhg_freeze(smy_pdhg* h, int instance);
  * 4 and 5 the counts of their infinite terms; 6 scale (max |t bound|, |t| > 1e-9, t = +-w/y_max);
  * 7 d_max = max |v_new - v|; 8 ||A x - r||^2 of the normalized difference; 9 its cost slope; 10 the
  * number of bound failures of the projected ray and of A times it. The host decides as the
- * reference does from these. flags[i] = 2 or 3 (no reference counterpart): the Farkas sums of the
- * dual's sign projection instead, its entries that make a row's term of f_min (2) or f_max (3)
- * infinite set to 0 (a y that passes is a Farkas certificate all the same). The ray test is
- * skipped, its slots 7-10 left 0, when no instance with flag 1 has a structural variable with an
- * infinite bound (a bounded LP has no improving ray). Flags above 3 are invalid. Synchronizes. */
+ * reference does from these. Other Farkas candidates (no reference counterpart; a y that passes
+ * the test is a Farkas certificate all the same): flags[i] = 2 or 3, the sums of the dual's sign
+ * projection, its entries that make a row's term of f_min (2) or f_max (3) infinite set to 0;
+ * flags 5, 6, 7: the same for the dual's change since the restart point, y_erg - y_rst, as it is
+ * (5) or projected (6, 7). The ray test is skipped, its slots 7-10 left 0, when no instance with
+ * flag 1 has a structural variable with an infinite bound (a bounded LP has no improving ray).
+ * Flags 4 and above 7 are invalid. Synchronizes. */
 int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out);
 /* The projected ray (n_i structural entries) of instance i's last detection. */
 int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray);
+/* The dual (m_i entries, the y_erg space) that the last detection with a flag above 1 tested for
+ * instance i: the Farkas certificate when its test passed. No reference counterpart. */
+int smy_pdhg_download_farkas(smy_pdhg* h, int instance, double* y);
 
 /* Adaptive step sizes (no reference counterpart: the reference's adaptive step is disabled code,
  * src/lp/pdlp.cpp:1051-1067). PDLP's rule (Applegate et al., NeurIPS 2021, sec. 3.1): an attempt

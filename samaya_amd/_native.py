@@ -28,7 +28,7 @@ EXPORTS = (
     "smy_pdhg_set_start_at", "smy_pdhg_download_at", "smy_pdhg_run_batch",
     "smy_pdhg_evaluate_batch", "smy_pdhg_restart_distance_batch", "smy_pdhg_restart_to_batch",
     "smy_pdhg_set_precision_batch", "smy_pdhg_freeze", "smy_pdhg_create_shared",
-    "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_run_adaptive",
+    "smy_pdhg_detect_batch", "smy_pdhg_download_ray", "smy_pdhg_download_farkas", "smy_pdhg_run_adaptive",
     "smy_pdhg_get_tile", "smy_pdhg_set_tile", "smy_pdhg_get_bin_tile", "smy_pdhg_set_bin_tile",
     "smy_pdhg_set_fp64_accumulation", "smy_pdhg_time_adaptive_kernels", "smy_pdhg_inject_fault",
     "smy_pdhg_equilibrate", "smy_pdlp_equilibrate", "smy_lp_dual_bound",
@@ -172,6 +172,7 @@ def load() -> C.CDLL:
     _proto(lib, "smy_pdhg_freeze", i32, vp, i32)
     _proto(lib, "smy_pdhg_detect_batch", i32, vp, vp, vp)
     _proto(lib, "smy_pdhg_download_ray", i32, vp, i32, vp)
+    _proto(lib, "smy_pdhg_download_farkas", i32, vp, i32, vp)
     _proto(lib, "smy_pdhg_run_adaptive", i32, vp, vp, i32)
     _proto(lib, "smy_pdhg_set_halpern", i32, vp, i32, f64)
     _proto(lib, "smy_pdhg_fixed_point_batch", i32, vp, vp)

@@ -34,6 +34,7 @@ struct DetectView {
   const T* __restrict__ v;
   const T* __restrict__ v_new;
   const T* __restrict__ y_erg;
+  const T* __restrict__ y_rst;           // the restart point's duals (flags 5-7)
   const double* __restrict__ rs;
   const double* __restrict__ cs;
   const double* __restrict__ cost;       // per variable (shared by the instances when Shared)
@@ -67,12 +68,15 @@ struct DetectView {
   __device__ bool skip(int inst) const { return flags[inst] <= 0; }
 };
 
-// Before phase 1 when some flag is 2 or 3: the dual the Farkas test takes, into a copy (ydst, the
-// layout of y_erg) that the phases then read for every flagged instance: y_erg itself (flag 1), or
-// its sign projection for the f_min (2) or f_max (3) test: the entry of a row whose term in that
-// sum would be infinite (f_min: -yf_i > 0 against an infinite lower bound of the row's logical
-// variable, or -yf_i < 0 against an infinite upper bound; f_max: the opposite signs) set to 0. The
-// reference tests y_erg only (pdlp.cpp:724-842); every y that passes the test is a certificate.
+// Before phase 1 when some flag is above 1: the dual the Farkas test takes, into a copy (ydst, the
+// layout of y_erg) that the phases then read for every flagged instance. Flag bit 2 (4): the dual's
+// change since the restart point, y_erg - y_rst (on an infeasible LP the iterates drift along the
+// certificate's direction and the difference sheds the point they started from), else y_erg.
+// Flag & 3 = 1: that vector as it is; 2 or 3: its sign projection for the f_min (2) or f_max (3)
+// test, the entry of a row whose term in that sum would be infinite (f_min: -yf_i > 0 against an
+// infinite lower bound of the row's logical variable, or -yf_i < 0 against an infinite upper
+// bound; f_max: the opposite signs) set to 0. The reference tests y_erg only (pdlp.cpp:724-842);
+// every y that passes the test is a certificate.
 template <typename T, bool Shared>
 struct FarkasProjectOp : DetectView<T, Shared> {
   static constexpr int kSums = 0;
@@ -81,13 +85,15 @@ struct FarkasProjectOp : DetectView<T, Shared> {
   __device__ int flag(int inst) const { return this->flags[inst] > 0 ? this->flags[inst] : -1; }
   __device__ void row_a(int i, int k, int f, double*, int inst) const {
     const size_t o = this->at(i, inst);
-    const T y = this->y_erg[o];
+    double y = static_cast<double>(this->y_erg[o]);
+    if (f & 4) y -= static_cast<double>(this->y_rst[o]);
+    const int orient = f & 3;
     bool cut = false;
-    if (f >= 2) {
-      const double s = f == 2 ? static_cast<double>(y) : -static_cast<double>(y);  // as for f_min
+    if (orient >= 2) {
+      const double s = orient == 2 ? y : -y;  // as for f_min
       cut = (s < 0.0 && !(this->lo(k, inst) > -INFINITY)) || (s > 0.0 && !(this->up(k, inst) < INFINITY));
     }
-    ydst[o] = cut ? T(0) : y;
+    ydst[o] = cut ? T(0) : static_cast<T>(y);
   }
   __device__ void col(int, int, double*, int) const {}
 };

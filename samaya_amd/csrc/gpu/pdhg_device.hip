@@ -1516,6 +1516,7 @@ Op<T, false> stacked_detect_op(smy_pdhg* h, int slot) {
   op.v = S.v[h->p];
   op.v_new = S.v[1 - h->p];
   op.y_erg = h->det_projected ? det_y<T>(h) : S.y_erg;
+  op.y_rst = S.y_rst;
   op.rs = h->rs;
   op.cs = h->cs;
   op.cost = h->cost_o;
@@ -1596,6 +1597,7 @@ int stacked_detect(smy_pdhg* h, const int* flags, double* out) {
   ensure_detect(h);
   stage_flags(h, flags);
   h->det_projected = std::any_of(flags, flags + h->K, [](int f) { return f >= 2; });
+  if (h->det_projected) h->det_y_ready = true;
   h->det_ray_test = false;
   for (int i = 0; i < h->K; ++i) h->det_ray_test = h->det_ray_test || (flags[i] == 1 && !h->inst[i].boxed);
   const bool f32 = any_flag(h, flags, SMY_FP32), f64 = any_flag(h, flags, SMY_FP64);
@@ -2080,7 +2082,7 @@ int smy_pdhg_set_start_at(smy_pdhg* h, int instance, const double* v, const doub
 int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out) {
   if (h == nullptr || flags == nullptr || out == nullptr) return fail(SMY_INVALID, "null argument");
   for (int i = 0; i < h->K; ++i)
-    if (flags[i] > 3) return fail(SMY_INVALID, "detection flags are 0, 1, 2 or 3");
+    if (flags[i] == 4 || flags[i] > 7) return fail(SMY_INVALID, "detection flags are 0-3 and 5-7");
   for (int i = 0; i < h->K; ++i)
     if (flags[i] > 0 && (h->inst[i].prec == kFrozen || !h->inst[i].has_original))
       return fail(SMY_STATE, "detection needs a running instance with set_original()");
@@ -2097,6 +2099,27 @@ int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray) {
     const Layout& L = h->inst[instance];
     if (L.n > 0)
       SMY_HIP(hipMemcpy(ray, h->det_ray + L.ox, L.n * sizeof(double), hipMemcpyDeviceToHost));
+    return SMY_OK;
+  });
+}
+
+int smy_pdhg_download_farkas(smy_pdhg* h, int instance, double* y) {
+  SMY_INSTANCE(h, instance);
+  if (y == nullptr) return fail(SMY_INVALID, "null argument");
+  if (!h->det_y_ready) return fail(SMY_STATE, "no detection with a flag above 1 has run");
+  if (h->shared) return guarded([&]() -> int { return shared_download_farkas(h, instance, y); });
+  return guarded([&]() -> int {
+    const Layout& L = h->inst[instance];
+    if (L.m == 0) return SMY_OK;
+    if (L.last_prec == SMY_FP64) {
+      if (h->det_y64 == nullptr) return fail(SMY_STATE, "no FP64 detection dual");
+      SMY_HIP(hipMemcpy(y, h->det_y64 + L.oy, L.m * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      if (h->det_y32 == nullptr) return fail(SMY_STATE, "no FP32 detection dual");
+      std::vector<float> t(static_cast<size_t>(L.m));
+      SMY_HIP(hipMemcpy(t.data(), h->det_y32 + L.oy, L.m * sizeof(float), hipMemcpyDeviceToHost));
+      std::copy(t.begin(), t.end(), y);
+    }
     return SMY_OK;
   });
 }

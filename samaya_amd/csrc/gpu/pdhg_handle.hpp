@@ -270,6 +270,7 @@ struct smy_pdhg {
   double* det_y64 = nullptr;
   float* det_y32 = nullptr;
   bool det_projected = false;
+  bool det_y_ready = false;  // det_y holds the duals of a detection with flags above 1
   // Whether the running detection takes the ray test: some instance with flag 1 is not boxed (the
   // ray slots of the others stay 0, so their test fails as it would).
   bool det_ray_test = true;
@@ -377,6 +378,7 @@ int shared_set_start(smy_pdhg* h, int k, const double* v, const double* y);
 int shared_time_kernels(smy_pdhg* h, int reps, double* dual_ms, double* primal_ms);
 int shared_detect(smy_pdhg* h, const int* flags, double* out);
 int shared_download_ray(smy_pdhg* h, int k, double* ray);
+int shared_download_farkas(smy_pdhg* h, int k, double* y);
 int shared_rule_decide(smy_pdhg* h, const smy_pdhg_rule_in* in, long long iter1, smy_pdhg_rule_out* out);
 
 }  // namespace smy

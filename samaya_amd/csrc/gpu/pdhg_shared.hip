@@ -786,6 +786,7 @@ Op<T, true> shared_detect_op(smy_pdhg* h, int slot) {
   op.v = S.v[h->p];
   op.v_new = S.v[1 - h->p];
   op.y_erg = h->det_projected ? det_y_shared<T>(h) : S.y_erg;
+  op.y_rst = S.y_rst;
   op.rs = s->rs;
   op.cs = s->cs;
   op.cost = s->cost_o;
@@ -868,6 +869,7 @@ int shared_detect(smy_pdhg* h, const int* flags, double* out) {
   ensure_detect_shared(h);
   stage_which(h, flags);
   h->det_projected = std::any_of(flags, flags + s->K, [](int f) { return f >= 2; });
+  if (h->det_projected) h->det_y_ready = true;
   h->det_ray_test = false;
   for (int k = 0; k < s->K; ++k) h->det_ray_test = h->det_ray_test || (flags[k] == 1 && !h->inst[k].boxed);
   const bool f32 = any_which(s, flags, SMY_FP32), f64 = any_which(s, flags, SMY_FP64);
@@ -918,6 +920,18 @@ int shared_detect(smy_pdhg* h, const int* flags, double* out) {
 
 int shared_download_ray(smy_pdhg* h, int k, double* ray) {
   column_to_host<double>(h, h->det_ray, h->shared->n, k, ray);
+  return SMY_OK;
+}
+
+int shared_download_farkas(smy_pdhg* h, int k, double* y) {
+  SharedBatch* s = h->shared;
+  if (s->state_prec[k] == SMY_FP64) {
+    if (h->det_y64 == nullptr) return fail(SMY_STATE, "no FP64 detection dual");
+    column_to_host<double>(h, h->det_y64, s->m, k, y);
+  } else {
+    if (h->det_y32 == nullptr) return fail(SMY_STATE, "no FP32 detection dual");
+    column_to_host<float>(h, h->det_y32, s->m, k, y);
+  }
   return SMY_OK;
 }
 

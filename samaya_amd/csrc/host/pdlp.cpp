@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <exception>
+#include <functional>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -177,6 +178,10 @@ class Device {
   void download_ray(int i, std::vector<double>& ray, int n) {
     ray.assign(static_cast<size_t>(n), 0.0);
     check(smy_pdhg_download_ray(h_, i, ray.data()));
+  }
+  void download_farkas(int i, std::vector<double>& y, int m) {
+    y.assign(static_cast<size_t>(m), 0.0);
+    check(smy_pdhg_download_farkas(h_, i, y.data()));
   }
   void advance() { check(smy_pdhg_advance(h_)); }
   void download(int i, std::vector<double>* v, std::vector<double>* v_new, std::vector<double>* y,
@@ -363,8 +368,9 @@ class Lane {
   // The same decisions from the device-side totals (11 numbers, smy_pdhg_detect_batch).
   bool farkas_from(long long iter, const double* o, const char* what = "") const;
   bool ray_from(long long iter, const double* o) const;
-  // The Farkas test's decision on the totals, without logging.
+  // The Farkas and ray tests' decisions on the totals, without logging.
   static bool farkas_passes(const double* o);
+  static bool ray_passes(const double* o);
   // The sign projection of a scaled dual for the f_min (orient 2) or f_max (3) test: entries that
   // make a row's term infinite set to 0 (smy_pdhg_detect_batch's flags 2 and 3).
   std::vector<double> sign_projected(const std::vector<double>& y, int orient) const;
@@ -684,6 +690,11 @@ bool Lane::detect_ray(long long iter, const std::vector<double>& v, const std::v
   return false;
 }
 
+bool Lane::ray_passes(const double* o) {
+  // src/lp/pdlp.cpp:852-1042 from the device totals.
+  return o[7] > 1e-6 && std::sqrt(o[8]) < 1e-6 && o[9] < -1e-8 && o[10] == 0.0;
+}
+
 bool Lane::farkas_passes(const double* o) {
   // src/lp/pdlp.cpp:724-842 from the device totals.
   if (!(o[1] > 1e-6)) return false;
@@ -707,9 +718,7 @@ bool Lane::farkas_from(long long iter, const double* o, const char* what) const 
 }
 
 bool Lane::ray_from(long long iter, const double* o) const {
-  // src/lp/pdlp.cpp:852-1042 from the device totals.
-  if (!(o[7] > 1e-6)) return false;
-  if (std::sqrt(o[8]) < 1e-6 && o[9] < -1e-8 && o[10] == 0.0) {
+  if (ray_passes(o)) {
     log(1, ": detected dual infeasibility (unbounded) at iter %lld (improving ray)", iter);
     return true;
   }
@@ -1052,28 +1061,45 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         any_detect = true;
       }
     if (any_detect && opt.device_detect) device->detect(dflag, dout);
-    // farkas_projection: the lanes whose Farkas test failed on infinite terms test the sign
-    // projection of their dual for f_min (flag 2), then for f_max (3), in one pass each.
-    std::vector<int> projected(static_cast<size_t>(K), 0);
+    // farkas_projection: the lanes whose Farkas test failed test other candidates, in one device
+    // pass per candidate for all of them (smy_pdhg_detect_batch flags): the sign projection of
+    // their dual for f_min (2) or f_max (3) when that sum had infinite terms; the dual's change
+    // since the restart point as it is (5), then projected (6, 7) when that had infinite terms.
+    // The first that passes is the certificate (downloaded as tested).
+    std::vector<int> found(static_cast<size_t>(K), 0);
+    std::vector<std::vector<double>> certificate(static_cast<size_t>(K));
     if (any_detect && opt.device_detect && opt.farkas_projection) {
-      for (int orient : {2, 3}) {
+      std::vector<double> last = dout;  // per lane: the numbers of its last candidate
+      const auto pass = [&](const std::function<int(int, const double*)>& pick) {
         std::vector<int> pflag(static_cast<size_t>(K), 0);
         bool any = false;
         for (int i = 0; i < K; ++i) {
-          const double* o = &dout[static_cast<size_t>(i) * kDetectSlotsHost];
-          if (dflag[i] && projected[i] == 0 && !Lane::farkas_passes(o) && o[1] > 1e-6 &&
-              o[orient == 2 ? 4 : 5] > 0.0) {
-            pflag[i] = orient;
-            any = true;
+          const double* o = &last[static_cast<size_t>(i) * kDetectSlotsHost];
+          const double* r = &dout[static_cast<size_t>(i) * kDetectSlotsHost];
+          if (dflag[i] && found[i] == 0 && !Lane::farkas_passes(r) && (lanes[i].boxed || !Lane::ray_passes(r))) {
+            pflag[i] = pick(i, o);
+            any = any || pflag[i] != 0;
           }
         }
-        if (!any) continue;
+        if (!any) return;
         std::vector<double> pout;
         device->detect(pflag, pout);
-        for (int i = 0; i < K; ++i)
-          if (pflag[i] && Lane::farkas_passes(&pout[static_cast<size_t>(i) * kDetectSlotsHost]))
-            projected[i] = orient;
-      }
+        for (int i = 0; i < K; ++i) {
+          if (pflag[i] == 0) continue;
+          const double* o = &pout[static_cast<size_t>(i) * kDetectSlotsHost];
+          std::copy(o, o + kDetectSlotsHost, last.begin() + static_cast<ptrdiff_t>(i) * kDetectSlotsHost);
+          if (Lane::farkas_passes(o)) {
+            found[i] = pflag[i];
+            device->download_farkas(i, certificate[i], lanes[i].lp_.m);
+          }
+        }
+      };
+      const auto raw = [&](int i) { return &dout[static_cast<size_t>(i) * kDetectSlotsHost]; };
+      pass([&](int i, const double*) { return raw(i)[1] > 1e-6 && raw(i)[4] > 0.0 ? 2 : 0; });
+      pass([&](int i, const double*) { return raw(i)[1] > 1e-6 && raw(i)[5] > 0.0 ? 3 : 0; });
+      pass([&](int, const double*) { return 5; });
+      pass([&](int, const double* o) { return o[1] > 1e-6 && o[4] > 0.0 ? 6 : 0; });
+      pass([&](int, const double* o) { return o[1] > 1e-6 && o[5] > 0.0 ? 7 : 0; });
     }
     for (int i = 0; any_detect && i < K; ++i) {
       if (!dflag[i]) continue;
@@ -1087,10 +1113,11 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
           L.res.dual_ray = y_erg;
           L.res.status = Status::kInfeasible;
           done = true;
-        } else if (projected[i] != 0) {
-          L.log(1, ": detected primal infeasibility at iter %lld (Farkas certificate, sign-projected dual)", iter);
-          device->download(i, nullptr, nullptr, nullptr, nullptr, nullptr, &y_erg, nt, m);
-          L.res.dual_ray = L.sign_projected(y_erg, projected[i]);
+        } else if (found[i] != 0) {
+          L.log(1, ": detected primal infeasibility at iter %lld (Farkas certificate, %s)", iter,
+                found[i] < 4 ? "sign-projected dual" : found[i] == 5 ? "the dual's change since the restart"
+                                                                     : "sign-projected change of the dual");
+          L.res.dual_ray = std::move(certificate[i]);
           L.res.status = Status::kInfeasible;
           done = true;
         } else if (!L.boxed && L.ray_from(iter, o)) {

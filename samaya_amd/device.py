@@ -345,6 +345,12 @@ class PdhgBatch:
         N.check(self.lib.smy_pdhg_download_ray(self.h, i, _p(ray)), "download_ray")
         return ray
 
+    def download_farkas(self, i: int) -> np.ndarray:
+        """The dual the last detection with a flag above 1 tested for instance i."""
+        y = np.zeros(self.sizes[i][1])
+        N.check(self.lib.smy_pdhg_download_farkas(self.h, i, _p(y)), "download_farkas")
+        return y
+
     def download(self, i: int):
         """dict of v, v_new, y, y_new, v_erg, y_erg of instance i (host copies)."""
         n, m = self.sizes[i]

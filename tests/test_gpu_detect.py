@@ -128,20 +128,32 @@ def test_detection_numbers_match_the_reference_formulas(gpu, layout):
         assert close(got[k], want), (k, got[k], want)
         assert close(b.download_ray(k), ray)
     # Flags 2 and 3: the Farkas numbers of the dual's sign projection (rows whose term of f_min,
-    # resp. f_max, would be infinite set to 0); no ray test.
-    for f in (2, 3):
+    # resp. f_max, would be infinite set to 0); 5, 6, 7: the same for the dual's change since the
+    # restart point; no ray test. The tested dual comes back with download_farkas.
+    rst = [b.download(k)["y_erg"].copy() for k in range(K)]  # the averages become y_rst
+    b.restart_to([1, -1, 1])
+    b.advance()
+    b.run([0.3, 0.2, 0.25], [0.2, 0.35, 0.3], 1.0, [0, 71, 0], 9)
+    b.advance()
+    b.run([0.3, 0.2, 0.25], [0.2, 0.35, 0.3], 1.0, [9, 80, 9], 5)
+    for f in (2, 3, 5, 6, 7):
         got = b.detect([f, 0, f])
         for k in (0, 2):
             st = b.download(k)
             y = st["y_erg"].copy()
-            s = y if f == 2 else -y
-            cut = ((s < 0) & ~np.isfinite(lows[k][n:])) | ((s > 0) & ~np.isfinite(ups[k][n:]))
-            y[cut] = 0.0
+            if f & 4:
+                y = y - rst[k]
+            s = y if (f & 3) != 3 else -y
+            if (f & 3) >= 2:
+                cut = ((s < 0) & ~np.isfinite(lows[k][n:])) | ((s > 0) & ~np.isfinite(ups[k][n:]))
+                y[cut] = 0.0
             want, _ = reference_numbers(lp, lows[k], ups[k], rs, cs, st["v"], st["v_new"], y)
             assert close(got[k][:7], want[:7]), (f, k, got[k], want)
             assert np.all(got[k][7:] == 0.0)
-    with pytest.raises(RuntimeError):
-        b.detect([4, 0, 0])
+            assert close(b.download_farkas(k), y)
+    for bad in (4, 8):
+        with pytest.raises(RuntimeError):
+            b.detect([bad, 0, 0])
 
 
 def test_bounded_instances_skip_the_ray_test(gpu):
