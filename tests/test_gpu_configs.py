@@ -101,11 +101,11 @@ def test_c2_fp64_reference_algorithm(gpu, c2_model):
           f"measures {r.primal_residual:.2e} {r.dual_residual:.2e} {r.gap:.2e}, dual bound {r.dual_bound!r}")
     assert r.status == smy.OPTIMAL == ref["status"]
     assert_within_tolerance(md, r)
-    # The same algorithm from the same start: the GPU sums in another order, so the runs part at
-    # the rounding level, but they stop within a few evaluation periods of each other, at points
-    # whose objectives agree to the tolerance.
-    assert abs(r.iterations - ref["iterations"]) <= 0.1 * ref["iterations"]
-    assert rel(r.objective, ref["objective"]) <= 1e-6
+    # The same algorithm from the same start. The GPU sums in another order, so in general the runs
+    # part at the rounding level; on this LP they keep the reference's restart schedule and stop at
+    # its iteration (100,352, measured), with objectives equal to ~1e-16.
+    assert r.iterations == ref["iterations"]
+    assert rel(r.objective, ref["objective"]) <= 1e-9
     # Weak duality: the bound from the answer's duals lies below the objective (minimization).
     assert r.dual_bound <= r.objective + 1e-9 * abs(r.objective)
     assert rel(r.dual_bound, r.objective) <= 1e-4
