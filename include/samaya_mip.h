@@ -38,7 +38,9 @@ typedef struct smy_mip_result {
   long long pruned_infeasible;
   long long pruned_bound;
   long long integral;       /* relaxations with an integral answer */
-  long long unresolved;
+  long long unresolved;     /* relaxations left without a usable point (their bounds kept) */
+  long long inexact;        /* relaxations stopped short of optimality, branched on their point
+                               with the parent's bound */
   long long lp_iterations;  /* PDLP iterations over all relaxations */
   double seconds;
   char message[256];        /* why a solve failed (SMY_ERROR) */

@@ -512,6 +512,7 @@ int smy_mip_solve(const smy_model* model, const unsigned char* integer, const sm
     result->pruned_bound = r.pruned_bound;
     result->integral = r.integral;
     result->unresolved = r.unresolved;
+    result->inexact = r.inexact;
     result->lp_iterations = r.lp_iterations;
     result->seconds = r.seconds;
     if (col_value != nullptr && !r.col_value.empty()) put(r.col_value, col_value);

@@ -109,12 +109,18 @@ MipResult solve_mip(const Model& md, const std::vector<char>& integer, const Mip
       ++out.pruned_infeasible;
       return;
     }
-    if (r.status != Status::kOptimal || r.col_value.size() != static_cast<size_t>(n)) {
+    if (r.col_value.size() != static_cast<size_t>(n)) {
       ++out.unresolved;
       closed_min = std::min(closed_min, node.bound);
       return;
     }
-    const double lb = std::max(node.bound, std::isfinite(r.dual_bound) ? sense * r.dual_bound : -kInf);
+    // A relaxation stopped short of optimality (a limit, or a ray test misled on a node that a
+    // bounded root makes bounded) still has a point: the node branches on it with its parent's
+    // bound, so no part of the search is dropped.
+    const bool exact = r.status == Status::kOptimal;
+    if (!exact) ++out.inexact;
+    const double lb =
+        std::max(node.bound, exact && std::isfinite(r.dual_bound) ? sense * r.dual_bound : -kInf);
     if (lb >= incumbent - gap_abs(incumbent)) {
       ++out.pruned_bound;
       if (lb < incumbent) closed_min = std::min(closed_min, lb);
@@ -229,7 +235,9 @@ MipResult solve_mip(const Model& md, const std::vector<char>& integer, const Mip
     out.objective = sense * incumbent;
     out.gap = (incumbent - bound) / (1.0 + std::fabs(incumbent));
   }
-  out.bound = std::isfinite(bound) ? sense * bound : sense * kInf;
+  // +inf (minimization sense) when every node was pruned infeasible; -inf when an unresolved
+  // node had no bound of its own.
+  out.bound = sense * bound;
   if (out.status == Status::kOptimal) {
     if (!std::isfinite(incumbent))
       out.status = out.unresolved > 0 ? Status::kNumericalError : Status::kInfeasible;

@@ -39,7 +39,9 @@ struct MipResult {
   long long pruned_infeasible = 0;
   long long pruned_bound = 0;     // at their solve, or before it (bound no better than the incumbent)
   long long integral = 0;         // relaxations whose answer was integral
-  long long unresolved = 0;       // relaxations that stopped without an answer (limits, errors)
+  long long unresolved = 0;       // relaxations left without a point, or with an integral one that
+                                  // missed a row (their bounds stay in the search's bound)
+  long long inexact = 0;          // relaxations stopped short of optimality, branched on their point
   long long lp_iterations = 0;
   double seconds = 0.0;
 };

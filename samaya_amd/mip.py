@@ -55,6 +55,7 @@ class MipResult:
     pruned_bound: int
     integral: int
     unresolved: int
+    inexact: int
     lp_iterations: int
     seconds: float
     message: str
@@ -78,4 +79,4 @@ def solve_mip(model: Model | NativeModel, integer=None, options: MipOptions | No
     has = not math.isnan(res.objective)
     return MipResult(res.status, res.objective, res.bound, res.gap, x if has else None, res.nodes,
                      res.waves, res.pruned_infeasible, res.pruned_bound, res.integral, res.unresolved,
-                     res.lp_iterations, res.seconds, res.message.decode(errors="replace"))
+                     res.inexact, res.lp_iterations, res.seconds, res.message.decode(errors="replace"))
