@@ -1069,15 +1069,14 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
     std::vector<int> found(static_cast<size_t>(K), 0);
     std::vector<std::vector<double>> certificate(static_cast<size_t>(K));
     if (any_detect && opt.device_detect && opt.farkas_projection) {
-      std::vector<double> last = dout;  // per lane: the numbers of its last candidate
-      const auto pass = [&](const std::function<int(int, const double*)>& pick) {
+      std::vector<double> diff(dout.size(), 0.0);  // per lane: the numbers of candidate 5
+      const auto pass = [&](const std::function<int(int)>& pick) {
         std::vector<int> pflag(static_cast<size_t>(K), 0);
         bool any = false;
         for (int i = 0; i < K; ++i) {
-          const double* o = &last[static_cast<size_t>(i) * kDetectSlotsHost];
           const double* r = &dout[static_cast<size_t>(i) * kDetectSlotsHost];
           if (dflag[i] && found[i] == 0 && !Lane::farkas_passes(r) && (lanes[i].boxed || !Lane::ray_passes(r))) {
-            pflag[i] = pick(i, o);
+            pflag[i] = pick(i);
             any = any || pflag[i] != 0;
           }
         }
@@ -1087,7 +1086,8 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         for (int i = 0; i < K; ++i) {
           if (pflag[i] == 0) continue;
           const double* o = &pout[static_cast<size_t>(i) * kDetectSlotsHost];
-          std::copy(o, o + kDetectSlotsHost, last.begin() + static_cast<ptrdiff_t>(i) * kDetectSlotsHost);
+          if (pflag[i] == 5)
+            std::copy(o, o + kDetectSlotsHost, diff.begin() + static_cast<ptrdiff_t>(i) * kDetectSlotsHost);
           if (Lane::farkas_passes(o)) {
             found[i] = pflag[i];
             device->download_farkas(i, certificate[i], lanes[i].lp_.m);
@@ -1095,11 +1095,12 @@ std::vector<PdlpResult> run_lanes(std::vector<Lane>& lanes, const PdlpOptions& o
         }
       };
       const auto raw = [&](int i) { return &dout[static_cast<size_t>(i) * kDetectSlotsHost]; };
-      pass([&](int i, const double*) { return raw(i)[1] > 1e-6 && raw(i)[4] > 0.0 ? 2 : 0; });
-      pass([&](int i, const double*) { return raw(i)[1] > 1e-6 && raw(i)[5] > 0.0 ? 3 : 0; });
-      pass([&](int, const double*) { return 5; });
-      pass([&](int, const double* o) { return o[1] > 1e-6 && o[4] > 0.0 ? 6 : 0; });
-      pass([&](int, const double* o) { return o[1] > 1e-6 && o[5] > 0.0 ? 7 : 0; });
+      const auto dif = [&](int i) { return &diff[static_cast<size_t>(i) * kDetectSlotsHost]; };
+      pass([&](int i) { return raw(i)[1] > 1e-6 && raw(i)[4] > 0.0 ? 2 : 0; });
+      pass([&](int i) { return raw(i)[1] > 1e-6 && raw(i)[5] > 0.0 ? 3 : 0; });
+      pass([&](int) { return 5; });
+      pass([&](int i) { return dif(i)[1] > 1e-6 && dif(i)[4] > 0.0 ? 6 : 0; });
+      pass([&](int i) { return dif(i)[1] > 1e-6 && dif(i)[5] > 0.0 ? 7 : 0; });
     }
     for (int i = 0; any_detect && i < K; ++i) {
       if (!dflag[i]) continue;
