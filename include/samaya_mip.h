@@ -41,6 +41,8 @@ typedef struct smy_mip_result {
   long long unresolved;     /* relaxations left without a usable point (their bounds kept) */
   long long inexact;        /* relaxations stopped short of optimality, branched on their point
                                with the parent's bound */
+  long long fixed_solves;   /* batches re-solving integral nodes' continuous parts (integers
+                               fixed, FP64, tolerance <= 1e-9) whose rounding missed a row */
   long long lp_iterations;  /* PDLP iterations over all relaxations */
   double seconds;
   char message[256];        /* why a solve failed (SMY_ERROR) */

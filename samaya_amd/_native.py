@@ -103,6 +103,7 @@ class SmyMipResult(C.Structure):
                 ("gap", C.c_double), ("nodes", C.c_longlong), ("waves", C.c_longlong),
                 ("pruned_infeasible", C.c_longlong), ("pruned_bound", C.c_longlong),
                 ("integral", C.c_longlong), ("unresolved", C.c_longlong), ("inexact", C.c_longlong),
+                ("fixed_solves", C.c_longlong),
                 ("lp_iterations", C.c_longlong), ("seconds", C.c_double),
                 ("message", C.c_char * 256)]
 

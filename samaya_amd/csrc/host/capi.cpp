@@ -513,6 +513,7 @@ int smy_mip_solve(const smy_model* model, const unsigned char* integer, const sm
     result->integral = r.integral;
     result->unresolved = r.unresolved;
     result->inexact = r.inexact;
+    result->fixed_solves = r.fixed_solves;
     result->lp_iterations = r.lp_iterations;
     result->seconds = r.seconds;
     if (col_value != nullptr && !r.col_value.empty()) put(r.col_value, col_value);

@@ -102,6 +102,47 @@ MipResult solve_mip(const Model& md, const std::vector<char>& integer, const Mip
   const std::vector<double> start_x = root.col_value, start_y = root.row_dual;
 
   std::vector<NodePtr> heap;
+  std::vector<NodePtr> to_fix;  // integral relaxations whose rounding missed a row
+  // The continuous part of the nodes in to_fix, integer columns fixed, solved in one batch to a
+  // tighter tolerance in FP64: a row-feasible answer is the node's optimum (an incumbent
+  // candidate); otherwise the node stays unresolved with its bound.
+  LpSolveOptions fix_opts = opt.lp;
+  fix_opts.pdlp.tol = std::min(opt.lp.pdlp.tol, 1e-9);
+  fix_opts.pdlp.precision = Precision::kFp64;
+  fix_opts.pdlp.halpern = false;
+  fix_opts.pdlp.adaptive_steps = false;
+  fix_opts.verify = true;  // polish rounds follow a rejected answer
+  fix_opts.polish_rounds = std::max(opt.lp.polish_rounds, 2);
+  const auto fix_and_solve = [&]() {
+    if (to_fix.empty()) return;
+    std::vector<const double*> lows, ups;
+    for (const NodePtr& node : to_fix) {
+      lows.push_back(node->lower.data());
+      ups.push_back(node->upper.data());
+    }
+    const std::vector<LpSolveResult> rs =
+        solve_lp_bounds_batch(md, lows, ups, fix_opts, start_x.data(), start_y.data());
+    ++out.fixed_solves;
+    for (size_t k = 0; k < to_fix.size(); ++k) {
+      const Node& node = *to_fix[k];
+      const LpSolveResult& r = rs[k];
+      out.lp_iterations += r.iterations;
+      const bool ok = r.status == Status::kOptimal && r.col_value.size() == static_cast<size_t>(n) &&
+                      feasible(md, r.col_value, node, opt.feasibility_tol);
+      if (ok) {
+        const double f = sense * objective_of(md, r.col_value);
+        if (f < incumbent) {
+          incumbent = f;
+          out.col_value = r.col_value;
+          log("new incumbent (integer columns fixed)");
+        }
+      } else if (r.status != Status::kInfeasible) {
+        ++out.unresolved;
+        closed_min = std::min(closed_min, node.bound);
+      }
+    }
+    to_fix.clear();
+  };
   // A solved relaxation of `node`: prune, update the incumbent, or branch.
   const auto handle = [&](const Node& node, const LpSolveResult& r) {
     if (&r != &root) out.lp_iterations += r.iterations;
@@ -150,12 +191,14 @@ MipResult solve_mip(const Model& md, const std::vector<char>& integer, const Mip
     }
     if (branch < 0) {
       // The relaxation's answer is integral: its rounding is the node's best point. If that point
-      // missed a row by more than the tolerance (an inexact relaxation), the node is counted as
-      // unresolved and its bound kept, rather than claimed.
+      // missed a row by more than the tolerance (the continuous columns keep PDLP's accuracy), the
+      // node's continuous part is solved again with its integer columns fixed (fix_and_solve).
       ++out.integral;
       if (!ok) {
-        ++out.unresolved;
-        closed_min = std::min(closed_min, lb);
+        auto fixed = std::make_unique<Node>(Node{lb, node.depth, node.lower, node.upper});
+        for (int j = 0; j < n; ++j)
+          if (integer[j]) fixed->lower[j] = fixed->upper[j] = x[j];
+        to_fix.push_back(std::move(fixed));
       }
       return;
     }
@@ -179,6 +222,7 @@ MipResult solve_mip(const Model& md, const std::vector<char>& integer, const Mip
   {
     const Node root_node{-kInf, 0, md.col_lower, md.col_upper};
     handle(root_node, root);
+    fix_and_solve();
   }
 
   out.status = Status::kOptimal;
@@ -218,6 +262,7 @@ MipResult solve_mip(const Model& md, const std::vector<char>& integer, const Mip
     ++out.waves;
     out.nodes += static_cast<long long>(wave.size());
     for (size_t k = 0; k < wave.size(); ++k) handle(*wave[k], rs[k]);
+    fix_and_solve();
     if (opt.log_level >= 1) {
       double open_min = kInf;
       for (const NodePtr& node : heap) open_min = std::min(open_min, node->bound);

@@ -42,6 +42,7 @@ struct MipResult {
   long long unresolved = 0;       // relaxations left without a point, or with an integral one that
                                   // missed a row (their bounds stay in the search's bound)
   long long inexact = 0;          // relaxations stopped short of optimality, branched on their point
+  long long fixed_solves = 0;     // batches of continuous parts with the integer columns fixed
   long long lp_iterations = 0;
   double seconds = 0.0;
 };

@@ -56,6 +56,7 @@ class MipResult:
     integral: int
     unresolved: int
     inexact: int
+    fixed_solves: int
     lp_iterations: int
     seconds: float
     message: str
@@ -79,4 +80,5 @@ def solve_mip(model: Model | NativeModel, integer=None, options: MipOptions | No
     has = not math.isnan(res.objective)
     return MipResult(res.status, res.objective, res.bound, res.gap, x if has else None, res.nodes,
                      res.waves, res.pruned_infeasible, res.pruned_bound, res.integral, res.unresolved,
-                     res.inexact, res.lp_iterations, res.seconds, res.message.decode(errors="replace"))
+                     res.inexact, res.fixed_solves, res.lp_iterations, res.seconds,
+                     res.message.decode(errors="replace"))
