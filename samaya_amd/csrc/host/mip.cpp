@@ -62,16 +62,132 @@ bool feasible(const Model& md, const std::vector<double>& x, const Node& node, d
   return true;
 }
 
+// Column bounds implied by the rows (activity-based propagation, up to 10 passes): for row i with
+// bounds [L, U] and entry a of column j, the other entries' least and greatest activities bound
+// a x_j from both sides. A new bound is relaxed by 1e-9 (1 + |bound|) plus 1e-11 of the row's
+// magnitude over |a| (far above the rounding of its sums) so that it never cuts off a feasible
+// point, and an integer column's is rounded inward (1e-6 slack). Every point that
+// satisfies the rows and the old bounds satisfies the new ones, so the MIP keeps its optimum,
+// while its relaxations get boxed columns (and two-sided rows, below), whose safe dual bounds are
+// finite. Returns false when some column's bounds cross (the MIP is infeasible).
+bool propagate_bounds(Model& md, const std::vector<char>& integer) {
+  const int m = md.m, n = md.n;
+  // Rows of A (CSR).
+  std::vector<int64_t> start(static_cast<size_t>(m) + 1, 0);
+  for (int64_t p = 0; p < md.A.start[n]; ++p) ++start[md.A.index[p] + 1];
+  for (int i = 0; i < m; ++i) start[i + 1] += start[i];
+  std::vector<int> col(static_cast<size_t>(md.A.start[n]));
+  std::vector<double> val(col.size());
+  {
+    std::vector<int64_t> next(start.begin(), start.end() - 1);
+    for (int j = 0; j < n; ++j)
+      for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+        const int64_t q = next[md.A.index[p]]++;
+        col[q] = j;
+        val[q] = md.A.value[p];
+      }
+  }
+  std::vector<double>& lo = md.col_lower;
+  std::vector<double>& up = md.col_upper;
+  const auto tighten = [&](int j, double b, double row_slack, bool upper) -> bool {
+    if (!std::isfinite(b)) return false;
+    const double slack = 1e-9 * (1.0 + std::fabs(b)) + row_slack;
+    if (upper) {
+      double v = b + slack;
+      if (integer[j]) v = std::floor(b + slack + 1e-6);
+      if (!(up[j] < kInf) || v < up[j] - 1e-9 * (1.0 + std::fabs(up[j]))) {
+        up[j] = v;
+        return true;
+      }
+    } else {
+      double v = b - slack;
+      if (integer[j]) v = std::ceil(b - slack - 1e-6);
+      if (!(lo[j] > -kInf) || v > lo[j] + 1e-9 * (1.0 + std::fabs(lo[j]))) {
+        lo[j] = v;
+        return true;
+      }
+    }
+    return false;
+  };
+  for (int pass = 0; pass < 10; ++pass) {
+    bool changed = false;
+    for (int i = 0; i < m; ++i) {
+      // Least and greatest activities: finite parts and counts of infinite terms.
+      double min_f = 0.0, max_f = 0.0, mag = 0.0;
+      int min_inf = 0, max_inf = 0;
+      for (int64_t q = start[i]; q < start[i + 1]; ++q) {
+        const double a = val[q], l = lo[col[q]], u = up[col[q]];
+        const double lmin = a > 0 ? l : u, lmax = a > 0 ? u : l;
+        if (std::isfinite(lmin)) {
+          min_f += a * lmin;
+          mag += std::fabs(a * lmin);
+        } else {
+          ++min_inf;
+        }
+        if (std::isfinite(lmax)) {
+          max_f += a * lmax;
+          mag += std::fabs(a * lmax);
+        } else {
+          ++max_inf;
+        }
+      }
+      if (std::isfinite(md.row_lower[i])) mag += std::fabs(md.row_lower[i]);
+      if (std::isfinite(md.row_upper[i])) mag += std::fabs(md.row_upper[i]);
+      for (int64_t q = start[i]; q < start[i + 1]; ++q) {
+        const int j = col[q];
+        const double a = val[q];
+        if (a == 0.0) continue;
+        const double lmin = a > 0 ? lo[j] : up[j], lmax = a > 0 ? up[j] : lo[j];
+        // The other entries' least / greatest activity (infinite unless every other term is finite).
+        const double rest_min = min_inf - (std::isfinite(lmin) ? 0 : 1) > 0 ? -kInf
+                                : min_f - (std::isfinite(lmin) ? a * lmin : 0.0);
+        const double rest_max = max_inf - (std::isfinite(lmax) ? 0 : 1) > 0 ? kInf
+                                : max_f - (std::isfinite(lmax) ? a * lmax : 0.0);
+        const double row_slack = 1e-11 * mag / std::fabs(a);
+        if (std::isfinite(md.row_upper[i]) && std::isfinite(rest_min))
+          changed |= tighten(j, (md.row_upper[i] - rest_min) / a, row_slack, a > 0);
+        if (std::isfinite(md.row_lower[i]) && std::isfinite(rest_max))
+          changed |= tighten(j, (md.row_lower[i] - rest_max) / a, row_slack, a < 0);
+      }
+    }
+    for (int j = 0; j < n; ++j)
+      if (lo[j] > up[j] + 1e-9 * (1.0 + std::fabs(up[j]))) return false;
+    if (!changed) break;
+  }
+  // A row's missing bound becomes its least / greatest activity over the column bounds (relaxed
+  // as above): every point within the column bounds satisfies it, and the rows' terms of the
+  // safe dual bound are then finite whatever the sign of their duals.
+  for (int i = 0; i < m; ++i) {
+    double min_a = 0.0, max_a = 0.0, mag = 0.0;
+    for (int64_t q = start[i]; q < start[i + 1]; ++q) {
+      const double a = val[q], l = lo[col[q]], u = up[col[q]];
+      min_a += a > 0 ? a * l : a * u;
+      max_a += a > 0 ? a * u : a * l;
+      mag += std::fabs(a) * std::max(std::fabs(l), std::fabs(u));
+    }
+    const double slack = 1e-9 * (1.0 + mag);
+    if (!(md.row_lower[i] > -kInf) && std::isfinite(min_a)) md.row_lower[i] = min_a - slack;
+    if (!(md.row_upper[i] < kInf) && std::isfinite(max_a)) md.row_upper[i] = max_a + slack;
+  }
+  return true;
+}
+
 }  // namespace
 
-MipResult solve_mip(const Model& md, const std::vector<char>& integer, const MipOptions& opt) {
+MipResult solve_mip(const Model& original, const std::vector<char>& integer, const MipOptions& opt) {
   const auto t0 = std::chrono::steady_clock::now();
   const auto seconds = [&]() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
+  MipResult out;
+  // The search runs on the model with the column bounds its rows imply.
+  Model md = original;
+  if (!propagate_bounds(md, integer)) {
+    out.status = Status::kInfeasible;
+    return out;
+  }
   const int n = md.n;
   const double sense = md.maximize ? -1.0 : 1.0;  // minimization sense: sense * objective
-  MipResult out;
   double incumbent = kInf;  // minimization sense
   // The smallest bound of the nodes closed without being beaten by the incumbent: pruned within
   // the gap tolerance, or left unresolved (their parent's bound).

@@ -482,6 +482,20 @@ def mip_fixtures() -> None:
     r = milp(np.array([1.0, 1.0]), constraints=LinearConstraint(np.array([[2.0, 2.0]]), [1.0], [1.0]),
              bounds=Bounds([0.0, 0.0], [1.0, 1.0]), integrality=np.ones(2))
     cases.append({"handmade": "2 x1 + 2 x2 = 1", "status": 1 if r.status == 2 else r.status})
+    # min x1 + x2 + x3 s.t. 2 x1 + 2 x2 + 2 x3 = 3, x integer in [0, 1]: no integer point, and the
+    # rows imply no tighter bounds (only branching shows it).
+    r = milp(np.ones(3), constraints=LinearConstraint(np.array([[2.0, 2.0, 2.0]]), [3.0], [3.0]),
+             bounds=Bounds(np.zeros(3), np.ones(3)), integrality=np.ones(3))
+    cases.append({"handmade": "2 x1 + 2 x2 + 2 x3 = 3", "status": 1 if r.status == 2 else r.status})
+    # The reference's own MIP instance (its tests/instances file, copied under golden/mps).
+    mps = smy.read_mps(HERE / "mps" / "mrpl_crude_small.mps")
+    A = sp.csc_matrix((mps.value, mps.index, mps.start), shape=(mps.m, mps.n))
+    r = milp(-mps.obj if mps.maximize else mps.obj,
+             constraints=LinearConstraint(A, mps.row_lower, mps.row_upper),
+             bounds=Bounds(mps.col_lower, mps.col_upper), integrality=np.asarray(mps.col_integer, int))
+    assert r.status == 0
+    cases.append({"mps": "mrpl_crude_small.mps", "status": 0,
+                  "objective": float((-r.fun if mps.maximize else r.fun) + mps.offset)})
     # The bench's MIP workload (bench.py --workload mip).
     md = G.knapsack(7101, 10, 60, 4)
     A = sp.csc_matrix((md.value, md.index, md.start), shape=(md.m, md.n))

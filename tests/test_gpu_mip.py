@@ -1,8 +1,9 @@
 """Branch and bound over batched PDLP node relaxations (smy_mip_solve) against HiGHS's MILP
 (golden mip_knapsack.json, scipy.optimize.milp): the same optimal objective (the data are
 integers, so a gap closed at 1e-4 leaves no other value), an incumbent that is integral and
-satisfies every row and bound, a bound that brackets it within the gap, and a MIP whose LP is
-feasible but which has no integer point reported infeasible."""
+satisfies every row and bound, a bound that brackets it within the gap, and MIPs whose LP is
+feasible but which have no integer point reported infeasible (by the rows' implied bounds, or by
+branching)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -12,6 +13,7 @@ from conftest import load_golden
 
 pytestmark = pytest.mark.gpu
 G = smy.generators
+GOLDEN_MPS = __import__("pathlib").Path(__file__).parent / "golden" / "mps"
 
 
 def knapsack_case(case):
@@ -53,18 +55,43 @@ def test_knapsack_mips_match_highs(gpu, halpern):
     assert waves > 0  # nodes were bounded in batches
 
 
-def test_integer_infeasible_mip(gpu):
-    # min x1 + x2 s.t. 2 x1 + 2 x2 = 1, x1, x2 integer in [0, 1]: the LP optimum is 0.5, no integer
-    # point exists, and every branch ends in an infeasible relaxation.
+def test_the_reference_mip_instance(gpu):
+    # mrpl_crude_small.mps from the reference's tests/instances (164 rows, 251 columns, 83 integer,
+    # columns not all boxed): the rows' implied bounds box every column, so the safe dual bounds
+    # prune, and the search closes at HiGHS's optimum.
     fx = load_golden("mip_knapsack")
-    case = [c for c in fx["cases"] if "handmade" in c][0]
-    assert case["status"] == smy.INFEASIBLE
-    md = smy.Model(1, 2, np.array([0, 1, 2], np.int64), np.array([0, 0], np.int32), np.array([2.0, 2.0]),
-                   np.array([1.0, 1.0]), np.zeros(2), np.ones(2), np.array([1.0]), np.array([1.0]),
-                   False, 0.0)
-    r = smy.solve_mip(md, [True, True], smy.MipOptions(lp=smy.PdlpOptions(tol=1e-8, verify=False)))
-    assert r.status == smy.INFEASIBLE, r
-    assert r.col_value is None and r.pruned_infeasible >= 2
+    want = [c for c in fx["cases"] if c.get("mps") == "mrpl_crude_small.mps"][0]["objective"]
+    md = smy.read_mps_native(GOLDEN_MPS / "mrpl_crude_small.mps")
+    opts = smy.MipOptions(lp=smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced",
+                                             halpern=True, verify=False), gap_tol=1e-4, time_limit=120.0)
+    r = smy.solve_mip(md, None, opts)
+    print(f"mrpl_crude_small: {r.objective!r} (HiGHS {want!r}), bound {r.bound!r}, {r.nodes} nodes, "
+          f"{r.waves} waves, {r.fixed_solves} fixed-integer batches, {r.seconds:.1f} s")
+    assert r.status == smy.OPTIMAL, r
+    assert abs(r.objective - want) <= 1e-6 * (1 + abs(want))
+    assert r.objective - 1e-6 * abs(r.objective) <= r.bound <= r.objective + 1e-4 * (1 + abs(r.objective))
+
+
+def equality_mip(k: int, rhs: float):
+    """min sum x s.t. 2 x_1 + ... + 2 x_k = rhs, x integer in [0, 1]."""
+    return smy.Model(1, k, np.arange(k + 1, dtype=np.int64), np.zeros(k, np.int32), np.full(k, 2.0),
+                     np.ones(k), np.zeros(k), np.ones(k), np.array([rhs]), np.array([rhs]), False, 0.0)
+
+
+def test_integer_infeasible_mips(gpu):
+    # Both have a feasible LP (x = 1/4 each, x = 1/2 each) and no integer point (HiGHS agrees,
+    # golden mip_knapsack). 2 x1 + 2 x2 = 1: the rows' implied bounds already cross (x_j <= 1/2
+    # rounds to 0, then x_j >= 1/2 to 1), before any relaxation. 2 x1 + 2 x2 + 2 x3 = 3: the implied
+    # bounds are the given ones, and the branches end in infeasible relaxations.
+    fx = load_golden("mip_knapsack")
+    hand = {c["handmade"]: c["status"] for c in fx["cases"] if "handmade" in c}
+    opts = smy.MipOptions(lp=smy.PdlpOptions(tol=1e-8, verify=False))
+    assert hand["2 x1 + 2 x2 = 1"] == smy.INFEASIBLE
+    r = smy.solve_mip(equality_mip(2, 1.0), [True] * 2, opts)
+    assert r.status == smy.INFEASIBLE and r.col_value is None and r.nodes == 0, r
+    assert hand["2 x1 + 2 x2 + 2 x3 = 3"] == smy.INFEASIBLE
+    r = smy.solve_mip(equality_mip(3, 3.0), [True] * 3, opts)
+    assert r.status == smy.INFEASIBLE and r.col_value is None and r.pruned_infeasible >= 2, r
 
 
 def test_mip_refuses_a_model_without_integer_columns(gpu):
