@@ -56,6 +56,16 @@ void smy_mip_params_default(smy_mip_params* params);
 int smy_mip_solve(const smy_model* model, const unsigned char* integer, const smy_mip_params* params,
                   smy_mip_result* result, double* col_value);
 
+/* The bounds the search starts from: column bounds tightened to those the rows imply
+ * (activity-based propagation, up to 10 passes, relaxed by margins that never cut off a point that
+ * satisfies the rows; integer columns rounded inward) and each one-sided row's missing bound set to
+ * its activity range. Outputs n and m doubles (any may be null); *crossed = 1 when some column's
+ * bounds cross (no point satisfies the rows; the outputs are then partial), else 0. integer: n
+ * flags or null (the model's own integrality). No reference counterpart (samaya's search
+ * propagates at every node, src/mip/branch_and_bound.hpp). */
+int smy_mip_propagate(const smy_model* model, const unsigned char* integer, double* col_lower,
+                      double* col_upper, double* row_lower, double* row_upper, int* crossed);
+
 #ifdef __cplusplus
 }
 #endif

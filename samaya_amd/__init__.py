@@ -13,7 +13,7 @@ from .lp import (Model, NativeModel, LpProblem, compute_scaling, make_problem, P
                  INFEASIBLE, UNBOUNDED, ITERATION_LIMIT, TIME_LIMIT, NUMERICAL_ERROR)
 from .mps import (MpsParseError, read_mps, read_mps_text, read_mps_native, read_mps_text_native,
                   mps_text, write_mps, solution_text, write_solution)
-from .mip import MipOptions, MipResult, solve_mip
+from .mip import MipOptions, MipResult, solve_mip, propagate_bounds
 from . import generators
 
 __all__ = [
@@ -25,5 +25,5 @@ __all__ = [
     "gpu_info", "STATUS_NAMES", "OPTIMAL", "INFEASIBLE", "UNBOUNDED", "ITERATION_LIMIT",
     "TIME_LIMIT", "NUMERICAL_ERROR", "generators", "MpsParseError", "read_mps", "read_mps_text",
     "read_mps_native", "read_mps_text_native", "mps_text", "write_mps", "solution_text",
-    "write_solution", "MipOptions", "MipResult", "solve_mip",
+    "write_solution", "MipOptions", "MipResult", "solve_mip", "propagate_bounds",
 ]

@@ -45,7 +45,7 @@ EXPORTS = (
     "smy_read_mps", "smy_read_mps_text", "smy_model_name", "smy_model_names",
     "smy_model_set_names", "smy_model_set_integrality", "smy_model_integrality", "smy_write_mps",
     "smy_write_mps_text", "smy_write_solution", "smy_solution_text",
-    "smy_mip_params_default", "smy_mip_solve",
+    "smy_mip_params_default", "smy_mip_solve", "smy_mip_propagate",
 )
 
 
@@ -225,6 +225,7 @@ def load() -> C.CDLL:
     _proto(lib, "smy_generate_fixings", i32, C.c_uint64, i32, f64, f64, vp, vp, vp, vp)
     _proto(lib, "smy_mip_params_default", None, C.POINTER(SmyMipParams))
     _proto(lib, "smy_mip_solve", i32, vp, vp, C.POINTER(SmyMipParams), C.POINTER(SmyMipResult), vp)
+    _proto(lib, "smy_mip_propagate", i32, vp, vp, vp, vp, vp, vp, C.POINTER(i32))
     _proto(lib, "smy_solve", i32, vp, C.POINTER(f64))
     _proto(lib, "smy_verify_lp", i32, vp, vp, vp, f64, f64, vp, C.c_char_p, C.c_size_t)
     _proto(lib, "smy_verify_infeasibility", i32, vp, vp, C.c_char_p, C.c_size_t)

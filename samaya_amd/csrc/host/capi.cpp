@@ -525,6 +525,29 @@ int smy_mip_solve(const smy_model* model, const unsigned char* integer, const sm
   }
 }
 
+int smy_mip_propagate(const smy_model* model, const unsigned char* integer, double* col_lower,
+                      double* col_upper, double* row_lower, double* row_upper, int* crossed) {
+  if (model == nullptr || crossed == nullptr) return SMY_INVALID;
+  const int n = model->model.n;
+  std::vector<char> flags(static_cast<size_t>(n), 0);
+  if (integer != nullptr) {
+    for (int j = 0; j < n; ++j) flags[j] = integer[j] != 0;
+  } else if (static_cast<int>(model->col_integer.size()) == n) {
+    flags.assign(model->col_integer.begin(), model->col_integer.end());
+  }
+  try {
+    Model md = model->model;
+    *crossed = propagate_bounds(md, flags) ? 0 : 1;
+    if (col_lower != nullptr) put(md.col_lower, col_lower);
+    if (col_upper != nullptr) put(md.col_upper, col_upper);
+    if (row_lower != nullptr) put(md.row_lower, row_lower);
+    if (row_upper != nullptr) put(md.row_upper, row_upper);
+    return SMY_OK;
+  } catch (const std::exception&) {
+    return SMY_ERROR;
+  }
+}
+
 int smy_generate_fixings(uint64_t seed, int n, double frac_min, double frac_max,
                          const double* lower, const double* upper, double* out_lower,
                          double* out_upper) {

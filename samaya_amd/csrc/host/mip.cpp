@@ -70,6 +70,8 @@ bool feasible(const Model& md, const std::vector<double>& x, const Node& node, d
 // satisfies the rows and the old bounds satisfies the new ones, so the MIP keeps its optimum,
 // while its relaxations get boxed columns (and two-sided rows, below), whose safe dual bounds are
 // finite. Returns false when some column's bounds cross (the MIP is infeasible).
+}  // namespace
+
 bool propagate_bounds(Model& md, const std::vector<char>& integer) {
   const int m = md.m, n = md.n;
   // Rows of A (CSR).
@@ -171,8 +173,6 @@ bool propagate_bounds(Model& md, const std::vector<char>& integer) {
   }
   return true;
 }
-
-}  // namespace
 
 MipResult solve_mip(const Model& original, const std::vector<char>& integer, const MipOptions& opt) {
   const auto t0 = std::chrono::steady_clock::now();

@@ -50,4 +50,8 @@ struct MipResult {
 // `integer`: n flags (nonzero: the column must take an integer value).
 MipResult solve_mip(const Model& model, const std::vector<char>& integer, const MipOptions& options);
 
+// The column bounds the rows imply (and the activity range as a one-sided row's missing bound),
+// written into `model`; false when some column's bounds cross (no point satisfies the rows).
+bool propagate_bounds(Model& model, const std::vector<char>& integer);
+
 }  // namespace smy::host

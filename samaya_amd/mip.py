@@ -82,3 +82,16 @@ def solve_mip(model: Model | NativeModel, integer=None, options: MipOptions | No
                      res.waves, res.pruned_infeasible, res.pruned_bound, res.integral, res.unresolved,
                      res.inexact, res.fixed_solves, res.lp_iterations, res.seconds,
                      res.message.decode(errors="replace"))
+
+
+def propagate_bounds(model: Model | NativeModel, integer=None):
+    """(col_lower, col_upper, row_lower, row_upper, crossed): the bounds solve_mip starts from
+    (smy_mip_propagate): column bounds the rows imply, and one-sided rows' activity ranges."""
+    nm = model if isinstance(model, NativeModel) else model.native()
+    flags = None if integer is None else np.ascontiguousarray(np.asarray(integer, dtype=bool), np.uint8)
+    lo, up = np.zeros(nm.n), np.zeros(nm.n)
+    rlo, rup = np.zeros(nm.m), np.zeros(nm.m)
+    crossed = C.c_int()
+    N.check(N.load().smy_mip_propagate(nm.h, _p(flags), _p(lo), _p(up), _p(rlo), _p(rup),
+                                       C.byref(crossed)), "mip_propagate")
+    return lo, up, rlo, rup, bool(crossed.value)
