@@ -72,25 +72,41 @@ bool feasible(const Model& md, const std::vector<double>& x, const Node& node, d
 // finite. Returns false when some column's bounds cross (the MIP is infeasible).
 }  // namespace
 
-bool propagate_bounds(Model& md, const std::vector<char>& integer) {
+namespace {
+
+// The rows of A (CSR), for propagation.
+struct Rows {
+  std::vector<int64_t> start;
+  std::vector<int> col;
+  std::vector<double> val;
+};
+
+Rows rows_of(const Model& md) {
   const int m = md.m, n = md.n;
-  // Rows of A (CSR).
-  std::vector<int64_t> start(static_cast<size_t>(m) + 1, 0);
-  for (int64_t p = 0; p < md.A.start[n]; ++p) ++start[md.A.index[p] + 1];
-  for (int i = 0; i < m; ++i) start[i + 1] += start[i];
-  std::vector<int> col(static_cast<size_t>(md.A.start[n]));
-  std::vector<double> val(col.size());
-  {
-    std::vector<int64_t> next(start.begin(), start.end() - 1);
-    for (int j = 0; j < n; ++j)
-      for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
-        const int64_t q = next[md.A.index[p]]++;
-        col[q] = j;
-        val[q] = md.A.value[p];
-      }
-  }
-  std::vector<double>& lo = md.col_lower;
-  std::vector<double>& up = md.col_upper;
+  Rows R;
+  R.start.assign(static_cast<size_t>(m) + 1, 0);
+  for (int64_t p = 0; p < md.A.start[n]; ++p) ++R.start[md.A.index[p] + 1];
+  for (int i = 0; i < m; ++i) R.start[i + 1] += R.start[i];
+  R.col.resize(static_cast<size_t>(md.A.start[n]));
+  R.val.resize(R.col.size());
+  std::vector<int64_t> next(R.start.begin(), R.start.end() - 1);
+  for (int j = 0; j < n; ++j)
+    for (int64_t p = md.A.start[j]; p < md.A.start[j + 1]; ++p) {
+      const int64_t q = next[md.A.index[p]]++;
+      R.col[q] = j;
+      R.val[q] = md.A.value[p];
+    }
+  return R;
+}
+
+// The passes of propagate_bounds over column bounds lo / up (the model's rows and row bounds);
+// false when some column's bounds cross.
+bool tighten_columns(const Model& md, const Rows& R, const std::vector<char>& integer,
+                     std::vector<double>& lo, std::vector<double>& up) {
+  const int m = md.m, n = md.n;
+  const std::vector<int64_t>& start = R.start;
+  const std::vector<int>& col = R.col;
+  const std::vector<double>& val = R.val;
   const auto tighten = [&](int j, double b, double row_slack, bool upper) -> bool {
     if (!std::isfinite(b)) return false;
     const double slack = 1e-9 * (1.0 + std::fabs(b)) + row_slack;
@@ -156,10 +172,23 @@ bool propagate_bounds(Model& md, const std::vector<char>& integer) {
       if (lo[j] > up[j] + 1e-9 * (1.0 + std::fabs(up[j]))) return false;
     if (!changed) break;
   }
+  return true;
+}
+
+}  // namespace
+
+bool propagate_bounds(Model& md, const std::vector<char>& integer) {
+  const Rows R = rows_of(md);
+  if (!tighten_columns(md, R, integer, md.col_lower, md.col_upper)) return false;
+  const std::vector<int64_t>& start = R.start;
+  const std::vector<int>& col = R.col;
+  const std::vector<double>& val = R.val;
+  const std::vector<double>& lo = md.col_lower;
+  const std::vector<double>& up = md.col_upper;
   // A row's missing bound becomes its least / greatest activity over the column bounds (relaxed
   // as above): every point within the column bounds satisfies it, and the rows' terms of the
   // safe dual bound are then finite whatever the sign of their duals.
-  for (int i = 0; i < m; ++i) {
+  for (int i = 0; i < md.m; ++i) {
     double min_a = 0.0, max_a = 0.0, mag = 0.0;
     for (int64_t q = start[i]; q < start[i + 1]; ++q) {
       const double a = val[q], l = lo[col[q]], u = up[col[q]];
@@ -188,6 +217,10 @@ MipResult solve_mip(const Model& original, const std::vector<char>& integer, con
   }
   const int n = md.n;
   const double sense = md.maximize ? -1.0 : 1.0;  // minimization sense: sense * objective
+  // Each node's bounds are propagated again before its relaxation (a branching often implies
+  // more), on models small enough that the host passes cost little beside the batch.
+  const bool node_propagation = md.A.nnz() <= 200000;
+  const Rows rows = node_propagation ? rows_of(md) : Rows{};
   double incumbent = kInf;  // minimization sense
   // The smallest bound of the nodes closed without being beaten by the incumbent: pruned within
   // the gap tolerance, or left unresolved (their parent's bound).
@@ -364,6 +397,10 @@ MipResult solve_mip(const Model& original, const std::vector<char>& integer, con
         if (node->bound < incumbent) closed_min = std::min(closed_min, node->bound);
         heap.clear();
         break;
+      }
+      if (node_propagation && !tighten_columns(md, rows, integer, node->lower, node->upper)) {
+        ++out.pruned_infeasible;  // the branching leaves no point that satisfies the rows
+        continue;
       }
       wave.push_back(std::move(node));
     }
