@@ -531,6 +531,7 @@ def main() -> int:
                                  "reference's own; FP64 polish rounds do not reach it either")
     if ref is not None:
         line["reference_fp64_objective"] = ref["objective"]
+        line["reference_fp64_iterations"] = ref["iterations"]
         line["rel_objective_error_vs_reference_fp64"] = relerr(last.objective, ref["objective"])
         if ref.get("highs_objective") is not None:
             line["highs_objective"] = ref["highs_objective"]
