@@ -78,7 +78,7 @@ struct PdlpOptions {
   // Restarted Halpern PDHG with reflection (Lu and Yang 2024, as in cuPDLPx) instead of restarted
   // PDHG with averages: constant steps 0.99 / L, the anchored step
   // z' = lambda ((1 + reflection) T(z) - reflection z) + (1 - lambda) z0, restarts decided on the
-  // fixed-point residual and made to T(z). Overrides adaptive_steps; batches use the stacked layout.
+  // fixed-point residual and made to T(z). Overrides adaptive_steps; both batch layouts take it.
   bool halpern = false;
   double reflection = 1.0;
   // The primal weight at a restart follows log(dy / dx) (the distances moved since the last
