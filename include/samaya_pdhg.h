@@ -197,7 +197,8 @@ int smy_pdhg_detect_batch(smy_pdhg* h, const int* flags, double* out);
 /* The projected ray (n_i structural entries) of instance i's last detection. */
 int smy_pdhg_download_ray(smy_pdhg* h, int instance, double* ray);
 /* The dual (m_i entries, the y_erg space) that the last detection with a flag above 1 tested for
- * instance i: the Farkas certificate when its test passed. No reference counterpart. */
+ * instance i, when that detection flagged instance i (its contents are unspecified otherwise): the
+ * Farkas certificate when its test passed. No reference counterpart. */
 int smy_pdhg_download_farkas(smy_pdhg* h, int instance, double* y);
 
 /* Adaptive step sizes (no reference counterpart: the reference's adaptive step is disabled code,
