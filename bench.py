@@ -299,7 +299,7 @@ def copy_calibration() -> int:
 
 
 BUDGET_S = 480.0  # the default run ends well inside 10 minutes: sub-results past this are skipped
-SUB_COST_S = {"c4": 110.0, "c5": 60.0}  # measured cost of each sub-result, with margin
+SUB_COST_S = {"c4": 110.0, "c5": 60.0, "mip": 30.0}  # measured cost of each sub-result, with margin
 
 
 def relerr(a, b) -> float | None:
@@ -319,7 +319,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-spmv", action="store_true")
     ap.add_argument("--no-batches", action="store_true",
-                    help="the default (c3) line without its c4 / c5 sub-results")
+                    help="the default (c3) line without its c4 / c5 / mip sub-results")
     ap.add_argument("--polish-rounds", type=int, default=0,
                     help="c3: also time one solve with this many FP64 polish rounds (the "
                          "continuation at tol / 10 after a verifier rejection)")
@@ -553,14 +553,15 @@ def main() -> int:
     if spmv is not None:
         line["spmv"] = spmv
     if not args.no_batches and args.scale == 1.0 and not capped:
-        for name, fn in (("c4", c4_result), ("c5", c5_result)):
+        for name, fn in (("c4", c4_result), ("c5", c5_result), ("mip", mip_result)):
             spent = time.perf_counter() - t_begin
             if spent + SUB_COST_S[name] > BUDGET_S:
                 line[name] = {"skipped": "budget", "elapsed_s": round(spent, 1)}
                 log(f"{name}: skipped ({spent:.0f} s spent of a {BUDGET_S:.0f} s budget)")
                 continue
             t1 = time.perf_counter()
-            kw = {"fp64_every": 32, "cpu_probe_every": 4} if name == "c4" else {"feasible_sample": 16, "infeasible_sample": 8}
+            kw = ({"fp64_every": 32, "cpu_probe_every": 4} if name == "c4" else
+                  {"feasible_sample": 16, "infeasible_sample": 8} if name == "c5" else {})
             line[name] = fn(args, **kw)
             line[name]["elapsed_s"] = round(time.perf_counter() - t1, 1)
     print(json.dumps(line), flush=True)
@@ -916,18 +917,18 @@ def batch_line(args, workload: str) -> int:
     return 0
 
 
-def mip_line(args) -> int:
-    """--workload mip: a knapsack MIP (m 10, n 60, all columns binary) by branch and bound whose
-    node relaxations are bounded in waves of 256 by batched PDLP (smy_mip_solve); value = seconds
-    until the gap closed at 1e-4. Compared with the same search bounding one node per batch (a
-    time-capped sample: nodes per second, scaled to the node count) and with HiGHS's optimum
-    (mip_knapsack.json)."""
+def mip_result(args, steps: int = 1) -> dict:
+    """A knapsack MIP (m 10, n 60, all columns binary) by branch and bound whose node relaxations
+    are bounded in waves of 256 by batched PDLP (smy_mip_solve): the fields of its line (value =
+    seconds until the gap closed at 1e-4). Compared with the same search bounding one node per
+    batch (a time-capped sample: nodes per second, scaled to the node count) and with HiGHS's
+    optimum (mip_knapsack.json)."""
     import samaya_amd as smy
     md = smy.generators.knapsack(7101, 10, 60, 4)
     lp = smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced", halpern=True, verify=False)
     smy.solve_mip(smy.generators.knapsack(7001, 3, 30, 3), np.ones(30, bool), smy.MipOptions(lp=lp, wave=64))
     runs = []
-    for _ in range(max(1, args.steps)):
+    for _ in range(max(1, steps)):
         r = smy.solve_mip(md, np.ones(md.n, bool), smy.MipOptions(lp=lp, wave=256))
         runs.append(r)
         log(f"mip: {r.status_name}, objective {r.objective}, bound {r.bound:.6f}, {r.nodes} nodes in "
@@ -939,7 +940,8 @@ def mip_line(args) -> int:
     fx = golden("mip_knapsack") or {}
     highs = fx.get("bench", {}).get("objective")
     value = float(np.mean([t.seconds for t in runs]))
-    extra = {
+    return {
+        "value": round(value, 4), "unit": "s", "higher_is_better": False,
         "status": r.status_name, "objective": r.objective, "bound": r.bound, "gap": r.gap,
         "nodes": r.nodes, "waves": r.waves, "nodes_per_s": round(r.nodes / r.seconds, 1),
         "pruned_bound": r.pruned_bound, "pruned_infeasible": r.pruned_infeasible,
@@ -951,10 +953,18 @@ def mip_line(args) -> int:
         "rel_objective_error_vs_highs": (abs(r.objective - highs) / (1 + abs(highs))
                                          if highs is not None else None),
         "roofline": None, "cpu_baseline": None,
+        "config": {"workload": "knapsack MIP m=10 n=60 (binary), branch and bound with waves of 256 "
+                               "node relaxations on batched PDLP (Halpern, mixed), gap 1e-4"},
     }
-    line = base_line(args, value, value * 1e3, "knapsack MIP m=10 n=60 (binary), branch and bound "
-                     "with waves of 256 node relaxations on batched PDLP (Halpern, mixed), gap 1e-4",
-                     extra)
+
+
+def mip_line(args) -> int:
+    """--workload mip: the MIP sub-result as its own line."""
+    res = mip_result(args, args.steps)
+    value = res.pop("value")
+    for k in ("unit", "higher_is_better"):
+        res.pop(k, None)
+    line = base_line(args, value, value * 1e3, res["config"]["workload"], res)
     line["warmup"] = 1
     print(json.dumps(line), flush=True)
     return 0
