@@ -100,3 +100,20 @@ def test_mip_refuses_a_model_without_integer_columns(gpu):
     md = G.knapsack(7001, 3, 30, 3)
     with pytest.raises(smy.NativeError):
         smy.solve_mip(md, np.zeros(md.n, bool))
+
+
+def test_minimization_form(gpu):
+    # The first knapsack as a minimization of the negated profits: the same search in the other
+    # sense reaches minus HiGHS's optimum, with its bound (now a lower bound) below the incumbent.
+    fx = load_golden("mip_knapsack")
+    case = [c for c in fx["cases"] if "knapsack" in c][0]
+    md = knapsack_case(case)
+    mn = smy.Model(md.m, md.n, md.start, md.index, md.value, -md.obj, md.col_lower, md.col_upper,
+                   md.row_lower, md.row_upper, False, 0.0)
+    opts = smy.MipOptions(lp=smy.PdlpOptions(tol=1e-6, precision="mixed", algorithm="enhanced",
+                                             halpern=True, verify=False), gap_tol=1e-4, wave=64)
+    r = smy.solve_mip(mn, np.ones(md.n, bool), opts)
+    want = -case["objective"]
+    assert r.status == smy.OPTIMAL, r
+    assert abs(r.objective - want) <= 1e-6 * (1 + abs(want))
+    assert r.objective - 1e-4 * (1 + abs(r.objective)) - 1e-9 <= r.bound <= r.objective + 1e-9
